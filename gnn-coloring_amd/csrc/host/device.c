@@ -1,0 +1,957 @@
+/*
+ * device.c -- C host layer: device objects (graphs, models, workspaces),
+ * stage sequencing and the Jones-Plassmann / SL round drivers. All compute
+ * runs in the HIP kernels behind gcxk.h; this file only allocates, copies,
+ * launches and polls.
+ */
+#include <hip/hip_runtime_api.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "internal.h"
+
+#define HIPC(expr)                                                              \
+    do {                                                                        \
+        if ((expr) != hipSuccess)                                               \
+            return GCX_ERR_HIP;                                                 \
+    } while (0)
+
+#define TRY(expr)                                                               \
+    do {                                                                        \
+        int rc_ = (expr);                                                       \
+        if (rc_ != GCX_OK)                                                      \
+            return rc_;                                                         \
+    } while (0)
+
+/* Rounds enqueued between two looks at the device's "done" flag. */
+#define ROUND_BATCH 32u
+
+static int g_init = 0;
+static int g_init_rc = GCX_OK;
+static hipStream_t g_stream = NULL;
+static uint32_t g_round_grid = 0;
+
+const char *gcx_status_string(int s)
+{
+    switch (s) {
+    case GCX_OK: return "ok";
+    case GCX_ERR_ARG: return "invalid argument";
+    case GCX_ERR_NOMEM: return "out of memory";
+    case GCX_ERR_HIP: return "HIP runtime error";
+    case GCX_ERR_CAPACITY: return "input exceeds capacity";
+    case GCX_ERR_IO: return "cannot open file";
+    case GCX_ERR_FORMAT: return "malformed file";
+    case GCX_ERR_NODEV: return "no gfx950 device";
+    case GCX_ERR_INVALID: return "validation failed";
+    case GCX_ERR_BUDGET: return "HBM budget exceeded";
+    default: return "unknown status";
+    }
+}
+
+const char *gcx_version(void) { return "gcx 0.1 (gfx950)"; }
+
+int gcx__ensure_init(void)
+{
+    if (g_init)
+        return g_init_rc;
+    g_init = 1;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1)
+        return g_init_rc = GCX_ERR_NODEV;
+    if (hipSetDevice(0) != hipSuccess)
+        return g_init_rc = GCX_ERR_NODEV;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, 0) != hipSuccess)
+        return g_init_rc = GCX_ERR_NODEV;
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        fprintf(stderr, "gcx: device 0 is %s, this build targets gfx950\n", prop.gcnArchName);
+        return g_init_rc = GCX_ERR_NODEV;
+    }
+    if (hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking) != hipSuccess)
+        return g_init_rc = GCX_ERR_HIP;
+    g_round_grid = gcxk_round_grid();
+    return g_init_rc = GCX_OK;
+}
+
+int gcx_init(void) { return gcx__ensure_init(); }
+
+void *gcx__default_stream(void) { return (void *)g_stream; }
+
+void *gcx__pick(void *stream) { return stream ? stream : (void *)g_stream; }
+
+/* ------------------------------------------------------------ streams/events */
+
+int gcx_stream_create(void **stream)
+{
+    TRY(gcx__ensure_init());
+    hipStream_t s;
+    HIPC(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = (void *)s;
+    return GCX_OK;
+}
+
+int gcx_stream_destroy(void *stream)
+{
+    if (stream)
+        HIPC(hipStreamDestroy((hipStream_t)stream));
+    return GCX_OK;
+}
+
+int gcx_stream_sync(void *stream)
+{
+    TRY(gcx__ensure_init());
+    HIPC(hipStreamSynchronize((hipStream_t)gcx__pick(stream)));
+    return GCX_OK;
+}
+
+int gcx_device_sync(void)
+{
+    TRY(gcx__ensure_init());
+    HIPC(hipDeviceSynchronize());
+    return GCX_OK;
+}
+
+int gcx_event_create(void **event)
+{
+    TRY(gcx__ensure_init());
+    hipEvent_t e;
+    HIPC(hipEventCreate(&e));
+    *event = (void *)e;
+    return GCX_OK;
+}
+
+int gcx_event_destroy(void *event)
+{
+    if (event)
+        HIPC(hipEventDestroy((hipEvent_t)event));
+    return GCX_OK;
+}
+
+int gcx_event_record(void *event, void *stream)
+{
+    HIPC(hipEventRecord((hipEvent_t)event, (hipStream_t)gcx__pick(stream)));
+    return GCX_OK;
+}
+
+int gcx_event_elapsed_ms(void *start, void *stop, float *ms)
+{
+    HIPC(hipEventSynchronize((hipEvent_t)stop));
+    HIPC(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+    return GCX_OK;
+}
+
+int gcx_mem_info(size_t *free_bytes, size_t *total_bytes)
+{
+    TRY(gcx__ensure_init());
+    HIPC(hipMemGetInfo(free_bytes, total_bytes));
+    return GCX_OK;
+}
+
+/* ------------------------------------------------------------------ profile */
+
+typedef struct {
+    int stage;
+    hipEvent_t a, b;
+} prof_rec;
+
+static int g_prof_on = 0;
+static prof_rec *g_prof = NULL;
+static size_t g_prof_n = 0, g_prof_cap = 0;
+static hipEvent_t *g_pool = NULL;
+static size_t g_pool_n = 0, g_pool_cap = 0;
+
+static hipEvent_t pool_get(void)
+{
+    if (g_pool_n)
+        return g_pool[--g_pool_n];
+    hipEvent_t e = NULL;
+    if (hipEventCreate(&e) != hipSuccess)
+        return NULL;
+    return e;
+}
+
+static void pool_put(hipEvent_t e)
+{
+    if (g_pool_n == g_pool_cap) {
+        size_t cap = g_pool_cap ? 2 * g_pool_cap : 64;
+        hipEvent_t *p = realloc(g_pool, cap * sizeof(*p));
+        if (!p) {
+            (void)hipEventDestroy(e);
+            return;
+        }
+        g_pool = p;
+        g_pool_cap = cap;
+    }
+    g_pool[g_pool_n++] = e;
+}
+
+int gcx_profile_enable(int on)
+{
+    g_prof_on = on != 0;
+    return GCX_OK;
+}
+
+int gcx__prof_begin(int stage, void *stream, void **token)
+{
+    *token = NULL;
+    if (!g_prof_on)
+        return GCX_OK;
+    if (g_prof_n == g_prof_cap) {
+        size_t cap = g_prof_cap ? 2 * g_prof_cap : 256;
+        prof_rec *p = realloc(g_prof, cap * sizeof(*p));
+        if (!p)
+            return GCX_ERR_NOMEM;
+        g_prof = p;
+        g_prof_cap = cap;
+    }
+    prof_rec *r = &g_prof[g_prof_n];
+    r->stage = stage;
+    r->a = pool_get();
+    r->b = pool_get();
+    if (!r->a || !r->b)
+        return GCX_ERR_HIP;
+    HIPC(hipEventRecord(r->a, (hipStream_t)stream));
+    g_prof_n++;
+    *token = (void *)(uintptr_t)g_prof_n; /* index + 1 */
+    return GCX_OK;
+}
+
+int gcx__prof_end(void *token, void *stream)
+{
+    if (!token)
+        return GCX_OK;
+    prof_rec *r = &g_prof[(uintptr_t)token - 1];
+    HIPC(hipEventRecord(r->b, (hipStream_t)stream));
+    return GCX_OK;
+}
+
+int gcx_profile_reset(void)
+{
+    for (size_t i = 0; i < g_prof_n; i++) {
+        pool_put(g_prof[i].a);
+        pool_put(g_prof[i].b);
+    }
+    g_prof_n = 0;
+    return GCX_OK;
+}
+
+int gcx_profile_read(double ms[GCX_ST_COUNT], uint64_t launches[GCX_ST_COUNT])
+{
+    for (int s = 0; s < GCX_ST_COUNT; s++) {
+        ms[s] = 0.0;
+        launches[s] = 0;
+    }
+    if (g_prof_n)
+        HIPC(hipDeviceSynchronize());
+    for (size_t i = 0; i < g_prof_n; i++) {
+        float t = 0.0f;
+        HIPC(hipEventElapsedTime(&t, g_prof[i].a, g_prof[i].b));
+        ms[g_prof[i].stage] += t;
+        launches[g_prof[i].stage] += 1;
+    }
+    return gcx_profile_reset();
+}
+
+/* -------------------------------------------------------------------- graph */
+
+uint64_t gcx_graph_footprint(uint32_t N, uint64_t M)
+{
+    return 4ull * ((uint64_t)N + 1) + 4ull * M + 4ull * N;
+}
+
+static int graph_finish(gcx_graph *g, void *stream)
+{
+    uint32_t *tmp = NULL;
+    HIPC(hipMalloc((void **)&g->hubs, 4ull * (g->N ? g->N : 1)));
+    HIPC(hipMalloc((void **)&tmp, 16));
+    int rc = gcxk_graph_stats(g->N, g->V, g->hubs, tmp, &g->nhub, &g->maxdeg, stream);
+    (void)hipFree(tmp);
+    return rc;
+}
+
+int gcx_graph_free(gcx_graph *g)
+{
+    if (!g)
+        return GCX_OK;
+    if (g->V)
+        (void)hipFree(g->V);
+    if (g->E)
+        (void)hipFree(g->E);
+    if (g->hubs)
+        (void)hipFree(g->hubs);
+    free(g);
+    return GCX_OK;
+}
+
+int gcx__graph_alloc(uint32_t N, uint64_t M, gcx_graph **out)
+{
+    TRY(gcx__ensure_init());
+    *out = NULL;
+    gcx_graph *g = calloc(1, sizeof(*g));
+    if (!g)
+        return GCX_ERR_NOMEM;
+    g->N = N;
+    g->M = M;
+    if (hipMalloc((void **)&g->V, 4ull * (N + 1ull)) != hipSuccess ||
+        hipMalloc((void **)&g->E, 4ull * (M ? M : 1)) != hipSuccess ||
+        hipMalloc((void **)&g->hubs, 4ull * (N ? N : 1)) != hipSuccess) {
+        gcx_graph_free(g);
+        return GCX_ERR_NOMEM;
+    }
+    *out = g;
+    return GCX_OK;
+}
+
+/* Copy a host CSR into buffers sized by gcx__graph_alloc (N and V[N] may be
+ * smaller than the allocation) and refresh the degree statistics. */
+int gcx__graph_fill(gcx_graph *g, uint32_t N, const uint32_t *V, const uint32_t *E, void *stream)
+{
+    if (!g || !V || N == 0 || (V[N] && !E))
+        return GCX_ERR_ARG;
+    hipStream_t s = (hipStream_t)gcx__pick(stream);
+    uint32_t *tmp = NULL;
+    HIPC(hipMemcpyAsync(g->V, V, 4ull * (N + 1ull), hipMemcpyHostToDevice, s));
+    if (V[N])
+        HIPC(hipMemcpyAsync(g->E, E, 4ull * V[N], hipMemcpyHostToDevice, s));
+    g->N = N;
+    g->M = V[N];
+    HIPC(hipMalloc((void **)&tmp, 16));
+    int ok = 0;
+    int rc = gcxk_csr_bounds(g->N, g->M, g->V, g->E, tmp, &ok, s);
+    if (rc == GCX_OK && !ok)
+        rc = GCX_ERR_ARG; /* indices out of range: refuse before any kernel reads them */
+    if (rc == GCX_OK)
+        rc = gcxk_graph_stats(g->N, g->V, g->hubs, tmp, &g->nhub, &g->maxdeg, s);
+    (void)hipFree(tmp);
+    return rc;
+}
+
+int gcx_graph_from_csr(uint32_t N, const uint32_t *V, const uint32_t *E, void *stream,
+                       gcx_graph **out)
+{
+    TRY(gcx__ensure_init());
+    *out = NULL;
+    if (N == 0 || !V || (V[N] && !E))
+        return GCX_ERR_ARG;
+    gcx_graph *g = NULL;
+    TRY(gcx__graph_alloc(N, V[N], &g));
+    int rc = gcx__graph_fill(g, N, V, E, stream);
+    if (rc != GCX_OK) {
+        gcx_graph_free(g);
+        return rc;
+    }
+    *out = g;
+    return GCX_OK;
+}
+
+static int graph_from_device_pairs(uint32_t N, uint64_t L, uint32_t *dI, uint32_t *dJ,
+                                   hipStream_t s, gcx_graph **out)
+{
+    gcx_graph *g = calloc(1, sizeof(*g));
+    if (!g)
+        return GCX_ERR_NOMEM;
+    g->N = N;
+    int rc = GCX_ERR_HIP;
+    if (hipMalloc((void **)&g->V, 4ull * (N + 1ull)) != hipSuccess)
+        goto fail;
+    if ((rc = gcxk_csr_from_pairs(N, L, dI, dJ, g->V, &g->E, &g->M, s)) != GCX_OK)
+        goto fail;
+    if ((rc = graph_finish(g, s)) != GCX_OK)
+        goto fail;
+    *out = g;
+    return GCX_OK;
+fail:
+    gcx_graph_free(g);
+    return rc;
+}
+
+int gcx_graph_from_pairs(uint32_t N, uint64_t L, const uint32_t *I, const uint32_t *J,
+                         void *stream, gcx_graph **out)
+{
+    TRY(gcx__ensure_init());
+    *out = NULL;
+    if (N == 0 || (L && (!I || !J)))
+        return GCX_ERR_ARG;
+    for (uint64_t k = 0; k < L; k++)
+        if (I[k] >= N || J[k] >= N)
+            return GCX_ERR_ARG;
+    hipStream_t s = (hipStream_t)gcx__pick(stream);
+    uint32_t *dI = NULL, *dJ = NULL;
+    int rc = GCX_ERR_HIP;
+    if (hipMalloc((void **)&dI, 4ull * (L ? L : 1)) != hipSuccess ||
+        hipMalloc((void **)&dJ, 4ull * (L ? L : 1)) != hipSuccess)
+        goto done;
+    if (L && (hipMemcpyAsync(dI, I, 4ull * L, hipMemcpyHostToDevice, s) != hipSuccess ||
+              hipMemcpyAsync(dJ, J, 4ull * L, hipMemcpyHostToDevice, s) != hipSuccess))
+        goto done;
+    rc = graph_from_device_pairs(N, L, dI, dJ, s, out);
+done:
+    (void)hipStreamSynchronize(s);
+    if (dI)
+        (void)hipFree(dI);
+    if (dJ)
+        (void)hipFree(dJ);
+    return rc;
+}
+
+int gcx_graph_rmat(int scale, uint32_t edge_factor, double a, double b, double c, uint64_t seed,
+                   void *stream, gcx_graph **out)
+{
+    TRY(gcx__ensure_init());
+    *out = NULL;
+    if (scale < 1 || scale > 30 || edge_factor == 0)
+        return GCX_ERR_ARG;
+    hipStream_t s = (hipStream_t)gcx__pick(stream);
+    const uint64_t L = (uint64_t)edge_factor << scale;
+    uint32_t thr[3];
+    gcx_rmat_thresholds(a, b, c, thr);
+    uint32_t *dI = NULL, *dJ = NULL;
+    int rc = GCX_ERR_HIP;
+    if (hipMalloc((void **)&dI, 4ull * L) != hipSuccess ||
+        hipMalloc((void **)&dJ, 4ull * L) != hipSuccess)
+        goto done;
+    if ((rc = gcxk_gen_rmat(scale, L, thr[0], thr[1], thr[2], seed, dI, dJ, s)) != GCX_OK)
+        goto done;
+    rc = graph_from_device_pairs(1u << scale, L, dI, dJ, s, out);
+done:
+    (void)hipStreamSynchronize(s);
+    if (dI)
+        (void)hipFree(dI);
+    if (dJ)
+        (void)hipFree(dJ);
+    return rc;
+}
+
+int gcx_graph_er(uint32_t n, uint64_t m, uint64_t seed, void *stream, gcx_graph **out)
+{
+    TRY(gcx__ensure_init());
+    *out = NULL;
+    if (n == 0)
+        return GCX_ERR_ARG;
+    hipStream_t s = (hipStream_t)gcx__pick(stream);
+    uint32_t *dI = NULL, *dJ = NULL;
+    int rc = GCX_ERR_HIP;
+    if (hipMalloc((void **)&dI, 4ull * (m ? m : 1)) != hipSuccess ||
+        hipMalloc((void **)&dJ, 4ull * (m ? m : 1)) != hipSuccess)
+        goto done;
+    if ((rc = gcxk_gen_er(n, m, seed, dI, dJ, s)) != GCX_OK)
+        goto done;
+    rc = graph_from_device_pairs(n, m, dI, dJ, s, out);
+done:
+    (void)hipStreamSynchronize(s);
+    if (dI)
+        (void)hipFree(dI);
+    if (dJ)
+        (void)hipFree(dJ);
+    return rc;
+}
+
+uint32_t gcx_graph_num_vertices(const gcx_graph *g) { return g ? g->N : 0; }
+uint64_t gcx_graph_num_entries(const gcx_graph *g) { return g ? g->M : 0; }
+uint32_t gcx_graph_max_degree(const gcx_graph *g) { return g ? g->maxdeg : 0; }
+const uint32_t *gcx_graph_V_dev(const gcx_graph *g) { return g ? g->V : NULL; }
+const uint32_t *gcx_graph_E_dev(const gcx_graph *g) { return g ? g->E : NULL; }
+
+int gcx_graph_download(const gcx_graph *g, uint32_t *V, uint32_t *E)
+{
+    if (!g)
+        return GCX_ERR_ARG;
+    if (V)
+        HIPC(hipMemcpy(V, g->V, 4ull * (g->N + 1ull), hipMemcpyDeviceToHost));
+    if (E && g->M)
+        HIPC(hipMemcpy(E, g->E, 4ull * g->M, hipMemcpyDeviceToHost));
+    return GCX_OK;
+}
+
+int gcx_graph_check(const gcx_graph *g, void *stream, int *valid)
+{
+    TRY(gcx__ensure_init());
+    if (!g || !valid)
+        return GCX_ERR_ARG;
+    uint32_t *tmp = NULL;
+    HIPC(hipMalloc((void **)&tmp, 16));
+    int rc = gcxk_graph_check(g->N, g->V, g->E, tmp, valid, gcx__pick(stream));
+    (void)hipFree(tmp);
+    return rc;
+}
+
+/* -------------------------------------------------------------------- model */
+
+#define LAYER_PARAMS 528
+
+int gcx_model_from_params(int params, const float *param, int md, gcx_model **out)
+{
+    TRY(gcx__ensure_init());
+    *out = NULL;
+    if (params < 2 * LAYER_PARAMS || params % LAYER_PARAMS || !param || md <= 0)
+        return GCX_ERR_FORMAT;
+    gcx_model *m = calloc(1, sizeof(*m));
+    if (!m)
+        return GCX_ERR_NOMEM;
+    m->params = params;
+    m->md = md;
+    m->layers = params / LAYER_PARAMS;
+    m->hparam = malloc(sizeof(float) * (size_t)params);
+    if (!m->hparam) {
+        free(m);
+        return GCX_ERR_NOMEM;
+    }
+    memcpy(m->hparam, param, sizeof(float) * (size_t)params);
+    if (hipMalloc((void **)&m->dparam, sizeof(float) * (size_t)params) != hipSuccess ||
+        hipMemcpy(m->dparam, param, sizeof(float) * (size_t)params, hipMemcpyHostToDevice) !=
+            hipSuccess) {
+        gcx_model_free(m);
+        return GCX_ERR_HIP;
+    }
+    *out = m;
+    return GCX_OK;
+}
+
+int gcx_model_load(const char *path, gcx_model **out)
+{
+    /* gnn.c:20-44: "params md", then params decimal floats (strtof rounding). */
+    *out = NULL;
+    FILE *f = fopen(path, "r");
+    if (!f)
+        return GCX_ERR_IO;
+    int params = 0, md = 0;
+    if (fscanf(f, "%d %d", &params, &md) != 2 || params < 2 * LAYER_PARAMS ||
+        params % LAYER_PARAMS || params > (1 << 24)) {
+        fclose(f);
+        return GCX_ERR_FORMAT;
+    }
+    float *w = malloc(sizeof(float) * (size_t)params);
+    if (!w) {
+        fclose(f);
+        return GCX_ERR_NOMEM;
+    }
+    for (int i = 0; i < params; i++) {
+        if (fscanf(f, "%f", w + i) != 1) {
+            free(w);
+            fclose(f);
+            return GCX_ERR_FORMAT;
+        }
+    }
+    fclose(f);
+    int rc = gcx_model_from_params(params, w, md, out);
+    free(w);
+    return rc;
+}
+
+int gcx_model_free(gcx_model *m)
+{
+    if (!m)
+        return GCX_OK;
+    if (m->dparam)
+        (void)hipFree(m->dparam);
+    free(m->hparam);
+    free(m);
+    return GCX_OK;
+}
+
+int gcx_model_layers(const gcx_model *m) { return m ? m->layers : 0; }
+
+/* ---------------------------------------------------------------- workspace */
+
+uint64_t gcx_workspace_footprint(uint32_t N, uint64_t M)
+{
+    (void)M;
+    /* key 8 + count 4 + color 4 + 4 lists x 4 + x,y 2 x 64 bytes per vertex */
+    return 160ull * ((uint64_t)N + 1) + 2 * sizeof(gcxk_ctl);
+}
+
+int gcx_workspace_free(gcx_workspace *ws)
+{
+    if (!ws)
+        return GCX_OK;
+    void *bufs[] = {ws->key, ws->count, ws->color, ws->q[0], ws->q[1],
+                    ws->h[0], ws->h[1], ws->x, ws->y, ws->ctl};
+    for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); i++)
+        if (bufs[i])
+            (void)hipFree(bufs[i]);
+    if (ws->hctl)
+        (void)hipHostFree(ws->hctl);
+    if (ws->aux)
+        (void)hipStreamDestroy((hipStream_t)ws->aux);
+    void *evs[] = {ws->ev_fork, ws->ev_join, ws->ev_poll[0], ws->ev_poll[1]};
+    for (size_t i = 0; i < 4; i++)
+        if (evs[i])
+            (void)hipEventDestroy((hipEvent_t)evs[i]);
+    free(ws);
+    return GCX_OK;
+}
+
+int gcx_workspace_create(uint32_t N, uint64_t M, gcx_workspace **out)
+{
+    TRY(gcx__ensure_init());
+    *out = NULL;
+    if (N == 0)
+        return GCX_ERR_ARG;
+    gcx_workspace *ws = calloc(1, sizeof(*ws));
+    if (!ws)
+        return GCX_ERR_NOMEM;
+    ws->capN = N;
+    ws->capM = M;
+    const size_t n = (size_t)N + 1;
+    int ok = hipMalloc(&ws->key, 8 * n) == hipSuccess &&
+             hipMalloc((void **)&ws->count, 4 * n) == hipSuccess &&
+             hipMalloc((void **)&ws->color, 4 * n) == hipSuccess &&
+             hipMalloc((void **)&ws->q[0], 4 * n) == hipSuccess &&
+             hipMalloc((void **)&ws->q[1], 4 * n) == hipSuccess &&
+             hipMalloc((void **)&ws->h[0], 4 * n) == hipSuccess &&
+             hipMalloc((void **)&ws->h[1], 4 * n) == hipSuccess &&
+             hipMalloc((void **)&ws->x, 64 * n) == hipSuccess &&
+             hipMalloc((void **)&ws->y, 64 * n) == hipSuccess &&
+             hipMalloc((void **)&ws->ctl, 2 * sizeof(gcxk_ctl)) == hipSuccess &&
+             hipHostMalloc((void **)&ws->hctl, 4 * sizeof(gcxk_ctl), hipHostMallocDefault) ==
+                 hipSuccess;
+    if (ok) {
+        hipStream_t aux;
+        hipEvent_t e[4];
+        ok = hipStreamCreateWithFlags(&aux, hipStreamNonBlocking) == hipSuccess;
+        if (ok)
+            ws->aux = (void *)aux;
+        for (int i = 0; ok && i < 4; i++) {
+            ok = hipEventCreateWithFlags(&e[i], hipEventDisableTiming) == hipSuccess;
+            if (ok) {
+                if (i == 0)
+                    ws->ev_fork = e[i];
+                else if (i == 1)
+                    ws->ev_join = e[i];
+                else
+                    ws->ev_poll[i - 2] = e[i];
+            }
+        }
+    }
+    if (!ok) {
+        gcx_workspace_free(ws);
+        return GCX_ERR_NOMEM;
+    }
+    memset(ws->hctl, 0, 4 * sizeof(gcxk_ctl));
+    *out = ws;
+    return GCX_OK;
+}
+
+static int ws_fits(const gcx_workspace *ws, const gcx_graph *g)
+{
+    return ws && g && g->N <= ws->capN;
+}
+
+/* ----------------------------------------------------------- round polling */
+
+/* Drive a device round loop: `enqueue(r)` launches round r; the device sets
+ * the flag at byte offset `flag_off` of the control block once a round sees
+ * nothing left to do. Two batches stay in flight so the GPU never idles on
+ * the host's check. */
+typedef int (*round_fn)(void *ctx, uint32_t r, hipStream_t s);
+
+static int drive_rounds(gcx_workspace *ws, hipStream_t s, round_fn enqueue, void *ctx,
+                        size_t flag_off, uint32_t max_rounds)
+{
+    uint32_t r = 0;
+    int k = 0;
+    for (;;) {
+        for (uint32_t i = 0; i < ROUND_BATCH; i++)
+            TRY(enqueue(ctx, r++, s));
+        gcxk_ctl *snap = ws->hctl + 2 + (k & 1);
+        HIPC(hipMemcpyAsync(snap, ws->ctl, sizeof(gcxk_ctl), hipMemcpyDeviceToHost, s));
+        HIPC(hipEventRecord((hipEvent_t)ws->ev_poll[k & 1], s));
+        if (k > 0) {
+            const int pk = (k - 1) & 1;
+            HIPC(hipEventSynchronize((hipEvent_t)ws->ev_poll[pk]));
+            const uint32_t flag = *(volatile uint32_t *)((char *)(ws->hctl + 2 + pk) + flag_off);
+            if (flag)
+                break;
+        }
+        if (r >= max_rounds)
+            return GCX_ERR_HIP; /* the loop must end within N + 1 rounds */
+        k++;
+    }
+    ws->last_launches = r;
+    return GCX_OK;
+}
+
+/* ---------------------------------------------------------------- ordering */
+
+typedef struct {
+    gcx_workspace *ws;
+    const gcx_graph *g;
+    int log_variant;
+    uint32_t grid;
+} sl_ctx;
+
+static int sl_round(void *p, uint32_t r, hipStream_t s)
+{
+    sl_ctx *c = (sl_ctx *)p;
+    gcx_workspace *ws = c->ws;
+    TRY(gcxk_sl_scan(ws->count, (uint32_t *)ws->key, ws->q[r & 1], ws->q[(r + 1) & 1], ws->h[0],
+                     ws->ctl, c->log_variant, r, c->grid, s));
+    return gcxk_sl_dec(c->g->V, c->g->E, ws->count, (uint32_t *)ws->key, ws->h[0], ws->ctl, r,
+                       c->grid, s);
+}
+
+static int order_sl(gcx_workspace *ws, const gcx_graph *g, int log_variant, hipStream_t s)
+{
+    HIPC(hipMemsetAsync(ws->ctl, 0, sizeof(gcxk_ctl), s));
+    TRY(gcxk_sl_init(g->N, g->V, ws->count, (uint32_t *)ws->key, ws->q[0], ws->ctl, s));
+    sl_ctx c = {ws, g, log_variant, g_round_grid};
+    TRY(drive_rounds(ws, s, sl_round, &c, offsetof(gcxk_ctl, sl_done), g->N + 2 * ROUND_BATCH + 2));
+    return GCX_OK;
+}
+
+static int order_gnn_fp32(gcx_workspace *ws, const gcx_graph *g, const gcx_model *m, hipStream_t s)
+{
+    const int L = m->layers;
+    const float *in = NULL;
+    float *bufs[2] = {ws->x, ws->y};
+    for (int l = 0; l < L; l++) {
+        const int kind = l == 0 ? 0 : (l == L - 1 ? 2 : 1);
+        const float *W = m->dparam + (size_t)l * LAYER_PARAMS;
+        const float *b = W + 512;
+        float *out = bufs[l & 1];
+        void *tok;
+        TRY(gcx__prof_begin(kind == 0 ? GCX_ST_GNN_IN : kind == 1 ? GCX_ST_GNN_HID : GCX_ST_GNN_OUT,
+                            s, &tok));
+        hipStream_t hs = s;
+        if (g->nhub) {
+            HIPC(hipEventRecord((hipEvent_t)ws->ev_fork, s));
+            HIPC(hipStreamWaitEvent((hipStream_t)ws->aux, (hipEvent_t)ws->ev_fork, 0));
+            hs = (hipStream_t)ws->aux;
+        }
+        TRY(gcxk_gnn_fp32_layer(kind, g->N, g->V, g->E, W, b, m->md, in, out,
+                                (uint32_t *)ws->key, g->hubs, g->nhub, s, hs));
+        if (g->nhub) {
+            HIPC(hipEventRecord((hipEvent_t)ws->ev_join, hs));
+            HIPC(hipStreamWaitEvent(s, (hipEvent_t)ws->ev_join, 0));
+        }
+        TRY(gcx__prof_end(tok, s));
+        in = out;
+    }
+    return GCX_OK;
+}
+
+int gcx_order(gcx_workspace *ws, const gcx_graph *g, int heuristic, const gcx_model *model,
+              int precision, void *stream)
+{
+    TRY(gcx__ensure_init());
+    if (!ws_fits(ws, g))
+        return ws && g ? GCX_ERR_CAPACITY : GCX_ERR_ARG;
+    hipStream_t s = (hipStream_t)gcx__pick(stream);
+    void *tok = NULL;
+    int rc = GCX_OK;
+    ws->curN = g->N;
+    switch (heuristic) {
+    case GCX_FF:
+    case GCX_LF:
+    case GCX_LLF:
+        TRY(gcx__prof_begin(GCX_ST_KEYS, s, &tok));
+        rc = heuristic == GCX_FF    ? gcxk_keys_ff(g->N, (uint32_t *)ws->key, s)
+             : heuristic == GCX_LF ? gcxk_keys_lf(g->N, g->V, (uint32_t *)ws->key, s)
+                                   : gcxk_keys_llf(g->N, g->V, (uint32_t *)ws->key, s);
+        TRY(gcx__prof_end(tok, s));
+        ws->key_kind = 0;
+        break;
+    case GCX_SL:
+    case GCX_SLL:
+        TRY(gcx__prof_begin(GCX_ST_SL, s, &tok));
+        rc = order_sl(ws, g, heuristic == GCX_SLL, s);
+        TRY(gcx__prof_end(tok, s));
+        ws->key_kind = 0;
+        break;
+    case GCX_GNN:
+        if (!model)
+            return GCX_ERR_ARG;
+        if (precision == GCX_FP32)
+            rc = order_gnn_fp32(ws, g, model, s);
+        else if (precision == GCX_BF16)
+            rc = gcx__order_gnn_bf16(ws, g, model, s);
+        else
+            return GCX_ERR_ARG;
+        ws->key_kind = 1;
+        break;
+    default:
+        return GCX_ERR_ARG;
+    }
+    if (rc == GCX_OK)
+        ws->key_bits = 32;
+    return rc;
+}
+
+int gcx_set_priorities(gcx_workspace *ws, const gcx_graph *g, const double *p, void *stream)
+{
+    TRY(gcx__ensure_init());
+    if (!ws_fits(ws, g) || !p)
+        return GCX_ERR_ARG;
+    hipStream_t s = (hipStream_t)gcx__pick(stream);
+    double *dp = (double *)ws->x; /* 64 bytes per vertex of scratch */
+    gcxk_ctl *vc = ws->ctl + 1;
+    HIPC(hipMemcpyAsync(dp, p, 8ull * g->N, hipMemcpyHostToDevice, s));
+    HIPC(hipMemsetAsync(vc, 0, sizeof(gcxk_ctl), s));
+    TRY(gcxk_keys_scan_doubles(g->N, dp, vc, s));
+    HIPC(hipMemcpyAsync(ws->hctl + 1, vc, sizeof(gcxk_ctl), hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    if (ws->hctl[1].nan_count)
+        return GCX_ERR_ARG;
+    ws->curN = g->N;
+    if (ws->hctl[1].nonfloat == 0) {
+        TRY(gcxk_keys_from_doubles32(g->N, dp, (uint32_t *)ws->key, s));
+        ws->key_bits = 32;
+        ws->key_kind = 1;
+    } else {
+        TRY(gcxk_keys_from_doubles64(g->N, dp, (uint64_t *)ws->key, s));
+        ws->key_bits = 64;
+        ws->key_kind = 2;
+    }
+    return GCX_OK;
+}
+
+/* ------------------------------------------------------------------- colour */
+
+typedef struct {
+    gcx_workspace *ws;
+    const gcx_graph *g;
+    uint32_t grid;
+} jp_ctx;
+
+static int jp_round(void *p, uint32_t r, hipStream_t s)
+{
+    jp_ctx *c = (jp_ctx *)p;
+    gcx_workspace *ws = c->ws;
+    return gcxk_jp_round(c->g->N, c->g->V, c->g->E, ws->count, ws->color, ws->q, ws->h, ws->ctl, r,
+                         c->grid, s);
+}
+
+int gcx_color(gcx_workspace *ws, const gcx_graph *g, void *stream)
+{
+    TRY(gcx__ensure_init());
+    if (!ws_fits(ws, g) || !ws->key_bits || ws->curN != g->N)
+        return GCX_ERR_ARG;
+    hipStream_t s = (hipStream_t)gcx__pick(stream);
+    void *tok = NULL;
+    HIPC(hipMemsetAsync(ws->ctl, 0, sizeof(gcxk_ctl), s));
+    TRY(gcx__prof_begin(GCX_ST_JP_INIT, s, &tok));
+    TRY(gcxk_jp_init(g->N, g->V, g->E, ws->key, ws->key_bits, ws->count, ws->color, ws->q[0],
+                     ws->h[0], g->hubs, g->nhub, ws->ctl, s));
+    TRY(gcx__prof_end(tok, s));
+    TRY(gcx__prof_begin(GCX_ST_JP_ROUNDS, s, &tok));
+    jp_ctx c = {ws, g, g_round_grid};
+    TRY(drive_rounds(ws, s, jp_round, &c, offsetof(gcxk_ctl, done), g->N + 2 * ROUND_BATCH + 2));
+    TRY(gcx__prof_end(tok, s));
+    ws->last_rounds = UINT32_MAX; /* read lazily from the device */
+    return GCX_OK;
+}
+
+int gcx_last_rounds(const gcx_workspace *ws, uint32_t *rounds)
+{
+    if (!ws || !rounds)
+        return GCX_ERR_ARG;
+    gcxk_ctl c;
+    HIPC(hipMemcpy(&c, ws->ctl, sizeof(c), hipMemcpyDeviceToHost));
+    *rounds = c.rounds;
+    return GCX_OK;
+}
+
+int gcx_last_launches(const gcx_workspace *ws, uint32_t *launches)
+{
+    if (!ws || !launches)
+        return GCX_ERR_ARG;
+    *launches = ws->last_launches;
+    return GCX_OK;
+}
+
+int gcx_recolor(gcx_workspace *ws, const gcx_graph *g, int iterations, void *stream)
+{
+    /* main.c:36-43: p[j] = color[j], then JP again; repeated. */
+    TRY(gcx__ensure_init());
+    if (!ws_fits(ws, g) || iterations < 0)
+        return GCX_ERR_ARG;
+    hipStream_t s = (hipStream_t)gcx__pick(stream);
+    for (int it = 0; it < iterations; it++) {
+        TRY(gcxk_keys_from_colors(g->N, ws->color, (uint32_t *)ws->key, s));
+        ws->key_bits = 32;
+        ws->key_kind = 0;
+        TRY(gcx_color(ws, g, s));
+    }
+    return GCX_OK;
+}
+
+int gcx_validate(gcx_workspace *ws, const gcx_graph *g, void *stream, int *max_color)
+{
+    TRY(gcx__ensure_init());
+    if (!ws_fits(ws, g) || !max_color)
+        return GCX_ERR_ARG;
+    hipStream_t s = (hipStream_t)gcx__pick(stream);
+    gcxk_ctl *vc = ws->ctl + 1;
+    HIPC(hipMemsetAsync(vc, 0, sizeof(gcxk_ctl), s));
+    TRY(gcxk_validate(g->N, g->V, g->E, ws->color, g->hubs, g->nhub, vc, s));
+    HIPC(hipMemcpyAsync(ws->hctl + 1, vc, sizeof(gcxk_ctl), hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    *max_color = ws->hctl[1].bad ? 0 : (ws->hctl[1].maxcolor > 0 ? ws->hctl[1].maxcolor : 1);
+    return GCX_OK;
+}
+
+int gcx_mex_check(gcx_workspace *ws, const gcx_graph *g, void *stream, uint64_t *violations)
+{
+    TRY(gcx__ensure_init());
+    if (!ws_fits(ws, g) || !violations || !ws->key_bits)
+        return GCX_ERR_ARG;
+    hipStream_t s = (hipStream_t)gcx__pick(stream);
+    gcxk_ctl *vc = ws->ctl + 1;
+    HIPC(hipMemsetAsync(vc, 0, sizeof(gcxk_ctl), s));
+    TRY(gcxk_mex_check(g->N, g->V, g->E, ws->key, ws->key_bits, ws->color, vc, s));
+    HIPC(hipMemcpyAsync(ws->hctl + 1, vc, sizeof(gcxk_ctl), hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    *violations = ws->hctl[1].viol;
+    return GCX_OK;
+}
+
+int gcx_download_colors(const gcx_workspace *ws, uint32_t N, int *color, void *stream)
+{
+    if (!ws || N > ws->capN || !color)
+        return GCX_ERR_ARG;
+    hipStream_t s = (hipStream_t)gcx__pick(stream);
+    HIPC(hipMemcpyAsync(color, ws->color, 4ull * N, hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    return GCX_OK;
+}
+
+int gcx_download_priorities(const gcx_workspace *ws, uint32_t N, double *p, void *stream)
+{
+    if (!ws || N > ws->capN || !p || !ws->key_bits)
+        return GCX_ERR_ARG;
+    hipStream_t s = (hipStream_t)gcx__pick(stream);
+    double *dp = (double *)ws->y;
+    if (ws->key_bits == 32)
+        TRY(gcxk_keys_to_doubles(N, (const uint32_t *)ws->key, ws->key_kind, dp, s));
+    else
+        TRY(gcxk_keys64_to_doubles(N, (const uint64_t *)ws->key, dp, s));
+    HIPC(hipMemcpyAsync(p, dp, 8ull * N, hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    return GCX_OK;
+}
+
+int gcx_upload_colors(gcx_workspace *ws, uint32_t N, const int *color, void *stream)
+{
+    TRY(gcx__ensure_init());
+    if (!ws || N > ws->capN || !color)
+        return GCX_ERR_ARG;
+    hipStream_t s = (hipStream_t)gcx__pick(stream);
+    HIPC(hipMemcpyAsync(ws->color, color, 4ull * N, hipMemcpyHostToDevice, s));
+    HIPC(hipStreamSynchronize(s));
+    return GCX_OK;
+}
+
+int gcx__order_ff_only(gcx_workspace *ws, uint32_t N, void *stream)
+{
+    TRY(gcx__ensure_init());
+    if (!ws || N > ws->capN)
+        return GCX_ERR_ARG;
+    TRY(gcxk_keys_ff(N, (uint32_t *)ws->key, gcx__pick(stream)));
+    ws->key_bits = 32;
+    ws->key_kind = 0;
+    ws->curN = N;
+    return GCX_OK;
+}
+
+int *gcx_workspace_colors_dev(gcx_workspace *ws) { return ws ? ws->color : NULL; }

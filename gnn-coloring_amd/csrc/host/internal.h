@@ -1,0 +1,60 @@
+/*
+ * internal.h -- private structures of the C host layer.
+ */
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+#include "gcx.h"
+#include "gcxk.h"
+
+struct gcx_graph {
+    uint32_t N;
+    uint64_t M;
+    uint32_t *V;      /* device, N+1 */
+    uint32_t *E;      /* device, M */
+    uint32_t *hubs;   /* device, nhub entries (degree >= GCXK_HUB_DEG) */
+    uint32_t nhub;
+    uint32_t maxdeg;
+};
+
+struct gcx_model {
+    int params, md, layers;
+    float *dparam; /* device copy of the weights */
+    float *hparam; /* host copy */
+};
+
+struct gcx_workspace {
+    uint32_t capN;
+    uint64_t capM;
+    void *key;        /* 8 * capN bytes: key32 or key64 view */
+    int key_bits;     /* 32 / 64 for the current priorities, 0 = none */
+    int key_kind;     /* 32-bit keys: 0 unsigned integer, 1 float order */
+    uint32_t *count;  /* pending predecessors (JP); degrees (SL) */
+    int *color;
+    uint32_t *q[2];   /* frontier / alive lists */
+    uint32_t *h[2];   /* hub frontier lists / SL removed list */
+    float *x, *y;     /* GNN rows, 16 floats (or 16 bf16) per vertex */
+    gcxk_ctl *ctl;    /* device control block */
+    gcxk_ctl *hctl;   /* pinned host mirror */
+    void *aux;        /* side stream for hub kernels */
+    void *ev_fork, *ev_join;
+    void *ev_poll[2];
+    uint32_t last_rounds;
+    uint32_t last_sl_rounds;
+    uint32_t last_launches; /* round kernels enqueued by the last round loop */
+    uint32_t curN;    /* vertex count of the graph the contents belong to */
+};
+
+/* device.c */
+int gcx__graph_alloc(uint32_t N, uint64_t M, gcx_graph **out);
+int gcx__graph_fill(gcx_graph *g, uint32_t N, const uint32_t *V, const uint32_t *E, void *stream);
+/* FF keys (N - id) for a workspace without a graph (first_fit.c:8-19). */
+int gcx__order_ff_only(gcx_workspace *ws, uint32_t N, void *stream);
+/* bf16.c */
+int gcx__order_gnn_bf16(gcx_workspace *ws, const gcx_graph *g, const gcx_model *m, void *stream);
+int gcx__ensure_init(void);
+void *gcx__default_stream(void);
+void *gcx__pick(void *stream);
+int gcx__prof_begin(int stage, void *stream, void **token);
+int gcx__prof_end(void *token, void *stream);

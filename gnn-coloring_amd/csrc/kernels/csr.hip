@@ -1,0 +1,439 @@
+// csr.hip -- device-side canonical CSR construction and graph checks.
+//
+// Reference path (fast_io.c:171-202, 450-455, 457-477): atomic degree count,
+// serial prefix sum, atomic scatter of both directions, per-row qsort, serial
+// removal of self loops and repeats. Here the same canonical CSR comes from
+// one radix sort: every pair (u, v) becomes the keys (u << b | v) and
+// (v << b | u) with b = bit length of N-1, self loops become the all-ones key
+// that sorts last; after sorting, the first occurrence of each key is kept,
+// its row is counted, and the row counts are scanned into V. Row order is
+// ascending v because the key's low bits are v.
+#include "common.hip.h"
+#include "gen.h"
+
+#include <cstring>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
+namespace {
+
+constexpr int kBlock = 256;
+
+__global__ __launch_bounds__(kBlock) void expand_pairs(uint64_t L, const uint32_t *__restrict__ I,
+                                                       const uint32_t *__restrict__ J, int b,
+                                                       uint64_t *__restrict__ keys)
+{
+    const uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (k >= L)
+        return;
+    const uint64_t u = I[k], v = J[k];
+    const uint64_t none = (b == 32) ? ~0ull : ((1ull << (2 * b)) - 1ull);
+    if (u == v) {
+        keys[2 * k] = none;
+        keys[2 * k + 1] = none;
+    } else {
+        keys[2 * k] = (u << b) | v;
+        keys[2 * k + 1] = (v << b) | u;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void mark_unique(uint64_t n, const uint64_t *__restrict__ keys,
+                                                      int b, uint32_t *__restrict__ flag,
+                                                      uint32_t *__restrict__ deg)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n)
+        return;
+    const uint64_t none = (b == 32) ? ~0ull : ((1ull << (2 * b)) - 1ull);
+    const uint64_t k = keys[i];
+    const bool keep = k != none && (i == 0 || keys[i - 1] != k);
+    flag[i] = keep ? 1u : 0u;
+    if (keep)
+        atomicAdd(deg + (uint32_t)(k >> b), 1u);
+}
+
+__global__ __launch_bounds__(kBlock) void scatter_unique(uint64_t n, const uint64_t *__restrict__ keys,
+                                                         const uint32_t *__restrict__ flag,
+                                                         const uint32_t *__restrict__ pos, int b,
+                                                         uint32_t *__restrict__ E)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n || !flag[i])
+        return;
+    const uint64_t mask = (b == 32) ? 0xFFFFFFFFull : ((1ull << b) - 1ull);
+    E[pos[i]] = (uint32_t)(keys[i] & mask);
+}
+
+__global__ __launch_bounds__(kBlock) void degree_stats(uint32_t N, const uint32_t *__restrict__ V,
+                                                       uint32_t *__restrict__ hubs,
+                                                       uint32_t *__restrict__ tmp)
+{
+    const uint32_t u = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t d = 0;
+    if (u < N) {
+        d = V[u + 1] - V[u];
+        if (d >= GCXK_HUB_DEG)
+            hubs[atomicAdd(tmp + 0, 1u)] = u;
+    }
+    d = group_max32<kWave>(d);
+    if (lane_id() == 0)
+        atomicMax(tmp + 1, d);
+}
+
+// Binary search for x in row `row`; a malformed row (decreasing or past the
+// end of E) reads nothing and counts as "absent".
+__device__ bool row_contains(uint32_t N, const uint32_t *__restrict__ V,
+                             const uint32_t *__restrict__ E, uint32_t row, uint32_t x)
+{
+    uint32_t lo = V[row], hi = V[row + 1];
+    if (hi < lo || hi > V[N])
+        return false;
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        const uint32_t y = E[mid];
+        if (y == x)
+            return true;
+        if (y < x)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return false;
+}
+
+// fast_io.c:509-540: V monotone, entries in range, strictly ascending rows
+// (no repeats), no self loops, every edge present in both directions.
+__global__ __launch_bounds__(kBlock) void check_csr(uint32_t N, const uint32_t *__restrict__ V,
+                                                    const uint32_t *__restrict__ E,
+                                                    uint32_t *__restrict__ bad)
+{
+    const uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t u = (uint32_t)(t / kWave);
+    if (u >= N)
+        return;
+    const uint32_t beg = V[u], end = V[u + 1];
+    uint32_t err = 0;
+    if (end < beg || end > V[N]) {
+        err = 1;
+    } else {
+        for (uint32_t i = beg + lane_id(); i < end; i += kWave) {
+            const uint32_t v = E[i];
+            if (v >= N || v == u || (i > beg && v <= E[i - 1]) || !row_contains(N, V, E, v, u))
+                err = 1;
+        }
+    }
+    const uint64_t any = __ballot(err != 0);
+    if (lane_id() == 0 && any)
+        atomicAdd(bad, 1u);
+}
+
+__global__ __launch_bounds__(kBlock) void gen_rmat_kernel(int scale, uint64_t L, gcx_rmat_thr thr,
+                                                          uint64_t seed, uint32_t *__restrict__ I,
+                                                          uint32_t *__restrict__ J)
+{
+    const uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (k < L)
+        gcx_rmat_pair(seed, scale, thr, k, I + k, J + k);
+}
+
+__global__ __launch_bounds__(kBlock) void gen_er_kernel(uint32_t n, uint64_t m, uint64_t seed,
+                                                        uint32_t *__restrict__ I,
+                                                        uint32_t *__restrict__ J)
+{
+    const uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (k < m)
+        gcx_er_pair(seed, n, k, I + k, J + k);
+}
+
+// Bounds check that every kernel indexing by V and E relies on: V[0] = 0,
+// V non-decreasing, V[N] = M and every entry of E below N.
+__global__ __launch_bounds__(kBlock) void csr_bounds(uint32_t N, uint64_t M,
+                                                     const uint32_t *__restrict__ V,
+                                                     const uint32_t *__restrict__ E,
+                                                     uint32_t *__restrict__ bad)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    uint32_t err = 0;
+    if (i < N)
+        err |= V[i + 1] < V[i] ? 1u : 0u;
+    if (i == 0)
+        err |= (V[0] != 0 || (uint64_t)V[N] != M) ? 1u : 0u;
+    if (i < M)
+        err |= E[i] >= N ? 1u : 0u;
+    const uint64_t any = __ballot(err != 0);
+    if (lane_id() == 0 && any)
+        atomicAdd(bad, 1u);
+}
+
+// One key per CSR entry: (row << b) | column, for sorting rows in place.
+__global__ __launch_bounds__(kBlock) void row_keys(uint32_t N, const uint32_t *__restrict__ V,
+                                                   const uint32_t *__restrict__ E, int b,
+                                                   uint64_t *__restrict__ keys)
+{
+    const uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t u = (uint32_t)(t / kWave);
+    if (u >= N)
+        return;
+    for (uint32_t i = V[u] + lane_id(); i < V[u + 1]; i += kWave)
+        keys[i] = ((uint64_t)u << b) | E[i];
+}
+
+__global__ __launch_bounds__(kBlock) void keys_low(uint64_t M, const uint64_t *__restrict__ keys,
+                                                   int b, uint32_t *__restrict__ E)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    const uint64_t mask = (b == 32) ? 0xFFFFFFFFull : ((1ull << b) - 1ull);
+    if (i < M)
+        E[i] = (uint32_t)(keys[i] & mask);
+}
+
+// graph_make_simple (fast_io.c:457-477): in a row sorted ascending, entry j
+// survives unless it is the row's own vertex or not larger than entry j-1.
+__global__ __launch_bounds__(kBlock) void simple_flags(uint32_t N, const uint32_t *__restrict__ V,
+                                                       const uint32_t *__restrict__ E,
+                                                       uint32_t *__restrict__ flag,
+                                                       uint32_t *__restrict__ deg)
+{
+    const uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t u = (uint32_t)(t / kWave);
+    if (u >= N)
+        return;
+    const uint32_t beg = V[u], end = V[u + 1];
+    uint32_t kept = 0;
+    for (uint32_t i = beg + lane_id(); i < end; i += kWave) {
+        const uint32_t v = E[i];
+        const bool keep = v != u && !(i > beg && v <= E[i - 1]);
+        flag[i] = keep ? 1u : 0u;
+        kept += keep ? 1u : 0u;
+    }
+    kept = group_sum32<kWave>(kept);
+    if (lane_id() == 0)
+        deg[u] = kept;
+}
+
+__global__ __launch_bounds__(kBlock) void compact_entries(uint64_t M, const uint32_t *__restrict__ E,
+                                                          const uint32_t *__restrict__ flag,
+                                                          const uint32_t *__restrict__ pos,
+                                                          uint32_t *__restrict__ out)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < M && flag[i])
+        out[pos[i]] = E[i];
+}
+
+int bit_length(uint32_t x)
+{
+    int b = 0;
+    while (x) {
+        b++;
+        x >>= 1;
+    }
+    return b;
+}
+
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf()
+    {
+        if (p)
+            (void)hipFree(p);
+    }
+    hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 8); }
+};
+
+} // namespace
+
+extern "C" int gcxk_csr_from_pairs(uint32_t N, uint64_t L, const uint32_t *dI, const uint32_t *dJ,
+                                   uint32_t *dV, uint32_t **dE, uint64_t *M, void *stream)
+{
+    hipStream_t s = as_stream(stream);
+    *dE = nullptr;
+    *M = 0;
+    if (N == 0)
+        return GCX_ERR_ARG;
+    const int b = bit_length(N - 1) ? bit_length(N - 1) : 1;
+    const uint64_t n = 2 * L;
+    DevBuf keys, keys2, flag, pos, deg, temp;
+    GCXK_CHECK(keys.alloc(n * sizeof(uint64_t)));
+    GCXK_CHECK(keys2.alloc(n * sizeof(uint64_t)));
+    GCXK_CHECK(deg.alloc(((size_t)N + 1) * sizeof(uint32_t)));
+    GCXK_CHECK(hipMemsetAsync(deg.p, 0, ((size_t)N + 1) * sizeof(uint32_t), s));
+    uint64_t *k_in = (uint64_t *)keys.p, *k_out = (uint64_t *)keys2.p;
+    if (L) {
+        expand_pairs<<<div_up(L, kBlock), kBlock, 0, s>>>(L, dI, dJ, b, k_in);
+        GCXK_LAUNCH_CHECK();
+        size_t tb = 0;
+        GCXK_CHECK(rocprim::radix_sort_keys(nullptr, tb, k_in, k_out, (size_t)n, 0, 2 * b, s));
+        GCXK_CHECK(temp.alloc(tb));
+        GCXK_CHECK(rocprim::radix_sort_keys(temp.p, tb, k_in, k_out, (size_t)n, 0, 2 * b, s));
+        GCXK_CHECK(flag.alloc(n * sizeof(uint32_t)));
+        GCXK_CHECK(pos.alloc(n * sizeof(uint32_t)));
+        mark_unique<<<div_up(n, kBlock), kBlock, 0, s>>>(n, k_out, b, (uint32_t *)flag.p,
+                                                          (uint32_t *)deg.p);
+        GCXK_LAUNCH_CHECK();
+        size_t tb2 = 0;
+        GCXK_CHECK(rocprim::exclusive_scan(nullptr, tb2, (uint32_t *)flag.p, (uint32_t *)pos.p, 0u,
+                                           (size_t)n, rocprim::plus<uint32_t>(), s));
+        DevBuf temp2;
+        GCXK_CHECK(temp2.alloc(tb2));
+        GCXK_CHECK(rocprim::exclusive_scan(temp2.p, tb2, (uint32_t *)flag.p, (uint32_t *)pos.p, 0u,
+                                           (size_t)n, rocprim::plus<uint32_t>(), s));
+    }
+    // V = exclusive scan of the row counts (N+1 entries; last = M).
+    size_t tb3 = 0;
+    GCXK_CHECK(rocprim::exclusive_scan(nullptr, tb3, (uint32_t *)deg.p, dV, 0u, (size_t)N + 1,
+                                       rocprim::plus<uint32_t>(), s));
+    DevBuf temp3;
+    GCXK_CHECK(temp3.alloc(tb3));
+    GCXK_CHECK(rocprim::exclusive_scan(temp3.p, tb3, (uint32_t *)deg.p, dV, 0u, (size_t)N + 1,
+                                       rocprim::plus<uint32_t>(), s));
+    uint32_t m32 = 0;
+    GCXK_CHECK(hipMemcpyAsync(&m32, dV + N, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    GCXK_CHECK(hipStreamSynchronize(s));
+    uint32_t *E = nullptr;
+    GCXK_CHECK(hipMalloc((void **)&E, (m32 ? m32 : 1) * sizeof(uint32_t)));
+    if (L && m32) {
+        scatter_unique<<<div_up(n, kBlock), kBlock, 0, s>>>(n, k_out, (uint32_t *)flag.p,
+                                                             (uint32_t *)pos.p, b, E);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(s);
+        if (e != hipSuccess) {
+            (void)hipFree(E);
+            return GCX_ERR_HIP;
+        }
+    }
+    *dE = E;
+    *M = m32;
+    return GCX_OK;
+}
+
+extern "C" int gcxk_graph_stats(uint32_t N, const uint32_t *dV, uint32_t *dhub, uint32_t *dtmp,
+                                uint32_t *nhub, uint32_t *maxdeg, void *stream)
+{
+    hipStream_t s = as_stream(stream);
+    GCXK_CHECK(hipMemsetAsync(dtmp, 0, 2 * sizeof(uint32_t), s));
+    if (N)
+        degree_stats<<<div_up(N, kBlock), kBlock, 0, s>>>(N, dV, dhub, dtmp);
+    GCXK_LAUNCH_CHECK();
+    uint32_t h[2];
+    GCXK_CHECK(hipMemcpyAsync(h, dtmp, sizeof(h), hipMemcpyDeviceToHost, s));
+    GCXK_CHECK(hipStreamSynchronize(s));
+    *nhub = h[0];
+    *maxdeg = h[1];
+    return GCX_OK;
+}
+
+extern "C" int gcxk_graph_check(uint32_t N, const uint32_t *dV, const uint32_t *dE, uint32_t *dtmp,
+                                int *valid, void *stream)
+{
+    hipStream_t s = as_stream(stream);
+    GCXK_CHECK(hipMemsetAsync(dtmp, 0, sizeof(uint32_t), s));
+    if (N)
+        check_csr<<<div_up((uint64_t)N * kWave, kBlock), kBlock, 0, s>>>(N, dV, dE, dtmp);
+    GCXK_LAUNCH_CHECK();
+    uint32_t bad = 0;
+    GCXK_CHECK(hipMemcpyAsync(&bad, dtmp, sizeof(bad), hipMemcpyDeviceToHost, s));
+    GCXK_CHECK(hipStreamSynchronize(s));
+    *valid = bad == 0;
+    return GCX_OK;
+}
+
+extern "C" int gcxk_gen_rmat(int scale, uint64_t L, uint32_t t0, uint32_t t1, uint32_t t2,
+                             uint64_t seed, uint32_t *dI, uint32_t *dJ, void *stream)
+{
+    if (L == 0)
+        return GCX_OK;
+    gcx_rmat_thr thr = {t0, t1, t2};
+    gen_rmat_kernel<<<div_up(L, kBlock), kBlock, 0, as_stream(stream)>>>(scale, L, thr, seed, dI,
+                                                                         dJ);
+    GCXK_LAUNCH_CHECK();
+    return GCX_OK;
+}
+
+extern "C" int gcxk_gen_er(uint32_t n, uint64_t m, uint64_t seed, uint32_t *dI, uint32_t *dJ,
+                           void *stream)
+{
+    if (m == 0)
+        return GCX_OK;
+    gen_er_kernel<<<div_up(m, kBlock), kBlock, 0, as_stream(stream)>>>(n, m, seed, dI, dJ);
+    GCXK_LAUNCH_CHECK();
+    return GCX_OK;
+}
+
+extern "C" int gcxk_csr_bounds(uint32_t N, uint64_t M, const uint32_t *dV, const uint32_t *dE,
+                               uint32_t *dtmp, int *ok, void *stream)
+{
+    hipStream_t s = as_stream(stream);
+    GCXK_CHECK(hipMemsetAsync(dtmp, 0, sizeof(uint32_t), s));
+    const uint64_t n = (uint64_t)N > M ? (uint64_t)N : M;
+    csr_bounds<<<div_up(n ? n : 1, kBlock), kBlock, 0, s>>>(N, M, dV, dE, dtmp);
+    GCXK_LAUNCH_CHECK();
+    uint32_t bad = 0;
+    GCXK_CHECK(hipMemcpyAsync(&bad, dtmp, sizeof(bad), hipMemcpyDeviceToHost, s));
+    GCXK_CHECK(hipStreamSynchronize(s));
+    *ok = bad == 0;
+    return GCX_OK;
+}
+
+extern "C" int gcxk_sort_rows(uint32_t N, const uint32_t *dV, uint32_t *dE, uint64_t M,
+                              void *stream)
+{
+    hipStream_t s = as_stream(stream);
+    if (M == 0 || N == 0)
+        return GCX_OK;
+    const int b = bit_length(N - 1) ? bit_length(N - 1) : 1;
+    DevBuf keys, keys2, temp;
+    GCXK_CHECK(keys.alloc(M * sizeof(uint64_t)));
+    GCXK_CHECK(keys2.alloc(M * sizeof(uint64_t)));
+    row_keys<<<div_up((uint64_t)N * kWave, kBlock), kBlock, 0, s>>>(N, dV, dE, b,
+                                                                    (uint64_t *)keys.p);
+    GCXK_LAUNCH_CHECK();
+    size_t tb = 0;
+    GCXK_CHECK(rocprim::radix_sort_keys(nullptr, tb, (uint64_t *)keys.p, (uint64_t *)keys2.p,
+                                        (size_t)M, 0, 2 * b, s));
+    GCXK_CHECK(temp.alloc(tb));
+    GCXK_CHECK(rocprim::radix_sort_keys(temp.p, tb, (uint64_t *)keys.p, (uint64_t *)keys2.p,
+                                        (size_t)M, 0, 2 * b, s));
+    keys_low<<<div_up(M, kBlock), kBlock, 0, s>>>(M, (uint64_t *)keys2.p, b, dE);
+    GCXK_LAUNCH_CHECK();
+    GCXK_CHECK(hipStreamSynchronize(s));
+    return GCX_OK;
+}
+
+extern "C" int gcxk_make_simple(uint32_t N, uint32_t *dV, const uint32_t *dE, uint64_t M,
+                                uint32_t *dE_out, uint64_t *M_out, void *stream)
+{
+    hipStream_t s = as_stream(stream);
+    DevBuf flag, pos, deg, temp, temp2;
+    GCXK_CHECK(flag.alloc(M * sizeof(uint32_t)));
+    GCXK_CHECK(pos.alloc(M * sizeof(uint32_t)));
+    GCXK_CHECK(deg.alloc(((size_t)N + 1) * sizeof(uint32_t)));
+    GCXK_CHECK(hipMemsetAsync(deg.p, 0, ((size_t)N + 1) * sizeof(uint32_t), s));
+    if (N)
+        simple_flags<<<div_up((uint64_t)N * kWave, kBlock), kBlock, 0, s>>>(
+            N, dV, dE, (uint32_t *)flag.p, (uint32_t *)deg.p);
+    GCXK_LAUNCH_CHECK();
+    if (M) {
+        size_t tb = 0;
+        GCXK_CHECK(rocprim::exclusive_scan(nullptr, tb, (uint32_t *)flag.p, (uint32_t *)pos.p, 0u,
+                                           (size_t)M, rocprim::plus<uint32_t>(), s));
+        GCXK_CHECK(temp.alloc(tb));
+        GCXK_CHECK(rocprim::exclusive_scan(temp.p, tb, (uint32_t *)flag.p, (uint32_t *)pos.p, 0u,
+                                           (size_t)M, rocprim::plus<uint32_t>(), s));
+        compact_entries<<<div_up(M, kBlock), kBlock, 0, s>>>(M, dE, (uint32_t *)flag.p,
+                                                             (uint32_t *)pos.p, dE_out);
+        GCXK_LAUNCH_CHECK();
+    }
+    size_t tb2 = 0;
+    GCXK_CHECK(rocprim::exclusive_scan(nullptr, tb2, (uint32_t *)deg.p, dV, 0u, (size_t)N + 1,
+                                       rocprim::plus<uint32_t>(), s));
+    GCXK_CHECK(temp2.alloc(tb2));
+    GCXK_CHECK(rocprim::exclusive_scan(temp2.p, tb2, (uint32_t *)deg.p, dV, 0u, (size_t)N + 1,
+                                       rocprim::plus<uint32_t>(), s));
+    uint32_t m32 = 0;
+    GCXK_CHECK(hipMemcpyAsync(&m32, dV + N, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    GCXK_CHECK(hipStreamSynchronize(s));
+    *M_out = m32;
+    return GCX_OK;
+}

@@ -1,0 +1,321 @@
+// gnn.hip -- GraphSAGE-16 inference (mean aggregation, concat self, ReLU),
+// FP32, reproducing the reference's operation order bit for bit
+// (fast_gnn.c:13-243):
+//   layer 0  A = [mean fdeg(v) x8, mean fid(v) x8, fdeg(u) x8, fid(u) x8]
+//            fdeg(v) = (float)deg(v) / (float)md, fid(v) = (float)((double)v / N)
+//   hidden   A = [mean over neighbours of x[v][0..15], x[u][0..15]]
+//   dense    c_j = b_j; c_j = fmaf(W[i][j], A[i], c_j) for i = 0..31; h = c > 0 ? c : +0
+//   output   t_k = h_k + h_{k+8}; p = ((t0+t1)+(t2+t3)) + ((t4+t5)+(t6+t7))
+// Neighbour sums run left to right in CSR order from +0 and are divided by
+// the degree when it is positive.
+//
+// Mapping: 16 lanes per vertex (lane j = feature j), 4 vertices per wave.
+// Each group fetches 16 neighbour rows at once (64 B coalesced per row) and
+// then adds them in order, so the sequential dependency is only on the adds.
+// Hubs (degree >= GCXK_HUB_DEG) get a 512-thread workgroup that stages 512
+// neighbour rows per step into LDS (double-buffered, feature-major) while one
+// 16-lane group performs the ordered sum; the hub kernel runs on a second
+// stream next to the group kernel.
+//
+// This file is compiled with -ffp-contract=off: the only fused operations
+// are the explicit fmaf calls of the dense step.
+#include "common.hip.h"
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int F = 16;
+constexpr int kHubBlock = 512;
+constexpr int kHubChunk = kHubBlock;
+
+enum LayerKind { kInput = 0, kHidden = 1, kOutput = 2 };
+
+__device__ __forceinline__ float relu(float c) { return c > 0.0f ? c : 0.0f; }
+
+__device__ __forceinline__ float fid(uint32_t v, uint32_t N)
+{
+    return (float)((double)v / (double)N);
+}
+
+// Dense step for lane j of a 16-lane group; A[i] for i < 16 is a_lo of lane
+// i, for i >= 16 a_hi of lane i - 16.
+__device__ __forceinline__ float dense_group(const float (&w)[2 * F], float bias, float a_lo,
+                                             float a_hi)
+{
+    float c = bias;
+#pragma unroll
+    for (int i = 0; i < F; i++)
+        c = fmaf(w[i], __shfl(a_lo, i, F), c);
+#pragma unroll
+    for (int i = 0; i < F; i++)
+        c = fmaf(w[F + i], __shfl(a_hi, i, F), c);
+    return relu(c);
+}
+
+// Layer-0 dense step: A = [sd x8, si x8, selfd x8, selfi x8].
+__device__ __forceinline__ float dense_input(const float (&w)[2 * F], float bias, float sd, float si,
+                                             float selfd, float selfi)
+{
+    float c = bias;
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+        c = fmaf(w[i], sd, c);
+#pragma unroll
+    for (int i = 8; i < 16; i++)
+        c = fmaf(w[i], si, c);
+#pragma unroll
+    for (int i = 16; i < 24; i++)
+        c = fmaf(w[i], selfd, c);
+#pragma unroll
+    for (int i = 24; i < 32; i++)
+        c = fmaf(w[i], selfi, c);
+    return relu(c);
+}
+
+// Output reduction over lanes 0..7 of the group; valid in lane 0.
+__device__ __forceinline__ float output_tree(float h)
+{
+    const float t = h + __shfl_down(h, 8, F);
+    const float s1 = t + __shfl_xor(t, 1, F);
+    const float s2 = s1 + __shfl_xor(s1, 2, F);
+    return s2 + __shfl_xor(s2, 4, F);
+}
+
+__device__ __forceinline__ void load_weights(const float *__restrict__ W, int j, float (&w)[2 * F])
+{
+#pragma unroll
+    for (int i = 0; i < 2 * F; i++)
+        w[i] = W[i * F + j];
+}
+
+// ------------------------------------------------------------ group kernels
+
+template <int kind>
+__global__ __launch_bounds__(kBlock) void sage_groups(uint32_t N, const uint32_t *__restrict__ V,
+                                                      const uint32_t *__restrict__ E,
+                                                      const float *__restrict__ W,
+                                                      const float *__restrict__ b, float fmd,
+                                                      const float *__restrict__ x,
+                                                      float *__restrict__ out,
+                                                      uint32_t *__restrict__ key)
+{
+    const uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t u = (uint32_t)(t / F);
+    const int j = (int)(t % F);
+    if (u >= N) // whole groups leave together (N is per-group uniform)
+        return;
+    const uint32_t beg = V[u], end = V[u + 1];
+    const uint32_t deg = end - beg;
+    if (deg >= GCXK_HUB_DEG)
+        return; // hub kernel
+    float w[2 * F];
+    load_weights(W, j, w);
+    const float bias = b[j];
+    const float fdeg_f = (float)deg;
+    float h;
+    if (kind == kInput) {
+        float sd = 0.0f, si = 0.0f;
+        for (uint32_t i0 = beg; i0 < end; i0 += F) {
+            const uint32_t nb = end - i0 < (uint32_t)F ? end - i0 : (uint32_t)F;
+            float fd = 0.0f, fi = 0.0f;
+            if ((uint32_t)j < nb) {
+                const uint32_t v = E[i0 + j];
+                fd = (float)(V[v + 1] - V[v]) / fmd;
+                fi = fid(v, N);
+            }
+            for (uint32_t k = 0; k < nb; k++) {
+                sd = sd + __shfl(fd, (int)k, F);
+                si = si + __shfl(fi, (int)k, F);
+            }
+        }
+        if (fdeg_f > 0.0f) {
+            sd = sd / fdeg_f;
+            si = si / fdeg_f;
+        }
+        h = dense_input(w, bias, sd, si, fdeg_f / fmd, fid(u, N));
+    } else {
+        float acc = 0.0f;
+        uint32_t i0 = beg;
+        for (; i0 + F <= end; i0 += F) {
+            const uint32_t idx = E[i0 + j];
+            float r[F];
+#pragma unroll
+            for (int k = 0; k < F; k++)
+                r[k] = x[(size_t)__shfl(idx, k, F) * F + j];
+#pragma unroll
+            for (int k = 0; k < F; k++)
+                acc = acc + r[k];
+        }
+        if (i0 < end) {
+            const uint32_t nb = end - i0;
+            const uint32_t idx = (uint32_t)j < nb ? E[i0 + j] : 0u;
+            float r[F];
+#pragma unroll
+            for (int k = 0; k < F; k++) {
+                const uint32_t v = __shfl(idx, k, F);
+                r[k] = (uint32_t)k < nb ? x[(size_t)v * F + j] : 0.0f;
+            }
+#pragma unroll
+            for (int k = 0; k < F; k++)
+                if ((uint32_t)k < nb)
+                    acc = acc + r[k];
+        }
+        if (fdeg_f > 0.0f)
+            acc = acc / fdeg_f;
+        h = dense_group(w, bias, acc, x[(size_t)u * F + j]);
+    }
+    if (kind == kOutput) {
+        const float p = output_tree(h);
+        if (j == 0)
+            key[u] = float_order_key(p);
+    } else {
+        out[(size_t)u * F + j] = h;
+    }
+}
+
+// -------------------------------------------------------------- hub kernel
+
+template <int kind>
+__global__ __launch_bounds__(kHubBlock) void sage_hubs(uint32_t N, const uint32_t *__restrict__ V,
+                                                       const uint32_t *__restrict__ E,
+                                                       const float *__restrict__ W,
+                                                       const float *__restrict__ b, float fmd,
+                                                       const float *__restrict__ x,
+                                                       float *__restrict__ out,
+                                                       uint32_t *__restrict__ key,
+                                                       const uint32_t *__restrict__ hubs)
+{
+    // Feature-major staging: buf[s][f][r] = feature f of row r of step s.
+    __shared__ float buf[2][F][kHubChunk];
+    const uint32_t u = hubs[blockIdx.x];
+    const uint32_t beg = V[u], end = V[u + 1];
+    const uint32_t deg = end - beg;
+    const int tid = threadIdx.x;
+    const bool summer = tid < F; // lanes 0..15 of wave 0 own the ordered sums
+    const int j = tid & (F - 1);
+    float acc = 0.0f;
+
+    auto stage = [&](uint32_t c, int s) {
+        const uint32_t i = beg + c * kHubChunk + tid;
+        if (i < end) {
+            const uint32_t v = E[i];
+            if (kind == kInput) {
+                buf[s][0][tid] = (float)(V[v + 1] - V[v]) / fmd;
+                buf[s][1][tid] = fid(v, N);
+            } else {
+                const float4 *row = reinterpret_cast<const float4 *>(x + (size_t)v * F);
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const float4 r4 = row[q];
+                    buf[s][4 * q + 0][tid] = r4.x;
+                    buf[s][4 * q + 1][tid] = r4.y;
+                    buf[s][4 * q + 2][tid] = r4.z;
+                    buf[s][4 * q + 3][tid] = r4.w;
+                }
+            }
+        }
+    };
+
+    const uint32_t nchunk = (deg + kHubChunk - 1) / kHubChunk;
+    stage(0, 0);
+    __syncthreads();
+    for (uint32_t c = 0; c < nchunk; c++) {
+        const int s = c & 1;
+        if (c + 1 < nchunk)
+            stage(c + 1, s ^ 1);
+        if (summer) {
+            const uint32_t rows = deg - c * kHubChunk < (uint32_t)kHubChunk ? deg - c * kHubChunk
+                                                                          : (uint32_t)kHubChunk;
+            if (kind == kInput) {
+                // lane 0 sums fdeg, lane 1 sums fid (two independent chains)
+                if (j < 2) {
+                    const float *col = buf[s][j];
+                    for (uint32_t r = 0; r < rows; r++)
+                        acc = acc + col[r];
+                }
+            } else {
+                const float *col = buf[s][j];
+                uint32_t r = 0;
+                for (; r + 4 <= rows; r += 4) {
+                    const float4 q = *reinterpret_cast<const float4 *>(col + r);
+                    acc = acc + q.x;
+                    acc = acc + q.y;
+                    acc = acc + q.z;
+                    acc = acc + q.w;
+                }
+                for (; r < rows; r++)
+                    acc = acc + col[r];
+            }
+        }
+        __syncthreads();
+    }
+    if (tid >= kWave)
+        return;
+    // Wave 0 finishes; lanes 16..63 mirror lanes 0..15 harmlessly.
+    float w[2 * F];
+    load_weights(W, j, w);
+    const float bias = b[j];
+    const float fdeg_f = (float)deg;
+    float h;
+    if (kind == kInput) {
+        float sd = __shfl(acc, 0, F), si = __shfl(acc, 1, F);
+        if (fdeg_f > 0.0f) {
+            sd = sd / fdeg_f;
+            si = si / fdeg_f;
+        }
+        h = dense_input(w, bias, sd, si, fdeg_f / fmd, fid(u, N));
+    } else {
+        float a = acc;
+        if (fdeg_f > 0.0f)
+            a = a / fdeg_f;
+        h = dense_group(w, bias, a, x[(size_t)u * F + j]);
+    }
+    if (tid < F) {
+        if (kind == kOutput) {
+            const float p = output_tree(h);
+            if (tid == 0)
+                key[u] = float_order_key(p);
+        } else {
+            out[(size_t)u * F + j] = h;
+        }
+    }
+}
+
+template <int kind>
+int launch_layer(uint32_t N, const uint32_t *V, const uint32_t *E, const float *W, const float *b,
+                 float fmd, const float *x, float *out, uint32_t *key, const uint32_t *hubs,
+                 uint32_t nhub, void *stream, void *hub_stream)
+{
+    if (N == 0)
+        return GCX_OK;
+    // Hubs go first on their own stream so that they overlap the groups.
+    if (nhub)
+        sage_hubs<kind><<<nhub, kHubBlock, 0, as_stream(hub_stream)>>>(N, V, E, W, b, fmd, x, out,
+                                                                       key, hubs);
+    sage_groups<kind><<<div_up((uint64_t)N * F, kBlock), kBlock, 0, as_stream(stream)>>>(
+        N, V, E, W, b, fmd, x, out, key);
+    GCXK_LAUNCH_CHECK();
+    return GCX_OK;
+}
+
+} // namespace
+
+extern "C" int gcxk_gnn_fp32_layer(int kind, uint32_t N, const uint32_t *dV, const uint32_t *dE,
+                                   const float *W, const float *b, int md, const float *x,
+                                   float *out, uint32_t *key, const uint32_t *hubs, uint32_t nhub,
+                                   void *stream, void *hub_stream)
+{
+    switch (kind) {
+    case 0:
+        return launch_layer<kInput>(N, dV, dE, W, b, (float)md, nullptr, out, nullptr, hubs, nhub,
+                                    stream, hub_stream);
+    case 1:
+        return launch_layer<kHidden>(N, dV, dE, W, b, 0.0f, x, out, nullptr, hubs, nhub, stream,
+                                     hub_stream);
+    case 2:
+        return launch_layer<kOutput>(N, dV, dE, W, b, 0.0f, x, nullptr, key, hubs, nhub, stream,
+                                     hub_stream);
+    default:
+        return GCX_ERR_ARG;
+    }
+}

@@ -1,0 +1,149 @@
+// keys.hip -- elementwise ordering keys (32-bit; larger = coloured earlier).
+//   FF  first_fit.c:8-19              key = N - id
+//   LF  largest_degree_first.c:12-16  key = deg
+//   LLF largest_degree_first.c:87-97  key = bit length of deg (0 for deg 0)
+// plus conversions between the reference's double priorities and keys.
+#include "common.hip.h"
+
+namespace {
+
+constexpr int kBlock = 256;
+
+__global__ __launch_bounds__(kBlock) void keys_ff(uint32_t N, uint32_t *__restrict__ key)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < N)
+        key[i] = N - i;
+}
+
+__global__ __launch_bounds__(kBlock) void keys_lf(uint32_t N, const uint32_t *__restrict__ V,
+                                                  uint32_t *__restrict__ key)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < N)
+        key[i] = V[i + 1] - V[i];
+}
+
+__global__ __launch_bounds__(kBlock) void keys_llf(uint32_t N, const uint32_t *__restrict__ V,
+                                                   uint32_t *__restrict__ key)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < N) {
+        const uint32_t d = V[i + 1] - V[i];
+        key[i] = d ? 32u - (uint32_t)__builtin_clz(d) : 0u;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void keys_from_colors(uint32_t N, const int *__restrict__ c,
+                                                           uint32_t *__restrict__ key)
+{
+    // Culberson step (main.c:38-40): p = colour; colours are >= 1.
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < N)
+        key[i] = (uint32_t)c[i];
+}
+
+__global__ __launch_bounds__(kBlock) void scan_doubles(uint32_t N, const double *__restrict__ p,
+                                                       gcxk_ctl *ctl)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    bool nan = false, nonfloat = false;
+    if (i < N) {
+        const double d = p[i];
+        nan = d != d;
+        nonfloat = !nan && (double)(float)d != d;
+    }
+    const uint64_t bn = __ballot(nan), bf = __ballot(nonfloat);
+    if (lane_id() == 0) {
+        if (bn)
+            atomicAdd(&ctl->nan_count, (uint32_t)__popcll(bn));
+        if (bf)
+            atomicAdd(&ctl->nonfloat, (uint32_t)__popcll(bf));
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void doubles_to_key32(uint32_t N, const double *__restrict__ p,
+                                                           uint32_t *__restrict__ key)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < N)
+        key[i] = float_order_key((float)p[i]);
+}
+
+__global__ __launch_bounds__(kBlock) void doubles_to_key64(uint32_t N, const double *__restrict__ p,
+                                                           uint64_t *__restrict__ key)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < N)
+        key[i] = double_order_key(p[i]);
+}
+
+__global__ __launch_bounds__(kBlock) void key32_to_doubles(uint32_t N, const uint32_t *__restrict__ key,
+                                                           int kind, double *__restrict__ p)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < N)
+        p[i] = kind == 0 ? (double)key[i] : (double)float_from_order_key(key[i]);
+}
+
+__global__ __launch_bounds__(kBlock) void key64_to_doubles(uint32_t N, const uint64_t *__restrict__ key,
+                                                           double *__restrict__ p)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < N)
+        p[i] = double_from_order_key(key[i]);
+}
+
+} // namespace
+
+#define ELEMENTWISE(kernel, ...)                                                \
+    do {                                                                        \
+        if (N == 0)                                                             \
+            return GCX_OK;                                                      \
+        kernel<<<div_up(N, kBlock), kBlock, 0, as_stream(stream)>>>(__VA_ARGS__); \
+        GCXK_LAUNCH_CHECK();                                                    \
+        return GCX_OK;                                                          \
+    } while (0)
+
+extern "C" int gcxk_keys_ff(uint32_t N, uint32_t *key, void *stream) { ELEMENTWISE(keys_ff, N, key); }
+
+extern "C" int gcxk_keys_lf(uint32_t N, const uint32_t *dV, uint32_t *key, void *stream)
+{
+    ELEMENTWISE(keys_lf, N, dV, key);
+}
+
+extern "C" int gcxk_keys_llf(uint32_t N, const uint32_t *dV, uint32_t *key, void *stream)
+{
+    ELEMENTWISE(keys_llf, N, dV, key);
+}
+
+extern "C" int gcxk_keys_from_colors(uint32_t N, const int *color, uint32_t *key, void *stream)
+{
+    ELEMENTWISE(keys_from_colors, N, color, key);
+}
+
+extern "C" int gcxk_keys_scan_doubles(uint32_t N, const double *p, gcxk_ctl *ctl, void *stream)
+{
+    ELEMENTWISE(scan_doubles, N, p, ctl);
+}
+
+extern "C" int gcxk_keys_from_doubles32(uint32_t N, const double *p, uint32_t *key, void *stream)
+{
+    ELEMENTWISE(doubles_to_key32, N, p, key);
+}
+
+extern "C" int gcxk_keys_from_doubles64(uint32_t N, const double *p, uint64_t *key, void *stream)
+{
+    ELEMENTWISE(doubles_to_key64, N, p, key);
+}
+
+extern "C" int gcxk_keys_to_doubles(uint32_t N, const uint32_t *key, int kind, double *p,
+                                    void *stream)
+{
+    ELEMENTWISE(key32_to_doubles, N, key, kind, p);
+}
+
+extern "C" int gcxk_keys64_to_doubles(uint32_t N, const uint64_t *key, double *p, void *stream)
+{
+    ELEMENTWISE(key64_to_doubles, N, key, p);
+}

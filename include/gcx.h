@@ -1,0 +1,166 @@
+/*
+ * gcx.h -- device-resident extension API of the MI355X colouring library.
+ *
+ * The reference-compatible entry points (coloring.h, fast_gnn.h, gnn.h,
+ * runner.h, the ordering headers, fast_io.h) take host arrays and copy them
+ * to the GPU on every call. This header adds what those signatures cannot
+ * express: graphs, weights and scratch that stay resident in HBM across
+ * calls, explicit HIP streams (passed as void *), status codes, BF16 GNN
+ * inference and batched colouring. It has no counterpart in the reference.
+ *
+ * All functions return GCX_OK (0) or a negative gcx_status. Streams are
+ * hipStream_t values passed as void * (NULL = the library's own stream).
+ */
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum gcx_status {
+    GCX_OK = 0,
+    GCX_ERR_ARG = -1,      /* invalid argument (size, NaN priority, ...) */
+    GCX_ERR_NOMEM = -2,    /* host or device allocation failed */
+    GCX_ERR_HIP = -3,      /* a HIP runtime call failed */
+    GCX_ERR_CAPACITY = -4, /* input larger than the workspace / scratch */
+    GCX_ERR_IO = -5,       /* file cannot be opened */
+    GCX_ERR_FORMAT = -6,   /* malformed or truncated file */
+    GCX_ERR_NODEV = -7,    /* no usable gfx950 device */
+    GCX_ERR_INVALID = -8,  /* a colouring or CSR failed validation */
+    GCX_ERR_BUDGET = -9    /* admission rejected: HBM budget exceeded */
+};
+
+/* Ordering heuristics (names follow the reference's runner labels). */
+enum gcx_heuristic {
+    GCX_FF = 0,   /* first fit:            p = N - id          */
+    GCX_LF = 1,   /* largest degree first: p = deg             */
+    GCX_LLF = 2,  /* largest log degree:   p = bitlength(deg)  */
+    GCX_SL = 3,   /* smallest degree last: peel level          */
+    GCX_SLL = 4,  /* smallest log degree last                  */
+    GCX_GNN = 5   /* GraphSAGE-16 priorities from a gcx_model  */
+};
+
+enum gcx_precision {
+    GCX_FP32 = 0, /* bit-exact with the reference's FP32 GNN */
+    GCX_BF16 = 1  /* BF16 rows + MFMA layer transforms, FP32 accumulate */
+};
+
+typedef struct gcx_graph gcx_graph;
+typedef struct gcx_model gcx_model;
+typedef struct gcx_workspace gcx_workspace;
+
+const char *gcx_status_string(int status);
+
+/* Initialise device 0 (must be gfx950). Called implicitly by every entry
+ * point; returns GCX_ERR_NODEV when no usable GPU is present. */
+int gcx_init(void);
+/* Library version string and the gfx target it was built for. */
+const char *gcx_version(void);
+
+/* ---- streams and timing (thin wrappers so callers need no HIP headers) */
+int gcx_stream_create(void **stream);
+int gcx_stream_destroy(void *stream);
+int gcx_stream_sync(void *stream);
+int gcx_device_sync(void);
+int gcx_event_create(void **event);
+int gcx_event_destroy(void *event);
+int gcx_event_record(void *event, void *stream);
+int gcx_event_elapsed_ms(void *start, void *stop, float *ms);
+/* Free and total HBM in bytes. */
+int gcx_mem_info(size_t *free_bytes, size_t *total_bytes);
+
+/* ---- device-resident graphs (canonical CSR: symmetric, sorted, simple) */
+int gcx_graph_from_csr(uint32_t N, const uint32_t *V, const uint32_t *E, void *stream,
+                       gcx_graph **out);
+/* Host pair list (0-based) -> canonical CSR built on the device. */
+int gcx_graph_from_pairs(uint32_t N, uint64_t L, const uint32_t *I, const uint32_t *J,
+                         void *stream, gcx_graph **out);
+/* Pairs generated on the device (same pairs as the host generators). */
+int gcx_graph_rmat(int scale, uint32_t edge_factor, double a, double b, double c,
+                   uint64_t seed, void *stream, gcx_graph **out);
+int gcx_graph_er(uint32_t n, uint64_t m, uint64_t seed, void *stream, gcx_graph **out);
+int gcx_graph_free(gcx_graph *g);
+uint32_t gcx_graph_num_vertices(const gcx_graph *g);
+uint64_t gcx_graph_num_entries(const gcx_graph *g);
+uint32_t gcx_graph_max_degree(const gcx_graph *g);
+int gcx_graph_download(const gcx_graph *g, uint32_t *V, uint32_t *E);
+/* Device CSR check (sorted, simple, symmetric, V[N] == M): 1 valid, 0 not. */
+int gcx_graph_check(const gcx_graph *g, void *stream, int *valid);
+/* Bytes of HBM a graph with N vertices and M entries occupies. */
+uint64_t gcx_graph_footprint(uint32_t N, uint64_t M);
+
+/* ---- GNN weights ("params md" + params floats, layers of 32x16 + 16) */
+int gcx_model_load(const char *path, gcx_model **out);
+int gcx_model_from_params(int params, const float *param, int md, gcx_model **out);
+int gcx_model_free(gcx_model *m);
+int gcx_model_layers(const gcx_model *m);
+
+/* ---- workspace: scratch for ordering + colouring of graphs up to (N, M) */
+int gcx_workspace_create(uint32_t N, uint64_t M, gcx_workspace **out);
+int gcx_workspace_free(gcx_workspace *ws);
+uint64_t gcx_workspace_footprint(uint32_t N, uint64_t M);
+
+/* Compute priorities for g into the workspace (model/precision used by
+ * GCX_GNN only). */
+int gcx_order(gcx_workspace *ws, const gcx_graph *g, int heuristic, const gcx_model *model,
+              int precision, void *stream);
+/* Load caller priorities (host doubles, N of them) into the workspace. */
+int gcx_set_priorities(gcx_workspace *ws, const gcx_graph *g, const double *p, void *stream);
+/* Jones-Plassmann colouring with the workspace's priorities (p descending,
+ * id ascending); colours 1..C land in the workspace. */
+int gcx_color(gcx_workspace *ws, const gcx_graph *g, void *stream);
+/* Culberson iterated greedy: `iterations` more passes, each with p = colour. */
+int gcx_recolor(gcx_workspace *ws, const gcx_graph *g, int iterations, void *stream);
+/* Proper-colouring check of the workspace colours: *max_color = C or 0. */
+int gcx_validate(gcx_workspace *ws, const gcx_graph *g, void *stream, int *max_color);
+/* Greedy-rule certificate: number of vertices whose colour is not the mex of
+ * their predecessors' colours under the workspace priorities. */
+int gcx_mex_check(gcx_workspace *ws, const gcx_graph *g, void *stream, uint64_t *violations);
+
+int gcx_download_colors(const gcx_workspace *ws, uint32_t N, int *color, void *stream);
+/* Load caller colours (host ints, N of them) into the workspace. */
+int gcx_upload_colors(gcx_workspace *ws, uint32_t N, const int *color, void *stream);
+/* Priorities as doubles (the reference's p). */
+int gcx_download_priorities(const gcx_workspace *ws, uint32_t N, double *p, void *stream);
+/* Number of JP rounds (DAG depth) of the last gcx_color call. */
+int gcx_last_rounds(const gcx_workspace *ws, uint32_t *rounds);
+/* Round kernels the last round loop enqueued (rounds + the empty tail). */
+int gcx_last_launches(const gcx_workspace *ws, uint32_t *launches);
+
+/* Device pointers for callers that bring their own kernels. */
+int *gcx_workspace_colors_dev(gcx_workspace *ws);
+const uint32_t *gcx_graph_V_dev(const gcx_graph *g);
+const uint32_t *gcx_graph_E_dev(const gcx_graph *g);
+
+/* ---- per-stage kernel timing (HIP events around each launch) ---- */
+enum gcx_stage {
+    GCX_ST_KEYS = 0,
+    GCX_ST_SL = 1,
+    GCX_ST_GNN_IN = 2,
+    GCX_ST_GNN_HID = 3,
+    GCX_ST_GNN_OUT = 4,
+    GCX_ST_JP_INIT = 5,
+    GCX_ST_JP_ROUNDS = 6,
+    GCX_ST_COUNT = 7
+};
+int gcx_profile_enable(int on);
+int gcx_profile_reset(void);
+/* Accumulated milliseconds and launch count per stage (synchronises). */
+int gcx_profile_read(double ms[GCX_ST_COUNT], uint64_t launches[GCX_ST_COUNT]);
+
+/* ---- host generators (pair lists; same pairs as the device versions) ---- */
+void gcx_rmat_thresholds(double a, double b, double c, uint32_t thr[3]);
+int gcx_gen_rmat_pairs(int scale, uint32_t edge_factor, double a, double b, double c,
+                       uint64_t seed, uint32_t *I, uint32_t *J);
+int gcx_gen_er_pairs(uint32_t n, uint64_t m, uint64_t seed, uint32_t *I, uint32_t *J);
+int gcx_gen_permutation(uint32_t n, uint64_t seed, uint32_t *perm);
+uint64_t gcx_stencil27_pair_count(uint32_t g);
+int gcx_gen_stencil27_pairs(uint32_t g, uint64_t seed, uint32_t *I, uint32_t *J);
+int gcx_gen_rgg2d_pairs(uint32_t n, double radius, uint64_t seed, uint64_t cap, uint32_t *I,
+                        uint32_t *J, uint64_t *L);
+
+#ifdef __cplusplus
+}
+#endif

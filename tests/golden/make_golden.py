@@ -1,0 +1,171 @@
+"""Write the golden vectors under tests/golden/ from the reference's own code.
+
+Runs in the development container only: it loads oracle/_ref/libref.so, the
+reference's C sources compiled by `make -C oracle ref` (from /root/reference),
+and the product library's host-side graph generators (no GPU needed).
+
+  python tests/golden/make_golden.py [--big]
+
+Outputs
+  tiny_<name>.npz   full vectors for hand-made graphs: CSR from the
+                    reference's .mtx path, priorities and colours for FF, LF,
+                    LLF, SL, SLL, GNN2-4, and a 5-pass Culberson chain
+  rmat10.npz        the same for R-MAT scale 10
+  digests.json      SHA-256 digests (CSR, priority bits, colours) and colour
+                    counts for R-MAT scale 16 (via a .mtx file, as the
+                    reference's own driver reads it), scale 20, and with
+                    --big scale 22 and the 160^3 27-point stencil
+  models.npz        the weights as parsed by the reference's gnn_setup
+"""
+from __future__ import annotations
+
+import argparse
+import hashlib
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "gnn-coloring_amd"))
+
+import gcx  # noqa: E402  (host generators only)
+import oracle  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+REF_TREE = "/root/reference"
+HEUR = ["FF", "LF", "LLF", "SL", "SLL", "GNN2", "GNN3", "GNN4"]
+
+
+def sha(a) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def write_mtx(path, N, I, J):
+    """Matrix Market pattern file with 1-based pairs (values omitted)."""
+    with open(path, "w") as f:
+        f.write("%%MatrixMarket matrix coordinate pattern general\n")
+        f.write(f"{N} {N} {len(I)}\n")
+        np.savetxt(f, np.stack([I.astype(np.int64) + 1, J.astype(np.int64) + 1], 1), fmt="%d")
+
+
+def ref_csr_via_mtx(ref, N, I, J):
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "g.mtx")
+        write_mtx(path, N, I, J)
+        return ref.mtx_csr(path)
+
+
+def all_heuristics(ref, models, V, E):
+    out = {}
+    for h in HEUR:
+        if h.startswith("GNN"):
+            param, md = models[int(h[3:])]
+            p = ref.gnn(param, V, E, md)
+        else:
+            p = ref.ordering(h, V, E)
+        c = ref.jp(V, E, p)
+        out[h] = (p, c, ref.validate(V, E, c))
+    return out
+
+
+def culberson(ref, V, E, p, K):
+    c = ref.jp(V, E, p)
+    counts = [ref.validate(V, E, c)]
+    for _ in range(1, K):
+        c = ref.jp(V, E, c.astype(np.float64))
+        counts.append(ref.validate(V, E, c))
+    return c, np.array(counts, np.int32)
+
+
+def tiny_graphs():
+    g = {}
+    g["path5"] = (5, [0, 1, 2, 3], [1, 2, 3, 4])
+    g["star6"] = (6, [0] * 5, [1, 2, 3, 4, 5])
+    k6 = [(a, b) for a in range(6) for b in range(a + 1, 6)]
+    g["clique6"] = (6, [a for a, _ in k6], [b for _, b in k6])
+    g["cycle7"] = (7, list(range(7)), [(i + 1) % 7 for i in range(7)])
+    g["bip34"] = (7, [a for a in range(3) for _ in range(4)], [b for _ in range(3) for b in range(3, 7)])
+    g["empty5"] = (5, [], [])
+    # isolated vertices 0, 5, 9 plus a triangle, a pendant path and repeats/self loops
+    g["mixed10"] = (10, [1, 2, 3, 3, 6, 7, 4, 4, 2], [2, 3, 1, 4, 7, 8, 4, 3, 1])
+    # SL threshold jump: a path P3 glued to a K5 (min degree jumps after peel)
+    k5 = [(a, b) for a in range(3, 8) for b in range(a + 1, 8)]
+    g["jump8"] = (8, [0, 1, 2] + [a for a, _ in k5], [1, 2, 3] + [b for _, b in k5])
+    n, I, J = gcx.er_pairs(300, 1200, seed=3)
+    g["er300"] = (n, I, J)
+    n, I, J = gcx.stencil27_pairs(6, seed=5)
+    g["stencil6"] = (n, I, J)
+    return g
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--big", action="store_true", help="also scale 22 and the 160^3 stencil")
+    args = ap.parse_args()
+    ref = oracle.Ref()
+    models = {l: ref.model_params(REF_TREE, l) for l in (2, 3, 4)}
+    np.savez_compressed(os.path.join(OUT, "models.npz"),
+                        **{f"m{l}": models[l][0] for l in models},
+                        **{f"md{l}": np.int32(models[l][1]) for l in models})
+    # -------- full vectors
+    graphs = tiny_graphs()
+    sc, I, J = gcx.rmat_pairs(10, 16, seed=7)
+    graphs["rmat10"] = (sc, I, J)
+    for name, (N, I, J) in graphs.items():
+        I = np.asarray(I, np.uint32)
+        J = np.asarray(J, np.uint32)
+        V, E = ref_csr_via_mtx(ref, N, I, J)
+        res = all_heuristics(ref, models, V, E)
+        arrs = {"N": np.int64(N), "I": I, "J": J, "V": V, "E": E}
+        for h, (p, c, C) in res.items():
+            arrs[f"p_{h}"] = p
+            arrs[f"color_{h}"] = c
+            arrs[f"ncolors_{h}"] = np.int32(C)
+        cc, counts = culberson(ref, V, E, res["GNN3"][0], 5)
+        arrs["culb_color"] = cc
+        arrs["culb_counts"] = counts
+        fname = f"{name}.npz" if name.startswith("rmat") else f"tiny_{name}.npz"
+        np.savez_compressed(os.path.join(OUT, fname), **arrs)
+        print(name, N, int(V[-1]), {h: int(r[2]) for h, r in res.items()}, flush=True)
+    # -------- digests
+    digests = {}
+    path = os.path.join(OUT, "digests.json")
+    if os.path.exists(path):
+        digests = json.load(open(path))
+    cases = [("rmat16", "rmat", 16, True), ("rmat20", "rmat", 20, False)]
+    if args.big:
+        cases += [("rmat22", "rmat", 22, False), ("stencil160", "stencil", 160, False)]
+    for name, kind, size, via_mtx in cases:
+        t = time.time()
+        if kind == "rmat":
+            N, I, J = gcx.rmat_pairs(size, 16, seed=1)
+            desc = {"kind": "rmat", "scale": size, "edge_factor": 16, "abc": [0.57, 0.19, 0.19],
+                    "seed": 1}
+        else:
+            N, I, J = gcx.stencil27_pairs(size, seed=1)
+            desc = {"kind": "stencil27", "g": size, "seed": 1}
+        if via_mtx:
+            V, E = ref_csr_via_mtx(ref, N, I, J)
+        else:
+            V, E = oracle.csr_from_pairs(N, I, J)
+        rec = {"graph": desc, "N": int(N), "M": int(V[-1]), "sha_V": sha(V), "sha_E": sha(E),
+               "csr_via": "reference mtx_parse_par+sort+make_simple" if via_mtx else "oracle"}
+        res = all_heuristics(ref, models, V, E)
+        for h, (p, c, C) in res.items():
+            rec[h] = {"sha_p": sha(p), "sha_color": sha(c), "ncolors": int(C)}
+        if name in ("rmat16", "rmat20"):
+            cc, counts = culberson(ref, V, E, res["GNN3"][0], 8)
+            rec["culb_GNN3_8"] = {"sha_color": sha(cc), "counts": counts.tolist()}
+        digests[name] = rec
+        print(name, rec["N"], rec["M"], {h: rec[h]["ncolors"] for h in HEUR},
+              "%.1fs" % (time.time() - t), flush=True)
+        json.dump(digests, open(path, "w"), indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,295 @@
+"""HIP path vs the oracle and the reference's golden vectors (needs gfx950).
+
+Bit-exact bar: CSR, integer priorities, FP32 GNN priorities (IEEE bits),
+colours and colour counts. Every call goes through the C ABI (libgcx.so).
+"""
+import hashlib
+
+import numpy as np
+import pytest
+
+import gcx
+import oracle
+from conftest import HEUR, TINY, load_case
+
+pytestmark = pytest.mark.gpu
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    gcx.init()  # raises (fails the test) when the HIP library cannot reach a gfx950
+    yield
+
+
+@pytest.fixture(scope="module")
+def gmodels():
+    return {l: gcx.Model.layers_file(l) for l in (2, 3, 4)}
+
+
+def order(ws, g, h, gmodels, precision=gcx.FP32):
+    if h.startswith("GNN"):
+        ws.order(g, gcx.GNN, gmodels[int(h[3:])], precision)
+    else:
+        ws.order(g, h)
+
+
+@pytest.mark.parametrize("name", TINY + ["rmat10"])
+def test_device_csr_build_matches_reference(name):
+    c = load_case(name)
+    g = gcx.Graph.from_pairs(int(c["N"]), c["I"], c["J"])
+    V, E = g.download()
+    np.testing.assert_array_equal(V, c["V"])
+    np.testing.assert_array_equal(E, c["E"])
+    assert g.check()
+
+
+@pytest.mark.parametrize("name", TINY + ["rmat10"])
+def test_priorities_colours_bit_exact(name, gmodels):
+    c = load_case(name)
+    N = int(c["N"])
+    g = gcx.Graph.from_csr(c["V"], c["E"])
+    ws = gcx.Workspace(N, len(c["E"]))
+    for h in HEUR:
+        order(ws, g, h, gmodels)
+        p = ws.priorities(N)
+        assert p.view(np.uint64).tolist() == c[f"p_{h}"].view(np.uint64).tolist(), h
+        ws.color(g)
+        col = ws.colors(N)
+        np.testing.assert_array_equal(col, c[f"color_{h}"], err_msg=h)
+        assert ws.validate(g) == int(c[f"ncolors_{h}"]), h
+        assert ws.mex_check(g) == 0, h
+
+
+@pytest.mark.parametrize("name", TINY + ["rmat10"])
+def test_culberson_chain(name, gmodels):
+    c = load_case(name)
+    N = int(c["N"])
+    g = gcx.Graph.from_csr(c["V"], c["E"])
+    ws = gcx.Workspace(N, len(c["E"]))
+    ws.order(g, gcx.GNN, gmodels[3])
+    ws.color(g)
+    counts = [ws.validate(g)]
+    for _ in range(4):
+        ws.recolor(g, 1)
+        counts.append(ws.validate(g))
+    np.testing.assert_array_equal(ws.colors(N), c["culb_color"])
+    assert counts == c["culb_counts"].tolist()
+
+
+@pytest.mark.parametrize("name", ["rmat16", "rmat20"])
+def test_digests_device_generated(name, digests, gmodels):
+    rec = digests[name]
+    gd = rec["graph"]
+    g = gcx.Graph.rmat(gd["scale"], gd["edge_factor"], *gd["abc"], seed=gd["seed"])
+    V, E = g.download()
+    assert sha(V) == rec["sha_V"] and sha(E) == rec["sha_E"]
+    ws = gcx.Workspace(g.N, g.M)
+    for h in HEUR:
+        order(ws, g, h, gmodels)
+        assert sha(ws.priorities(g.N)) == rec[h]["sha_p"], h
+        ws.color(g)
+        assert sha(ws.colors(g.N)) == rec[h]["sha_color"], h
+        assert ws.validate(g) == rec[h]["ncolors"], h
+    order(ws, g, "GNN3", gmodels)
+    ws.color(g)
+    ws.recolor(g, 7)
+    assert sha(ws.colors(g.N)) == rec["culb_GNN3_8"]["sha_color"]
+
+
+def test_device_generator_equals_host_generator():
+    N, I, J = gcx.rmat_pairs(14, 16, seed=5)
+    gh = gcx.Graph.from_pairs(N, I, J)
+    gd = gcx.Graph.rmat(14, 16, seed=5)
+    Vh, Eh = gh.download()
+    Vd, Ed = gd.download()
+    np.testing.assert_array_equal(Vh, Vd)
+    np.testing.assert_array_equal(Eh, Ed)
+    n, I, J = gcx.er_pairs(5000, 40000, seed=6)
+    Vh, Eh = gcx.Graph.from_pairs(n, I, J).download()
+    Vd, Ed = gcx.Graph.er(5000, 40000, seed=6).download()
+    np.testing.assert_array_equal(Vh, Vd)
+    np.testing.assert_array_equal(Eh, Ed)
+
+
+# ------------------------------------------------------------- edge cases
+
+def clique(n):
+    return oracle.csr_from_pairs(n, *zip(*[(a, b) for a in range(n) for b in range(a + 1, n)]))
+
+
+@pytest.mark.parametrize("n", [65, 130, 1100])
+def test_cliques_need_more_than_64_colours(n, gmodels):
+    # n > 64 forces the windowed mex; n = 1100 makes every vertex a hub
+    V, E = clique(n)
+    g = gcx.Graph.from_csr(V, E)
+    ws = gcx.Workspace(n, len(E))
+    for h in ["FF", "LF", "SL", "GNN2"]:
+        order(ws, g, h, gmodels)
+        p = ws.priorities(n)
+        ws.color(g)
+        col = ws.colors(n)
+        want, rounds = oracle.greedy(V, E, p)
+        np.testing.assert_array_equal(col, want)
+        assert sorted(col.tolist()) == list(range(1, n + 1))
+        assert ws.validate(g) == n and ws.mex_check(g) == 0
+        assert ws.rounds() == rounds == n
+
+
+@pytest.mark.parametrize("leaves", [1023, 1024, 5000])
+def test_star_hub_paths(leaves, gmodels):
+    # the centre crosses the hub threshold (degree >= 1024) at 1024 leaves
+    n = leaves + 1
+    V, E = oracle.csr_from_pairs(n, np.zeros(leaves, np.uint32), np.arange(1, n, dtype=np.uint32))
+    g = gcx.Graph.from_csr(V, E)
+    ws = gcx.Workspace(n, len(E))
+    param, md = oracle.load_model(gcx.MODEL_DIR + "/m3.gnn")
+    for h in ["FF", "LF", "GNN3"]:
+        order(ws, g, h, gmodels)
+        p = ws.priorities(n)
+        if h == "GNN3":
+            want_p = oracle.gnn_fp32(param, V, E, md)
+            assert p.view(np.uint64).tolist() == want_p.view(np.uint64).tolist()
+        ws.color(g)
+        want, _ = oracle.greedy(V, E, p)
+        np.testing.assert_array_equal(ws.colors(n), want)
+
+
+def test_hub_heavy_random_graph(gmodels):
+    # several hubs with overlapping neighbourhoods + many small vertices
+    rng = np.random.default_rng(21)
+    n = 20000
+    hubs = rng.choice(n, 12, replace=False)
+    I = np.concatenate([np.repeat(hubs, 3000), rng.integers(0, n, 60000)]).astype(np.uint32)
+    J = np.concatenate([rng.integers(0, n, 36000), rng.integers(0, n, 60000)]).astype(np.uint32)
+    V, E = oracle.csr_from_pairs(n, I, J)
+    g = gcx.Graph.from_pairs(n, I, J)
+    Vg, Eg = g.download()
+    np.testing.assert_array_equal(Vg, V)
+    np.testing.assert_array_equal(Eg, E)
+    ws = gcx.Workspace(n, len(E))
+    models = {l: oracle.load_model(f"{gcx.MODEL_DIR}/m{l}.gnn") for l in (2, 3, 4)}
+    for h in HEUR:
+        order(ws, g, h, gmodels)
+        p = ws.priorities(n)
+        if h.startswith("GNN"):
+            want_p = oracle.gnn_fp32(*models[int(h[3:])][:1], V, E, models[int(h[3:])][1])
+        else:
+            want_p = oracle.key(h, V, E)[0]
+        assert p.view(np.uint64).tolist() == want_p.view(np.uint64).tolist(), h
+        ws.color(g)
+        want, _ = oracle.greedy(V, E, p)
+        np.testing.assert_array_equal(ws.colors(n), want, err_msg=h)
+
+
+def test_single_vertex_and_isolated(gmodels):
+    for n in [1, 2, 17]:
+        V = np.zeros(n + 1, np.uint32)
+        E = np.zeros(0, np.uint32)
+        g = gcx.Graph.from_csr(V, E)
+        ws = gcx.Workspace(n, 0)
+        for h in HEUR:
+            order(ws, g, h, gmodels)
+            ws.color(g)
+            assert ws.colors(n).tolist() == [1] * n
+            assert ws.validate(g) == 1
+
+
+def test_caller_priorities_with_ties_and_doubles():
+    c = load_case("rmat10")
+    V, E = c["V"], c["E"]
+    N = len(V) - 1
+    g = gcx.Graph.from_csr(V, E)
+    ws = gcx.Workspace(N, len(E))
+    rng = np.random.default_rng(3)
+    cases = [rng.integers(0, 3, N).astype(np.float64),          # heavy ties -> id tie-break
+             rng.standard_normal(N),                             # not floats -> 64-bit keys
+             -np.arange(N, dtype=np.float64),                    # negative values
+             np.where(rng.random(N) < 0.5, 0.0, -0.0)]           # signed zeros compare equal
+    for p in cases:
+        ws.set_priorities(g, p)
+        ws.color(g)
+        want, _ = oracle.greedy(V, E, p)
+        np.testing.assert_array_equal(ws.colors(N), want)
+        assert ws.mex_check(g) == 0
+    with pytest.raises(gcx.GcxError) as e:
+        bad = np.zeros(N)
+        bad[5] = np.nan
+        ws.set_priorities(g, bad)
+    assert e.value.status == gcx.ERR_ARG
+
+
+def test_errors_are_reported():
+    c = load_case("er300")
+    g = gcx.Graph.from_csr(c["V"], c["E"])
+    small = gcx.Workspace(10, 10)
+    with pytest.raises(gcx.GcxError) as e:
+        small.order(g, "FF")
+    assert e.value.status == gcx.ERR_CAPACITY
+    E = c["E"].copy()
+    E[3] = 100000  # out of range: refused before any kernel reads it
+    with pytest.raises(gcx.GcxError) as e:
+        gcx.Graph.from_csr(c["V"], E)
+    assert e.value.status == gcx.ERR_ARG
+    with pytest.raises(gcx.GcxError):
+        gcx.Graph.from_pairs(10, np.array([1], np.uint32), np.array([10], np.uint32))
+
+
+def test_determinism_bits(gmodels):
+    g = gcx.Graph.rmat(14, 16, seed=2)
+    ws = gcx.Workspace(g.N, g.M)
+    outs = []
+    for _ in range(3):
+        ws.order(g, gcx.GNN, gmodels[3])
+        p = ws.priorities(g.N)
+        ws.color(g)
+        outs.append((p.tobytes(), ws.colors(g.N).tobytes()))
+    assert outs[0] == outs[1] == outs[2]
+
+
+# ------------------------------------------------- the reference's C API
+
+def test_reference_api_parity(models):
+    c = load_case("rmat10")
+    V, E = c["V"], c["E"]
+    for h in ["FF", "LF", "LLF", "SL", "SLL"]:
+        p = gcx.ref.ordering(h, V, E)
+        assert p.view(np.uint64).tolist() == c[f"p_{h}"].view(np.uint64).tolist(), h
+        np.testing.assert_array_equal(gcx.ref.coloring_par(h, V, E), c[f"color_{h}"])
+        np.testing.assert_array_equal(gcx.ref.jones_plassmann_coloring(V, E, p), c[f"color_{h}"])
+        np.testing.assert_array_equal(gcx.ref.greedy_coloring(V, E, p), c[f"color_{h}"])
+    for l in (2, 3, 4):
+        param, md = models[l]
+        p = gcx.ref.cgnn_graph_sage_16_par(param, V, E, md)
+        assert p.view(np.uint64).tolist() == c[f"p_GNN{l}"].view(np.uint64).tolist()
+    col = c["color_GNN3"]
+    assert gcx.ref.validate_coloring_par(V, E, col) == int(c["ncolors_GNN3"])
+    bad = col.copy()
+    bad[E[0]] = bad[0]
+    assert gcx.ref.validate_coloring_par(V, E, bad) == 0
+
+
+def test_reference_csr_utilities():
+    c = load_case("mixed10")
+    N = int(c["N"])
+    # unsorted, with repeats and self loops: what mtx_to_graph would hand over
+    I = np.concatenate([c["I"], c["J"]])
+    J = np.concatenate([c["J"], c["I"]])
+    order = np.argsort(I, kind="stable")
+    rows, cols = I[order], J[order]
+    V = np.zeros(N + 1, np.uint32)
+    np.add.at(V, rows + 1, 1)
+    V = np.cumsum(V).astype(np.uint32)
+    # rows scrambled (descending) to give the sort something to do
+    E = np.concatenate([np.sort(cols[V[u]:V[u + 1]])[::-1] for u in range(N)]).astype(np.uint32)
+    Es = gcx.ref.graph_sort_edges_par(V, E)
+    for u in range(N):
+        assert Es[V[u]:V[u + 1]].tolist() == sorted(E[V[u]:V[u + 1]].tolist())
+    V2, E2 = gcx.ref.graph_make_simple(V, Es)
+    np.testing.assert_array_equal(V2, c["V"])
+    np.testing.assert_array_equal(E2, c["E"])
+    assert gcx.ref.graph_validate_par(V2, E2) == 1
+    assert gcx.ref.graph_validate_par(V, Es) == 0

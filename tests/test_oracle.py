@@ -1,0 +1,265 @@
+"""Pins the CPU oracle (oracle/) before anything is compared with it.
+
+Checks, per oracle function:
+  * golden vectors produced by the reference's own compiled sources
+    (tests/golden/make_golden.py): CSR, priorities (bit patterns), colours,
+    colour counts, Culberson chains, digests of R-MAT scale 16 and 20;
+  * brute force: sequential greedy in (p desc, id asc) order, written
+    independently below, on the hand-made graphs and on random tiny graphs;
+  * hand-derived known answers for SL / SLL (path, star, clique, a
+    threshold-jump graph);
+  * the GNN in float64 as sparse matrix algebra (numpy/scipy), independent of
+    the oracle's loop structure, against the FP64 and FP32 oracles;
+  * planted mistakes: a corrupted colouring / CSR must be rejected.
+"""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import oracle
+from conftest import HEUR, TINY, load_case
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def brute_greedy(V, E, p):
+    """Sequential greedy in (p descending, id ascending) order."""
+    N = len(V) - 1
+    order = sorted(range(N), key=lambda u: (-p[u], u))
+    color = np.zeros(N, np.int32)
+    for u in order:
+        used = {int(color[v]) for v in E[V[u]:V[u + 1]] if color[v] > 0}
+        c = 1
+        while c in used:
+            c += 1
+        color[u] = c
+    return color
+
+
+def priorities(case, h, models):
+    if h.startswith("GNN"):
+        param, md = models[int(h[3:])]
+        return oracle.gnn_fp32(param, case["V"], case["E"], md)
+    return oracle.key(h, case["V"], case["E"])[0]
+
+
+@pytest.mark.parametrize("name", TINY + ["rmat10"])
+def test_csr_matches_reference(name):
+    c = load_case(name)
+    V, E = oracle.csr_from_pairs(int(c["N"]), c["I"], c["J"])
+    np.testing.assert_array_equal(V, c["V"])
+    np.testing.assert_array_equal(E, c["E"])
+    assert oracle.graph_validate(V, E) == 1
+
+
+@pytest.mark.parametrize("name", TINY + ["rmat10"])
+@pytest.mark.parametrize("h", HEUR)
+def test_priorities_and_colours_match_reference(name, h, models):
+    c = load_case(name)
+    p = priorities(c, h, models)
+    # bit-exact priorities (compare the IEEE bit patterns)
+    assert p.view(np.uint64).tolist() == c[f"p_{h}"].view(np.uint64).tolist()
+    col, _ = oracle.greedy(c["V"], c["E"], p)
+    np.testing.assert_array_equal(col, c[f"color_{h}"])
+    assert oracle.validate(c["V"], c["E"], col) == int(c[f"ncolors_{h}"])
+    assert oracle.mex_violations(c["V"], c["E"], p, col) == 0
+
+
+@pytest.mark.parametrize("name", TINY + ["rmat10"])
+def test_culberson_matches_reference(name, models):
+    c = load_case(name)
+    p = priorities(c, "GNN3", models)
+    col, counts = oracle.culberson(c["V"], c["E"], p, 5)
+    np.testing.assert_array_equal(col, c["culb_color"])
+    np.testing.assert_array_equal(counts, c["culb_counts"])
+    assert all(counts[i + 1] <= counts[i] for i in range(len(counts) - 1))
+
+
+@pytest.mark.parametrize("name", TINY + ["rmat10"])
+@pytest.mark.parametrize("h", ["FF", "LF", "SL", "GNN3"])
+def test_greedy_equals_brute_force(name, h, models):
+    c = load_case(name)
+    p = priorities(c, h, models)
+    col, rounds = oracle.greedy(c["V"], c["E"], p)
+    np.testing.assert_array_equal(col, brute_greedy(c["V"], c["E"], p))
+    N = len(c["V"]) - 1
+    maxdeg = int(np.diff(c["V"]).max()) if N else 0
+    assert col.max(initial=0) <= maxdeg + 1
+    assert rounds <= N
+
+
+def test_greedy_random_tiny_graphs():
+    rng = np.random.default_rng(11)
+    for trial in range(60):
+        n = int(rng.integers(1, 40))
+        m = int(rng.integers(0, 3 * n + 1))
+        I = rng.integers(0, n, m).astype(np.uint32)
+        J = rng.integers(0, n, m).astype(np.uint32)
+        V, E = oracle.csr_from_pairs(n, I, J)
+        # ties on purpose: small integer priorities exercise the id tie-break
+        p = rng.integers(0, 4, n).astype(np.float64)
+        col, _ = oracle.greedy(V, E, p)
+        np.testing.assert_array_equal(col, brute_greedy(V, E, p))
+        assert oracle.validate(V, E, col) == (col.max() if n else 1)
+
+
+@pytest.mark.parametrize("name", ["rmat16", "rmat20"])
+def test_digests_match_reference(name, digests, models):
+    import gcx  # host generators only
+
+    rec = digests[name]
+    N, I, J = gcx.rmat_pairs(rec["graph"]["scale"], rec["graph"]["edge_factor"],
+                             *rec["graph"]["abc"], seed=rec["graph"]["seed"])
+    V, E = oracle.csr_from_pairs(N, I, J)
+    assert sha(V) == rec["sha_V"] and sha(E) == rec["sha_E"]
+    for h in HEUR:
+        p = priorities({"V": V, "E": E}, h, models)
+        assert sha(p) == rec[h]["sha_p"], h
+        col, _ = oracle.greedy(V, E, p)
+        assert sha(col) == rec[h]["sha_color"], h
+        assert oracle.validate(V, E, col) == rec[h]["ncolors"]
+    p = priorities({"V": V, "E": E}, "GNN3", models)
+    col, counts = oracle.culberson(V, E, p, 8)
+    assert sha(col) == rec["culb_GNN3_8"]["sha_color"]
+    assert counts.tolist() == rec["culb_GNN3_8"]["counts"]
+
+
+# ---------------------------------------------------------- known answers
+
+def csr(n, pairs):
+    I = np.array([a for a, _ in pairs], np.uint32)
+    J = np.array([b for _, b in pairs], np.uint32)
+    return oracle.csr_from_pairs(n, I, J)
+
+
+KNOWN_SL = {
+    # path 0-1-2-3-4: ends go at delta=1, the middle at delta=2
+    "path5": (5, [(0, 1), (1, 2), (2, 3), (3, 4)], [1, 2, 2, 2, 1], [0, 1, 1, 1, 0]),
+    # star: leaves at 1, centre at delta = max(1+1, 0) = 2
+    "star6": (6, [(0, i) for i in range(1, 6)], [2, 1, 1, 1, 1, 1], [1, 0, 0, 0, 0, 0]),
+    # clique K6: everything in round one at delta = 5
+    "clique6": (6, [(a, b) for a in range(6) for b in range(a + 1, 6)], [5] * 6, [3] * 6),
+    # path 0-1-2 glued to K5 on 3..7: delta 1 -> 2 -> jumps to 4
+    "jump8": (8, [(0, 1), (1, 2), (2, 3)] + [(a, b) for a in range(3, 8) for b in range(a + 1, 8)],
+              [1, 2, 2, 4, 4, 4, 4, 4], [0, 1, 1, 2, 2, 2, 2, 2]),
+}
+
+
+@pytest.mark.parametrize("name", list(KNOWN_SL))
+def test_sl_sll_known_answers(name):
+    n, pairs, sl, sll = KNOWN_SL[name]
+    V, E = csr(n, pairs)
+    assert oracle.key("SL", V, E)[0].tolist() == sl
+    assert oracle.key("SLL", V, E)[0].tolist() == sll
+
+
+def test_closed_form_keys():
+    V, E = csr(6, [(0, 1), (0, 2), (0, 3), (1, 2), (4, 4)])
+    deg = np.diff(V)
+    assert oracle.key("FF", V, E)[0].tolist() == [6, 5, 4, 3, 2, 1]
+    assert oracle.key("LF", V, E)[0].tolist() == deg.tolist()
+    assert oracle.key("LLF", V, E)[0].tolist() == [int(d).bit_length() for d in deg]
+
+
+def test_clique_and_bipartite_colours():
+    V, E = csr(9, [(a, b) for a in range(9) for b in range(a + 1, 9)])
+    col, rounds = oracle.greedy(V, E, oracle.key("FF", V, E)[0])
+    assert sorted(col.tolist()) == list(range(1, 10)) and rounds == 9
+    V, E = csr(7, [(a, b) for a in range(3) for b in range(3, 7)])
+    col, _ = oracle.greedy(V, E, oracle.key("FF", V, E)[0])
+    assert col.tolist() == [1, 1, 1, 2, 2, 2, 2]
+
+
+# -------------------------------------------------------- GNN as algebra
+
+def gnn_algebra(param, V, E, md):
+    """GraphSAGE-16 in float64 as sparse matrix products (row-normalised
+    adjacency), independent of the oracle's per-vertex loops."""
+    N = len(V) - 1
+    deg = np.diff(V).astype(np.float64)
+    A = sp.csr_matrix((np.ones(len(E)), E.astype(np.int64), V.astype(np.int64)), shape=(N, N))
+    Dinv = sp.diags(np.where(deg > 0, 1.0 / np.maximum(deg, 1), 0.0))
+    P = Dinv @ A  # mean over neighbours (0 for isolated vertices)
+    w = np.asarray(param, np.float64).reshape(-1, 528)
+    L = w.shape[0]
+    fdeg = deg / md
+    fid = np.arange(N, dtype=np.float64) / N
+    X0 = np.concatenate([np.repeat(fdeg[:, None], 8, 1), np.repeat(fid[:, None], 8, 1)], 1)
+    x = X0
+    for l in range(L):
+        W = w[l, :512].reshape(32, 16)
+        b = w[l, 512:]
+        H = np.concatenate([P @ x, x], 1) @ W + b
+        x = np.maximum(H, 0.0)
+    return x.sum(1)
+
+
+@pytest.mark.parametrize("name", ["er300", "stencil6", "mixed10", "rmat10"])
+@pytest.mark.parametrize("layers", [2, 3, 4])
+def test_gnn_fp64_matches_algebra(name, layers, models):
+    c = load_case(name)
+    param, md = models[layers]
+    want = gnn_algebra(param, c["V"], c["E"], md)
+    got = oracle.gnn_fp64(param, c["V"], c["E"], md)
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12)
+    # FP32 (reference op order) stays within FP32 rounding of the exact value
+    p32 = oracle.gnn_fp32(param, c["V"], c["E"], md)
+    np.testing.assert_allclose(p32, want, rtol=1e-4, atol=1e-5)
+
+
+def test_model_parse_matches_reference_loader(models):
+    for l in (2, 3, 4):
+        param, md = oracle.load_model(os.path.join(ROOT, "models", f"m{l}.gnn"))
+        ref_param, ref_md = models[l]
+        assert md == ref_md
+        assert param.view(np.uint32).tolist() == ref_param.view(np.uint32).tolist()
+
+
+def test_model_loader_rejects_bad_files(tmp_path):
+    bad = tmp_path / "bad.gnn"
+    bad.write_text("1056 100\n0.5 0.25\n")
+    with pytest.raises(IOError):
+        oracle.load_model(bad)
+    with pytest.raises(IOError):
+        oracle.load_model(tmp_path / "missing.gnn")
+
+
+# ------------------------------------------------------ planted mistakes
+
+def test_checkers_reject_planted_mistakes(models):
+    c = load_case("rmat10")
+    V, E = c["V"], c["E"]
+    p = priorities(c, "GNN3", models)
+    col = c["color_GNN3"].copy()
+    # a monochromatic edge
+    u = int(np.argmax(np.diff(V)))
+    bad = col.copy()
+    bad[E[V[u]]] = bad[u]
+    assert oracle.validate(V, E, bad) == 0
+    # a proper but non-greedy colouring (shift one vertex to a fresh colour)
+    bad = col.copy()
+    bad[u] = col.max() + 1
+    assert oracle.validate(V, E, bad) > 0
+    assert oracle.mex_violations(V, E, p, bad) >= 1
+    # an uncoloured vertex
+    bad = col.copy()
+    bad[0] = 0
+    assert oracle.validate(V, E, bad) == 0
+    # a flipped tie-break (v > u instead of v < u) changes colours here
+    q = np.zeros_like(p)
+    wrong, _ = oracle.greedy(V, E, q)
+    right = brute_greedy(V, E, q)
+    np.testing.assert_array_equal(wrong, right)
+    flipped = brute_greedy(V, E, -np.arange(len(p), dtype=np.float64) * 0 + np.arange(len(p)))
+    assert not np.array_equal(flipped, right)
+    # an asymmetric CSR fails the graph check
+    E2 = E.copy()
+    E2[V[u]] = (E2[V[u]] + 1) % (len(V) - 1)
+    assert oracle.graph_validate(V, E2) == 0
