@@ -1,14 +1,98 @@
 /*
- * bf16.c -- BF16 GraphSAGE ordering (rows stored as bf16, FP32 neighbour
- * aggregation, layer transforms on MFMA with FP32 accumulation).
+ * bf16.c -- BF16 GraphSAGE ordering: weight packing for the MFMA operand
+ * layout and the layer sequence (kernels in gnn_bf16.hip).
  */
+#include <hip/hip_runtime_api.h>
+#include <stdlib.h>
+#include <string.h>
+
 #include "internal.h"
+
+#define LAYER_PARAMS 528
+
+/* Round to nearest even (Inf/NaN kept). */
+static uint16_t to_bf16(float f)
+{
+    uint32_t b;
+    memcpy(&b, &f, 4);
+    if ((b & 0x7F800000u) == 0x7F800000u)
+        return (uint16_t)((b >> 16) | ((b & 0xFFFFu) ? 0x40u : 0u));
+    b += 0x7FFFu + ((b >> 16) & 1u);
+    return (uint16_t)(b >> 16);
+}
+
+/* Per layer: Wb = W[k][j] in bf16 (512 values, k-major), and the two MFMA
+ * B operands in lane order: pack[m*256 + 4*lane + e] = W[16m + 4*(lane/16) + e][lane%16]
+ * (v_mfma_f32_16x16x16_bf16: B[k][j] lives in lane j + 16*(k/4), element k%4). */
+int gcx__model_pack_bf16(gcx_model *m)
+{
+    const int L = m->layers;
+    uint16_t *wb = malloc(sizeof(uint16_t) * 512 * (size_t)L);
+    uint16_t *pk = malloc(sizeof(uint16_t) * 512 * (size_t)L);
+    if (!wb || !pk) {
+        free(wb);
+        free(pk);
+        return GCX_ERR_NOMEM;
+    }
+    for (int l = 0; l < L; l++) {
+        const float *W = m->hparam + (size_t)l * LAYER_PARAMS;
+        for (int k = 0; k < 512; k++)
+            wb[512 * l + k] = to_bf16(W[k]);
+        for (int mm = 0; mm < 2; mm++)
+            for (int lane = 0; lane < 64; lane++)
+                for (int e = 0; e < 4; e++) {
+                    const int k = 16 * mm + 4 * (lane / 16) + e, j = lane % 16;
+                    pk[512 * l + 256 * mm + 4 * lane + e] = wb[512 * l + 16 * k + j];
+                }
+    }
+    int rc = GCX_OK;
+    if (hipMalloc((void **)&m->dWb, sizeof(uint16_t) * 512 * (size_t)L) != hipSuccess ||
+        hipMalloc((void **)&m->dWpack, sizeof(uint16_t) * 512 * (size_t)L) != hipSuccess ||
+        hipMemcpy(m->dWb, wb, sizeof(uint16_t) * 512 * (size_t)L, hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        hipMemcpy(m->dWpack, pk, sizeof(uint16_t) * 512 * (size_t)L, hipMemcpyHostToDevice) !=
+            hipSuccess)
+        rc = GCX_ERR_HIP;
+    free(wb);
+    free(pk);
+    return rc;
+}
 
 int gcx__order_gnn_bf16(gcx_workspace *ws, const gcx_graph *g, const gcx_model *m, void *stream)
 {
-    (void)ws;
-    (void)g;
-    (void)m;
-    (void)stream;
-    return GCX_ERR_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    const int L = m->layers;
+    uint16_t *bufs[2] = {(uint16_t *)ws->x, (uint16_t *)ws->y};
+    const uint16_t *in = NULL;
+    for (int l = 0; l < L; l++) {
+        const int kind = l == 0 ? 0 : (l == L - 1 ? 2 : 1);
+        const float *b = m->dparam + (size_t)l * LAYER_PARAMS + 512;
+        uint16_t *out = bufs[l & 1];
+        void *tok;
+        int rc = gcx__prof_begin(kind == 0 ? GCX_ST_GNN_IN : kind == 1 ? GCX_ST_GNN_HID : GCX_ST_GNN_OUT,
+                                 s, &tok);
+        if (rc != GCX_OK)
+            return rc;
+        hipStream_t hs = s;
+        if (g->nhub) {
+            if (hipEventRecord((hipEvent_t)ws->ev_fork, s) != hipSuccess ||
+                hipStreamWaitEvent((hipStream_t)ws->aux, (hipEvent_t)ws->ev_fork, 0) != hipSuccess)
+                return GCX_ERR_HIP;
+            hs = (hipStream_t)ws->aux;
+        }
+        rc = gcxk_gnn_bf16_layer(kind, g->N, g->V, g->E, m->dWpack + 512 * (size_t)l,
+                                 m->dWb + 512 * (size_t)l, b, m->md, in, out, (uint32_t *)ws->key,
+                                 g->hubs, g->nhub, s, hs);
+        if (rc != GCX_OK)
+            return rc;
+        if (g->nhub) {
+            if (hipEventRecord((hipEvent_t)ws->ev_join, hs) != hipSuccess ||
+                hipStreamWaitEvent(s, (hipEvent_t)ws->ev_join, 0) != hipSuccess)
+                return GCX_ERR_HIP;
+        }
+        if ((rc = gcx__prof_end(tok, s)) != GCX_OK)
+            return rc;
+        in = out;
+    }
+    return GCX_OK;
 }

@@ -260,12 +260,70 @@ uint64_t gcx_graph_footprint(uint32_t N, uint64_t M)
     return 4ull * ((uint64_t)N + 1) + 4ull * M + 4ull * N;
 }
 
+static int cmp_u32(const void *a, const void *b)
+{
+    const uint32_t x = *(const uint32_t *)a, y = *(const uint32_t *)b;
+    return (x > y) - (x < y);
+}
+
+/* Degree statistics: max degree, hub list (ascending ids) and the hubs'
+ * 4096-edge chunk table used by the JP orientation pass. */
+static int graph_stats(gcx_graph *g, hipStream_t s, uint32_t *tmp)
+{
+    TRY(gcxk_graph_stats(g->N, g->V, g->hubs, tmp, &g->nhub, &g->maxdeg, s));
+    if (g->hchunk) {
+        (void)hipFree(g->hchunk);
+        g->hchunk = NULL;
+    }
+    g->nhchunk = 0;
+    if (g->nhub == 0)
+        return GCX_OK;
+    const uint32_t n = g->nhub;
+    uint32_t *ids = malloc(sizeof(uint32_t) * n), *deg = malloc(sizeof(uint32_t) * n);
+    uint32_t *chunk = malloc(sizeof(uint32_t) * (n + 1ull));
+    uint32_t *ddeg = NULL;
+    int rc = GCX_ERR_HIP;
+    if (!ids || !deg || !chunk) {
+        rc = GCX_ERR_NOMEM;
+        goto done;
+    }
+    if (hipMemcpyAsync(ids, g->hubs, 4ull * n, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        goto done;
+    qsort(ids, n, sizeof(uint32_t), cmp_u32);
+    if (hipMemcpyAsync(g->hubs, ids, 4ull * n, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMalloc((void **)&ddeg, 4ull * n) != hipSuccess)
+        goto done;
+    if ((rc = gcxk_gather_degrees(g->V, g->hubs, n, ddeg, s)) != GCX_OK)
+        goto done;
+    rc = GCX_ERR_HIP;
+    if (hipMemcpyAsync(deg, ddeg, 4ull * n, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        goto done;
+    chunk[0] = 0;
+    for (uint32_t h = 0; h < n; h++)
+        chunk[h + 1] = chunk[h] + (deg[h] + GCXK_HUB_CHUNK_EDGES - 1) / GCXK_HUB_CHUNK_EDGES;
+    if (hipMalloc((void **)&g->hchunk, 4ull * (n + 1)) != hipSuccess ||
+        hipMemcpyAsync(g->hchunk, chunk, 4ull * (n + 1), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        goto done;
+    g->nhchunk = chunk[n];
+    rc = GCX_OK;
+done:
+    if (ddeg)
+        (void)hipFree(ddeg);
+    free(ids);
+    free(deg);
+    free(chunk);
+    return rc;
+}
+
 static int graph_finish(gcx_graph *g, void *stream)
 {
     uint32_t *tmp = NULL;
     HIPC(hipMalloc((void **)&g->hubs, 4ull * (g->N ? g->N : 1)));
     HIPC(hipMalloc((void **)&tmp, 16));
-    int rc = gcxk_graph_stats(g->N, g->V, g->hubs, tmp, &g->nhub, &g->maxdeg, stream);
+    int rc = graph_stats(g, (hipStream_t)stream, tmp);
     (void)hipFree(tmp);
     return rc;
 }
@@ -280,6 +338,8 @@ int gcx_graph_free(gcx_graph *g)
         (void)hipFree(g->E);
     if (g->hubs)
         (void)hipFree(g->hubs);
+    if (g->hchunk)
+        (void)hipFree(g->hchunk);
     free(g);
     return GCX_OK;
 }
@@ -322,7 +382,7 @@ int gcx__graph_fill(gcx_graph *g, uint32_t N, const uint32_t *V, const uint32_t 
     if (rc == GCX_OK && !ok)
         rc = GCX_ERR_ARG; /* indices out of range: refuse before any kernel reads them */
     if (rc == GCX_OK)
-        rc = gcxk_graph_stats(g->N, g->V, g->hubs, tmp, &g->nhub, &g->maxdeg, s);
+        rc = graph_stats(g, s, tmp);
     (void)hipFree(tmp);
     return rc;
 }
@@ -504,6 +564,11 @@ int gcx_model_from_params(int params, const float *param, int md, gcx_model **ou
         gcx_model_free(m);
         return GCX_ERR_HIP;
     }
+    int rc = gcx__model_pack_bf16(m);
+    if (rc != GCX_OK) {
+        gcx_model_free(m);
+        return rc;
+    }
     *out = m;
     return GCX_OK;
 }
@@ -545,6 +610,10 @@ int gcx_model_free(gcx_model *m)
         return GCX_OK;
     if (m->dparam)
         (void)hipFree(m->dparam);
+    if (m->dWb)
+        (void)hipFree(m->dWb);
+    if (m->dWpack)
+        (void)hipFree(m->dWpack);
     free(m->hparam);
     free(m);
     return GCX_OK;
@@ -556,17 +625,22 @@ int gcx_model_layers(const gcx_model *m) { return m ? m->layers : 0; }
 
 uint64_t gcx_workspace_footprint(uint32_t N, uint64_t M)
 {
-    (void)M;
-    /* key 8 + count 4 + color 4 + 4 lists x 4 + x,y 2 x 64 bytes per vertex */
-    return 160ull * ((uint64_t)N + 1) + 2 * sizeof(gcxk_ctl);
+    /* per vertex: key 8, count 4, color 4, npred 4, mask64 8, 4 lists x 4,
+     * hpos 8, rows x,y 2 x 64; per entry: E_split 4; plus the overflow colour
+     * words and two item queues */
+    const uint64_t n = (uint64_t)N + 1;
+    return 184ull * n + 4ull * M + 4ull * gcxk_jp_ovf_words(N, M) +
+           2ull * sizeof(gcxk_item) * gcxk_jp_item_capacity(N, M) + 2 * sizeof(gcxk_ctl);
 }
 
 int gcx_workspace_free(gcx_workspace *ws)
 {
     if (!ws)
         return GCX_OK;
-    void *bufs[] = {ws->key, ws->count, ws->color, ws->q[0], ws->q[1],
-                    ws->h[0], ws->h[1], ws->x, ws->y, ws->ctl};
+    void *bufs[] = {ws->key,    ws->count,  ws->color,    ws->q[0],     ws->q[1],
+                    ws->h[0],   ws->h[1],   ws->x,        ws->y,        ws->ctl,
+                    ws->npred,  ws->mask64, ws->ovf,      ws->esplit,   ws->items[0],
+                    ws->items[1], ws->hpos};
     for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); i++)
         if (bufs[i])
             (void)hipFree(bufs[i]);
@@ -604,6 +678,15 @@ int gcx_workspace_create(uint32_t N, uint64_t M, gcx_workspace **out)
              hipMalloc((void **)&ws->x, 64 * n) == hipSuccess &&
              hipMalloc((void **)&ws->y, 64 * n) == hipSuccess &&
              hipMalloc((void **)&ws->ctl, 2 * sizeof(gcxk_ctl)) == hipSuccess &&
+             hipMalloc((void **)&ws->npred, 4 * n) == hipSuccess &&
+             hipMalloc((void **)&ws->mask64, 8 * n) == hipSuccess &&
+             hipMalloc((void **)&ws->hpos, 8 * n) == hipSuccess &&
+             hipMalloc((void **)&ws->ovf, 4 * gcxk_jp_ovf_words(N, M)) == hipSuccess &&
+             hipMalloc((void **)&ws->esplit, 4 * (M ? M : 1)) == hipSuccess &&
+             hipMalloc((void **)&ws->items[0], sizeof(gcxk_item) * gcxk_jp_item_capacity(N, M)) ==
+                 hipSuccess &&
+             hipMalloc((void **)&ws->items[1], sizeof(gcxk_item) * gcxk_jp_item_capacity(N, M)) ==
+                 hipSuccess &&
              hipHostMalloc((void **)&ws->hctl, 4 * sizeof(gcxk_ctl), hipHostMallocDefault) ==
                  hipSuccess;
     if (ok) {
@@ -635,7 +718,7 @@ int gcx_workspace_create(uint32_t N, uint64_t M, gcx_workspace **out)
 
 static int ws_fits(const gcx_workspace *ws, const gcx_graph *g)
 {
-    return ws && g && g->N <= ws->capN;
+    return ws && g && g->N <= ws->capN && g->M <= ws->capM;
 }
 
 /* ----------------------------------------------------------- round polling */
@@ -818,8 +901,8 @@ static int jp_round(void *p, uint32_t r, hipStream_t s)
 {
     jp_ctx *c = (jp_ctx *)p;
     gcx_workspace *ws = c->ws;
-    return gcxk_jp_round(c->g->N, c->g->V, c->g->E, ws->count, ws->color, ws->q, ws->h, ws->ctl, r,
-                         c->grid, s);
+    return gcxk_jp_round(c->g->V, ws->npred, ws->esplit, ws->count, ws->color, ws->mask64, ws->ovf,
+                         ws->items, ws->ctl, r, c->grid, s);
 }
 
 int gcx_color(gcx_workspace *ws, const gcx_graph *g, void *stream)
@@ -829,16 +912,16 @@ int gcx_color(gcx_workspace *ws, const gcx_graph *g, void *stream)
         return GCX_ERR_ARG;
     hipStream_t s = (hipStream_t)gcx__pick(stream);
     void *tok = NULL;
-    HIPC(hipMemsetAsync(ws->ctl, 0, sizeof(gcxk_ctl), s));
     TRY(gcx__prof_begin(GCX_ST_JP_INIT, s, &tok));
-    TRY(gcxk_jp_init(g->N, g->V, g->E, ws->key, ws->key_bits, ws->count, ws->color, ws->q[0],
-                     ws->h[0], g->hubs, g->nhub, ws->ctl, s));
+    TRY(gcxk_jp_clear(g->N, g->M, ws->color, ws->mask64, ws->ovf, g->hubs, g->nhub, ws->count,
+                      ws->npred, ws->hpos, ws->ctl, s));
+    TRY(gcxk_jp_init(g->N, g->V, g->E, ws->key, ws->key_bits, ws->count, ws->npred, ws->esplit,
+                     ws->items[0], g->hubs, g->nhub, g->hchunk, g->nhchunk, ws->hpos, ws->ctl, s));
     TRY(gcx__prof_end(tok, s));
     TRY(gcx__prof_begin(GCX_ST_JP_ROUNDS, s, &tok));
     jp_ctx c = {ws, g, g_round_grid};
     TRY(drive_rounds(ws, s, jp_round, &c, offsetof(gcxk_ctl, done), g->N + 2 * ROUND_BATCH + 2));
     TRY(gcx__prof_end(tok, s));
-    ws->last_rounds = UINT32_MAX; /* read lazily from the device */
     return GCX_OK;
 }
 

@@ -16,12 +16,16 @@ struct gcx_graph {
     uint32_t *hubs;   /* device, nhub entries (degree >= GCXK_HUB_DEG) */
     uint32_t nhub;
     uint32_t maxdeg;
+    uint32_t *hchunk; /* device, nhub+1: prefix of 4096-edge chunks per hub */
+    uint32_t nhchunk;
 };
 
 struct gcx_model {
     int params, md, layers;
     float *dparam; /* device copy of the weights */
     float *hparam; /* host copy */
+    uint16_t *dWb;    /* bf16 weights, 512 per layer, W[k][j] k-major */
+    uint16_t *dWpack; /* bf16 MFMA B operands, 512 per layer, lane order */
 };
 
 struct gcx_workspace {
@@ -32,8 +36,14 @@ struct gcx_workspace {
     int key_kind;     /* 32-bit keys: 0 unsigned integer, 1 float order */
     uint32_t *count;  /* pending predecessors (JP); degrees (SL) */
     int *color;
-    uint32_t *q[2];   /* frontier / alive lists */
-    uint32_t *h[2];   /* hub frontier lists / SL removed list */
+    uint32_t *q[2];   /* SL alive lists */
+    uint32_t *h[2];   /* SL removed list */
+    uint32_t *npred;  /* JP: predecessors per vertex */
+    uint64_t *mask64; /* JP: colours 1..64 used by coloured predecessors */
+    uint32_t *ovf;    /* JP: colour-set overflow words (colours > 64) */
+    uint32_t *esplit; /* JP: rows split into predecessors | successors (capM) */
+    gcxk_item *items[2]; /* JP: work-item queues */
+    uint32_t *hpos;   /* JP: per-hub E_split positions (2 per vertex slot) */
     float *x, *y;     /* GNN rows, 16 floats (or 16 bf16) per vertex */
     gcxk_ctl *ctl;    /* device control block */
     gcxk_ctl *hctl;   /* pinned host mirror */
@@ -52,6 +62,7 @@ int gcx__graph_fill(gcx_graph *g, uint32_t N, const uint32_t *V, const uint32_t 
 /* FF keys (N - id) for a workspace without a graph (first_fit.c:8-19). */
 int gcx__order_ff_only(gcx_workspace *ws, uint32_t N, void *stream);
 /* bf16.c */
+int gcx__model_pack_bf16(gcx_model *m);
 int gcx__order_gnn_bf16(gcx_workspace *ws, const gcx_graph *g, const gcx_model *m, void *stream);
 int gcx__ensure_init(void);
 void *gcx__default_stream(void);

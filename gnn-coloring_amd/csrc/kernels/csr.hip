@@ -221,6 +221,15 @@ __global__ __launch_bounds__(kBlock) void compact_entries(uint64_t M, const uint
         out[pos[i]] = E[i];
 }
 
+__global__ __launch_bounds__(kBlock) void gather_degrees(const uint32_t *__restrict__ V,
+                                                         const uint32_t *__restrict__ list, uint32_t n,
+                                                         uint32_t *__restrict__ out)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n)
+        out[i] = V[list[i] + 1] - V[list[i]];
+}
+
 int bit_length(uint32_t x)
 {
     int b = 0;
@@ -435,5 +444,15 @@ extern "C" int gcxk_make_simple(uint32_t N, uint32_t *dV, const uint32_t *dE, ui
     GCXK_CHECK(hipMemcpyAsync(&m32, dV + N, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     GCXK_CHECK(hipStreamSynchronize(s));
     *M_out = m32;
+    return GCX_OK;
+}
+
+extern "C" int gcxk_gather_degrees(const uint32_t *dV, const uint32_t *list, uint32_t n,
+                                   uint32_t *out, void *stream)
+{
+    if (n == 0)
+        return GCX_OK;
+    gather_degrees<<<div_up(n, kBlock), kBlock, 0, as_stream(stream)>>>(dV, list, n, out);
+    GCXK_LAUNCH_CHECK();
     return GCX_OK;
 }

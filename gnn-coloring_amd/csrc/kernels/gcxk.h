@@ -97,24 +97,54 @@ int gcxk_gnn_fp32_layer(int kind, uint32_t N, const uint32_t *dV, const uint32_t
                         const float *W, const float *b, int md, const float *x, float *out,
                         uint32_t *key, const uint32_t *hubs, uint32_t nhub, void *stream,
                         void *hub_stream);
-/* BF16 rows (16 x bf16 = 32 B), FP32 aggregation, MFMA layer transform. */
-int gcxk_gnn_bf16_input(uint32_t N, const uint32_t *dV, const uint32_t *dE, const float *W,
-                        const float *b, int md, uint16_t *out, void *stream);
-int gcxk_gnn_bf16_hidden(uint32_t N, const uint32_t *dV, const uint32_t *dE, const float *W,
-                         const float *b, const uint16_t *x, uint16_t *out, void *stream);
-int gcxk_gnn_bf16_output(uint32_t N, const uint32_t *dV, const uint32_t *dE, const float *W,
-                         const float *b, const uint16_t *x, uint32_t *key, void *stream);
+/* BF16 rows (16 x bf16 = 32 B), FP32 aggregation, MFMA layer transform.
+ * Wpack: the layer's two MFMA B operands in lane order (512 bf16); Wb: the
+ * layer's weights W[k][j] in bf16 (512); b: FP32 biases (16). */
+int gcxk_gnn_bf16_layer(int kind, uint32_t N, const uint32_t *dV, const uint32_t *dE,
+                        const uint16_t *Wpack, const uint16_t *Wb, const float *b, int md,
+                        const uint16_t *x, uint16_t *out, uint32_t *key, const uint32_t *hubs,
+                        uint32_t nhub, void *stream, void *hub_stream);
 
 /* ---- Jones-Plassmann (jp.hip) ---- */
-/* key_bits = 32 or 64. Counts predecessors, zeroes colours, queues roots
- * into slot 0 (small queue q0 / hub queue h0). ctl must be zeroed. */
+/* Work item: (vertex, chunk of 64 successor edges). Stored as two uint32. */
+typedef struct gcxk_item {
+    uint32_t u, k;
+} gcxk_item;
+
+/* Overflow colour-set words for N vertices / M entries (colours > 64). */
+static inline uint64_t gcxk_jp_ovf_words(uint32_t N, uint64_t M)
+{
+    return M / 32u + (uint64_t)N + 2u;
+}
+/* Item queue capacity: every vertex contributes max(1, ceil(nsucc / 64)). */
+static inline uint64_t gcxk_jp_item_capacity(uint32_t N, uint64_t M)
+{
+    return (uint64_t)N + M / 64u + 2u;
+}
+/* Hub chunks for the orientation pass: 4096 edges each. */
+#define GCXK_HUB_CHUNK_EDGES 4096u
+
+/* Zero colours, colour sets (mask64: N words, ovf: gcxk_jp_ovf_words), the
+ * hubs' counters and per-hub positions (hpos: 2 per hub), and the control
+ * block. A plain kernel, so the sequence can be captured in a graph. */
+int gcxk_jp_clear(uint32_t N, uint64_t M, int *color, uint64_t *mask64, uint32_t *ovf,
+                  const uint32_t *hubs, uint32_t nhub, uint32_t *count, uint32_t *npred,
+                  uint32_t *hpos, gcxk_ctl *ctl, void *stream);
+/* Orientation: key_bits = 32 or 64. E_split rows = predecessors then
+ * successors; count = npred = predecessor counts; roots queued as items in
+ * slot 0. hchunk: per-hub prefix of 4096-edge chunks (nhub + 1 entries),
+ * nhchunk = total chunks. */
 int gcxk_jp_init(uint32_t N, const uint32_t *dV, const uint32_t *dE, const void *key,
-                 int key_bits, uint32_t *count, int *color, uint32_t *q0, uint32_t *h0,
-                 const uint32_t *hubs, uint32_t nhub, gcxk_ctl *ctl, void *stream);
-/* One JP round r: colours frontier slot r, queues the next frontier. */
-int gcxk_jp_round(uint32_t N, const uint32_t *dV, const uint32_t *dE, uint32_t *count,
-                  int *color, uint32_t *const q[2], uint32_t *const h[2], gcxk_ctl *ctl,
-                  uint32_t r, uint32_t grid, void *stream);
+                 int key_bits, uint32_t *count, uint32_t *npred, uint32_t *esplit,
+                 gcxk_item *q0, const uint32_t *hubs, uint32_t nhub, const uint32_t *hchunk,
+                 uint32_t nhchunk, uint32_t *hpos, gcxk_ctl *ctl, void *stream);
+/* One JP round r: colours the items of slot r, queues the next round's. */
+int gcxk_jp_round(const uint32_t *dV, const uint32_t *npred, const uint32_t *esplit,
+                  uint32_t *count, int *color, uint64_t *mask64, uint32_t *ovf,
+                  gcxk_item *const q[2], gcxk_ctl *ctl, uint32_t r, uint32_t grid, void *stream);
+/* Degrees of the listed vertices (for the hub chunk table). */
+int gcxk_gather_degrees(const uint32_t *dV, const uint32_t *list, uint32_t n, uint32_t *out,
+                        void *stream);
 /* Proper-colouring check: ctl->bad, ctl->maxcolor (ctl fields zeroed by caller). */
 int gcxk_validate(uint32_t N, const uint32_t *dV, const uint32_t *dE, const int *color,
                   const uint32_t *hubs, uint32_t nhub, gcxk_ctl *ctl, void *stream);

@@ -180,6 +180,58 @@ def culberson(V, E, p, K):
     return color[:N], counts
 
 
+def bf16_round(x):
+    """Round float32 values to bfloat16 (nearest even), returned as float32."""
+    b = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    b = (b + 0x7FFF + ((b >> 16) & 1)) & 0xFFFF0000
+    return b.astype(np.uint32).view(np.float32)
+
+
+def gnn_bf16_emulated(param, V, E, md=100, rounding=True):
+    """The BF16 GraphSAGE contract evaluated on the CPU: operands of each
+    dense step ([mean | self] and W) rounded to bf16, products accumulated
+    exactly (float64 here), FP32 bias, ReLU, next-layer rows rounded to bf16,
+    output = sum of the 16 features. The reference has no BF16 path, so this
+    is pinned by `rounding=False`, which must equal the FP64 network."""
+    import scipy.sparse as sp
+
+    if rounding:
+        def rnd(a):  # float32 value rounded to bf16, as float64
+            return bf16_round(np.asarray(a, np.float32)).astype(np.float64)
+
+        def f32(a):
+            return np.asarray(a, np.float32).astype(np.float64)
+    else:
+        def rnd(a):
+            return np.asarray(a, np.float64)
+        f32 = rnd
+    V = np.asarray(V, np.int64)
+    E = np.asarray(E, np.int64)
+    N = len(V) - 1
+    deg = np.diff(V)
+    A = sp.csr_matrix((np.ones(len(E)), E, V), shape=(N, N))
+    inv = np.where(deg > 0, 1.0 / np.maximum(deg, 1), 0.0)
+    w = np.asarray(param, np.float32).reshape(-1, 528)
+    fdeg = f32(deg.astype(np.float64) / md)
+    fidv = f32(np.arange(N, dtype=np.float64) / N)
+    x = None
+    for l in range(w.shape[0]):
+        W = rnd(w[l, :512]).reshape(32, 16)
+        b = w[l, 512:].astype(np.float64)
+        if l == 0:
+            sd = rnd((A @ fdeg) * inv)
+            si = rnd((A @ fidv) * inv)
+            Aop = np.stack([sd] * 8 + [si] * 8 + [rnd(fdeg)] * 8 + [rnd(fidv)] * 8, 1)
+        else:
+            Aop = np.concatenate([rnd((A @ x) * inv[:, None]), x], 1)
+        H = np.maximum(Aop @ W + b, 0.0)
+        if l < w.shape[0] - 1:
+            x = rnd(H)
+        else:
+            return H.sum(1)
+    return None
+
+
 # ------------------------------------------------------------------ reference
 
 class Ref:

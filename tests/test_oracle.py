@@ -263,3 +263,18 @@ def test_checkers_reject_planted_mistakes(models):
     E2 = E.copy()
     E2[V[u]] = (E2[V[u]] + 1) % (len(V) - 1)
     assert oracle.graph_validate(V, E2) == 0
+
+
+@pytest.mark.parametrize("name", ["er300", "rmat10", "mixed10"])
+@pytest.mark.parametrize("layers", [2, 3, 4])
+def test_bf16_emulation_structure(name, layers, models):
+    """With bf16 rounding switched off the BF16 contract is the FP64 network;
+    with it on, priorities stay within bf16 error of FP64."""
+    c = load_case(name)
+    param, md = models[layers]
+    exact = oracle.gnn_fp64(param, c["V"], c["E"], md)
+    plain = oracle.gnn_bf16_emulated(param, c["V"], c["E"], md, rounding=False)
+    np.testing.assert_allclose(plain, exact, rtol=1e-12, atol=1e-12)
+    rounded = oracle.gnn_bf16_emulated(param, c["V"], c["E"], md)
+    rel = np.abs(rounded - exact) / np.maximum(np.abs(exact), 1e-3)
+    assert rel.mean() < 2e-2 and not np.array_equal(rounded, exact)
