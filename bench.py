@@ -70,6 +70,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--scale", type=int, default=22)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--schedule", choices=["auto", "rounds", "ordered"], default="auto")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is "
@@ -87,13 +88,30 @@ def main():
     model = gcx.Model.layers_file(3)
     ws = gcx.Workspace(N, M)
     stream = gcx.Stream()
+    # streaming-copy reference: 1 GiB read + 1 GiB written by a plain kernel
+    copy_ms = min(gcx.calibration_copy(1 << 30) for _ in range(3))
+    copy_gbs = 2 * (1 << 30) / (copy_ms * 1e-3) / 1e9
 
     def step():
         ws.order(g, gcx.GNN, model, gcx.FP32, stream)
         ws.color(g, stream)
 
-    for _ in range(args.warmup):
-        step()
+    # schedule choice: time each JP schedule on warm-up steps, keep the faster
+    sched_names = {"rounds": (gcx.SCHED_ROUNDS, 0), "ordered": (gcx.SCHED_ORDERED, 0)}
+    cands = list(sched_names) if args.schedule == "auto" else [args.schedule]
+    sched_ms = {}
+    depth = None
+    for name in cands:
+        ws.set_schedule(*sched_names[name])
+        for _ in range(max(args.warmup, 1)):
+            t0 = time.perf_counter()
+            step()
+            stream.sync()
+            sched_ms[name] = min(sched_ms.get(name, 1e30), 1000 * (time.perf_counter() - t0))
+        if name == "rounds":
+            depth = ws.rounds()
+    best = min(sched_ms, key=sched_ms.get)
+    ws.set_schedule(*sched_names[best])
     stream.sync()
     gcx.profile_enable(True)
     gcx.profile_read()  # clears
@@ -110,7 +128,7 @@ def main():
     prof = gcx.profile_read()
     gcx.profile_enable(False)
     launches = ws.launches()
-    rounds = ws.rounds()
+    fell_back = ws.fell_back()
 
     ms_per_step = 1000.0 * wall / args.steps
     value = M / (wall / args.steps)
@@ -163,14 +181,18 @@ def main():
         "config": {"workload": f"rmat{args.scale}_gnn3_jp_fp32",
                    "graph": f"R-MAT scale {args.scale}, edge factor 16, a/b/c/d=.57/.19/.19/.05, "
                             "seed 1, symmetrised, simple, int32 CSR",
-                   "N": N, "M": M, "heuristic": "GNN3", "precision": "fp32 (bit-exact)"},
+                   "N": N, "M": M, "heuristic": "GNN3", "precision": "fp32 (bit-exact)",
+                   "jp_schedule": best},
+        "schedule_warmup_ms": sched_ms,
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "avg_launch_ms": avg_ms, "launches": n_launch,
                      "algorithmic_bytes_per_launch": per_launch},
+        "copy_kernel_gbs": copy_gbs,
         "colors": colors,
-        "jp_rounds": rounds,
-        "jp_round_launches": launches,
+        "jp_dag_depth": depth,
+        "jp_launches_per_step": launches,
+        "jp_ordered_fell_back": fell_back,
         "stage_ms_per_step": {k: v / args.steps for k, v in stage_ms.items()},
         "event_ms_per_step": ev_ms / args.steps,
         "bit_exact_vs_reference": golden,

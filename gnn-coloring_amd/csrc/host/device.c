@@ -9,6 +9,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include "gen.h"
 #include "internal.h"
 
 #define HIPC(expr)                                                              \
@@ -184,6 +185,36 @@ static void pool_put(hipEvent_t e)
         g_pool_cap = cap;
     }
     g_pool[g_pool_n++] = e;
+}
+
+int gcx_calibration_copy(uint64_t bytes, float *ms)
+{
+    TRY(gcx__ensure_init());
+    bytes &= ~(uint64_t)15;
+    void *a = NULL, *b = NULL;
+    hipEvent_t e0 = NULL, e1 = NULL;
+    int rc = GCX_ERR_HIP;
+    if (hipMalloc(&a, bytes ? bytes : 16) != hipSuccess || hipMalloc(&b, bytes ? bytes : 16) != hipSuccess ||
+        hipMemsetAsync(a, 1, bytes, g_stream) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess || hipEventRecord(e0, g_stream) != hipSuccess)
+        goto done;
+    if ((rc = gcxk_copy_bytes(bytes, a, b, g_stream)) != GCX_OK)
+        goto done;
+    rc = GCX_ERR_HIP;
+    if (hipEventRecord(e1, g_stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(ms, e0, e1) != hipSuccess)
+        goto done;
+    rc = GCX_OK;
+done:
+    if (e0)
+        (void)hipEventDestroy(e0);
+    if (e1)
+        (void)hipEventDestroy(e1);
+    if (a)
+        (void)hipFree(a);
+    if (b)
+        (void)hipFree(b);
+    return rc;
 }
 
 int gcx_profile_enable(int on)
@@ -507,6 +538,26 @@ done:
     return rc;
 }
 
+int gcx_graph_rgg(uint32_t n, double radius, uint64_t seed, void *stream, gcx_graph **out)
+{
+    TRY(gcx__ensure_init());
+    *out = NULL;
+    if (n == 0 || !(radius > 0.0))
+        return GCX_ERR_ARG;
+    hipStream_t s = (hipStream_t)gcx__pick(stream);
+    uint32_t *dI = NULL, *dJ = NULL;
+    uint64_t L = 0;
+    int rc = gcxk_gen_rgg(n, gcx_rgg_cells(radius), gcx_rgg_r2(radius), seed, &dI, &dJ, &L, s);
+    if (rc == GCX_OK)
+        rc = graph_from_device_pairs(n, L, dI, dJ, s, out);
+    (void)hipStreamSynchronize(s);
+    if (dI)
+        (void)hipFree(dI);
+    if (dJ)
+        (void)hipFree(dJ);
+    return rc;
+}
+
 uint32_t gcx_graph_num_vertices(const gcx_graph *g) { return g ? g->N : 0; }
 uint64_t gcx_graph_num_entries(const gcx_graph *g) { return g ? g->M : 0; }
 uint32_t gcx_graph_max_degree(const gcx_graph *g) { return g ? g->maxdeg : 0; }
@@ -623,24 +674,33 @@ int gcx_model_layers(const gcx_model *m) { return m ? m->layers : 0; }
 
 /* ---------------------------------------------------------------- workspace */
 
+/* Sort scratch of the ordered JP schedule that does not fit in the x rows. */
+static size_t order_extra(uint32_t N)
+{
+    const size_t need = gcxk_jp_order_scratch(N, 64);
+    const size_t in_x = 64ull * ((size_t)N + 1);
+    return need > in_x ? need : 0;
+}
+
 uint64_t gcx_workspace_footprint(uint32_t N, uint64_t M)
 {
-    /* per vertex: key 8, count 4, color 4, npred 4, mask64 8, 4 lists x 4,
+    /* per vertex: key 8, count 4, color 4, npred 4, state 8, 4 lists x 4,
      * hpos 8, rows x,y 2 x 64; per entry: E_split 4; plus the overflow colour
-     * words and two item queues */
+     * words, two item queues and any sort scratch beyond the x rows */
     const uint64_t n = (uint64_t)N + 1;
-    return 184ull * n + 4ull * M + 4ull * gcxk_jp_ovf_words(N, M) +
-           2ull * sizeof(gcxk_item) * gcxk_jp_item_capacity(N, M) + 2 * sizeof(gcxk_ctl);
+    return 180ull * n + 4ull * M + 4ull * gcxk_jp_ovf_words(N, M) +
+           2ull * sizeof(gcxk_item) * gcxk_jp_item_capacity(N, M) + 2 * sizeof(gcxk_ctl) +
+           order_extra(N);
 }
 
 int gcx_workspace_free(gcx_workspace *ws)
 {
     if (!ws)
         return GCX_OK;
-    void *bufs[] = {ws->key,    ws->count,  ws->color,    ws->q[0],     ws->q[1],
-                    ws->h[0],   ws->h[1],   ws->x,        ws->y,        ws->ctl,
-                    ws->npred,  ws->mask64, ws->ovf,      ws->esplit,   ws->items[0],
-                    ws->items[1], ws->hpos};
+    void *bufs[] = {ws->key,    ws->count, ws->color,    ws->q[0],     ws->q[1],
+                    ws->h[0],   ws->h[1],  ws->x,        ws->y,        ws->ctl,
+                    ws->npred,  ws->state, ws->ovf,      ws->esplit,   ws->items[0],
+                    ws->items[1], ws->hpos, ws->order_buf};
     for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); i++)
         if (bufs[i])
             (void)hipFree(bufs[i]);
@@ -679,7 +739,7 @@ int gcx_workspace_create(uint32_t N, uint64_t M, gcx_workspace **out)
              hipMalloc((void **)&ws->y, 64 * n) == hipSuccess &&
              hipMalloc((void **)&ws->ctl, 2 * sizeof(gcxk_ctl)) == hipSuccess &&
              hipMalloc((void **)&ws->npred, 4 * n) == hipSuccess &&
-             hipMalloc((void **)&ws->mask64, 8 * n) == hipSuccess &&
+             hipMalloc((void **)&ws->state, 8 * n) == hipSuccess &&
              hipMalloc((void **)&ws->hpos, 8 * n) == hipSuccess &&
              hipMalloc((void **)&ws->ovf, 4 * gcxk_jp_ovf_words(N, M)) == hipSuccess &&
              hipMalloc((void **)&ws->esplit, 4 * (M ? M : 1)) == hipSuccess &&
@@ -689,6 +749,12 @@ int gcx_workspace_create(uint32_t N, uint64_t M, gcx_workspace **out)
                  hipSuccess &&
              hipHostMalloc((void **)&ws->hctl, 4 * sizeof(gcxk_ctl), hipHostMallocDefault) ==
                  hipSuccess;
+    if (ok) {
+        const size_t extra = order_extra(N);
+        ws->order_bytes = extra ? extra : 64 * n;
+        if (extra)
+            ok = hipMalloc(&ws->order_buf, extra) == hipSuccess;
+    }
     if (ok) {
         hipStream_t aux;
         hipEvent_t e[4];
@@ -895,14 +961,67 @@ typedef struct {
     gcx_workspace *ws;
     const gcx_graph *g;
     uint32_t grid;
+    int mb;
 } jp_ctx;
 
 static int jp_round(void *p, uint32_t r, hipStream_t s)
 {
     jp_ctx *c = (jp_ctx *)p;
     gcx_workspace *ws = c->ws;
-    return gcxk_jp_round(c->g->V, ws->npred, ws->esplit, ws->count, ws->color, ws->mask64, ws->ovf,
+    return gcxk_jp_round(c->g->V, ws->npred, ws->esplit, ws->state, ws->color, ws->ovf, c->mb,
                          ws->items, ws->ctl, r, c->grid, s);
+}
+
+/* Orientation pass shared by both schedules. */
+static int jp_prepare(gcx_workspace *ws, const gcx_graph *g, int mb, hipStream_t s)
+{
+    TRY(gcxk_jp_clear(g->N, g->M, ws->ovf, g->hubs, g->nhub, ws->state, ws->npred, ws->hpos,
+                      ws->ctl, s));
+    return gcxk_jp_init(g->N, g->V, g->E, ws->key, ws->key_bits, mb, ws->npred, ws->esplit,
+                        ws->state, ws->color, g->hubs, g->nhub, g->hchunk, g->nhchunk, ws->hpos,
+                        s);
+}
+
+static int color_rounds(gcx_workspace *ws, const gcx_graph *g, int mb, hipStream_t s)
+{
+    void *tok = NULL;
+    TRY(gcx__prof_begin(GCX_ST_JP_INIT, s, &tok));
+    TRY(jp_prepare(ws, g, mb, s));
+    TRY(gcxk_jp_roots(g->N, g->V, ws->npred, ws->items[0], ws->ctl, s));
+    TRY(gcx__prof_end(tok, s));
+    TRY(gcx__prof_begin(GCX_ST_JP_ROUNDS, s, &tok));
+    jp_ctx c = {ws, g, g_round_grid, mb};
+    TRY(drive_rounds(ws, s, jp_round, &c, offsetof(gcxk_ctl, done), g->N + 2 * ROUND_BATCH + 2));
+    return gcx__prof_end(tok, s);
+}
+
+/* Ordered launches never overlap on the device: each waits for the previous
+ * one (a persistent launch needs all its workgroups resident). */
+static hipEvent_t g_ordered_done;
+
+static int color_ordered(gcx_workspace *ws, const gcx_graph *g, int mb, hipStream_t s)
+{
+    void *tok = NULL;
+    if (!g_ordered_done)
+        HIPC(hipEventCreateWithFlags(&g_ordered_done, hipEventDisableTiming));
+    TRY(gcx__prof_begin(GCX_ST_JP_INIT, s, &tok));
+    TRY(jp_prepare(ws, g, mb, s));
+    void *scratch = ws->order_buf ? ws->order_buf : (void *)ws->x;
+    uint32_t *cnt = (uint32_t *)ws->y, *incl = cnt + ws->capN;
+    TRY(gcxk_jp_order_items(g->N, g->V, ws->npred, ws->key, ws->key_bits, scratch,
+                            ws->order_bytes, cnt, incl, ws->items[0], ws->ctl, s));
+    TRY(gcx__prof_end(tok, s));
+    TRY(gcx__prof_begin(GCX_ST_JP_ROUNDS, s, &tok));
+    HIPC(hipStreamWaitEvent(s, g_ordered_done, 0));
+    /* 2 s per wait: far beyond any legitimate wait, and it still ends the kernel */
+    TRY(gcxk_jp_ordered(g->V, ws->npred, ws->esplit, ws->state, ws->color, ws->ovf, mb,
+                        ws->items[0], ws->ctl, ws->ordered_grid, 200000000ull, s));
+    HIPC(hipEventRecord(g_ordered_done, s));
+    TRY(gcx__prof_end(tok, s));
+    HIPC(hipMemcpyAsync(ws->hctl + 2, ws->ctl, sizeof(gcxk_ctl), hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    ws->last_launches = 1;
+    return ws->hctl[2].err ? GCX_ERR_INVALID : GCX_OK;
 }
 
 int gcx_color(gcx_workspace *ws, const gcx_graph *g, void *stream)
@@ -911,17 +1030,31 @@ int gcx_color(gcx_workspace *ws, const gcx_graph *g, void *stream)
     if (!ws_fits(ws, g) || !ws->key_bits || ws->curN != g->N)
         return GCX_ERR_ARG;
     hipStream_t s = (hipStream_t)gcx__pick(stream);
-    void *tok = NULL;
-    TRY(gcx__prof_begin(GCX_ST_JP_INIT, s, &tok));
-    TRY(gcxk_jp_clear(g->N, g->M, ws->color, ws->mask64, ws->ovf, g->hubs, g->nhub, ws->count,
-                      ws->npred, ws->hpos, ws->ctl, s));
-    TRY(gcxk_jp_init(g->N, g->V, g->E, ws->key, ws->key_bits, ws->count, ws->npred, ws->esplit,
-                     ws->items[0], g->hubs, g->nhub, g->hchunk, g->nhchunk, ws->hpos, ws->ctl, s));
-    TRY(gcx__prof_end(tok, s));
-    TRY(gcx__prof_begin(GCX_ST_JP_ROUNDS, s, &tok));
-    jp_ctx c = {ws, g, g_round_grid};
-    TRY(drive_rounds(ws, s, jp_round, &c, offsetof(gcxk_ctl, done), g->N + 2 * ROUND_BATCH + 2));
-    TRY(gcx__prof_end(tok, s));
+    const int mb = gcxk_jp_mask_bits(g->maxdeg);
+    ws->last_fallback = 0;
+    if (ws->schedule == GCX_SCHED_ORDERED && g->N) {
+        const int rc = color_ordered(ws, g, mb, s);
+        if (rc != GCX_ERR_INVALID)
+            return rc;
+        ws->last_fallback = 1; /* a wait hit its bound: redo with rounds */
+    }
+    return color_rounds(ws, g, mb, s);
+}
+
+int gcx_set_schedule(gcx_workspace *ws, int schedule, uint32_t param)
+{
+    if (!ws || schedule < GCX_SCHED_ROUNDS || schedule > GCX_SCHED_ORDERED)
+        return GCX_ERR_ARG;
+    ws->schedule = schedule;
+    ws->ordered_grid = schedule == GCX_SCHED_ORDERED ? param : 0;
+    return GCX_OK;
+}
+
+int gcx_last_fallback(const gcx_workspace *ws, uint32_t *fell_back)
+{
+    if (!ws || !fell_back)
+        return GCX_ERR_ARG;
+    *fell_back = ws->last_fallback;
     return GCX_OK;
 }
 
@@ -931,7 +1064,7 @@ int gcx_last_rounds(const gcx_workspace *ws, uint32_t *rounds)
         return GCX_ERR_ARG;
     gcxk_ctl c;
     HIPC(hipMemcpy(&c, ws->ctl, sizeof(c), hipMemcpyDeviceToHost));
-    *rounds = c.rounds;
+    *rounds = ws->schedule == GCX_SCHED_ORDERED && !ws->last_fallback ? 0 : c.rounds;
     return GCX_OK;
 }
 
@@ -982,7 +1115,8 @@ int gcx_mex_check(gcx_workspace *ws, const gcx_graph *g, void *stream, uint64_t 
     hipStream_t s = (hipStream_t)gcx__pick(stream);
     gcxk_ctl *vc = ws->ctl + 1;
     HIPC(hipMemsetAsync(vc, 0, sizeof(gcxk_ctl), s));
-    TRY(gcxk_mex_check(g->N, g->V, g->E, ws->key, ws->key_bits, ws->color, vc, s));
+    TRY(gcxk_mex_check(g->N, g->V, g->E, ws->key, ws->key_bits, ws->color, g->hubs, g->nhub, vc,
+                       s));
     HIPC(hipMemcpyAsync(ws->hctl + 1, vc, sizeof(gcxk_ctl), hipMemcpyDeviceToHost, s));
     HIPC(hipStreamSynchronize(s));
     *violations = ws->hctl[1].viol;

@@ -133,30 +133,23 @@ int gcx_gen_stencil27_pairs(uint32_t g, uint64_t seed, uint32_t *I, uint32_t *J)
 int gcx_gen_rgg2d_pairs(uint32_t n, double radius, uint64_t seed, uint64_t cap,
                         uint32_t *I, uint32_t *J, uint64_t *L)
 {
-    /* Points: 32-bit fixed-point coordinates from hash(seed, "RGG", i).
-     * Pairs (i, j), i < j, with squared distance < radius^2, found through a
-     * uniform cell grid of side >= radius. Output order: by i, then j. */
+    /* Points: 31-bit fixed-point coordinates (gcx_rgg_point). Pairs (i, j),
+     * i < j, with integer squared distance < r2 = floor(radius^2 * 2^62),
+     * found through a uniform cell grid of side >= radius. Output order: by
+     * i, then by neighbouring cell, then by j. */
     if (n == 0 || radius <= 0.0)
         return GCX_ERR_ARG;
     uint32_t *px = malloc(sizeof(uint32_t) * n), *py = malloc(sizeof(uint32_t) * n);
-    for (uint32_t i = 0; i < n; i++) {
-        const uint64_t h = gcx_hash3(seed, 0x524747ull, i);
-        px[i] = (uint32_t)(h >> 32);
-        py[i] = (uint32_t)h;
-    }
-    uint32_t cells = (uint32_t)floor(1.0 / radius);
-    if (cells < 1)
-        cells = 1;
-    if (cells > 65536)
-        cells = 65536;
+    for (uint32_t i = 0; i < n; i++)
+        gcx_rgg_point(seed, i, px + i, py + i);
+    const uint32_t cells = gcx_rgg_cells(radius);
+    const uint64_t r2 = gcx_rgg_r2(radius);
     const uint64_t nc = (uint64_t)cells * cells;
     uint32_t *start = calloc(nc + 1, sizeof(uint32_t));
     uint32_t *order = malloc(sizeof(uint32_t) * n);
     uint32_t *cell_of = malloc(sizeof(uint32_t) * n);
     for (uint32_t i = 0; i < n; i++) {
-        const uint32_t cx = (uint32_t)(((uint64_t)px[i] * cells) >> 32);
-        const uint32_t cy = (uint32_t)(((uint64_t)py[i] * cells) >> 32);
-        cell_of[i] = cy * cells + cx;
+        cell_of[i] = gcx_rgg_cell(px[i], py[i], cells);
         start[cell_of[i] + 1]++;
     }
     for (uint64_t c = 0; c < nc; c++)
@@ -166,9 +159,6 @@ int gcx_gen_rgg2d_pairs(uint32_t n, double radius, uint64_t seed, uint64_t cap,
     for (uint32_t i = 0; i < n; i++)
         order[fill[cell_of[i]]++] = i;
     free(fill);
-    /* Squared radius in units of 2^-32 per axis, as an exact 128-bit-safe
-     * comparison done in double on integer differences < 2^32. */
-    const double r2 = radius * radius * 18446744073709551616.0;
     uint64_t w = 0;
     int overflow = 0;
     for (uint32_t i = 0; i < n && !overflow; i++) {
@@ -183,9 +173,7 @@ int gcx_gen_rgg2d_pairs(uint32_t n, double radius, uint64_t seed, uint64_t cap,
                     const uint32_t j = order[k];
                     if (j <= i)
                         continue;
-                    const double ddx = (double)px[i] - (double)px[j];
-                    const double ddy = (double)py[i] - (double)py[j];
-                    if (ddx * ddx + ddy * ddy < r2) {
+                    if (gcx_rgg_close(px[i], py[i], px[j], py[j], r2)) {
                         if (w >= cap) {
                             overflow = 1;
                             break;

@@ -71,4 +71,48 @@ GCX_HD void gcx_er_pair(uint64_t seed, uint32_t n, uint64_t k, uint32_t *u, uint
     *v = gcx_bounded((uint32_t)h, n);
 }
 
+/* 2D random geometric graph point i: 31-bit fixed-point coordinates. */
+GCX_HD void gcx_rgg_point(uint64_t seed, uint64_t i, uint32_t *x, uint32_t *y)
+{
+    const uint64_t h = gcx_hash3(seed, 0x524747ull, i);
+    *x = (uint32_t)(h >> 33);
+    *y = (uint32_t)(h & 0x7FFFFFFFull);
+}
+
+/* Cell of a point on a cells x cells grid (row-major, y then x). */
+GCX_HD uint32_t gcx_rgg_cell(uint32_t x, uint32_t y, uint32_t cells)
+{
+    const uint32_t cx = (uint32_t)(((uint64_t)x * cells) >> 31);
+    const uint32_t cy = (uint32_t)(((uint64_t)y * cells) >> 31);
+    return cy * cells + cx;
+}
+
+/* Exact integer distance test (differences < 2^31, so no overflow). */
+GCX_HD int gcx_rgg_close(uint32_t xi, uint32_t yi, uint32_t xj, uint32_t yj, uint64_t r2)
+{
+    const uint64_t dx = xi > xj ? (uint64_t)(xi - xj) : (uint64_t)(xj - xi);
+    const uint64_t dy = yi > yj ? (uint64_t)(yi - yj) : (uint64_t)(yj - yi);
+    return dx * dx + dy * dy < r2;
+}
+
+#if !defined(__HIPCC__)
+/* Host-side RGG parameters: grid side (cells of width >= radius) and the
+ * squared radius in units of 2^-62. */
+static inline uint32_t gcx_rgg_cells(double radius)
+{
+    double c = radius > 0.0 ? 1.0 / radius : 1.0;
+    if (c < 1.0)
+        c = 1.0;
+    if (c > 65536.0)
+        c = 65536.0;
+    return (uint32_t)c;
+}
+
+static inline uint64_t gcx_rgg_r2(double radius)
+{
+    const double r2 = radius * radius * 4611686018427387904.0;
+    return r2 >= 9.2e18 ? 9200000000000000000ull : (uint64_t)r2;
+}
+#endif
+
 #undef GCX_HD

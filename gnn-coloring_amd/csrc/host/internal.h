@@ -34,15 +34,15 @@ struct gcx_workspace {
     void *key;        /* 8 * capN bytes: key32 or key64 view */
     int key_bits;     /* 32 / 64 for the current priorities, 0 = none */
     int key_kind;     /* 32-bit keys: 0 unsigned integer, 1 float order */
-    uint32_t *count;  /* pending predecessors (JP); degrees (SL) */
+    uint32_t *count;  /* SL: remaining degrees */
     int *color;
     uint32_t *q[2];   /* SL alive lists */
     uint32_t *h[2];   /* SL removed list */
     uint32_t *npred;  /* JP: predecessors per vertex */
-    uint64_t *mask64; /* JP: colours 1..64 used by coloured predecessors */
-    uint32_t *ovf;    /* JP: colour-set overflow words (colours > 64) */
+    uint64_t *state;  /* JP: pending predecessors << mb | colours 1..mb taken */
+    uint32_t *ovf;    /* JP: colour-set overflow words (colours > mb) */
     uint32_t *esplit; /* JP: rows split into predecessors | successors (capM) */
-    gcxk_item *items[2]; /* JP: work-item queues */
+    gcxk_item *items[2]; /* JP: round queues; items[0] = greedy-order items */
     uint32_t *hpos;   /* JP: per-hub E_split positions (2 per vertex slot) */
     float *x, *y;     /* GNN rows, 16 floats (or 16 bf16) per vertex */
     gcxk_ctl *ctl;    /* device control block */
@@ -53,6 +53,11 @@ struct gcx_workspace {
     uint32_t last_rounds;
     uint32_t last_sl_rounds;
     uint32_t last_launches; /* round kernels enqueued by the last round loop */
+    int schedule;           /* GCX_SCHED_ROUNDS or GCX_SCHED_ORDERED */
+    uint32_t ordered_grid;  /* workgroups of the ordered launch (0 = all resident) */
+    uint32_t last_fallback; /* 1 when the last ordered launch hit its wait bound */
+    void *order_buf;        /* sort scratch when it does not fit in x */
+    size_t order_bytes;     /* bytes of sort scratch available */
     uint32_t curN;    /* vertex count of the graph the contents belong to */
 };
 

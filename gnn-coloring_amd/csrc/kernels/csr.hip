@@ -230,6 +230,76 @@ __global__ __launch_bounds__(kBlock) void gather_degrees(const uint32_t *__restr
         out[i] = V[list[i] + 1] - V[list[i]];
 }
 
+// ------------------------------------------------ random geometric (2D) pairs
+
+__global__ __launch_bounds__(kBlock) void rgg_cells(uint32_t n, uint64_t seed, uint32_t cells,
+                                                    uint32_t *__restrict__ px, uint32_t *__restrict__ py,
+                                                    uint32_t *__restrict__ cell_of,
+                                                    uint32_t *__restrict__ cellcnt)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n)
+        return;
+    uint32_t x, y;
+    gcx_rgg_point(seed, i, &x, &y);
+    px[i] = x;
+    py[i] = y;
+    const uint32_t c = gcx_rgg_cell(x, y, cells);
+    cell_of[i] = c;
+    atomicAdd(cellcnt + c, 1u);
+}
+
+__global__ __launch_bounds__(kBlock) void rgg_fill(uint32_t n, const uint32_t *__restrict__ cell_of,
+                                                   uint32_t *__restrict__ fill,
+                                                   uint32_t *__restrict__ order)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n)
+        order[atomicAdd(fill + cell_of[i], 1u)] = i;
+}
+
+// Pass 0 counts, pass 1 writes the pairs (i, j), j > i, of point i.
+template <int pass>
+__global__ __launch_bounds__(kBlock) void rgg_pairs(uint32_t n, uint32_t cells, uint64_t r2,
+                                                    const uint32_t *__restrict__ px,
+                                                    const uint32_t *__restrict__ py,
+                                                    const uint32_t *__restrict__ cell_of,
+                                                    const uint32_t *__restrict__ cellstart,
+                                                    const uint32_t *__restrict__ order,
+                                                    uint32_t *__restrict__ cnt,
+                                                    const uint64_t *__restrict__ off,
+                                                    uint32_t *__restrict__ I, uint32_t *__restrict__ J)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n)
+        return;
+    const uint32_t xi = px[i], yi = py[i], c = cell_of[i];
+    const int64_t cx = c % cells, cy = c / cells;
+    uint32_t k = 0;
+    uint64_t w = pass ? off[i] : 0;
+    for (int dy = -1; dy <= 1; dy++)
+        for (int dx = -1; dx <= 1; dx++) {
+            const int64_t X = cx + dx, Y = cy + dy;
+            if (X < 0 || Y < 0 || X >= cells || Y >= cells)
+                continue;
+            const uint32_t cc = (uint32_t)(Y * cells + X);
+            for (uint32_t s = cellstart[cc]; s < cellstart[cc + 1]; s++) {
+                const uint32_t j = order[s];
+                if (j > i && gcx_rgg_close(xi, yi, px[j], py[j], r2)) {
+                    if (pass) {
+                        I[w] = i;
+                        J[w] = j;
+                        w++;
+                    } else {
+                        k++;
+                    }
+                }
+            }
+        }
+    if (!pass)
+        cnt[i] = k;
+}
+
 int bit_length(uint32_t x)
 {
     int b = 0;
@@ -454,5 +524,78 @@ extern "C" int gcxk_gather_degrees(const uint32_t *dV, const uint32_t *list, uin
         return GCX_OK;
     gather_degrees<<<div_up(n, kBlock), kBlock, 0, as_stream(stream)>>>(dV, list, n, out);
     GCXK_LAUNCH_CHECK();
+    return GCX_OK;
+}
+
+extern "C" int gcxk_gen_rgg(uint32_t n, uint32_t cells, uint64_t r2, uint64_t seed, uint32_t **dI,
+                            uint32_t **dJ, uint64_t *L, void *stream)
+{
+    hipStream_t s = as_stream(stream);
+    *dI = *dJ = nullptr;
+    *L = 0;
+    if (n == 0 || cells == 0)
+        return GCX_ERR_ARG;
+    const uint64_t nc = (uint64_t)cells * cells;
+    if (nc >= UINT32_MAX)
+        return GCX_ERR_ARG;
+    DevBuf px, py, cellof, cellcnt, cellstart, fill, order, cnt, off, t1, t2;
+    GCXK_CHECK(px.alloc(4ull * n));
+    GCXK_CHECK(py.alloc(4ull * n));
+    GCXK_CHECK(cellof.alloc(4ull * n));
+    GCXK_CHECK(order.alloc(4ull * n));
+    GCXK_CHECK(cellcnt.alloc(4ull * (nc + 1)));
+    GCXK_CHECK(cellstart.alloc(4ull * (nc + 1)));
+    GCXK_CHECK(fill.alloc(4ull * (nc + 1)));
+    GCXK_CHECK(cnt.alloc(4ull * (n + 1ull)));
+    GCXK_CHECK(off.alloc(8ull * (n + 1ull)));
+    GCXK_CHECK(hipMemsetAsync(cellcnt.p, 0, 4ull * (nc + 1), s));
+    GCXK_CHECK(hipMemsetAsync(cnt.p, 0, 4ull * (n + 1ull), s));
+    uint32_t *P = (uint32_t *)px.p, *Q = (uint32_t *)py.p, *CO = (uint32_t *)cellof.p;
+    rgg_cells<<<div_up(n, kBlock), kBlock, 0, s>>>(n, seed, cells, P, Q, CO, (uint32_t *)cellcnt.p);
+    GCXK_LAUNCH_CHECK();
+    size_t tb = 0;
+    GCXK_CHECK(rocprim::exclusive_scan(nullptr, tb, (uint32_t *)cellcnt.p, (uint32_t *)cellstart.p,
+                                       0u, (size_t)nc + 1, rocprim::plus<uint32_t>(), s));
+    GCXK_CHECK(t1.alloc(tb));
+    GCXK_CHECK(rocprim::exclusive_scan(t1.p, tb, (uint32_t *)cellcnt.p, (uint32_t *)cellstart.p, 0u,
+                                       (size_t)nc + 1, rocprim::plus<uint32_t>(), s));
+    GCXK_CHECK(hipMemcpyAsync(fill.p, cellstart.p, 4ull * (nc + 1), hipMemcpyDeviceToDevice, s));
+    rgg_fill<<<div_up(n, kBlock), kBlock, 0, s>>>(n, CO, (uint32_t *)fill.p, (uint32_t *)order.p);
+    rgg_pairs<0><<<div_up(n, kBlock), kBlock, 0, s>>>(n, cells, r2, P, Q, CO,
+                                                     (const uint32_t *)cellstart.p,
+                                                     (const uint32_t *)order.p, (uint32_t *)cnt.p,
+                                                     nullptr, nullptr, nullptr);
+    GCXK_LAUNCH_CHECK();
+    size_t tb2 = 0;
+    GCXK_CHECK(rocprim::exclusive_scan(nullptr, tb2, (uint32_t *)cnt.p, (uint64_t *)off.p, 0ull,
+                                       (size_t)n + 1, rocprim::plus<uint64_t>(), s));
+    GCXK_CHECK(t2.alloc(tb2));
+    GCXK_CHECK(rocprim::exclusive_scan(t2.p, tb2, (uint32_t *)cnt.p, (uint64_t *)off.p, 0ull,
+                                       (size_t)n + 1, rocprim::plus<uint64_t>(), s));
+    uint64_t total = 0;
+    GCXK_CHECK(hipMemcpyAsync(&total, (uint64_t *)off.p + n, 8, hipMemcpyDeviceToHost, s));
+    GCXK_CHECK(hipStreamSynchronize(s));
+    uint32_t *I = nullptr, *J = nullptr;
+    if (hipMalloc((void **)&I, 4ull * (total ? total : 1)) != hipSuccess ||
+        hipMalloc((void **)&J, 4ull * (total ? total : 1)) != hipSuccess) {
+        if (I)
+            (void)hipFree(I);
+        return GCX_ERR_NOMEM;
+    }
+    rgg_pairs<1><<<div_up(n, kBlock), kBlock, 0, s>>>(n, cells, r2, P, Q, CO,
+                                                     (const uint32_t *)cellstart.p,
+                                                     (const uint32_t *)order.p, nullptr,
+                                                     (const uint64_t *)off.p, I, J);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        (void)hipFree(I);
+        (void)hipFree(J);
+        return GCX_ERR_HIP;
+    }
+    *dI = I;
+    *dJ = J;
+    *L = total;
     return GCX_OK;
 }

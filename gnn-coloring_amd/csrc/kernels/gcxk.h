@@ -31,6 +31,8 @@ typedef struct {
     uint32_t sl_delta[3];   /* SL: threshold of the round */
     uint32_t sl_rounds;
     uint32_t sl_done;
+    uint32_t total_items;   /* JP (ordered): work items in greedy order */
+    uint32_t err;           /* JP (ordered): a wave hit its wait limit */
 } gcxk_ctl;
 
 /* ---- graph helpers (csr.hip) ---- */
@@ -60,6 +62,9 @@ int gcxk_make_simple(uint32_t N, uint32_t *dV, const uint32_t *dE, uint64_t M, u
 int gcxk_gen_rmat(int scale, uint64_t L, uint32_t t0, uint32_t t1, uint32_t t2, uint64_t seed,
                   uint32_t *dI, uint32_t *dJ, void *stream);
 int gcxk_gen_er(uint32_t n, uint64_t m, uint64_t seed, uint32_t *dI, uint32_t *dJ, void *stream);
+/* 2D random geometric graph: allocates *dI, *dJ (caller frees), *L pairs. */
+int gcxk_gen_rgg(uint32_t n, uint32_t cells, uint64_t r2, uint64_t seed, uint32_t **dI,
+                 uint32_t **dJ, uint64_t *L, void *stream);
 
 /* ---- ordering keys (keys.hip); key32: larger = earlier ---- */
 int gcxk_keys_ff(uint32_t N, uint32_t *key, void *stream);
@@ -74,6 +79,9 @@ int gcxk_keys_from_doubles64(uint32_t N, const double *p, uint64_t *key, void *s
 /* key32 -> doubles: kind 0 = unsigned integer, 1 = float order key. */
 int gcxk_keys_to_doubles(uint32_t N, const uint32_t *key, int kind, double *p, void *stream);
 int gcxk_keys64_to_doubles(uint32_t N, const uint64_t *key, double *p, void *stream);
+
+/* Streaming copy of `bytes` (multiple of 16) for byte-counter calibration. */
+int gcxk_copy_bytes(uint64_t bytes, const void *src, void *dst, void *stream);
 
 /* ---- SL / SLL peeling (sl.hip) ---- */
 /* key[u] = UINT32_MAX marks "not yet removed"; a removed vertex's key is the
@@ -111,7 +119,8 @@ typedef struct gcxk_item {
     uint32_t u, k;
 } gcxk_item;
 
-/* Overflow colour-set words for N vertices / M entries (colours > 64). */
+/* Overflow colour-set words for N vertices / M entries (colours past the
+ * state word's colour bits, which are at least 32). */
 static inline uint64_t gcxk_jp_ovf_words(uint32_t N, uint64_t M)
 {
     return M / 32u + (uint64_t)N + 2u;
@@ -124,24 +133,48 @@ static inline uint64_t gcxk_jp_item_capacity(uint32_t N, uint64_t M)
 /* Hub chunks for the orientation pass: 4096 edges each. */
 #define GCXK_HUB_CHUNK_EDGES 4096u
 
-/* Zero colours, colour sets (mask64: N words, ovf: gcxk_jp_ovf_words), the
- * hubs' counters and per-hub positions (hpos: 2 per hub), and the control
- * block. A plain kernel, so the sequence can be captured in a graph. */
-int gcxk_jp_clear(uint32_t N, uint64_t M, int *color, uint64_t *mask64, uint32_t *ovf,
-                  const uint32_t *hubs, uint32_t nhub, uint32_t *count, uint32_t *npred,
-                  uint32_t *hpos, gcxk_ctl *ctl, void *stream);
-/* Orientation: key_bits = 32 or 64. E_split rows = predecessors then
- * successors; count = npred = predecessor counts; roots queued as items in
- * slot 0. hchunk: per-hub prefix of 4096-edge chunks (nhub + 1 entries),
- * nhchunk = total chunks. */
+/* Colour bits in the per-vertex JP state word: 64 - bitlength(maxdeg); the
+ * rest of the word counts pending predecessors. */
+int gcxk_jp_mask_bits(uint32_t maxdeg);
+/* Zero the overflow colour words (gcxk_jp_ovf_words), the hubs' state words,
+ * predecessor counts and per-hub positions (hpos: 2 per hub), and the
+ * control block. A plain kernel, so the sequence can be captured in a graph. */
+int gcxk_jp_clear(uint32_t N, uint64_t M, uint32_t *ovf, const uint32_t *hubs, uint32_t nhub,
+                  uint64_t *state, uint32_t *npred, uint32_t *hpos, gcxk_ctl *ctl, void *stream);
+/* Orientation: key_bits = 32 or 64, mb = gcxk_jp_mask_bits(maxdeg). E_split
+ * rows = predecessors then successors; npred = predecessor counts; state =
+ * npred << mb; color = 1 for degree-0 vertices, else 0. hchunk: per-hub
+ * prefix of 4096-edge chunks (nhub + 1 entries), nhchunk = total chunks. */
 int gcxk_jp_init(uint32_t N, const uint32_t *dV, const uint32_t *dE, const void *key,
-                 int key_bits, uint32_t *count, uint32_t *npred, uint32_t *esplit,
-                 gcxk_item *q0, const uint32_t *hubs, uint32_t nhub, const uint32_t *hchunk,
-                 uint32_t nhchunk, uint32_t *hpos, gcxk_ctl *ctl, void *stream);
+                 int key_bits, int mb, uint32_t *npred, uint32_t *esplit, uint64_t *state,
+                 int *color, const uint32_t *hubs, uint32_t nhub, const uint32_t *hchunk,
+                 uint32_t nhchunk, uint32_t *hpos, void *stream);
+/* Round schedule: queue the roots (vertices with successors and no
+ * predecessors) as round-0 items in q0 / ctl->qn[0]. */
+int gcxk_jp_roots(uint32_t N, const uint32_t *dV, const uint32_t *npred, gcxk_item *q0,
+                  gcxk_ctl *ctl, void *stream);
 /* One JP round r: colours the items of slot r, queues the next round's. */
 int gcxk_jp_round(const uint32_t *dV, const uint32_t *npred, const uint32_t *esplit,
-                  uint32_t *count, int *color, uint64_t *mask64, uint32_t *ovf,
-                  gcxk_item *const q[2], gcxk_ctl *ctl, uint32_t r, uint32_t grid, void *stream);
+                  uint64_t *state, int *color, uint32_t *ovf, int mb, gcxk_item *const q[2],
+                  gcxk_ctl *ctl, uint32_t r, uint32_t grid, void *stream);
+/* Ordered schedule, step 1: items of every vertex in greedy order (p
+ * descending, id ascending) into `items`, their number into
+ * ctl->total_items. scratch: gcxk_jp_order_scratch(N, key_bits) bytes; cnt,
+ * incl: N words each. */
+size_t gcxk_jp_order_scratch(uint32_t N, int key_bits);
+int gcxk_jp_order_items(uint32_t N, const uint32_t *dV, const uint32_t *npred, const void *key,
+                        int key_bits, void *scratch, size_t scratch_bytes, uint32_t *cnt,
+                        uint32_t *incl, gcxk_item *items, gcxk_ctl *ctl, void *stream);
+/* Workgroups of the ordered kernel resident on the device at once (0 on
+ * error). */
+uint32_t gcxk_jp_ordered_grid(void);
+/* Ordered schedule, step 2: one cooperative launch of `grid` workgroups (0 =
+ * gcxk_jp_ordered_grid()); wave w colours items w, w + W, ... waiting on each
+ * item's state word. spin_ticks bounds every wait (100 MHz ticks); exceeding
+ * it sets ctl->err and ends the kernel. */
+int gcxk_jp_ordered(const uint32_t *dV, const uint32_t *npred, const uint32_t *esplit,
+                    uint64_t *state, int *color, uint32_t *ovf, int mb, const gcxk_item *items,
+                    gcxk_ctl *ctl, uint32_t grid, uint64_t spin_ticks, void *stream);
 /* Degrees of the listed vertices (for the hub chunk table). */
 int gcxk_gather_degrees(const uint32_t *dV, const uint32_t *list, uint32_t n, uint32_t *out,
                         void *stream);
@@ -150,7 +183,8 @@ int gcxk_validate(uint32_t N, const uint32_t *dV, const uint32_t *dE, const int 
                   const uint32_t *hubs, uint32_t nhub, gcxk_ctl *ctl, void *stream);
 /* Greedy-rule certificate: ctl->viol. */
 int gcxk_mex_check(uint32_t N, const uint32_t *dV, const uint32_t *dE, const void *key,
-                   int key_bits, const int *color, gcxk_ctl *ctl, void *stream);
+                   int key_bits, const int *color, const uint32_t *hubs, uint32_t nhub,
+                   gcxk_ctl *ctl, void *stream);
 
 /* Number of workgroups the round kernel should use (fills the GPU). */
 uint32_t gcxk_round_grid(void);

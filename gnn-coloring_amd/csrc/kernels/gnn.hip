@@ -27,6 +27,9 @@ constexpr int kBlock = 256;
 constexpr int F = 16;
 constexpr int kHubBlock = 512;
 constexpr int kHubChunk = kHubBlock;
+// Row stride padding of the feature-major hub staging buffer: 4 floats put
+// the 16 summing lanes' 16-byte reads on distinct LDS banks.
+constexpr int kHubPad = 4;
 
 enum LayerKind { kInput = 0, kHidden = 1, kOutput = 2 };
 
@@ -115,17 +118,30 @@ __global__ __launch_bounds__(kBlock) void sage_groups(uint32_t N, const uint32_t
     float h;
     if (kind == kInput) {
         float sd = 0.0f, si = 0.0f;
-        for (uint32_t i0 = beg; i0 < end; i0 += F) {
-            const uint32_t nb = end - i0 < (uint32_t)F ? end - i0 : (uint32_t)F;
-            float fd = 0.0f, fi = 0.0f;
-            if ((uint32_t)j < nb) {
-                const uint32_t v = E[i0 + j];
-                fd = (float)(V[v + 1] - V[v]) / fmd;
-                fi = fid(v, N);
+        // 4 x 16 neighbours per step: lane j computes the features of
+        // neighbours i0 + 16t + j (independent gathers), then the group adds
+        // them strictly in CSR order.
+        for (uint32_t i0 = beg; i0 < end; i0 += 4 * F) {
+            float fd[4], fi[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const uint32_t i = i0 + F * t + j;
+                fd[t] = 0.0f;
+                fi[t] = 0.0f;
+                if (i < end) {
+                    const uint32_t v = E[i];
+                    fd[t] = (float)(V[v + 1] - V[v]) / fmd;
+                    fi[t] = fid(v, N);
+                }
             }
-            for (uint32_t k = 0; k < nb; k++) {
-                sd = sd + __shfl(fd, (int)k, F);
-                si = si + __shfl(fi, (int)k, F);
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const uint32_t s0 = i0 + F * t;
+                const uint32_t nb = s0 >= end ? 0u : (end - s0 < (uint32_t)F ? end - s0 : (uint32_t)F);
+                for (uint32_t k = 0; k < nb; k++) {
+                    sd = sd + __shfl(fd[t], (int)k, F);
+                    si = si + __shfl(fi[t], (int)k, F);
+                }
             }
         }
         if (fdeg_f > 0.0f) {
@@ -186,7 +202,7 @@ __global__ __launch_bounds__(kHubBlock) void sage_hubs(uint32_t N, const uint32_
                                                        const uint32_t *__restrict__ hubs)
 {
     // Feature-major staging: buf[s][f][r] = feature f of row r of step s.
-    __shared__ float buf[2][F][kHubChunk];
+    __shared__ __attribute__((aligned(16))) float buf[2][F][kHubChunk + kHubPad];
     const uint32_t u = hubs[blockIdx.x];
     const uint32_t beg = V[u], end = V[u + 1];
     const uint32_t deg = end - beg;
@@ -226,22 +242,24 @@ __global__ __launch_bounds__(kHubBlock) void sage_hubs(uint32_t N, const uint32_
         if (summer) {
             const uint32_t rows = deg - c * kHubChunk < (uint32_t)kHubChunk ? deg - c * kHubChunk
                                                                           : (uint32_t)kHubChunk;
-            if (kind == kInput) {
-                // lane 0 sums fdeg, lane 1 sums fid (two independent chains)
-                if (j < 2) {
-                    const float *col = buf[s][j];
-                    for (uint32_t r = 0; r < rows; r++)
-                        acc = acc + col[r];
-                }
-            } else {
+            // Layer 0: lane 0 sums fdeg, lane 1 sums fid. Other layers: lane j
+            // sums feature j. 32 rows of LDS reads are issued ahead of the
+            // ordered adds so the add chain does not wait on each read.
+            if (kind != kInput || j < 2) {
                 const float *col = buf[s][j];
                 uint32_t r = 0;
-                for (; r + 4 <= rows; r += 4) {
-                    const float4 q = *reinterpret_cast<const float4 *>(col + r);
-                    acc = acc + q.x;
-                    acc = acc + q.y;
-                    acc = acc + q.z;
-                    acc = acc + q.w;
+                for (; r + 32 <= rows; r += 32) {
+                    float4 q[8];
+#pragma unroll
+                    for (int t = 0; t < 8; t++)
+                        q[t] = *reinterpret_cast<const float4 *>(col + r + 4 * t);
+#pragma unroll
+                    for (int t = 0; t < 8; t++) {
+                        acc = acc + q[t].x;
+                        acc = acc + q[t].y;
+                        acc = acc + q[t].z;
+                        acc = acc + q[t].w;
+                    }
                 }
                 for (; r < rows; r++)
                     acc = acc + col[r];

@@ -5,41 +5,107 @@
 // p[v] > p[u] || (p[v] == p[u] && v < u); each vertex takes the smallest
 // colour >= 1 unused by its predecessors. The result equals sequential greedy
 // in (p descending, id ascending) order, so it does not depend on how the
-// rounds below are scheduled.
+// work below is scheduled.
 //
 // Orientation (init): every row is split into predecessors first and
-// successors last (E_split, as coloring.c:241-264 does); count[u] = npred[u]
-// = number of predecessors; vertices with none are the first frontier.
+// successors last (E_split, as coloring.c:241-264 does); npred[u] = number of
+// predecessors.
 //
-// Push-based rounds: when u takes colour c it ORs bit c-1 into the colour set
-// of each successor v and decrements count[v]; the successor's colour set is
-// therefore complete when its count reaches zero, and its mex costs one
-// 64-bit word (colours 1..64) or a short scan of its overflow words (colours
-// 65..deg+1, a per-vertex region at word offset V[v]/32 + v). A ready vertex
-// becomes ceil(nsucc/64) work items (at least one); one wave handles one
-// item, i.e. up to 64 successor edges, so a hub's successors are spread over
-// many waves of the same round. Every item of u recomputes the (identical)
-// mex; item 0 stores the colour.
+// Per-vertex state word: state[v] = (pending predecessors << mb) | colour
+// bits, bit c-1 set when a predecessor took colour c (1 <= c <= mb), where
+// mb = 64 - bitlength(max degree) leaves just enough high bits for the count.
+// Colours mb+1 .. deg+1 live in per-vertex overflow words at word offset
+// V[v]/32 + v. When u takes colour c <= mb it ORs bit c-1 into state[v] of
+// each successor and then subtracts 1 << mb from the same word: both are
+// atomics on one location, so the value a consumer reads with count zero
+// already holds every predecessor's bit, and the consumer's single atomic
+// load yields both "ready" and the colour set. For c > mb the overflow OR is
+// waited for (s_waitcnt) before the decrement.
+//
+// A vertex with s successors is ceil(s/64) work items (at least one); one
+// wave handles one item, i.e. up to 64 successor edges, so a hub's
+// successors are spread over many waves. Every item of u recomputes the
+// (identical) mex; item 0 stores the colour. Vertices of degree 0 take
+// colour 1 at init and have no items.
+//
+// Two schedules, same colours:
+//   rounds  -- one launch per DAG level; the frontier is an item queue.
+//   ordered -- items sorted in greedy order; one persistent launch whose
+//              wave w takes items w, w + W, w + 2W, ... and waits on each
+//              item's state word. The earliest unfinished item is always
+//              ready and its wave is at it, so the launch completes when all
+//              W waves are resident; a wall-clock bound on every wait ends
+//              the kernel with ctl->err set if they are not.
 #include "common.hip.h"
+
+#include <cstring>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 namespace {
 
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kWave;
-constexpr int kGroup = 16;             // lanes per vertex in whole-graph passes
-constexpr uint32_t kChunk = 64;        // successor edges per work item
+constexpr int kGroup = 16;      // lanes per vertex in whole-graph passes
+constexpr uint32_t kChunk = 64; // successor edges per work item
+constexpr int kRootBlock = 1024;
 
 __device__ __forceinline__ uint32_t ovf_base(uint32_t Vv, uint32_t v) { return Vv / 32u + v; }
 
-// Overflow words of a vertex of degree d: colours 65 .. d+1.
-__device__ __forceinline__ uint32_t ovf_words(uint32_t d)
+// Overflow words of a vertex of degree d: colours mb+1 .. d+1.
+__device__ __forceinline__ uint32_t ovf_words(uint32_t d, int mb)
 {
-    return d >= 64u ? (d + 1u - 64u + 31u) / 32u : 0u;
+    return d + 1u > (uint32_t)mb ? (d + 1u - (uint32_t)mb + 31u) / 32u : 0u;
 }
 
 __device__ __forceinline__ uint32_t n_items(uint32_t nsucc)
 {
     return nsucc ? (nsucc + kChunk - 1u) / kChunk : 1u;
+}
+
+__device__ __forceinline__ uint64_t load_state(const uint64_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint32_t load_word(const uint32_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Smallest colour not in u's set; s = u's state word with count zero. Wave
+// uniform (all lanes call with the same u). mex <= npred + 1 <= deg + 1, so a
+// free bit lies inside the overflow region when the word is full.
+__device__ __forceinline__ uint32_t mex_of(uint64_t s, int mb, uint32_t u, uint32_t Vu,
+                                           uint32_t deg, const uint32_t *__restrict__ ovf)
+{
+    const uint64_t low = (1ull << mb) - 1ull;
+    const uint64_t free = ~s & low;
+    if (free)
+        return (uint32_t)__builtin_ctzll(free) + 1u;
+    const uint32_t base = ovf_base(Vu, u), words = ovf_words(deg, mb);
+    const int lane = lane_id();
+    for (uint32_t wb = 0; wb < words; wb += kWave) {
+        const uint32_t wi = wb + (uint32_t)lane;
+        const uint32_t w = wi < words ? load_word(&ovf[base + wi]) : 0xFFFFFFFFu;
+        const uint64_t b = __ballot(w != 0xFFFFFFFFu);
+        if (b) {
+            const int fl = __ffsll((long long)b) - 1;
+            const uint32_t wf = __shfl(w, fl);
+            return (uint32_t)mb + 1u + 32u * (wb + (uint32_t)fl) + (uint32_t)__builtin_ctz(~wf);
+        }
+    }
+    return 0u; // unreachable for a consistent state
+}
+
+// Record colour c > mb in successor v's overflow words (colours past v's
+// degree + 1 can never be v's mex and are dropped).
+__device__ __forceinline__ void or_overflow(const uint32_t *__restrict__ V, uint32_t *ovf,
+                                            uint32_t v, uint32_t c, int mb)
+{
+    const uint32_t Vv = V[v], b = c - (uint32_t)mb - 1u;
+    if (b < 32u * ovf_words(V[v + 1] - Vv, mb))
+        atomicOr(&ovf[ovf_base(Vv, v) + b / 32u], 1u << (b % 32u));
 }
 
 // Wave-wide: lanes with `want` append items (v, 0..cnt-1) to q. All lanes call.
@@ -67,25 +133,21 @@ __device__ __forceinline__ void push_items(bool want, uint32_t v, uint32_t cnt, 
         q[start + k] = make_uint2(v, k);
 }
 
-// Zero colours, colour sets, hub counters and the control block.
-__global__ __launch_bounds__(kBlock) void jp_clear(uint32_t N, uint64_t nwords, int *__restrict__ color,
-                                                   uint64_t *__restrict__ mask64,
-                                                   uint32_t *__restrict__ ovf,
+// Zero the overflow words, the hubs' counters and per-hub positions, and the
+// control block. (Every other vertex's state is written by the orientation
+// pass.) A plain kernel, so the sequence can be captured in a graph.
+__global__ __launch_bounds__(kBlock) void jp_clear(uint64_t nwords, uint32_t *__restrict__ ovf,
                                                    const uint32_t *__restrict__ hubs, uint32_t nhub,
-                                                   uint32_t *__restrict__ count,
+                                                   uint64_t *__restrict__ state,
                                                    uint32_t *__restrict__ npred,
                                                    uint32_t *__restrict__ hpos, gcxk_ctl *ctl)
 {
     const uint64_t stride = (uint64_t)gridDim.x * kBlock;
     const uint64_t t0 = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    for (uint64_t i = t0; i < N; i += stride) {
-        color[i] = 0;
-        mask64[i] = 0ull;
-    }
     for (uint64_t i = t0; i < nwords; i += stride)
         ovf[i] = 0u;
     for (uint64_t h = t0; h < nhub; h += stride) {
-        count[hubs[h]] = 0u;
+        state[hubs[h]] = 0ull;
         npred[hubs[h]] = 0u;
         hpos[2 * h] = 0u;
         hpos[2 * h + 1] = 0u;
@@ -94,14 +156,16 @@ __global__ __launch_bounds__(kBlock) void jp_clear(uint32_t N, uint64_t nwords, 
         reinterpret_cast<uint32_t *>(ctl)[t0] = 0u;
 }
 
+// Orientation of non-hub rows, 16 lanes per vertex. Also writes every
+// vertex's colour slot (1 for degree 0, else 0) and, for non-hubs, the state.
 template <typename K>
 __global__ __launch_bounds__(kBlock) void jp_init_groups(uint32_t N, const uint32_t *__restrict__ V,
                                                          const uint32_t *__restrict__ E,
                                                          const K *__restrict__ key,
-                                                         uint32_t *__restrict__ count,
                                                          uint32_t *__restrict__ npred,
                                                          uint32_t *__restrict__ esplit,
-                                                         uint2 *__restrict__ q0, gcxk_ctl *ctl)
+                                                         uint64_t *__restrict__ state,
+                                                         int *__restrict__ color, int mb)
 {
     const uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     const uint32_t u = (uint32_t)(t / kGroup);
@@ -115,54 +179,61 @@ __global__ __launch_bounds__(kBlock) void jp_init_groups(uint32_t N, const uint3
     }
     const bool hub = (end - beg) >= GCXK_HUB_DEG;
     uint32_t np = 0, ns = 0;
-    if (active && !hub) {
+    if (active && !hub && end > beg) {
         const K ku = key[u];
         const uint32_t below = (1u << gl) - 1u;
-        for (uint32_t i0 = beg; i0 < end; i0 += kGroup) {
-            const uint32_t i = i0 + gl;
-            const bool in = i < end;
-            bool pr = false;
-            uint32_t v = 0;
-            if (in) {
-                v = E[i];
-                pr = precedes(key[v], v, ku, u);
+        for (uint32_t i0 = beg; i0 < end; i0 += 2 * kGroup) {
+            uint32_t v[2];
+            bool in[2], pr[2];
+#pragma unroll
+            for (int s = 0; s < 2; s++) {
+                const uint32_t i = i0 + kGroup * s + gl;
+                in[s] = i < end;
+                v[s] = in[s] ? E[i] : 0u;
             }
-            const uint32_t gp = (uint32_t)(__ballot(in && pr) >> (kGroup * gw)) & 0xFFFFu;
-            const uint32_t gs = (uint32_t)(__ballot(in && !pr) >> (kGroup * gw)) & 0xFFFFu;
-            if (in) {
-                if (pr)
-                    esplit[beg + np + __popc(gp & below)] = v;
-                else
-                    esplit[end - 1u - ns - __popc(gs & below)] = v;
+#pragma unroll
+            for (int s = 0; s < 2; s++)
+                pr[s] = in[s] && precedes(key[v[s]], v[s], ku, u);
+#pragma unroll
+            for (int s = 0; s < 2; s++) {
+                const uint32_t gp = (uint32_t)(__ballot(pr[s]) >> (kGroup * gw)) & 0xFFFFu;
+                const uint32_t gs = (uint32_t)(__ballot(in[s] && !pr[s]) >> (kGroup * gw)) & 0xFFFFu;
+                if (in[s]) {
+                    if (pr[s])
+                        esplit[beg + np + __popc(gp & below)] = v[s];
+                    else
+                        esplit[end - 1u - ns - __popc(gs & below)] = v[s];
+                }
+                np += __popc(gp);
+                ns += __popc(gs);
             }
-            np += __popc(gp);
-            ns += __popc(gs);
         }
     }
-    const bool lead = active && !hub && gl == 0;
-    if (lead) {
-        count[u] = np;
-        npred[u] = np;
+    if (active && gl == 0) {
+        color[u] = end == beg ? 1 : 0;
+        if (!hub) {
+            npred[u] = np;
+            state[u] = (uint64_t)np << mb;
+        }
     }
-    push_items(lead && np == 0, u, n_items(end - beg - np), q0, &ctl->qn[0]);
 }
 
 // Hubs: chunk b of hub h covers edges [V[u] + b*kHubEdges, ...); positions in
 // E_split come from per-hub counters (predecessors from the front,
 // successors from the back); the order inside each part is irrelevant.
-constexpr uint32_t kHubEdges = 4096;
+constexpr uint32_t kHubEdges = GCXK_HUB_CHUNK_EDGES;
 
 template <typename K>
 __global__ __launch_bounds__(kBlock) void jp_init_hub_chunks(const uint32_t *__restrict__ V,
                                                              const uint32_t *__restrict__ E,
                                                              const K *__restrict__ key,
-                                                             uint32_t *__restrict__ count,
                                                              uint32_t *__restrict__ npred,
+                                                             uint64_t *__restrict__ state,
                                                              uint32_t *__restrict__ esplit,
                                                              const uint32_t *__restrict__ hubs,
                                                              const uint32_t *__restrict__ hchunk,
                                                              uint32_t nhub,
-                                                             uint32_t *__restrict__ hpos)
+                                                             uint32_t *__restrict__ hpos, int mb)
 {
     // hub of this block: last h with hchunk[h] <= blockIdx.x
     uint32_t lo = 0, hi = nhub;
@@ -208,38 +279,65 @@ __global__ __launch_bounds__(kBlock) void jp_init_hub_chunks(const uint32_t *__r
         npb += (uint32_t)__popcll(bp);
     }
     if (lane == 0 && npb) {
-        atomicAdd(&count[u], npb);
         atomicAdd(&npred[u], npb);
+        atomicAdd((unsigned long long *)&state[u], (unsigned long long)npb << mb);
     }
 }
 
-__global__ __launch_bounds__(kBlock) void jp_hub_roots(const uint32_t *__restrict__ V,
-                                                       const uint32_t *__restrict__ npred,
-                                                       const uint32_t *__restrict__ hubs,
-                                                       uint32_t nhub, uint2 *__restrict__ q0,
-                                                       gcxk_ctl *ctl)
+// ------------------------------------------------------------ round schedule
+
+// Round-0 queue: every vertex with successors and no predecessors. One queue
+// reservation per workgroup.
+__global__ __launch_bounds__(kRootBlock) void jp_collect_roots(uint32_t N,
+                                                               const uint32_t *__restrict__ V,
+                                                               const uint32_t *__restrict__ npred,
+                                                               uint2 *__restrict__ q0,
+                                                               gcxk_ctl *ctl)
 {
-    const uint32_t h = blockIdx.x * kBlock + threadIdx.x;
-    bool root = false;
-    uint32_t u = 0, cnt = 0;
-    if (h < nhub) {
-        u = hubs[h];
-        root = npred[u] == 0;
-        cnt = n_items(V[u + 1] - V[u] - npred[u]);
+    __shared__ uint32_t wsum[kRootBlock / kWave];
+    __shared__ uint32_t bbase;
+    const uint32_t u = blockIdx.x * kRootBlock + threadIdx.x;
+    const int lane = lane_id(), w = threadIdx.x / kWave;
+    uint32_t cnt = 0;
+    if (u < N) {
+        const uint32_t d = V[u + 1] - V[u], np = npred[u];
+        if (d > 0 && np == 0)
+            cnt = n_items(d);
     }
-    push_items(root, u, cnt, q0, &ctl->qn[0]);
+    uint32_t incl = cnt;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const uint32_t y = __shfl_up(incl, off);
+        if (lane >= off)
+            incl += y;
+    }
+    if (lane == kWave - 1)
+        wsum[w] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (int i = 0; i < kRootBlock / kWave; i++) {
+            const uint32_t x = wsum[i];
+            wsum[i] = tot;
+            tot += x;
+        }
+        bbase = tot ? atomicAdd(&ctl->qn[0], tot) : 0u;
+    }
+    __syncthreads();
+    const uint32_t start = bbase + wsum[w] + incl - cnt;
+    for (uint32_t k = 0; k < cnt; k++)
+        q0[start + k] = make_uint2(u, k);
 }
 
 __global__ __launch_bounds__(kBlock) void jp_round_kernel(const uint32_t *__restrict__ V,
                                                           const uint32_t *__restrict__ npred,
                                                           const uint32_t *__restrict__ esplit,
-                                                          uint32_t *__restrict__ count,
+                                                          uint64_t *__restrict__ state,
                                                           int *__restrict__ color,
-                                                          uint64_t *__restrict__ mask64,
                                                           uint32_t *__restrict__ ovf,
                                                           const uint2 *__restrict__ qin,
                                                           uint2 *__restrict__ qout, gcxk_ctl *ctl,
-                                                          uint32_t r)
+                                                          uint32_t r, int mb)
 {
     const uint32_t si = r % 3, so = (r + 1) % 3, sr = (r + 2) % 3;
     const uint32_t n = ctl->qn[si];
@@ -254,6 +352,7 @@ __global__ __launch_bounds__(kBlock) void jp_round_kernel(const uint32_t *__rest
     if (n == 0)
         return;
     const int lane = lane_id();
+    const uint64_t one = 1ull << mb;
     const uint32_t w0 = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
     const uint32_t nw = gridDim.x * kWavesPerBlock;
     for (uint32_t idx = w0; idx < n; idx += nw) {
@@ -261,41 +360,26 @@ __global__ __launch_bounds__(kBlock) void jp_round_kernel(const uint32_t *__rest
         const uint32_t u = it.x, k = it.y;
         const uint32_t Vu = V[u], Vu1 = V[u + 1];
         const uint32_t np = npred[u];
-        const uint64_t m = mask64[u];
-        uint32_t c;
-        if (m != ~0ull) {
-            c = (uint32_t)__builtin_ctzll(~m) + 1u;
-        } else {
-            // mex <= npred + 1 <= deg + 1, so a free bit lies inside the region
-            const uint32_t base = ovf_base(Vu, u), words = ovf_words(Vu1 - Vu);
-            c = 0;
-            for (uint32_t wb = 0; wb < words; wb += kWave) {
-                const uint32_t wi = wb + lane;
-                const uint32_t w = wi < words ? ovf[base + wi] : 0xFFFFFFFFu;
-                const uint64_t b = __ballot(w != 0xFFFFFFFFu);
-                if (b) {
-                    const int fl = __ffsll((long long)b) - 1;
-                    const uint32_t wf = __shfl(w, fl);
-                    c = 65u + 32u * (wb + (uint32_t)fl) + (uint32_t)__builtin_ctz(~wf);
-                    break;
-                }
-            }
-        }
+        // u's count reached zero in an earlier launch: a plain load is current
+        const uint32_t c = mex_of(state[u], mb, u, Vu, Vu1 - Vu, ovf);
         if (k == 0 && lane == 0)
             color[u] = (int)c;
         const uint32_t i = Vu + np + k * kChunk + (uint32_t)lane;
+        const bool has = i < Vu1;
+        const uint32_t v = has ? esplit[i] : 0u;
         bool rel = false;
-        uint32_t v = 0;
-        if (i < Vu1) {
-            v = esplit[i];
-            if (c <= 64u) {
-                atomicOr((unsigned long long *)&mask64[v], 1ull << (c - 1u));
-            } else {
-                const uint32_t Vv = V[v], b = c - 65u;
-                if (b < 32u * ovf_words(V[v + 1] - Vv))
-                    atomicOr(&ovf[ovf_base(Vv, v) + b / 32u], 1u << (b % 32u));
+        if (c <= (uint32_t)mb) {
+            if (has) {
+                atomicOr((unsigned long long *)&state[v], 1ull << (c - 1u));
+                const uint64_t old = atomicAdd((unsigned long long *)&state[v], 0ull - one);
+                rel = (old >> mb) == 1ull;
             }
-            rel = atomicSub(&count[v], 1u) == 1u;
+        } else {
+            if (has)
+                or_overflow(V, ovf, v, c, mb);
+            __builtin_amdgcn_s_waitcnt(0);
+            if (has)
+                rel = (atomicAdd((unsigned long long *)&state[v], 0ull - one) >> mb) == 1ull;
         }
         uint32_t cnt = 0;
         if (rel)
@@ -304,18 +388,123 @@ __global__ __launch_bounds__(kBlock) void jp_round_kernel(const uint32_t *__rest
     }
 }
 
-// ---------------------------------------------------------------- validate
+// ----------------------------------------------------------- ordered schedule
 
+// Sort input: complemented keys (so an ascending stable sort yields p
+// descending, id ascending) and the ids.
+template <typename K>
+__global__ __launch_bounds__(kBlock) void jp_order_keys(uint32_t N, const K *__restrict__ key,
+                                                        K *__restrict__ nkey,
+                                                        uint32_t *__restrict__ ids)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < N) {
+        nkey[i] = ~key[i];
+        ids[i] = i;
+    }
+}
+
+// Items per sorted position (0 for degree-0 vertices).
+__global__ __launch_bounds__(kBlock) void jp_item_counts(uint32_t N, const uint32_t *__restrict__ V,
+                                                         const uint32_t *__restrict__ npred,
+                                                         const uint32_t *__restrict__ ids,
+                                                         uint32_t *__restrict__ cnt)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < N) {
+        const uint32_t u = ids[i];
+        const uint32_t d = V[u + 1] - V[u];
+        cnt[i] = d ? n_items(d - npred[u]) : 0u;
+    }
+}
+
+// Items in greedy order; incl = inclusive prefix of the counts.
+__global__ __launch_bounds__(kBlock) void jp_expand(uint32_t N, const uint32_t *__restrict__ ids,
+                                                    const uint32_t *__restrict__ incl,
+                                                    uint2 *__restrict__ items, gcxk_ctl *ctl)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= N)
+        return;
+    const uint32_t end = incl[i], beg = i ? incl[i - 1] : 0u;
+    const uint32_t u = ids[i];
+    for (uint32_t j = beg; j < end; j++)
+        items[j] = make_uint2(u, j - beg);
+    if (i == N - 1)
+        ctl->total_items = end;
+}
+
+__global__ __launch_bounds__(kBlock) void jp_ordered_kernel(const uint32_t *__restrict__ V,
+                                                            const uint32_t *__restrict__ npred,
+                                                            const uint32_t *__restrict__ esplit,
+                                                            uint64_t *__restrict__ state,
+                                                            int *__restrict__ color,
+                                                            uint32_t *__restrict__ ovf,
+                                                            const uint2 *__restrict__ items,
+                                                            gcxk_ctl *ctl, int mb,
+                                                            uint64_t spin_ticks)
+{
+    const uint32_t n = ctl->total_items;
+    const int lane = lane_id();
+    const uint64_t one = 1ull << mb;
+    const uint32_t w0 = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+    const uint32_t nw = gridDim.x * kWavesPerBlock;
+    for (uint32_t j = w0; j < n; j += nw) {
+        const uint2 it = items[j];
+        const uint32_t u = it.x, k = it.y;
+        uint64_t s = load_state(&state[u]);
+        const uint32_t Vu = V[u], Vu1 = V[u + 1];
+        const uint32_t np = npred[u];
+        const uint32_t i = Vu + np + k * kChunk + (uint32_t)lane;
+        const bool has = i < Vu1;
+        const uint32_t v = has ? esplit[i] : 0u;
+        if (s >= one) {
+            const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+            uint32_t polls = 0;
+            do {
+                __builtin_amdgcn_s_sleep(1);
+                s = load_state(&state[u]);
+                if ((++polls & 255u) == 0u &&
+                    (__builtin_amdgcn_s_memrealtime() - t0 > spin_ticks || load_word(&ctl->err))) {
+                    if (lane == 0)
+                        __hip_atomic_store(&ctl->err, 1u, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                    return;
+                }
+            } while (s >= one);
+        }
+        const uint32_t c = mex_of(s, mb, u, Vu, Vu1 - Vu, ovf);
+        if (k == 0 && lane == 0)
+            color[u] = (int)c;
+        if (c <= (uint32_t)mb) {
+            if (has) {
+                atomicOr((unsigned long long *)&state[v], 1ull << (c - 1u));
+                atomicAdd((unsigned long long *)&state[v], 0ull - one);
+            }
+        } else {
+            if (has)
+                or_overflow(V, ovf, v, c, mb);
+            __builtin_amdgcn_s_waitcnt(0);
+            if (has)
+                atomicAdd((unsigned long long *)&state[v], 0ull - one);
+        }
+    }
+}
+
+// ------------------------------------------------------------- validation
+
+// Grid-stride over vertices, 16 lanes each; one atomic per workgroup.
 __global__ __launch_bounds__(kBlock) void validate_groups(uint32_t N, const uint32_t *__restrict__ V,
                                                           const uint32_t *__restrict__ E,
                                                           const int *__restrict__ color,
                                                           gcxk_ctl *ctl)
 {
-    const uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    const uint32_t u = (uint32_t)(t / kGroup);
-    const int gl = (int)(t % kGroup);
+    __shared__ uint32_t sbad[kWavesPerBlock], stop[kWavesPerBlock];
+    const int gl = threadIdx.x % kGroup;
+    const uint32_t gpb = kBlock / kGroup;
     uint32_t bad = 0, top = 0;
-    if (u < N) {
+    for (uint64_t u = (uint64_t)blockIdx.x * gpb + threadIdx.x / kGroup; u < N;
+         u += (uint64_t)gridDim.x * gpb) {
         const uint32_t beg = V[u], end = V[u + 1];
         const int cu = color[u];
         if (end - beg < GCXK_HUB_DEG) {
@@ -324,15 +513,27 @@ __global__ __launch_bounds__(kBlock) void validate_groups(uint32_t N, const uint
         }
         if (gl == 0) {
             bad += cu <= 0 ? 1u : 0u;
-            top = cu > 0 ? (uint32_t)cu : 0u;
+            top = cu > 0 && (uint32_t)cu > top ? (uint32_t)cu : top;
         }
     }
     bad = group_sum32<kWave>(bad);
     top = group_max32<kWave>(top);
+    const int w = threadIdx.x / kWave;
     if (lane_id() == 0) {
-        if (bad)
-            atomicAdd(&ctl->bad, bad);
-        atomicMax(&ctl->maxcolor, (int)top);
+        sbad[w] = bad;
+        stop[w] = top;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t b = 0, t = 0;
+        for (int i = 0; i < kWavesPerBlock; i++) {
+            b += sbad[i];
+            t = stop[i] > t ? stop[i] : t;
+        }
+        if (b)
+            atomicAdd(&ctl->bad, b);
+        if (t)
+            atomicMax(&ctl->maxcolor, (int)t);
     }
 }
 
@@ -355,24 +556,66 @@ __global__ __launch_bounds__(kBlock) void validate_hubs(const uint32_t *__restri
 
 // ------------------------------------------------------------- certificate
 
+// mex over the predecessors' colours, one 64-colour window per pass. 16 lanes
+// per non-hub vertex; hubs take a whole workgroup each.
 template <typename K>
-__global__ __launch_bounds__(kBlock) void mex_check_kernel(uint32_t N, const uint32_t *__restrict__ V,
+__global__ __launch_bounds__(kBlock) void mex_check_groups(uint32_t N, const uint32_t *__restrict__ V,
                                                            const uint32_t *__restrict__ E,
                                                            const K *__restrict__ key,
                                                            const int *__restrict__ color,
                                                            gcxk_ctl *ctl)
 {
-    const uint32_t u = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
-    if (u >= N)
+    const uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t u = (uint32_t)(t / kGroup);
+    const int gl = (int)(t % kGroup);
+    if (u >= N) // whole groups leave together
         return;
-    const int lane = lane_id();
     const uint32_t beg = V[u], end = V[u + 1];
+    if (end - beg >= GCXK_HUB_DEG)
+        return;
     const K ku = key[u];
     uint32_t base = 0;
     uint64_t mask;
     for (;;) {
         mask = 0;
-        for (uint32_t i = beg + lane; i < end; i += kWave) {
+        for (uint32_t i = beg + gl; i < end; i += kGroup) {
+            const uint32_t v = E[i];
+            if (precedes(key[v], v, ku, u)) {
+                const uint32_t c = (uint32_t)color[v];
+                if (c > base && c <= base + 64)
+                    mask |= 1ull << (c - base - 1);
+            }
+        }
+        mask = group_or64<kGroup>(mask);
+        if (mask != ~0ull)
+            break;
+        base += 64;
+    }
+    const uint32_t mex = base + (uint32_t)__builtin_ctzll(~mask) + 1u;
+    if (gl == 0 && (uint32_t)color[u] != mex)
+        atomicAdd((unsigned long long *)&ctl->viol, 1ull);
+}
+
+template <typename K>
+__global__ __launch_bounds__(kBlock) void mex_check_hubs(const uint32_t *__restrict__ V,
+                                                         const uint32_t *__restrict__ E,
+                                                         const K *__restrict__ key,
+                                                         const int *__restrict__ color,
+                                                         const uint32_t *__restrict__ hubs,
+                                                         gcxk_ctl *ctl)
+{
+    __shared__ unsigned long long smask;
+    const uint32_t u = hubs[blockIdx.x];
+    const uint32_t beg = V[u], end = V[u + 1];
+    const K ku = key[u];
+    uint32_t base = 0;
+    uint64_t mask;
+    for (;;) {
+        if (threadIdx.x == 0)
+            smask = 0ull;
+        __syncthreads();
+        mask = 0;
+        for (uint32_t i = beg + threadIdx.x; i < end; i += kBlock) {
             const uint32_t v = E[i];
             if (precedes(key[v], v, ku, u)) {
                 const uint32_t c = (uint32_t)color[v];
@@ -381,75 +624,220 @@ __global__ __launch_bounds__(kBlock) void mex_check_kernel(uint32_t N, const uin
             }
         }
         mask = group_or64<kWave>(mask);
+        if (lane_id() == 0 && mask)
+            atomicOr(&smask, (unsigned long long)mask);
+        __syncthreads();
+        mask = smask;
+        __syncthreads();
         if (mask != ~0ull)
             break;
         base += 64;
     }
     const uint32_t mex = base + (uint32_t)__builtin_ctzll(~mask) + 1u;
-    if (lane == 0 && (uint32_t)color[u] != mex)
+    if (threadIdx.x == 0 && (uint32_t)color[u] != mex)
         atomicAdd((unsigned long long *)&ctl->viol, 1ull);
+}
+
+// Workgroups of the ordered kernel that fit on the device at once.
+uint32_t g_ordered_grid = 0;
+
+// Byte offset of the sort / scan temporary storage inside the scratch:
+// after two key arrays and two id arrays, rounded up to 256 bytes.
+size_t order_tmp_offset(uint32_t N, size_t key_size)
+{
+    return (2 * key_size * N + 8ull * N + 255) & ~(size_t)255;
+}
+
+template <typename K>
+int sort_order(uint32_t N, const void *key, void *scratch, size_t scratch_bytes, uint32_t **ids,
+               hipStream_t s)
+{
+    // scratch: nkey[2][N] (K), id[2][N], then rocprim temporary storage
+    char *p = static_cast<char *>(scratch);
+    K *k0 = reinterpret_cast<K *>(p);
+    K *k1 = k0 + N;
+    uint32_t *i0 = reinterpret_cast<uint32_t *>(k1 + N);
+    uint32_t *i1 = i0 + N;
+    const size_t used = order_tmp_offset(N, sizeof(K));
+    char *tmp = p + used;
+    if (used > scratch_bytes)
+        return GCX_ERR_CAPACITY;
+    size_t tmp_bytes = scratch_bytes - used;
+    jp_order_keys<K><<<div_up(N, kBlock), kBlock, 0, s>>>(N, static_cast<const K *>(key), k0, i0);
+    GCXK_LAUNCH_CHECK();
+    rocprim::double_buffer<K> kb(k0, k1);
+    rocprim::double_buffer<uint32_t> vb(i0, i1);
+    size_t need = 0;
+    GCXK_CHECK(rocprim::radix_sort_pairs(nullptr, need, kb, vb, N, 0, (unsigned)(8 * sizeof(K)), s));
+    if (need > tmp_bytes)
+        return GCX_ERR_CAPACITY;
+    GCXK_CHECK(rocprim::radix_sort_pairs(tmp, need, kb, vb, N, 0, (unsigned)(8 * sizeof(K)), s));
+    *ids = vb.current();
+    return GCX_OK;
 }
 
 } // namespace
 
 extern "C" uint32_t gcxk_round_grid(void) { return 1024u; }
 
-extern "C" int gcxk_jp_clear(uint32_t N, uint64_t M, int *color, uint64_t *mask64, uint32_t *ovf,
-                             const uint32_t *hubs, uint32_t nhub, uint32_t *count,
-                             uint32_t *npred, uint32_t *hpos, gcxk_ctl *ctl, void *stream)
+extern "C" int gcxk_jp_mask_bits(uint32_t maxdeg)
+{
+    const int cb = maxdeg ? 32 - __builtin_clz(maxdeg) : 1;
+    return 64 - cb;
+}
+
+extern "C" int gcxk_jp_clear(uint32_t N, uint64_t M, uint32_t *ovf, const uint32_t *hubs,
+                             uint32_t nhub, uint64_t *state, uint32_t *npred, uint32_t *hpos,
+                             gcxk_ctl *ctl, void *stream)
 {
     const uint64_t nwords = gcxk_jp_ovf_words(N, M);
-    uint64_t n = nwords > N ? nwords : N;
+    uint64_t n = nwords > nhub ? nwords : nhub;
     if (n < 64)
         n = 64;
     uint32_t grid = div_up(n, kBlock);
     if (grid > 4096)
         grid = 4096;
-    jp_clear<<<grid, kBlock, 0, as_stream(stream)>>>(N, nwords, color, mask64, ovf, hubs, nhub,
-                                                     count, npred, hpos, ctl);
+    jp_clear<<<grid, kBlock, 0, as_stream(stream)>>>(nwords, ovf, hubs, nhub, state, npred, hpos,
+                                                     ctl);
     GCXK_LAUNCH_CHECK();
     return GCX_OK;
 }
 
 extern "C" int gcxk_jp_init(uint32_t N, const uint32_t *dV, const uint32_t *dE, const void *key,
-                            int key_bits, uint32_t *count, uint32_t *npred, uint32_t *esplit,
-                            gcxk_item *q0i, const uint32_t *hubs, uint32_t nhub,
+                            int key_bits, int mb, uint32_t *npred, uint32_t *esplit,
+                            uint64_t *state, int *color, const uint32_t *hubs, uint32_t nhub,
                             const uint32_t *hchunk, uint32_t nhchunk, uint32_t *hpos,
-                            gcxk_ctl *ctl, void *stream)
+                            void *stream)
 {
     if (N == 0)
         return GCX_OK;
-    uint2 *q0 = reinterpret_cast<uint2 *>(q0i);
     hipStream_t s = as_stream(stream);
     const uint32_t grid = div_up((uint64_t)N * kGroup, kBlock);
     if (key_bits == 32) {
-        jp_init_groups<uint32_t><<<grid, kBlock, 0, s>>>(N, dV, dE, (const uint32_t *)key, count,
-                                                         npred, esplit, q0, ctl);
+        jp_init_groups<uint32_t><<<grid, kBlock, 0, s>>>(N, dV, dE, (const uint32_t *)key, npred,
+                                                         esplit, state, color, mb);
         if (nhub)
             jp_init_hub_chunks<uint32_t><<<nhchunk, kBlock, 0, s>>>(
-                dV, dE, (const uint32_t *)key, count, npred, esplit, hubs, hchunk, nhub, hpos);
+                dV, dE, (const uint32_t *)key, npred, state, esplit, hubs, hchunk, nhub, hpos, mb);
     } else {
-        jp_init_groups<uint64_t><<<grid, kBlock, 0, s>>>(N, dV, dE, (const uint64_t *)key, count,
-                                                         npred, esplit, q0, ctl);
+        jp_init_groups<uint64_t><<<grid, kBlock, 0, s>>>(N, dV, dE, (const uint64_t *)key, npred,
+                                                         esplit, state, color, mb);
         if (nhub)
             jp_init_hub_chunks<uint64_t><<<nhchunk, kBlock, 0, s>>>(
-                dV, dE, (const uint64_t *)key, count, npred, esplit, hubs, hchunk, nhub, hpos);
+                dV, dE, (const uint64_t *)key, npred, state, esplit, hubs, hchunk, nhub, hpos, mb);
     }
-    if (nhub)
-        jp_hub_roots<<<div_up(nhub, kBlock), kBlock, 0, s>>>(dV, npred, hubs, nhub, q0, ctl);
+    GCXK_LAUNCH_CHECK();
+    return GCX_OK;
+}
+
+extern "C" int gcxk_jp_roots(uint32_t N, const uint32_t *dV, const uint32_t *npred, gcxk_item *q0,
+                             gcxk_ctl *ctl, void *stream)
+{
+    if (N == 0)
+        return GCX_OK;
+    jp_collect_roots<<<div_up(N, kRootBlock), kRootBlock, 0, as_stream(stream)>>>(
+        N, dV, npred, reinterpret_cast<uint2 *>(q0), ctl);
     GCXK_LAUNCH_CHECK();
     return GCX_OK;
 }
 
 extern "C" int gcxk_jp_round(const uint32_t *dV, const uint32_t *npred, const uint32_t *esplit,
-                             uint32_t *count, int *color, uint64_t *mask64, uint32_t *ovf,
+                             uint64_t *state, int *color, uint32_t *ovf, int mb,
                              gcxk_item *const q[2], gcxk_ctl *ctl, uint32_t r, uint32_t grid,
                              void *stream)
 {
     jp_round_kernel<<<grid, kBlock, 0, as_stream(stream)>>>(
-        dV, npred, esplit, count, color, mask64, ovf, reinterpret_cast<const uint2 *>(q[r & 1]),
-        reinterpret_cast<uint2 *>(q[(r + 1) & 1]), ctl, r);
+        dV, npred, esplit, state, color, ovf, reinterpret_cast<const uint2 *>(q[r & 1]),
+        reinterpret_cast<uint2 *>(q[(r + 1) & 1]), ctl, r, mb);
     GCXK_LAUNCH_CHECK();
+    return GCX_OK;
+}
+
+extern "C" size_t gcxk_jp_order_scratch(uint32_t N, int key_bits)
+{
+    const size_t ks = key_bits == 32 ? 4 : 8;
+    size_t need = 0;
+    if (N) {
+        if (key_bits == 32) {
+            rocprim::double_buffer<uint32_t> kb(nullptr, nullptr);
+            rocprim::double_buffer<uint32_t> vb(nullptr, nullptr);
+            if (rocprim::radix_sort_pairs(nullptr, need, kb, vb, N, 0, 32u) != hipSuccess)
+                return 0;
+        } else {
+            rocprim::double_buffer<uint64_t> kb(nullptr, nullptr);
+            rocprim::double_buffer<uint32_t> vb(nullptr, nullptr);
+            if (rocprim::radix_sort_pairs(nullptr, need, kb, vb, N, 0, 64u) != hipSuccess)
+                return 0;
+        }
+    }
+    size_t scan = 0;
+    if (N && rocprim::inclusive_scan(nullptr, scan, (const uint32_t *)nullptr, (uint32_t *)nullptr,
+                                     N, rocprim::plus<uint32_t>()) != hipSuccess)
+        return 0;
+    need = need > scan ? need : scan;
+    return order_tmp_offset(N, ks) + need;
+}
+
+extern "C" int gcxk_jp_order_items(uint32_t N, const uint32_t *dV, const uint32_t *npred,
+                                   const void *key, int key_bits, void *scratch,
+                                   size_t scratch_bytes, uint32_t *cnt, uint32_t *incl,
+                                   gcxk_item *items, gcxk_ctl *ctl, void *stream)
+{
+    if (N == 0)
+        return GCX_OK;
+    hipStream_t s = as_stream(stream);
+    uint32_t *ids = nullptr;
+    const int rc = key_bits == 32 ? sort_order<uint32_t>(N, key, scratch, scratch_bytes, &ids, s)
+                                  : sort_order<uint64_t>(N, key, scratch, scratch_bytes, &ids, s);
+    if (rc != GCX_OK)
+        return rc;
+    jp_item_counts<<<div_up(N, kBlock), kBlock, 0, s>>>(N, dV, npred, ids, cnt);
+    GCXK_LAUNCH_CHECK();
+    // the sort's temporary storage is free again: reuse it for the scan
+    const size_t ks = key_bits == 32 ? 4 : 8;
+    char *tmp = static_cast<char *>(scratch) + order_tmp_offset(N, ks);
+    size_t tmp_bytes = scratch_bytes - order_tmp_offset(N, ks), need = 0;
+    GCXK_CHECK(rocprim::inclusive_scan(nullptr, need, cnt, incl, N, rocprim::plus<uint32_t>(), s));
+    if (need > tmp_bytes)
+        return GCX_ERR_CAPACITY;
+    GCXK_CHECK(rocprim::inclusive_scan(tmp, need, cnt, incl, N, rocprim::plus<uint32_t>(), s));
+    jp_expand<<<div_up(N, kBlock), kBlock, 0, s>>>(N, ids, incl, reinterpret_cast<uint2 *>(items),
+                                                   ctl);
+    GCXK_LAUNCH_CHECK();
+    return GCX_OK;
+}
+
+extern "C" uint32_t gcxk_jp_ordered_grid(void)
+{
+    if (!g_ordered_grid) {
+        int per_cu = 0, dev = 0;
+        hipDeviceProp_t prop;
+        if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, jp_ordered_kernel, kBlock, 0) !=
+                hipSuccess ||
+            per_cu <= 0)
+            return 0;
+        g_ordered_grid = (uint32_t)per_cu * (uint32_t)prop.multiProcessorCount;
+    }
+    return g_ordered_grid;
+}
+
+extern "C" int gcxk_jp_ordered(const uint32_t *dV, const uint32_t *npred, const uint32_t *esplit,
+                               uint64_t *state, int *color, uint32_t *ovf, int mb,
+                               const gcxk_item *items, gcxk_ctl *ctl, uint32_t grid,
+                               uint64_t spin_ticks, void *stream)
+{
+    const uint32_t cap = gcxk_jp_ordered_grid();
+    if (!cap)
+        return GCX_ERR_HIP;
+    if (grid == 0 || grid > cap)
+        grid = cap;
+    const uint2 *it = reinterpret_cast<const uint2 *>(items);
+    void *args[] = {(void *)&dV,  (void *)&npred, (void *)&esplit, (void *)&state,
+                    (void *)&color, (void *)&ovf, (void *)&it,     (void *)&ctl,
+                    (void *)&mb,  (void *)&spin_ticks};
+    GCXK_CHECK(hipLaunchCooperativeKernel((const void *)jp_ordered_kernel, dim3(grid), dim3(kBlock),
+                                          args, 0, as_stream(stream)));
     return GCX_OK;
 }
 
@@ -458,8 +846,10 @@ extern "C" int gcxk_validate(uint32_t N, const uint32_t *dV, const uint32_t *dE,
 {
     if (N == 0)
         return GCX_OK;
-    validate_groups<<<div_up((uint64_t)N * kGroup, kBlock), kBlock, 0, as_stream(stream)>>>(
-        N, dV, dE, color, ctl);
+    uint32_t grid = div_up((uint64_t)N * kGroup, kBlock);
+    if (grid > 2048)
+        grid = 2048;
+    validate_groups<<<grid, kBlock, 0, as_stream(stream)>>>(N, dV, dE, color, ctl);
     if (nhub)
         validate_hubs<<<nhub, kBlock, 0, as_stream(stream)>>>(dV, dE, color, hubs, ctl);
     GCXK_LAUNCH_CHECK();
@@ -467,17 +857,26 @@ extern "C" int gcxk_validate(uint32_t N, const uint32_t *dV, const uint32_t *dE,
 }
 
 extern "C" int gcxk_mex_check(uint32_t N, const uint32_t *dV, const uint32_t *dE, const void *key,
-                              int key_bits, const int *color, gcxk_ctl *ctl, void *stream)
+                              int key_bits, const int *color, const uint32_t *hubs, uint32_t nhub,
+                              gcxk_ctl *ctl, void *stream)
 {
     if (N == 0)
         return GCX_OK;
-    const uint32_t grid = div_up(N, kWavesPerBlock);
-    if (key_bits == 32)
-        mex_check_kernel<uint32_t><<<grid, kBlock, 0, as_stream(stream)>>>(
-            N, dV, dE, (const uint32_t *)key, color, ctl);
-    else
-        mex_check_kernel<uint64_t><<<grid, kBlock, 0, as_stream(stream)>>>(
-            N, dV, dE, (const uint64_t *)key, color, ctl);
+    hipStream_t s = as_stream(stream);
+    const uint32_t grid = div_up((uint64_t)N * kGroup, kBlock);
+    if (key_bits == 32) {
+        mex_check_groups<uint32_t><<<grid, kBlock, 0, s>>>(N, dV, dE, (const uint32_t *)key, color,
+                                                           ctl);
+        if (nhub)
+            mex_check_hubs<uint32_t><<<nhub, kBlock, 0, s>>>(dV, dE, (const uint32_t *)key, color,
+                                                             hubs, ctl);
+    } else {
+        mex_check_groups<uint64_t><<<grid, kBlock, 0, s>>>(N, dV, dE, (const uint64_t *)key, color,
+                                                           ctl);
+        if (nhub)
+            mex_check_hubs<uint64_t><<<nhub, kBlock, 0, s>>>(dV, dE, (const uint64_t *)key, color,
+                                                             hubs, ctl);
+    }
     GCXK_LAUNCH_CHECK();
     return GCX_OK;
 }

@@ -94,7 +94,26 @@ __global__ __launch_bounds__(kBlock) void key64_to_doubles(uint32_t N, const uin
         p[i] = double_from_order_key(key[i]);
 }
 
+// Plain streaming copy, used to calibrate HBM byte counters.
+__global__ __launch_bounds__(kBlock) void copy_u32x4(uint64_t n4, const uint4 *__restrict__ src,
+                                                     uint4 *__restrict__ dst)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n4; i += stride)
+        dst[i] = src[i];
+}
+
 } // namespace
+
+extern "C" int gcxk_copy_bytes(uint64_t bytes, const void *src, void *dst, void *stream)
+{
+    const uint64_t n4 = bytes / 16u;
+    if (n4 == 0)
+        return GCX_OK;
+    copy_u32x4<<<8192, kBlock, 0, as_stream(stream)>>>(n4, (const uint4 *)src, (uint4 *)dst);
+    GCXK_LAUNCH_CHECK();
+    return GCX_OK;
+}
 
 #define ELEMENTWISE(kernel, ...)                                                \
     do {                                                                        \

@@ -23,6 +23,7 @@ OK, ERR_ARG, ERR_NOMEM, ERR_HIP, ERR_CAPACITY, ERR_IO, ERR_FORMAT, ERR_NODEV, ER
 
 FF, LF, LLF, SL, SLL, GNN = range(6)
 FP32, BF16 = 0, 1
+SCHED_ROUNDS, SCHED_ORDERED = 0, 1
 HEURISTICS = {"FF": FF, "LF": LF, "LLF": LLF, "SL": SL, "SLL": SLL}
 STAGES = ["keys", "sl", "gnn_in", "gnn_hid", "gnn_out", "jp_init", "jp_rounds"]
 
@@ -62,6 +63,7 @@ def _load():
         "gcx_graph_rmat": (C.c_int, [C.c_int, C.c_uint32, C.c_double, C.c_double, C.c_double,
                                      C.c_uint64, vp, C.POINTER(vp)]),
         "gcx_graph_er": (C.c_int, [C.c_uint32, C.c_uint64, C.c_uint64, vp, C.POINTER(vp)]),
+        "gcx_graph_rgg": (C.c_int, [C.c_uint32, C.c_double, C.c_uint64, vp, C.POINTER(vp)]),
         "gcx_graph_free": (C.c_int, [vp]),
         "gcx_graph_num_vertices": (C.c_uint32, [vp]),
         "gcx_graph_num_entries": (C.c_uint64, [vp]),
@@ -80,6 +82,7 @@ def _load():
         "gcx_set_priorities": (C.c_int, [vp, vp, f64p, vp]),
         "gcx_color": (C.c_int, [vp, vp, vp]),
         "gcx_recolor": (C.c_int, [vp, vp, C.c_int, vp]),
+        "gcx_set_schedule": (C.c_int, [vp, C.c_int, C.c_uint32]),
         "gcx_validate": (C.c_int, [vp, vp, vp, i32p]),
         "gcx_mex_check": (C.c_int, [vp, vp, vp, C.POINTER(C.c_uint64)]),
         "gcx_download_colors": (C.c_int, [vp, C.c_uint32, i32p, vp]),
@@ -87,7 +90,9 @@ def _load():
         "gcx_download_priorities": (C.c_int, [vp, C.c_uint32, f64p, vp]),
         "gcx_last_rounds": (C.c_int, [vp, u32p]),
         "gcx_last_launches": (C.c_int, [vp, u32p]),
+        "gcx_last_fallback": (C.c_int, [vp, u32p]),
         "gcx_profile_enable": (C.c_int, [C.c_int]),
+        "gcx_calibration_copy": (C.c_int, [C.c_uint64, C.POINTER(C.c_float)]),
         "gcx_profile_reset": (C.c_int, []),
         "gcx_profile_read": (C.c_int, [f64p, C.POINTER(C.c_uint64)]),
         "gcx_rmat_thresholds": (None, [C.c_double, C.c_double, C.c_double, u32p]),
@@ -267,6 +272,12 @@ class Graph:
         _chk(_lib.gcx_graph_er(n, m, seed, _sh(stream), C.byref(h)), "ER graph")
         return cls(h)
 
+    @classmethod
+    def rgg(cls, n, radius, seed=1, stream=None):
+        h = vp()
+        _chk(_lib.gcx_graph_rgg(n, radius, seed, _sh(stream), C.byref(h)), "RGG graph")
+        return cls(h)
+
     @property
     def N(self) -> int:
         return int(_lib.gcx_graph_num_vertices(self.handle))
@@ -376,6 +387,12 @@ class Workspace:
     def color(self, g: Graph, stream=None):
         _chk(_lib.gcx_color(self.handle, g.handle, _sh(stream)), "color")
 
+    def set_schedule(self, schedule, grid=0):
+        """JP schedule: SCHED_ROUNDS (launch per DAG level) or SCHED_ORDERED
+        (greedy-order items, one persistent launch of `grid` workgroups, 0 =
+        all that are resident at once)."""
+        _chk(_lib.gcx_set_schedule(self.handle, schedule, grid), "schedule")
+
     def recolor(self, g: Graph, iterations, stream=None):
         _chk(_lib.gcx_recolor(self.handle, g.handle, iterations, _sh(stream)), "recolor")
 
@@ -415,6 +432,12 @@ class Workspace:
         _chk(_lib.gcx_last_launches(self.handle, C.byref(r)), "launches")
         return int(r.value)
 
+    def fell_back(self) -> bool:
+        """True when the last colour() redid an ordered launch with rounds."""
+        r = C.c_uint32()
+        _chk(_lib.gcx_last_fallback(self.handle, C.byref(r)), "fallback")
+        return bool(r.value)
+
     def free(self):
         if self.handle:
             _lib.gcx_workspace_free(self.handle)
@@ -428,6 +451,13 @@ class Workspace:
 
 
 # ---------------------------------------------------------------- profiling
+
+def calibration_copy(nbytes):
+    """Run the streaming copy kernel once over `nbytes`; returns milliseconds."""
+    ms = C.c_float()
+    _chk(_lib.gcx_calibration_copy(nbytes, C.byref(ms)), "calibration copy")
+    return float(ms.value)
+
 
 def profile_enable(on=True):
     _chk(_lib.gcx_profile_enable(1 if on else 0), "profile")

@@ -81,6 +81,9 @@ int gcx_graph_from_pairs(uint32_t N, uint64_t L, const uint32_t *I, const uint32
 int gcx_graph_rmat(int scale, uint32_t edge_factor, double a, double b, double c,
                    uint64_t seed, void *stream, gcx_graph **out);
 int gcx_graph_er(uint32_t n, uint64_t m, uint64_t seed, void *stream, gcx_graph **out);
+/* 2D random geometric graph: n points in the unit square, an edge for every
+ * pair closer than `radius` (same pairs as gcx_gen_rgg2d_pairs). */
+int gcx_graph_rgg(uint32_t n, double radius, uint64_t seed, void *stream, gcx_graph **out);
 int gcx_graph_free(gcx_graph *g);
 uint32_t gcx_graph_num_vertices(const gcx_graph *g);
 uint64_t gcx_graph_num_entries(const gcx_graph *g);
@@ -111,6 +114,18 @@ int gcx_set_priorities(gcx_workspace *ws, const gcx_graph *g, const double *p, v
 /* Jones-Plassmann colouring with the workspace's priorities (p descending,
  * id ascending); colours 1..C land in the workspace. */
 int gcx_color(gcx_workspace *ws, const gcx_graph *g, void *stream);
+/* JP schedule; both give the same colours.
+ *   ROUNDS:  one launch per DAG level over a compacted frontier.
+ *   ORDERED: work items sorted in greedy order, then one persistent
+ *            (cooperative) launch of `param` workgroups (0 = as many as are
+ *            resident at once) in which each wave takes a fixed stride of
+ *            the items and waits on their predecessor counts. Only one
+ *            ordered launch runs on the device at a time. If a wait exceeds
+ *            its bound the call redoes the colouring with ROUNDS. */
+enum gcx_schedule { GCX_SCHED_ROUNDS = 0, GCX_SCHED_ORDERED = 1 };
+int gcx_set_schedule(gcx_workspace *ws, int schedule, uint32_t param);
+/* 1 when the last gcx_color fell back from ORDERED to ROUNDS. */
+int gcx_last_fallback(const gcx_workspace *ws, uint32_t *fell_back);
 /* Culberson iterated greedy: `iterations` more passes, each with p = colour. */
 int gcx_recolor(gcx_workspace *ws, const gcx_graph *g, int iterations, void *stream);
 /* Proper-colouring check of the workspace colours: *max_color = C or 0. */
@@ -124,9 +139,11 @@ int gcx_download_colors(const gcx_workspace *ws, uint32_t N, int *color, void *s
 int gcx_upload_colors(gcx_workspace *ws, uint32_t N, const int *color, void *stream);
 /* Priorities as doubles (the reference's p). */
 int gcx_download_priorities(const gcx_workspace *ws, uint32_t N, double *p, void *stream);
-/* Number of JP rounds (DAG depth) of the last gcx_color call. */
+/* Number of JP rounds (DAG depth) of the last gcx_color call under ROUNDS
+ * (0 under ORDERED, which has no rounds). */
 int gcx_last_rounds(const gcx_workspace *ws, uint32_t *rounds);
-/* Round kernels the last round loop enqueued (rounds + the empty tail). */
+/* Kernels the last colouring loop enqueued (ROUNDS: rounds + the empty
+ * tail; ORDERED: 1). */
 int gcx_last_launches(const gcx_workspace *ws, uint32_t *launches);
 
 /* Device pointers for callers that bring their own kernels. */
@@ -145,6 +162,9 @@ enum gcx_stage {
     GCX_ST_JP_ROUNDS = 6,
     GCX_ST_COUNT = 7
 };
+/* Copy `bytes` between two fresh HBM buffers with a plain streaming kernel
+ * (for calibrating hardware byte counters); *ms = kernel time. */
+int gcx_calibration_copy(uint64_t bytes, float *ms);
 int gcx_profile_enable(int on);
 int gcx_profile_reset(void);
 /* Accumulated milliseconds and launch count per stage (synchronises). */

@@ -1,0 +1,135 @@
+"""The ordered persistent JP schedule must give exactly the colours of the
+round schedule and of the reference (needs gfx950)."""
+import hashlib
+
+import numpy as np
+import pytest
+
+import gcx
+import oracle
+from conftest import HEUR, TINY, load_case
+
+pytestmark = pytest.mark.gpu
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    gcx.init()
+    yield
+
+
+@pytest.fixture(scope="module")
+def gmodels():
+    return {l: gcx.Model.layers_file(l) for l in (2, 3, 4)}
+
+
+def order(ws, g, h, gmodels):
+    if h.startswith("GNN"):
+        ws.order(g, gcx.GNN, gmodels[int(h[3:])])
+    else:
+        ws.order(g, h)
+
+
+# grid 0 = every resident workgroup; small grids give each wave long item
+# strides (many waits per wave), grid 1 = a single workgroup of 4 waves
+SCHEDULES = [(gcx.SCHED_ORDERED, 0), (gcx.SCHED_ORDERED, 1), (gcx.SCHED_ORDERED, 7)]
+
+
+@pytest.mark.parametrize("sched", SCHEDULES)
+@pytest.mark.parametrize("name", TINY + ["rmat10"])
+def test_ordered_matches_reference(name, sched, gmodels):
+    c = load_case(name)
+    N = int(c["N"])
+    g = gcx.Graph.from_csr(c["V"], c["E"])
+    ws = gcx.Workspace(N, len(c["E"]))
+    ws.set_schedule(*sched)
+    for h in HEUR:
+        order(ws, g, h, gmodels)
+        ws.color(g)
+        assert not ws.fell_back(), h
+        np.testing.assert_array_equal(ws.colors(N), c[f"color_{h}"], err_msg=h)
+        assert ws.validate(g) == int(c[f"ncolors_{h}"])
+        assert ws.mex_check(g) == 0
+
+
+@pytest.mark.parametrize("name", ["rmat16", "rmat20"])
+def test_ordered_digests(name, digests, gmodels):
+    rec = digests[name]
+    gd = rec["graph"]
+    g = gcx.Graph.rmat(gd["scale"], gd["edge_factor"], *gd["abc"], seed=gd["seed"])
+    ws = gcx.Workspace(g.N, g.M)
+    ws.set_schedule(gcx.SCHED_ORDERED)
+    for h in HEUR:
+        order(ws, g, h, gmodels)
+        ws.color(g)
+        assert not ws.fell_back(), h
+        assert sha(ws.colors(g.N)) == rec[h]["sha_color"], h
+    order(ws, g, "GNN3", gmodels)
+    ws.color(g)
+    ws.recolor(g, 7)
+    assert not ws.fell_back()
+    assert sha(ws.colors(g.N)) == rec["culb_GNN3_8"]["sha_color"]
+
+
+@pytest.mark.parametrize("n", [130, 1100])
+def test_ordered_cliques(n, gmodels):
+    """Cliques push colours far past the state word into the overflow words."""
+    V, E = oracle.csr_from_pairs(n, *zip(*[(a, b) for a in range(n) for b in range(a + 1, n)]))
+    g = gcx.Graph.from_csr(V, E)
+    ws = gcx.Workspace(n, len(E))
+    for sched in [(gcx.SCHED_ROUNDS, 0), (gcx.SCHED_ORDERED, 0), (gcx.SCHED_ORDERED, 3)]:
+        ws.set_schedule(*sched)
+        for h in ["FF", "LF", "GNN2"]:
+            order(ws, g, h, gmodels)
+            p = ws.priorities(n)
+            ws.color(g)
+            want, _ = oracle.greedy(V, E, p)
+            np.testing.assert_array_equal(ws.colors(n), want, err_msg=f"{h} {sched}")
+
+
+def test_ordered_star_and_hubs(gmodels):
+    rng = np.random.default_rng(5)
+    n = 30000
+    hubs = rng.choice(n, 8, replace=False)
+    I = np.concatenate([np.repeat(hubs, 5000), rng.integers(0, n, 80000)]).astype(np.uint32)
+    J = np.concatenate([rng.integers(0, n, 40000), rng.integers(0, n, 80000)]).astype(np.uint32)
+    V, E = oracle.csr_from_pairs(n, I, J)
+    g = gcx.Graph.from_csr(V, E)
+    ws = gcx.Workspace(n, len(E))
+    for sched in [(gcx.SCHED_ORDERED, 0), (gcx.SCHED_ORDERED, 2)]:
+        ws.set_schedule(*sched)
+        for h in ["FF", "LF", "SLL", "GNN4"]:
+            order(ws, g, h, gmodels)
+            p = ws.priorities(n)
+            ws.color(g)
+            want, _ = oracle.greedy(V, E, p)
+            np.testing.assert_array_equal(ws.colors(n), want, err_msg=f"{h} {sched}")
+
+
+def test_ordered_caller_priorities_key64():
+    """Priorities that are not floats take the 64-bit key path of the sort."""
+    c = load_case("rmat10")
+    N = int(c["N"])
+    g = gcx.Graph.from_csr(c["V"], c["E"])
+    ws = gcx.Workspace(N, len(c["E"]))
+    rng = np.random.default_rng(3)
+    p = rng.integers(0, 50, N).astype(np.float64) + 0.1  # not floats, many ties
+    want, _ = oracle.greedy(c["V"], c["E"], p)
+    for sched in [gcx.SCHED_ROUNDS, gcx.SCHED_ORDERED]:
+        ws.set_schedule(sched)
+        ws.set_priorities(g, p)
+        ws.color(g)
+        np.testing.assert_array_equal(ws.colors(N), want)
+
+
+def test_rgg_device_equals_host():
+    for n, r, seed in [(3000, 0.04, 1), (20000, 0.011, 2), (1, 0.5, 3), (500, 0.9, 4)]:
+        n_, I, J = gcx.rgg2d_pairs(n, r, seed=seed)
+        Vh, Eh = gcx.Graph.from_pairs(n, I, J).download()
+        Vd, Ed = gcx.Graph.rgg(n, r, seed=seed).download()
+        np.testing.assert_array_equal(Vh, Vd)
+        np.testing.assert_array_equal(Eh, Ed)
