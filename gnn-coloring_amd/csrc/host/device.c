@@ -288,7 +288,7 @@ int gcx_profile_read(double ms[GCX_ST_COUNT], uint64_t launches[GCX_ST_COUNT])
 
 uint64_t gcx_graph_footprint(uint32_t N, uint64_t M)
 {
-    return 4ull * ((uint64_t)N + 1) + 4ull * M + 4ull * N;
+    return 4ull * ((uint64_t)N + 1) + 4ull * M + 8ull * N;
 }
 
 static int cmp_u32(const void *a, const void *b)
@@ -302,6 +302,9 @@ static int cmp_u32(const void *a, const void *b)
 static int graph_stats(gcx_graph *g, hipStream_t s, uint32_t *tmp)
 {
     TRY(gcxk_graph_stats(g->N, g->V, g->hubs, tmp, &g->nhub, &g->maxdeg, s));
+    if (!g->perm)
+        HIPC(hipMalloc((void **)&g->perm, 4ull * (g->N ? g->N : 1)));
+    TRY(gcxk_degree_order(g->N, g->V, g->maxdeg, g->perm, s));
     if (g->hchunk) {
         (void)hipFree(g->hchunk);
         g->hchunk = NULL;
@@ -371,6 +374,8 @@ int gcx_graph_free(gcx_graph *g)
         (void)hipFree(g->hubs);
     if (g->hchunk)
         (void)hipFree(g->hchunk);
+    if (g->perm)
+        (void)hipFree(g->perm);
     free(g);
     return GCX_OK;
 }
@@ -386,7 +391,8 @@ int gcx__graph_alloc(uint32_t N, uint64_t M, gcx_graph **out)
     g->M = M;
     if (hipMalloc((void **)&g->V, 4ull * (N + 1ull)) != hipSuccess ||
         hipMalloc((void **)&g->E, 4ull * (M ? M : 1)) != hipSuccess ||
-        hipMalloc((void **)&g->hubs, 4ull * (N ? N : 1)) != hipSuccess) {
+        hipMalloc((void **)&g->hubs, 4ull * (N ? N : 1)) != hipSuccess ||
+        hipMalloc((void **)&g->perm, 4ull * (N ? N : 1)) != hipSuccess) {
         gcx_graph_free(g);
         return GCX_ERR_NOMEM;
     }
@@ -684,11 +690,11 @@ static size_t order_extra(uint32_t N)
 
 uint64_t gcx_workspace_footprint(uint32_t N, uint64_t M)
 {
-    /* per vertex: key 8, count 4, color 4, npred 4, state 8, 4 lists x 4,
+    /* per vertex: key 8, count 4, color 4, npred 4, state 16, 4 lists x 4,
      * hpos 8, rows x,y 2 x 64; per entry: E_split 4; plus the overflow colour
      * words, two item queues and any sort scratch beyond the x rows */
     const uint64_t n = (uint64_t)N + 1;
-    return 180ull * n + 4ull * M + 4ull * gcxk_jp_ovf_words(N, M) +
+    return 188ull * n + 4ull * M + 4ull * gcxk_jp_ovf_words(N, M) +
            2ull * sizeof(gcxk_item) * gcxk_jp_item_capacity(N, M) + 2 * sizeof(gcxk_ctl) +
            order_extra(N);
 }
@@ -739,7 +745,7 @@ int gcx_workspace_create(uint32_t N, uint64_t M, gcx_workspace **out)
              hipMalloc((void **)&ws->y, 64 * n) == hipSuccess &&
              hipMalloc((void **)&ws->ctl, 2 * sizeof(gcxk_ctl)) == hipSuccess &&
              hipMalloc((void **)&ws->npred, 4 * n) == hipSuccess &&
-             hipMalloc((void **)&ws->state, 8 * n) == hipSuccess &&
+             hipMalloc((void **)&ws->state, 16 * n) == hipSuccess &&
              hipMalloc((void **)&ws->hpos, 8 * n) == hipSuccess &&
              hipMalloc((void **)&ws->ovf, 4 * gcxk_jp_ovf_words(N, M)) == hipSuccess &&
              hipMalloc((void **)&ws->esplit, 4 * (M ? M : 1)) == hipSuccess &&
@@ -869,7 +875,7 @@ static int order_gnn_fp32(gcx_workspace *ws, const gcx_graph *g, const gcx_model
             hs = (hipStream_t)ws->aux;
         }
         TRY(gcxk_gnn_fp32_layer(kind, g->N, g->V, g->E, W, b, m->md, in, out,
-                                (uint32_t *)ws->key, g->hubs, g->nhub, s, hs));
+                                (uint32_t *)ws->key, g->hubs, g->nhub, g->perm, s, hs));
         if (g->nhub) {
             HIPC(hipEventRecord((hipEvent_t)ws->ev_join, hs));
             HIPC(hipStreamWaitEvent(s, (hipEvent_t)ws->ev_join, 0));
@@ -973,20 +979,19 @@ static int jp_round(void *p, uint32_t r, hipStream_t s)
 }
 
 /* Orientation pass shared by both schedules. */
-static int jp_prepare(gcx_workspace *ws, const gcx_graph *g, int mb, hipStream_t s)
+static int jp_prepare(gcx_workspace *ws, const gcx_graph *g, int mb, int ordered, hipStream_t s)
 {
-    TRY(gcxk_jp_clear(g->N, g->M, ws->ovf, g->hubs, g->nhub, ws->state, ws->npred, ws->hpos,
-                      ws->ctl, s));
-    return gcxk_jp_init(g->N, g->V, g->E, ws->key, ws->key_bits, mb, ws->npred, ws->esplit,
-                        ws->state, ws->color, g->hubs, g->nhub, g->hchunk, g->nhchunk, ws->hpos,
-                        s);
+    TRY(gcxk_jp_clear(g->N, g->M, ws->ovf, g->nhub, ws->hpos, ws->ctl, s));
+    return gcxk_jp_init(g->N, g->V, g->E, ws->key, ws->key_bits, mb, g->perm, ws->npred,
+                        ws->esplit, ws->state, ws->color, g->hubs, g->nhub, g->hchunk, g->nhchunk,
+                        ws->hpos, ordered, s);
 }
 
 static int color_rounds(gcx_workspace *ws, const gcx_graph *g, int mb, hipStream_t s)
 {
     void *tok = NULL;
     TRY(gcx__prof_begin(GCX_ST_JP_INIT, s, &tok));
-    TRY(jp_prepare(ws, g, mb, s));
+    TRY(jp_prepare(ws, g, mb, 0, s));
     TRY(gcxk_jp_roots(g->N, g->V, ws->npred, ws->items[0], ws->ctl, s));
     TRY(gcx__prof_end(tok, s));
     TRY(gcx__prof_begin(GCX_ST_JP_ROUNDS, s, &tok));
@@ -1005,7 +1010,7 @@ static int color_ordered(gcx_workspace *ws, const gcx_graph *g, int mb, hipStrea
     if (!g_ordered_done)
         HIPC(hipEventCreateWithFlags(&g_ordered_done, hipEventDisableTiming));
     TRY(gcx__prof_begin(GCX_ST_JP_INIT, s, &tok));
-    TRY(jp_prepare(ws, g, mb, s));
+    TRY(jp_prepare(ws, g, mb, 1, s));
     void *scratch = ws->order_buf ? ws->order_buf : (void *)ws->x;
     uint32_t *cnt = (uint32_t *)ws->y, *incl = cnt + ws->capN;
     TRY(gcxk_jp_order_items(g->N, g->V, ws->npred, ws->key, ws->key_bits, scratch,

@@ -320,6 +320,19 @@ struct DevBuf {
     hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 8); }
 };
 
+// Sort keys for the degree order: maxdeg - deg (so ascending = degree
+// descending) and the vertex ids.
+__global__ __launch_bounds__(kBlock) void degree_keys(uint32_t N, const uint32_t *__restrict__ V,
+                                                      uint32_t maxdeg, uint32_t *__restrict__ k,
+                                                      uint32_t *__restrict__ id)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < N) {
+        k[i] = maxdeg - (V[i + 1] - V[i]);
+        id[i] = i;
+    }
+}
+
 } // namespace
 
 extern "C" int gcxk_csr_from_pairs(uint32_t N, uint64_t L, const uint32_t *dI, const uint32_t *dJ,
@@ -385,6 +398,41 @@ extern "C" int gcxk_csr_from_pairs(uint32_t N, uint64_t L, const uint32_t *dI, c
     *dE = E;
     *M = m32;
     return GCX_OK;
+}
+
+extern "C" int gcxk_degree_order(uint32_t N, const uint32_t *dV, uint32_t maxdeg, uint32_t *perm,
+                                 void *stream)
+{
+    if (N == 0)
+        return GCX_OK;
+    hipStream_t s = as_stream(stream);
+    const unsigned bits = maxdeg ? 32u - (unsigned)__builtin_clz(maxdeg) : 1u;
+    uint32_t *k0 = nullptr, *k1 = nullptr, *id = nullptr;
+    void *tmp = nullptr;
+    size_t tb = 0;
+    int rc = GCX_ERR_HIP;
+    if (hipMalloc((void **)&k0, 4ull * N) != hipSuccess ||
+        hipMalloc((void **)&k1, 4ull * N) != hipSuccess ||
+        hipMalloc((void **)&id, 4ull * N) != hipSuccess)
+        goto done;
+    degree_keys<<<div_up(N, kBlock), kBlock, 0, s>>>(N, dV, maxdeg, k0, id);
+    if (hipGetLastError() != hipSuccess ||
+        rocprim::radix_sort_pairs(nullptr, tb, k0, k1, id, perm, N, 0u, bits, s) != hipSuccess ||
+        hipMalloc(&tmp, tb ? tb : 1) != hipSuccess ||
+        rocprim::radix_sort_pairs(tmp, tb, k0, k1, id, perm, N, 0u, bits, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        goto done;
+    rc = GCX_OK;
+done:
+    if (k0)
+        (void)hipFree(k0);
+    if (k1)
+        (void)hipFree(k1);
+    if (id)
+        (void)hipFree(id);
+    if (tmp)
+        (void)hipFree(tmp);
+    return rc;
 }
 
 extern "C" int gcxk_graph_stats(uint32_t N, const uint32_t *dV, uint32_t *dhub, uint32_t *dtmp,

@@ -45,6 +45,11 @@ int gcxk_csr_from_pairs(uint32_t N, uint64_t L, const uint32_t *dI, const uint32
  * degree. Uses dtmp (>= 8 bytes device scratch). */
 int gcxk_graph_stats(uint32_t N, const uint32_t *dV, uint32_t *dhub, uint32_t *dtmp,
                      uint32_t *nhub, uint32_t *maxdeg, void *stream);
+/* Degree order: perm = vertex ids by degree descending, id ascending (so the
+ * nhub hubs come first). Temporaries are allocated and freed inside;
+ * synchronises. */
+int gcxk_degree_order(uint32_t N, const uint32_t *dV, uint32_t maxdeg, uint32_t *perm,
+                      void *stream);
 /* CSR validity (sorted, simple, symmetric): *valid = 1 or 0. */
 int gcxk_graph_check(uint32_t N, const uint32_t *dV, const uint32_t *dE, uint32_t *dtmp,
                      int *valid, void *stream);
@@ -99,19 +104,20 @@ int gcxk_sl_dec(const uint32_t *dV, const uint32_t *dE, uint32_t *deg, const uin
 /* ---- GraphSAGE-16 (gnn.hip) ---- */
 /* FP32, bit-exact with the reference op order. kind 0 = input layer (x
  * unused), 1 = hidden (x -> out), 2 = output (x -> key, float order keys).
- * Rows are 16 floats. Hub rows run on hub_stream, the rest on stream; the
+ * Rows are 16 floats. Hub rows run on hub_stream, the rest on stream in the
+ * degree order perm (gcxk_degree_order; non-hubs are perm[nhub .. N)); the
  * caller joins the two streams before the next layer. */
 int gcxk_gnn_fp32_layer(int kind, uint32_t N, const uint32_t *dV, const uint32_t *dE,
                         const float *W, const float *b, int md, const float *x, float *out,
-                        uint32_t *key, const uint32_t *hubs, uint32_t nhub, void *stream,
-                        void *hub_stream);
+                        uint32_t *key, const uint32_t *hubs, uint32_t nhub, const uint32_t *perm,
+                        void *stream, void *hub_stream);
 /* BF16 rows (16 x bf16 = 32 B), FP32 aggregation, MFMA layer transform.
  * Wpack: the layer's two MFMA B operands in lane order (512 bf16); Wb: the
  * layer's weights W[k][j] in bf16 (512); b: FP32 biases (16). */
 int gcxk_gnn_bf16_layer(int kind, uint32_t N, const uint32_t *dV, const uint32_t *dE,
                         const uint16_t *Wpack, const uint16_t *Wb, const float *b, int md,
                         const uint16_t *x, uint16_t *out, uint32_t *key, const uint32_t *hubs,
-                        uint32_t nhub, void *stream, void *hub_stream);
+                        uint32_t nhub, const uint32_t *perm, void *stream, void *hub_stream);
 
 /* ---- Jones-Plassmann (jp.hip) ---- */
 /* Work item: (vertex, chunk of 64 successor edges). Stored as two uint32. */
@@ -136,19 +142,23 @@ static inline uint64_t gcxk_jp_item_capacity(uint32_t N, uint64_t M)
 /* Colour bits in the per-vertex JP state word: 64 - bitlength(maxdeg); the
  * rest of the word counts pending predecessors. */
 int gcxk_jp_mask_bits(uint32_t maxdeg);
-/* Zero the overflow colour words (gcxk_jp_ovf_words), the hubs' state words,
- * predecessor counts and per-hub positions (hpos: 2 per hub), and the
- * control block. A plain kernel, so the sequence can be captured in a graph. */
-int gcxk_jp_clear(uint32_t N, uint64_t M, uint32_t *ovf, const uint32_t *hubs, uint32_t nhub,
-                  uint64_t *state, uint32_t *npred, uint32_t *hpos, gcxk_ctl *ctl, void *stream);
-/* Orientation: key_bits = 32 or 64, mb = gcxk_jp_mask_bits(maxdeg). E_split
- * rows = predecessors then successors; npred = predecessor counts; state =
- * npred << mb; color = 1 for degree-0 vertices, else 0. hchunk: per-hub
- * prefix of 4096-edge chunks (nhub + 1 entries), nhchunk = total chunks. */
+/* Zero the overflow colour words (gcxk_jp_ovf_words), the per-hub positions
+ * (hpos: 2 per hub) and the control block. A plain kernel, so the sequence
+ * can be captured in a graph. */
+int gcxk_jp_clear(uint32_t N, uint64_t M, uint32_t *ovf, uint32_t nhub, uint32_t *hpos,
+                  gcxk_ctl *ctl, void *stream);
+/* Orientation: key_bits = 32 or 64, mb = gcxk_jp_mask_bits(maxdeg), perm =
+ * the graph's degree order (hubs first). E_split rows = predecessors then
+ * successors; npred = predecessor counts; color = 1 for degree-0 vertices,
+ * else 0; state (2N words) = npred << mb per vertex for the round schedule
+ * (ordered = 0), or two zero report words per vertex for the ordered one.
+ * hchunk: per-hub prefix of 4096-edge chunks (nhub + 1 entries), nhchunk =
+ * total chunks. */
 int gcxk_jp_init(uint32_t N, const uint32_t *dV, const uint32_t *dE, const void *key,
-                 int key_bits, int mb, uint32_t *npred, uint32_t *esplit, uint64_t *state,
-                 int *color, const uint32_t *hubs, uint32_t nhub, const uint32_t *hchunk,
-                 uint32_t nhchunk, uint32_t *hpos, void *stream);
+                 int key_bits, int mb, const uint32_t *perm, uint32_t *npred, uint32_t *esplit,
+                 uint64_t *state, int *color, const uint32_t *hubs, uint32_t nhub,
+                 const uint32_t *hchunk, uint32_t nhchunk, uint32_t *hpos, int ordered,
+                 void *stream);
 /* Round schedule: queue the roots (vertices with successors and no
  * predecessors) as round-0 items in q0 / ctl->qn[0]. */
 int gcxk_jp_roots(uint32_t N, const uint32_t *dV, const uint32_t *npred, gcxk_item *q0,

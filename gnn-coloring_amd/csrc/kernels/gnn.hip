@@ -9,10 +9,10 @@
 // Neighbour sums run left to right in CSR order from +0 and are divided by
 // the degree when it is positive.
 //
-// Mapping: 16 lanes per vertex (lane j = feature j), 4 vertices per wave.
-// Each group fetches 16 neighbour rows at once (64 B coalesced per row) and
-// then adds them in order, so the sequential dependency is only on the adds.
-// Hubs (degree >= GCXK_HUB_DEG) get a 512-thread workgroup that stages 512
+// Mapping: vertices below the hub degree run in degree order (the graph's
+// perm), 4 lanes per vertex and 16 vertices per wave (see sage_quads); the
+// sequential dependency is only on the ordered adds, the row loads are
+// issued ahead. Hubs (degree >= GCXK_HUB_DEG) get a 512-thread workgroup that stages 512
 // neighbour rows per step into LDS (double-buffered, feature-major) while one
 // 16-lane group performs the ordered sum; the hub kernel runs on a second
 // stream next to the group kernel.
@@ -91,101 +91,160 @@ __device__ __forceinline__ void load_weights(const float *__restrict__ W, int j,
         w[i] = W[i * F + j];
 }
 
-// ------------------------------------------------------------ group kernels
+// ------------------------------------------------------------- quad kernel
+
+// Non-hub vertices in degree order (perm[p0 .. p1)), 4 lanes per vertex:
+// lane q owns features 4q .. 4q+3 (one 16-byte slice of each 64-byte row), so
+// one load instruction fetches 16 neighbour rows and a wave's 16 vertices
+// have near-equal degrees. Each lane keeps four independent ordered sums.
+// The weights sit in LDS as float4 W[i][4q .. 4q+3]; the dense step takes
+// A[i] from the owning lane of the quad.
+constexpr int kQuads = kBlock / 4; // vertices per workgroup step
+constexpr int kUnroll = 8;         // neighbour rows in flight per lane
 
 template <int kind>
-__global__ __launch_bounds__(kBlock) void sage_groups(uint32_t N, const uint32_t *__restrict__ V,
-                                                      const uint32_t *__restrict__ E,
-                                                      const float *__restrict__ W,
-                                                      const float *__restrict__ b, float fmd,
-                                                      const float *__restrict__ x,
-                                                      float *__restrict__ out,
-                                                      uint32_t *__restrict__ key)
+__global__ __launch_bounds__(kBlock) void sage_quads(uint32_t N, const uint32_t *__restrict__ V,
+                                                     const uint32_t *__restrict__ E,
+                                                     const float *__restrict__ W,
+                                                     const float *__restrict__ b, float fmd,
+                                                     const float *__restrict__ x,
+                                                     float *__restrict__ out,
+                                                     uint32_t *__restrict__ key,
+                                                     const uint32_t *__restrict__ perm, uint32_t p0,
+                                                     uint32_t p1)
 {
-    const uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    const uint32_t u = (uint32_t)(t / F);
-    const int j = (int)(t % F);
-    if (u >= N) // whole groups leave together (N is per-group uniform)
-        return;
-    const uint32_t beg = V[u], end = V[u + 1];
-    const uint32_t deg = end - beg;
-    if (deg >= GCXK_HUB_DEG)
-        return; // hub kernel
-    float w[2 * F];
-    load_weights(W, j, w);
-    const float bias = b[j];
-    const float fdeg_f = (float)deg;
-    float h;
-    if (kind == kInput) {
-        float sd = 0.0f, si = 0.0f;
-        // 4 x 16 neighbours per step: lane j computes the features of
-        // neighbours i0 + 16t + j (independent gathers), then the group adds
-        // them strictly in CSR order.
-        for (uint32_t i0 = beg; i0 < end; i0 += 4 * F) {
-            float fd[4], fi[4];
+    __shared__ float4 Ws[2 * F][4];
+    __shared__ float4 bs[4];
+    for (int t = threadIdx.x; t < 2 * F * F; t += kBlock)
+        reinterpret_cast<float *>(Ws)[t] = W[t];
+    if (threadIdx.x < F)
+        reinterpret_cast<float *>(bs)[threadIdx.x] = b[threadIdx.x];
+    __syncthreads();
+    const int q = threadIdx.x & 3;
+    const float4 *X = reinterpret_cast<const float4 *>(x);
+    // whole quads leave the loop together, so the width-4 shuffles stay valid
+    for (uint32_t g = p0 + blockIdx.x * kQuads + threadIdx.x / 4; g < p1;
+         g += gridDim.x * kQuads) {
+        const uint32_t u = perm[g];
+        const uint32_t beg = V[u], end = V[u + 1];
+        const float fdeg_f = (float)(end - beg);
+        const float4 bb = bs[q];
+        float c[4] = {bb.x, bb.y, bb.z, bb.w};
+        if (kind == kInput) {
+            // lane q computes the features of neighbours i0 + 4t + q, then
+            // every lane adds all 16 in CSR order
+            float sd = 0.0f, si = 0.0f;
+            for (uint32_t i0 = beg; i0 < end; i0 += 16) {
+                float fd[4], fi[4];
 #pragma unroll
-            for (int t = 0; t < 4; t++) {
-                const uint32_t i = i0 + F * t + j;
-                fd[t] = 0.0f;
-                fi[t] = 0.0f;
-                if (i < end) {
-                    const uint32_t v = E[i];
-                    fd[t] = (float)(V[v + 1] - V[v]) / fmd;
-                    fi[t] = fid(v, N);
+                for (int t = 0; t < 4; t++) {
+                    const uint32_t i = i0 + 4 * t + q;
+                    fd[t] = 0.0f;
+                    fi[t] = 0.0f;
+                    if (i < end) {
+                        const uint32_t v = E[i];
+                        fd[t] = (float)(V[v + 1] - V[v]) / fmd;
+                        fi[t] = fid(v, N);
+                    }
+                }
+                const uint32_t nb = end - i0 < 16u ? end - i0 : 16u;
+#pragma unroll
+                for (int t = 0; t < 4; t++)
+#pragma unroll
+                    for (int m = 0; m < 4; m++) {
+                        const float a = __shfl(fd[t], m, 4), e = __shfl(fi[t], m, 4);
+                        if ((uint32_t)(4 * t + m) < nb) {
+                            sd = sd + a;
+                            si = si + e;
+                        }
+                    }
+            }
+            if (fdeg_f > 0.0f) {
+                sd = sd / fdeg_f;
+                si = si / fdeg_f;
+            }
+            const float selfd = fdeg_f / fmd, selfi = fid(u, N);
+#pragma unroll
+            for (int i = 0; i < 2 * F; i++) {
+                const float a = i < 8 ? sd : i < 16 ? si : i < 24 ? selfd : selfi;
+                const float4 w = Ws[i][q];
+                c[0] = fmaf(w.x, a, c[0]);
+                c[1] = fmaf(w.y, a, c[1]);
+                c[2] = fmaf(w.z, a, c[2]);
+                c[3] = fmaf(w.w, a, c[3]);
+            }
+        } else {
+            float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
+            uint32_t i = beg;
+            for (; i + kUnroll <= end; i += kUnroll) {
+                uint32_t v[kUnroll];
+#pragma unroll
+                for (int t = 0; t < kUnroll; t++)
+                    v[t] = E[i + t];
+                float4 r[kUnroll];
+#pragma unroll
+                for (int t = 0; t < kUnroll; t++)
+                    r[t] = X[(size_t)v[t] * 4 + q];
+#pragma unroll
+                for (int t = 0; t < kUnroll; t++) {
+                    s0 = s0 + r[t].x;
+                    s1 = s1 + r[t].y;
+                    s2 = s2 + r[t].z;
+                    s3 = s3 + r[t].w;
                 }
             }
-#pragma unroll
-            for (int t = 0; t < 4; t++) {
-                const uint32_t s0 = i0 + F * t;
-                const uint32_t nb = s0 >= end ? 0u : (end - s0 < (uint32_t)F ? end - s0 : (uint32_t)F);
-                for (uint32_t k = 0; k < nb; k++) {
-                    sd = sd + __shfl(fd[t], (int)k, F);
-                    si = si + __shfl(fi[t], (int)k, F);
-                }
+            for (; i < end; i++) {
+                const float4 r = X[(size_t)E[i] * 4 + q];
+                s0 = s0 + r.x;
+                s1 = s1 + r.y;
+                s2 = s2 + r.z;
+                s3 = s3 + r.w;
             }
-        }
-        if (fdeg_f > 0.0f) {
-            sd = sd / fdeg_f;
-            si = si / fdeg_f;
-        }
-        h = dense_input(w, bias, sd, si, fdeg_f / fmd, fid(u, N));
-    } else {
-        float acc = 0.0f;
-        uint32_t i0 = beg;
-        for (; i0 + F <= end; i0 += F) {
-            const uint32_t idx = E[i0 + j];
-            float r[F];
+            if (fdeg_f > 0.0f) {
+                s0 = s0 / fdeg_f;
+                s1 = s1 / fdeg_f;
+                s2 = s2 / fdeg_f;
+                s3 = s3 / fdeg_f;
+            }
+            const float4 self = X[(size_t)u * 4 + q];
+            const float ag[4] = {s0, s1, s2, s3};
+            const float sf[4] = {self.x, self.y, self.z, self.w};
 #pragma unroll
-            for (int k = 0; k < F; k++)
-                r[k] = x[(size_t)__shfl(idx, k, F) * F + j];
-#pragma unroll
-            for (int k = 0; k < F; k++)
-                acc = acc + r[k];
-        }
-        if (i0 < end) {
-            const uint32_t nb = end - i0;
-            const uint32_t idx = (uint32_t)j < nb ? E[i0 + j] : 0u;
-            float r[F];
-#pragma unroll
-            for (int k = 0; k < F; k++) {
-                const uint32_t v = __shfl(idx, k, F);
-                r[k] = (uint32_t)k < nb ? x[(size_t)v * F + j] : 0.0f;
+            for (int i = 0; i < F; i++) {
+                const float a = __shfl(ag[i % 4], i / 4, 4);
+                const float4 w = Ws[i][q];
+                c[0] = fmaf(w.x, a, c[0]);
+                c[1] = fmaf(w.y, a, c[1]);
+                c[2] = fmaf(w.z, a, c[2]);
+                c[3] = fmaf(w.w, a, c[3]);
             }
 #pragma unroll
-            for (int k = 0; k < F; k++)
-                if ((uint32_t)k < nb)
-                    acc = acc + r[k];
+            for (int i = 0; i < F; i++) {
+                const float a = __shfl(sf[i % 4], i / 4, 4);
+                const float4 w = Ws[F + i][q];
+                c[0] = fmaf(w.x, a, c[0]);
+                c[1] = fmaf(w.y, a, c[1]);
+                c[2] = fmaf(w.z, a, c[2]);
+                c[3] = fmaf(w.w, a, c[3]);
+            }
         }
-        if (fdeg_f > 0.0f)
-            acc = acc / fdeg_f;
-        h = dense_group(w, bias, acc, x[(size_t)u * F + j]);
-    }
-    if (kind == kOutput) {
-        const float p = output_tree(h);
-        if (j == 0)
-            key[u] = float_order_key(p);
-    } else {
-        out[(size_t)u * F + j] = h;
+        float h[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            h[k] = relu(c[k]);
+        if (kind == kOutput) {
+            // t_k = h_k + h_{k+8}: lanes 0,1 hold h_0..7, lanes 2,3 hold h_8..15
+            float t[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                t[k] = h[k] + __shfl(h[k], (q & 1) + 2, 4);
+            const float half = (t[0] + t[1]) + (t[2] + t[3]);
+            const float p = half + __shfl(half, 1, 4);
+            if (q == 0)
+                key[u] = float_order_key(p);
+        } else {
+            reinterpret_cast<float4 *>(out)[(size_t)u * 4 + q] = make_float4(h[0], h[1], h[2], h[3]);
+        }
     }
 }
 
@@ -299,19 +358,28 @@ __global__ __launch_bounds__(kHubBlock) void sage_hubs(uint32_t N, const uint32_
     }
 }
 
+// Workgroups of the quad kernel: enough to fill the device several times
+// over, looping over the vertices.
+constexpr uint32_t kQuadGrid = 2048;
+
 template <int kind>
 int launch_layer(uint32_t N, const uint32_t *V, const uint32_t *E, const float *W, const float *b,
                  float fmd, const float *x, float *out, uint32_t *key, const uint32_t *hubs,
-                 uint32_t nhub, void *stream, void *hub_stream)
+                 uint32_t nhub, const uint32_t *perm, void *stream, void *hub_stream)
 {
     if (N == 0)
         return GCX_OK;
-    // Hubs go first on their own stream so that they overlap the groups.
+    // Hubs go first on their own stream so that they overlap the quads.
     if (nhub)
         sage_hubs<kind><<<nhub, kHubBlock, 0, as_stream(hub_stream)>>>(N, V, E, W, b, fmd, x, out,
                                                                        key, hubs);
-    sage_groups<kind><<<div_up((uint64_t)N * F, kBlock), kBlock, 0, as_stream(stream)>>>(
-        N, V, E, W, b, fmd, x, out, key);
+    const uint32_t rest = N - nhub;
+    if (rest) {
+        uint32_t grid = div_up(rest, kQuads);
+        grid = grid < kQuadGrid ? grid : kQuadGrid;
+        sage_quads<kind><<<grid, kBlock, 0, as_stream(stream)>>>(N, V, E, W, b, fmd, x, out, key,
+                                                                 perm, nhub, N);
+    }
     GCXK_LAUNCH_CHECK();
     return GCX_OK;
 }
@@ -321,18 +389,18 @@ int launch_layer(uint32_t N, const uint32_t *V, const uint32_t *E, const float *
 extern "C" int gcxk_gnn_fp32_layer(int kind, uint32_t N, const uint32_t *dV, const uint32_t *dE,
                                    const float *W, const float *b, int md, const float *x,
                                    float *out, uint32_t *key, const uint32_t *hubs, uint32_t nhub,
-                                   void *stream, void *hub_stream)
+                                   const uint32_t *perm, void *stream, void *hub_stream)
 {
     switch (kind) {
     case 0:
         return launch_layer<kInput>(N, dV, dE, W, b, (float)md, nullptr, out, nullptr, hubs, nhub,
-                                    stream, hub_stream);
+                                    perm, stream, hub_stream);
     case 1:
-        return launch_layer<kHidden>(N, dV, dE, W, b, 0.0f, x, out, nullptr, hubs, nhub, stream,
-                                     hub_stream);
+        return launch_layer<kHidden>(N, dV, dE, W, b, 0.0f, x, out, nullptr, hubs, nhub, perm,
+                                     stream, hub_stream);
     case 2:
-        return launch_layer<kOutput>(N, dV, dE, W, b, 0.0f, x, nullptr, key, hubs, nhub, stream,
-                                     hub_stream);
+        return launch_layer<kOutput>(N, dV, dE, W, b, 0.0f, x, nullptr, key, hubs, nhub, perm,
+                                     stream, hub_stream);
     default:
         return GCX_ERR_ARG;
     }

@@ -7,7 +7,9 @@
 // FP32 bias; ReLU in FP32; the next layer's rows are rounded to bf16; the
 // output layer sums the 16 FP32 features into the priority.
 //
-// Mapping: a 256-thread workgroup owns 64 consecutive vertices. Phase 1:
+// Mapping: a 256-thread workgroup owns 64 vertices that are consecutive in
+// the graph's degree order (perm, hubs excluded), so their gathers take
+// near-equal time. Phase 1:
 // four lanes per vertex stride its neighbours, each loading whole 32-byte
 // rows (two 16-byte loads) into 16 FP32 partial sums, then a 2-step xor
 // reduction combines the four partials. The bf16 operand tile [64 x 32]
@@ -65,20 +67,26 @@ __global__ __launch_bounds__(kBlock) void sage_bf16_tiles(uint32_t N, const uint
                                                           const float *__restrict__ bias, float fmd,
                                                           const uint16_t *__restrict__ x,
                                                           uint16_t *__restrict__ out,
-                                                          uint32_t *__restrict__ key)
+                                                          uint32_t *__restrict__ key,
+                                                          const uint32_t *__restrict__ perm,
+                                                          uint32_t p0)
 {
     // Operand tile: 64 vertices x 32 bf16, padded to 36 to spread LDS banks.
     __shared__ uint16_t A[kTile][36];
+    __shared__ uint32_t tu[kTile]; // vertex of each tile row, N = none
     const int t = threadIdx.x;
     const int vi = t / kSub, sub = t % kSub;
-    const uint32_t u = blockIdx.x * kTile + vi;
+    const uint32_t pos = p0 + blockIdx.x * kTile + vi;
+    const bool skip = pos >= N;
+    const uint32_t u = skip ? 0u : perm[pos];
     uint32_t beg = 0, end = 0;
-    if (u < N) {
+    if (!skip) {
         beg = V[u];
         end = V[u + 1];
     }
+    if (sub == 0)
+        tu[vi] = skip ? N : u;
     const uint32_t deg = end - beg;
-    const bool skip = u >= N || deg >= GCXK_HUB_DEG;
     const float fdeg = (float)deg;
     if (kind == kInput) {
         float sd = 0.0f, si = 0.0f;
@@ -143,8 +151,8 @@ __global__ __launch_bounds__(kBlock) void sage_bf16_tiles(uint32_t N, const uint
     f32x4 c = {bj, bj, bj, bj};
     c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0, b0, c, 0, 0, 0);
     c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, b1, c, 0, 0, 0);
-    // c[e] = H[vertex 4q+e of this wave][feature i]
-    const uint32_t vbase = blockIdx.x * kTile + 16 * w + 4 * q;
+    // c[e] = H[tile row 16w + 4q + e][feature i]
+    const int rbase = 16 * w + 4 * q;
     if (kind == kOutput) {
 #pragma unroll
         for (int e = 0; e < 4; e++) {
@@ -153,15 +161,15 @@ __global__ __launch_bounds__(kBlock) void sage_bf16_tiles(uint32_t N, const uint
             h += __shfl_xor(h, 2, 16);
             h += __shfl_xor(h, 4, 16);
             h += __shfl_xor(h, 8, 16);
-            const uint32_t v = vbase + e;
-            if (i == 0 && v < N && (V[v + 1] - V[v]) < GCXK_HUB_DEG)
+            const uint32_t v = tu[rbase + e];
+            if (i == 0 && v < N)
                 key[v] = float_order_key(h);
         }
     } else {
 #pragma unroll
         for (int e = 0; e < 4; e++) {
-            const uint32_t v = vbase + e;
-            if (v < N && (V[v + 1] - V[v]) < GCXK_HUB_DEG)
+            const uint32_t v = tu[rbase + e];
+            if (v < N)
                 out[(size_t)v * F + i] = to_bf16(c[e] > 0.0f ? c[e] : 0.0f);
         }
     }
@@ -246,15 +254,17 @@ __global__ __launch_bounds__(kHubBlock) void sage_bf16_hubs(uint32_t N, const ui
 template <int kind>
 int launch(uint32_t N, const uint32_t *V, const uint32_t *E, const uint16_t *Wpack,
            const uint16_t *Wb, const float *b, float fmd, const uint16_t *x, uint16_t *out,
-           uint32_t *key, const uint32_t *hubs, uint32_t nhub, void *stream, void *hub_stream)
+           uint32_t *key, const uint32_t *hubs, uint32_t nhub, const uint32_t *perm,
+           void *stream, void *hub_stream)
 {
     if (N == 0)
         return GCX_OK;
     if (nhub)
         sage_bf16_hubs<kind><<<nhub, kHubBlock, 0, as_stream(hub_stream)>>>(N, V, E, Wb, b, fmd, x,
                                                                             out, key, hubs);
-    sage_bf16_tiles<kind><<<div_up(N, kTile), kBlock, 0, as_stream(stream)>>>(N, V, E, Wpack, b,
-                                                                              fmd, x, out, key);
+    if (N > nhub)
+        sage_bf16_tiles<kind><<<div_up(N - nhub, kTile), kBlock, 0, as_stream(stream)>>>(
+            N, V, E, Wpack, b, fmd, x, out, key, perm, nhub);
     GCXK_LAUNCH_CHECK();
     return GCX_OK;
 }
@@ -264,20 +274,20 @@ int launch(uint32_t N, const uint32_t *V, const uint32_t *E, const uint16_t *Wpa
 extern "C" int gcxk_gnn_bf16_layer(int kind, uint32_t N, const uint32_t *dV, const uint32_t *dE,
                                    const uint16_t *Wpack, const uint16_t *Wb, const float *b,
                                    int md, const uint16_t *x, uint16_t *out, uint32_t *key,
-                                   const uint32_t *hubs, uint32_t nhub, void *stream,
-                                   void *hub_stream)
+                                   const uint32_t *hubs, uint32_t nhub, const uint32_t *perm,
+                                   void *stream, void *hub_stream)
 {
     const float fmd = (float)md;
     switch (kind) {
     case 0:
         return launch<kInput>(N, dV, dE, Wpack, Wb, b, fmd, nullptr, out, nullptr, hubs, nhub,
-                              stream, hub_stream);
+                              perm, stream, hub_stream);
     case 1:
-        return launch<kHidden>(N, dV, dE, Wpack, Wb, b, fmd, x, out, nullptr, hubs, nhub, stream,
-                               hub_stream);
+        return launch<kHidden>(N, dV, dE, Wpack, Wb, b, fmd, x, out, nullptr, hubs, nhub, perm,
+                               stream, hub_stream);
     case 2:
-        return launch<kOutput>(N, dV, dE, Wpack, Wb, b, fmd, x, nullptr, key, hubs, nhub, stream,
-                               hub_stream);
+        return launch<kOutput>(N, dV, dE, Wpack, Wb, b, fmd, x, nullptr, key, hubs, nhub, perm,
+                               stream, hub_stream);
     default:
         return GCX_ERR_ARG;
     }

@@ -22,6 +22,10 @@
 // load yields both "ready" and the colour set. For c > mb the overflow OR is
 // waited for (s_waitcnt) before the decrement.
 //
+// The ordered schedule keeps two such words per vertex, each with its own
+// report counter (see jp_ordered_kernel), so colours up to 2mb need no
+// overflow words and no wait.
+//
 // A vertex with s successors is ceil(s/64) work items (at least one); one
 // wave handles one item, i.e. up to 64 successor edges, so a hub's
 // successors are spread over many waves. Every item of u recomputes the
@@ -52,10 +56,11 @@ constexpr int kRootBlock = 1024;
 
 __device__ __forceinline__ uint32_t ovf_base(uint32_t Vv, uint32_t v) { return Vv / 32u + v; }
 
-// Overflow words of a vertex of degree d: colours mb+1 .. d+1.
-__device__ __forceinline__ uint32_t ovf_words(uint32_t d, int mb)
+// Overflow words of a vertex of degree d whose state words hold colours
+// 1 .. cw: colours cw+1 .. d+1.
+__device__ __forceinline__ uint32_t ovf_words(uint32_t d, uint32_t cw)
 {
-    return d + 1u > (uint32_t)mb ? (d + 1u - (uint32_t)mb + 31u) / 32u : 0u;
+    return d + 1u > cw ? (d + 1u - cw + 31u) / 32u : 0u;
 }
 
 __device__ __forceinline__ uint32_t n_items(uint32_t nsucc)
@@ -73,17 +78,12 @@ __device__ __forceinline__ uint32_t load_word(const uint32_t *p)
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Smallest colour not in u's set; s = u's state word with count zero. Wave
-// uniform (all lanes call with the same u). mex <= npred + 1 <= deg + 1, so a
-// free bit lies inside the overflow region when the word is full.
-__device__ __forceinline__ uint32_t mex_of(uint64_t s, int mb, uint32_t u, uint32_t Vu,
-                                           uint32_t deg, const uint32_t *__restrict__ ovf)
+// Smallest colour > cw not in u's overflow words. Wave uniform (all lanes call
+// with the same u). mex <= npred + 1 <= deg + 1, so a free bit exists.
+__device__ __forceinline__ uint32_t mex_overflow(uint32_t cw, uint32_t u, uint32_t Vu, uint32_t deg,
+                                                 const uint32_t *__restrict__ ovf)
 {
-    const uint64_t low = (1ull << mb) - 1ull;
-    const uint64_t free = ~s & low;
-    if (free)
-        return (uint32_t)__builtin_ctzll(free) + 1u;
-    const uint32_t base = ovf_base(Vu, u), words = ovf_words(deg, mb);
+    const uint32_t base = ovf_base(Vu, u), words = ovf_words(deg, cw);
     const int lane = lane_id();
     for (uint32_t wb = 0; wb < words; wb += kWave) {
         const uint32_t wi = wb + (uint32_t)lane;
@@ -92,19 +92,29 @@ __device__ __forceinline__ uint32_t mex_of(uint64_t s, int mb, uint32_t u, uint3
         if (b) {
             const int fl = __ffsll((long long)b) - 1;
             const uint32_t wf = __shfl(w, fl);
-            return (uint32_t)mb + 1u + 32u * (wb + (uint32_t)fl) + (uint32_t)__builtin_ctz(~wf);
+            return cw + 1u + 32u * (wb + (uint32_t)fl) + (uint32_t)__builtin_ctz(~wf);
         }
     }
     return 0u; // unreachable for a consistent state
 }
 
-// Record colour c > mb in successor v's overflow words (colours past v's
+// Smallest colour not in u's set (one state word s with colours 1..mb).
+__device__ __forceinline__ uint32_t mex_of(uint64_t s, int mb, uint32_t u, uint32_t Vu,
+                                           uint32_t deg, const uint32_t *__restrict__ ovf)
+{
+    const uint64_t free = ~s & ((1ull << mb) - 1ull);
+    if (free)
+        return (uint32_t)__builtin_ctzll(free) + 1u;
+    return mex_overflow((uint32_t)mb, u, Vu, deg, ovf);
+}
+
+// Record colour c > cw in successor v's overflow words (colours past v's
 // degree + 1 can never be v's mex and are dropped).
 __device__ __forceinline__ void or_overflow(const uint32_t *__restrict__ V, uint32_t *ovf,
-                                            uint32_t v, uint32_t c, int mb)
+                                            uint32_t v, uint32_t c, uint32_t cw)
 {
-    const uint32_t Vv = V[v], b = c - (uint32_t)mb - 1u;
-    if (b < 32u * ovf_words(V[v + 1] - Vv, mb))
+    const uint32_t Vv = V[v], b = c - cw - 1u;
+    if (b < 32u * ovf_words(V[v + 1] - Vv, cw))
         atomicOr(&ovf[ovf_base(Vv, v) + b / 32u], 1u << (b % 32u));
 }
 
@@ -133,71 +143,83 @@ __device__ __forceinline__ void push_items(bool want, uint32_t v, uint32_t cnt, 
         q[start + k] = make_uint2(v, k);
 }
 
-// Zero the overflow words, the hubs' counters and per-hub positions, and the
-// control block. (Every other vertex's state is written by the orientation
-// pass.) A plain kernel, so the sequence can be captured in a graph.
+// Zero the overflow words, the per-hub positions and the control block.
+// (State words and predecessor counts are written by the orientation pass.)
+// A plain kernel, so the sequence can be captured in a graph.
 __global__ __launch_bounds__(kBlock) void jp_clear(uint64_t nwords, uint32_t *__restrict__ ovf,
-                                                   const uint32_t *__restrict__ hubs, uint32_t nhub,
-                                                   uint64_t *__restrict__ state,
-                                                   uint32_t *__restrict__ npred,
-                                                   uint32_t *__restrict__ hpos, gcxk_ctl *ctl)
+                                                   uint32_t nhub, uint32_t *__restrict__ hpos,
+                                                   gcxk_ctl *ctl)
 {
     const uint64_t stride = (uint64_t)gridDim.x * kBlock;
     const uint64_t t0 = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     for (uint64_t i = t0; i < nwords; i += stride)
         ovf[i] = 0u;
-    for (uint64_t h = t0; h < nhub; h += stride) {
-        state[hubs[h]] = 0ull;
-        npred[hubs[h]] = 0u;
-        hpos[2 * h] = 0u;
-        hpos[2 * h + 1] = 0u;
-    }
+    for (uint64_t h = t0; h < 2ull * nhub; h += stride)
+        hpos[h] = 0u;
     if (t0 < sizeof(gcxk_ctl) / 4)
         reinterpret_cast<uint32_t *>(ctl)[t0] = 0u;
 }
 
-// Orientation of non-hub rows, 16 lanes per vertex. Also writes every
-// vertex's colour slot (1 for degree 0, else 0) and, for non-hubs, the state.
+// Orientation in degree order (perm), 4 lanes per vertex so a wave's 16
+// vertices have near-equal degrees. Also writes every vertex's colour slot
+// (1 for degree 0, else 0), npred and state: rounds layout state[u] =
+// npred << mb, ordered layout state[2u] = state[2u+1] = 0 (report counts).
+// The hubs (perm[0 .. nhub)) get zeros here; jp_init_hub_chunks splits their
+// rows and adds their counts.
+constexpr int kQuad = 4;
+constexpr int kQuadSteps = 4; // 16 edges in flight per vertex
+
 template <typename K>
-__global__ __launch_bounds__(kBlock) void jp_init_groups(uint32_t N, const uint32_t *__restrict__ V,
-                                                         const uint32_t *__restrict__ E,
-                                                         const K *__restrict__ key,
-                                                         uint32_t *__restrict__ npred,
-                                                         uint32_t *__restrict__ esplit,
-                                                         uint64_t *__restrict__ state,
-                                                         int *__restrict__ color, int mb)
+__global__ __launch_bounds__(kBlock) void jp_init_quads(uint32_t N, const uint32_t *__restrict__ V,
+                                                        const uint32_t *__restrict__ E,
+                                                        const K *__restrict__ key,
+                                                        const uint32_t *__restrict__ perm,
+                                                        uint32_t nhub,
+                                                        uint32_t *__restrict__ npred,
+                                                        uint32_t *__restrict__ esplit,
+                                                        uint64_t *__restrict__ state,
+                                                        int *__restrict__ color, int mb,
+                                                        int ordered)
 {
     const uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    const uint32_t u = (uint32_t)(t / kGroup);
-    const int gl = (int)(t % kGroup);
-    const int gw = lane_id() / kGroup;
-    const bool active = u < N;
-    uint32_t beg = 0, end = 0;
-    if (active) {
-        beg = V[u];
-        end = V[u + 1];
+    const uint32_t pi = (uint32_t)(t / kQuad);
+    const int gl = (int)(t % kQuad);
+    const int gsh = (lane_id() / kQuad) * kQuad; // first lane of the quad
+    if (pi >= N) // whole quads leave together
+        return;
+    const uint32_t u = perm[pi];
+    const uint32_t beg = V[u], end = V[u + 1];
+    if (pi < nhub) {
+        if (gl == 0) {
+            color[u] = 0;
+            npred[u] = 0;
+            if (ordered)
+                reinterpret_cast<ulonglong2 *>(state)[u] = make_ulonglong2(0ull, 0ull);
+            else
+                state[u] = 0ull;
+        }
+        return;
     }
-    const bool hub = (end - beg) >= GCXK_HUB_DEG;
     uint32_t np = 0, ns = 0;
-    if (active && !hub && end > beg) {
+    if (end > beg) {
         const K ku = key[u];
         const uint32_t below = (1u << gl) - 1u;
-        for (uint32_t i0 = beg; i0 < end; i0 += 2 * kGroup) {
-            uint32_t v[2];
-            bool in[2], pr[2];
+        for (uint32_t i0 = beg; i0 < end; i0 += kQuad * kQuadSteps) {
+            uint32_t v[kQuadSteps];
+            bool in[kQuadSteps], pr[kQuadSteps];
 #pragma unroll
-            for (int s = 0; s < 2; s++) {
-                const uint32_t i = i0 + kGroup * s + gl;
+            for (int s = 0; s < kQuadSteps; s++) {
+                const uint32_t i = i0 + kQuad * s + gl;
                 in[s] = i < end;
                 v[s] = in[s] ? E[i] : 0u;
             }
 #pragma unroll
-            for (int s = 0; s < 2; s++)
+            for (int s = 0; s < kQuadSteps; s++)
                 pr[s] = in[s] && precedes(key[v[s]], v[s], ku, u);
 #pragma unroll
-            for (int s = 0; s < 2; s++) {
-                const uint32_t gp = (uint32_t)(__ballot(pr[s]) >> (kGroup * gw)) & 0xFFFFu;
-                const uint32_t gs = (uint32_t)(__ballot(in[s] && !pr[s]) >> (kGroup * gw)) & 0xFFFFu;
+            for (int s = 0; s < kQuadSteps; s++) {
+                const uint32_t gp = (uint32_t)(__ballot(pr[s]) >> gsh) & 0xFu;
+                const uint32_t gs = (uint32_t)(__ballot(in[s] && !pr[s]) >> gsh) & 0xFu;
                 if (in[s]) {
                     if (pr[s])
                         esplit[beg + np + __popc(gp & below)] = v[s];
@@ -209,12 +231,13 @@ __global__ __launch_bounds__(kBlock) void jp_init_groups(uint32_t N, const uint3
             }
         }
     }
-    if (active && gl == 0) {
+    if (gl == 0) {
         color[u] = end == beg ? 1 : 0;
-        if (!hub) {
-            npred[u] = np;
+        npred[u] = np;
+        if (ordered)
+            reinterpret_cast<ulonglong2 *>(state)[u] = make_ulonglong2(0ull, 0ull);
+        else
             state[u] = (uint64_t)np << mb;
-        }
     }
 }
 
@@ -280,7 +303,8 @@ __global__ __launch_bounds__(kBlock) void jp_init_hub_chunks(const uint32_t *__r
     }
     if (lane == 0 && npb) {
         atomicAdd(&npred[u], npb);
-        atomicAdd((unsigned long long *)&state[u], (unsigned long long)npb << mb);
+        if (state) // rounds layout only
+            atomicAdd((unsigned long long *)&state[u], (unsigned long long)npb << mb);
     }
 }
 
@@ -434,10 +458,17 @@ __global__ __launch_bounds__(kBlock) void jp_expand(uint32_t N, const uint32_t *
         ctl->total_items = end;
 }
 
+// Two state words per vertex (st[2v], st[2v+1]): each is (reports << mb) |
+// colour bits, word 0 for colours 1..mb, word 1 for mb+1..2mb. A predecessor
+// with colour c reports into the word that holds c (OR, then add 1 << mb, on
+// the same location); colours above 2mb go to the overflow words, waited for,
+// then reported in word 0. u is ready when the two report counts sum to
+// npred[u]: counts only grow and each report sits in the word of its bit, so
+// two separate loads that sum to npred have seen every bit.
 __global__ __launch_bounds__(kBlock) void jp_ordered_kernel(const uint32_t *__restrict__ V,
                                                             const uint32_t *__restrict__ npred,
                                                             const uint32_t *__restrict__ esplit,
-                                                            uint64_t *__restrict__ state,
+                                                            uint64_t *__restrict__ st,
                                                             int *__restrict__ color,
                                                             uint32_t *__restrict__ ovf,
                                                             const uint2 *__restrict__ items,
@@ -446,24 +477,26 @@ __global__ __launch_bounds__(kBlock) void jp_ordered_kernel(const uint32_t *__re
 {
     const uint32_t n = ctl->total_items;
     const int lane = lane_id();
-    const uint64_t one = 1ull << mb;
+    const uint64_t one = 1ull << mb, low = one - 1ull;
+    const uint32_t cw = 2u * (uint32_t)mb;
     const uint32_t w0 = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
     const uint32_t nw = gridDim.x * kWavesPerBlock;
     for (uint32_t j = w0; j < n; j += nw) {
         const uint2 it = items[j];
         const uint32_t u = it.x, k = it.y;
-        uint64_t s = load_state(&state[u]);
+        uint64_t s0 = load_state(&st[2 * (size_t)u]), s1 = load_state(&st[2 * (size_t)u + 1]);
         const uint32_t Vu = V[u], Vu1 = V[u + 1];
-        const uint32_t np = npred[u];
-        const uint32_t i = Vu + np + k * kChunk + (uint32_t)lane;
+        const uint64_t np = npred[u];
+        const uint32_t i = Vu + (uint32_t)np + k * kChunk + (uint32_t)lane;
         const bool has = i < Vu1;
         const uint32_t v = has ? esplit[i] : 0u;
-        if (s >= one) {
+        if ((s0 >> mb) + (s1 >> mb) != np) {
             const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
             uint32_t polls = 0;
             do {
                 __builtin_amdgcn_s_sleep(1);
-                s = load_state(&state[u]);
+                s0 = load_state(&st[2 * (size_t)u]);
+                s1 = load_state(&st[2 * (size_t)u + 1]);
                 if ((++polls & 255u) == 0u &&
                     (__builtin_amdgcn_s_memrealtime() - t0 > spin_ticks || load_word(&ctl->err))) {
                     if (lane == 0)
@@ -471,22 +504,32 @@ __global__ __launch_bounds__(kBlock) void jp_ordered_kernel(const uint32_t *__re
                                            __HIP_MEMORY_SCOPE_AGENT);
                     return;
                 }
-            } while (s >= one);
+            } while ((s0 >> mb) + (s1 >> mb) != np);
         }
-        const uint32_t c = mex_of(s, mb, u, Vu, Vu1 - Vu, ovf);
+        uint32_t c;
+        const uint64_t f0 = ~s0 & low, f1 = ~s1 & low;
+        if (f0)
+            c = (uint32_t)__builtin_ctzll(f0) + 1u;
+        else if (f1)
+            c = (uint32_t)mb + (uint32_t)__builtin_ctzll(f1) + 1u;
+        else
+            c = mex_overflow(cw, u, Vu, Vu1 - Vu, ovf);
         if (k == 0 && lane == 0)
             color[u] = (int)c;
-        if (c <= (uint32_t)mb) {
+        if (c <= cw) {
             if (has) {
-                atomicOr((unsigned long long *)&state[v], 1ull << (c - 1u));
-                atomicAdd((unsigned long long *)&state[v], 0ull - one);
+                const uint32_t wsel = c > (uint32_t)mb ? 1u : 0u;
+                const uint32_t bit = c - 1u - wsel * (uint32_t)mb;
+                unsigned long long *w = (unsigned long long *)&st[2 * (size_t)v + wsel];
+                atomicOr(w, 1ull << bit);
+                atomicAdd(w, (unsigned long long)one);
             }
         } else {
             if (has)
-                or_overflow(V, ovf, v, c, mb);
+                or_overflow(V, ovf, v, c, cw);
             __builtin_amdgcn_s_waitcnt(0);
             if (has)
-                atomicAdd((unsigned long long *)&state[v], 0ull - one);
+                atomicAdd((unsigned long long *)&st[2 * (size_t)v], (unsigned long long)one);
         }
     }
 }
@@ -686,45 +729,46 @@ extern "C" int gcxk_jp_mask_bits(uint32_t maxdeg)
     return 64 - cb;
 }
 
-extern "C" int gcxk_jp_clear(uint32_t N, uint64_t M, uint32_t *ovf, const uint32_t *hubs,
-                             uint32_t nhub, uint64_t *state, uint32_t *npred, uint32_t *hpos,
+extern "C" int gcxk_jp_clear(uint32_t N, uint64_t M, uint32_t *ovf, uint32_t nhub, uint32_t *hpos,
                              gcxk_ctl *ctl, void *stream)
 {
     const uint64_t nwords = gcxk_jp_ovf_words(N, M);
-    uint64_t n = nwords > nhub ? nwords : nhub;
+    uint64_t n = nwords > 2ull * nhub ? nwords : 2ull * nhub;
     if (n < 64)
         n = 64;
     uint32_t grid = div_up(n, kBlock);
     if (grid > 4096)
         grid = 4096;
-    jp_clear<<<grid, kBlock, 0, as_stream(stream)>>>(nwords, ovf, hubs, nhub, state, npred, hpos,
-                                                     ctl);
+    jp_clear<<<grid, kBlock, 0, as_stream(stream)>>>(nwords, ovf, nhub, hpos, ctl);
     GCXK_LAUNCH_CHECK();
     return GCX_OK;
 }
 
 extern "C" int gcxk_jp_init(uint32_t N, const uint32_t *dV, const uint32_t *dE, const void *key,
-                            int key_bits, int mb, uint32_t *npred, uint32_t *esplit,
-                            uint64_t *state, int *color, const uint32_t *hubs, uint32_t nhub,
-                            const uint32_t *hchunk, uint32_t nhchunk, uint32_t *hpos,
-                            void *stream)
+                            int key_bits, int mb, const uint32_t *perm, uint32_t *npred,
+                            uint32_t *esplit, uint64_t *state, int *color, const uint32_t *hubs,
+                            uint32_t nhub, const uint32_t *hchunk, uint32_t nhchunk,
+                            uint32_t *hpos, int ordered, void *stream)
 {
     if (N == 0)
         return GCX_OK;
     hipStream_t s = as_stream(stream);
-    const uint32_t grid = div_up((uint64_t)N * kGroup, kBlock);
+    const uint32_t grid = div_up((uint64_t)N * kQuad, kBlock);
+    uint64_t *hstate = ordered ? nullptr : state;
     if (key_bits == 32) {
-        jp_init_groups<uint32_t><<<grid, kBlock, 0, s>>>(N, dV, dE, (const uint32_t *)key, npred,
-                                                         esplit, state, color, mb);
+        jp_init_quads<uint32_t><<<grid, kBlock, 0, s>>>(N, dV, dE, (const uint32_t *)key, perm,
+                                                        nhub, npred, esplit, state, color, mb,
+                                                        ordered);
         if (nhub)
             jp_init_hub_chunks<uint32_t><<<nhchunk, kBlock, 0, s>>>(
-                dV, dE, (const uint32_t *)key, npred, state, esplit, hubs, hchunk, nhub, hpos, mb);
+                dV, dE, (const uint32_t *)key, npred, hstate, esplit, hubs, hchunk, nhub, hpos, mb);
     } else {
-        jp_init_groups<uint64_t><<<grid, kBlock, 0, s>>>(N, dV, dE, (const uint64_t *)key, npred,
-                                                         esplit, state, color, mb);
+        jp_init_quads<uint64_t><<<grid, kBlock, 0, s>>>(N, dV, dE, (const uint64_t *)key, perm,
+                                                        nhub, npred, esplit, state, color, mb,
+                                                        ordered);
         if (nhub)
             jp_init_hub_chunks<uint64_t><<<nhchunk, kBlock, 0, s>>>(
-                dV, dE, (const uint64_t *)key, npred, state, esplit, hubs, hchunk, nhub, hpos, mb);
+                dV, dE, (const uint64_t *)key, npred, hstate, esplit, hubs, hchunk, nhub, hpos, mb);
     }
     GCXK_LAUNCH_CHECK();
     return GCX_OK;
