@@ -690,11 +690,11 @@ static size_t order_extra(uint32_t N)
 
 uint64_t gcx_workspace_footprint(uint32_t N, uint64_t M)
 {
-    /* per vertex: key 8, count 4, color 4, npred 4, state 16, 4 lists x 4,
+    /* per vertex: key 8, count 4, color 4, npred 4, state 8, 4 lists x 4,
      * hpos 8, rows x,y 2 x 64; per entry: E_split 4; plus the overflow colour
      * words, two item queues and any sort scratch beyond the x rows */
     const uint64_t n = (uint64_t)N + 1;
-    return 188ull * n + 4ull * M + 4ull * gcxk_jp_ovf_words(N, M) +
+    return 180ull * n + 4ull * M + 4ull * gcxk_jp_ovf_words(N, M) +
            2ull * sizeof(gcxk_item) * gcxk_jp_item_capacity(N, M) + 2 * sizeof(gcxk_ctl) +
            order_extra(N);
 }
@@ -745,7 +745,7 @@ int gcx_workspace_create(uint32_t N, uint64_t M, gcx_workspace **out)
              hipMalloc((void **)&ws->y, 64 * n) == hipSuccess &&
              hipMalloc((void **)&ws->ctl, 2 * sizeof(gcxk_ctl)) == hipSuccess &&
              hipMalloc((void **)&ws->npred, 4 * n) == hipSuccess &&
-             hipMalloc((void **)&ws->state, 16 * n) == hipSuccess &&
+             hipMalloc((void **)&ws->state, 8 * n) == hipSuccess &&
              hipMalloc((void **)&ws->hpos, 8 * n) == hipSuccess &&
              hipMalloc((void **)&ws->ovf, 4 * gcxk_jp_ovf_words(N, M)) == hipSuccess &&
              hipMalloc((void **)&ws->esplit, 4 * (M ? M : 1)) == hipSuccess &&
@@ -979,19 +979,19 @@ static int jp_round(void *p, uint32_t r, hipStream_t s)
 }
 
 /* Orientation pass shared by both schedules. */
-static int jp_prepare(gcx_workspace *ws, const gcx_graph *g, int mb, int ordered, hipStream_t s)
+static int jp_prepare(gcx_workspace *ws, const gcx_graph *g, int mb, hipStream_t s)
 {
     TRY(gcxk_jp_clear(g->N, g->M, ws->ovf, g->nhub, ws->hpos, ws->ctl, s));
     return gcxk_jp_init(g->N, g->V, g->E, ws->key, ws->key_bits, mb, g->perm, ws->npred,
                         ws->esplit, ws->state, ws->color, g->hubs, g->nhub, g->hchunk, g->nhchunk,
-                        ws->hpos, ordered, s);
+                        ws->hpos, s);
 }
 
 static int color_rounds(gcx_workspace *ws, const gcx_graph *g, int mb, hipStream_t s)
 {
     void *tok = NULL;
     TRY(gcx__prof_begin(GCX_ST_JP_INIT, s, &tok));
-    TRY(jp_prepare(ws, g, mb, 0, s));
+    TRY(jp_prepare(ws, g, mb, s));
     TRY(gcxk_jp_roots(g->N, g->V, ws->npred, ws->items[0], ws->ctl, s));
     TRY(gcx__prof_end(tok, s));
     TRY(gcx__prof_begin(GCX_ST_JP_ROUNDS, s, &tok));
@@ -1010,7 +1010,7 @@ static int color_ordered(gcx_workspace *ws, const gcx_graph *g, int mb, hipStrea
     if (!g_ordered_done)
         HIPC(hipEventCreateWithFlags(&g_ordered_done, hipEventDisableTiming));
     TRY(gcx__prof_begin(GCX_ST_JP_INIT, s, &tok));
-    TRY(jp_prepare(ws, g, mb, 1, s));
+    TRY(jp_prepare(ws, g, mb, s));
     void *scratch = ws->order_buf ? ws->order_buf : (void *)ws->x;
     uint32_t *cnt = (uint32_t *)ws->y, *incl = cnt + ws->capN;
     TRY(gcxk_jp_order_items(g->N, g->V, ws->npred, ws->key, ws->key_bits, scratch,

@@ -40,7 +40,7 @@ struct gcx_workspace {
     uint32_t *q[2];   /* SL alive lists */
     uint32_t *h[2];   /* SL removed list */
     uint32_t *npred;  /* JP: predecessors per vertex */
-    uint64_t *state;  /* JP: 2 words per vertex (layouts in jp.hip) */
+    uint64_t *state;  /* JP: pending predecessors << mb | colours 1..mb taken */
     uint32_t *ovf;    /* JP: colour-set overflow words (colours > mb) */
     uint32_t *esplit; /* JP: rows split into predecessors | successors (capM) */
     gcxk_item *items[2]; /* JP: round queues; items[0] = greedy-order items */

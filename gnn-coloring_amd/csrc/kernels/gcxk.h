@@ -149,16 +149,13 @@ int gcxk_jp_clear(uint32_t N, uint64_t M, uint32_t *ovf, uint32_t nhub, uint32_t
                   gcxk_ctl *ctl, void *stream);
 /* Orientation: key_bits = 32 or 64, mb = gcxk_jp_mask_bits(maxdeg), perm =
  * the graph's degree order (hubs first). E_split rows = predecessors then
- * successors; npred = predecessor counts; color = 1 for degree-0 vertices,
- * else 0; state (2N words) = npred << mb per vertex for the round schedule
- * (ordered = 0), or two zero report words per vertex for the ordered one.
- * hchunk: per-hub prefix of 4096-edge chunks (nhub + 1 entries), nhchunk =
- * total chunks. */
+ * successors; npred = predecessor counts; state = npred << mb; color = 1 for
+ * degree-0 vertices, else 0. hchunk: per-hub prefix of 4096-edge chunks
+ * (nhub + 1 entries), nhchunk = total chunks. */
 int gcxk_jp_init(uint32_t N, const uint32_t *dV, const uint32_t *dE, const void *key,
                  int key_bits, int mb, const uint32_t *perm, uint32_t *npred, uint32_t *esplit,
                  uint64_t *state, int *color, const uint32_t *hubs, uint32_t nhub,
-                 const uint32_t *hchunk, uint32_t nhchunk, uint32_t *hpos, int ordered,
-                 void *stream);
+                 const uint32_t *hchunk, uint32_t nhchunk, uint32_t *hpos, void *stream);
 /* Round schedule: queue the roots (vertices with successors and no
  * predecessors) as round-0 items in q0 / ctl->qn[0]. */
 int gcxk_jp_roots(uint32_t N, const uint32_t *dV, const uint32_t *npred, gcxk_item *q0,
@@ -175,12 +172,12 @@ size_t gcxk_jp_order_scratch(uint32_t N, int key_bits);
 int gcxk_jp_order_items(uint32_t N, const uint32_t *dV, const uint32_t *npred, const void *key,
                         int key_bits, void *scratch, size_t scratch_bytes, uint32_t *cnt,
                         uint32_t *incl, gcxk_item *items, gcxk_ctl *ctl, void *stream);
-/* Workgroups of the ordered kernel resident on the device at once (0 on
- * error). */
+/* Default workgroups of the ordered kernel: 4 per CU, at most what is
+ * resident at once (0 on error). */
 uint32_t gcxk_jp_ordered_grid(void);
 /* Ordered schedule, step 2: one cooperative launch of `grid` workgroups (0 =
- * gcxk_jp_ordered_grid()); wave w colours items w, w + W, ... waiting on each
- * item's state word. spin_ticks bounds every wait (100 MHz ticks); exceeding
+ * gcxk_jp_ordered_grid(); larger values are capped at the resident maximum);
+ * wave w colours items w, w + W, ... waiting on each item's state word. spin_ticks bounds every wait (100 MHz ticks); exceeding
  * it sets ctl->err and ends the kernel. */
 int gcxk_jp_ordered(const uint32_t *dV, const uint32_t *npred, const uint32_t *esplit,
                     uint64_t *state, int *color, uint32_t *ovf, int mb, const gcxk_item *items,

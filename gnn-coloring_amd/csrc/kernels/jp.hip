@@ -22,10 +22,6 @@
 // load yields both "ready" and the colour set. For c > mb the overflow OR is
 // waited for (s_waitcnt) before the decrement.
 //
-// The ordered schedule keeps two such words per vertex, each with its own
-// report counter (see jp_ordered_kernel), so colours up to 2mb need no
-// overflow words and no wait.
-//
 // A vertex with s successors is ceil(s/64) work items (at least one); one
 // wave handles one item, i.e. up to 64 successor edges, so a hub's
 // successors are spread over many waves. Every item of u recomputes the
@@ -162,10 +158,9 @@ __global__ __launch_bounds__(kBlock) void jp_clear(uint64_t nwords, uint32_t *__
 
 // Orientation in degree order (perm), 4 lanes per vertex so a wave's 16
 // vertices have near-equal degrees. Also writes every vertex's colour slot
-// (1 for degree 0, else 0), npred and state: rounds layout state[u] =
-// npred << mb, ordered layout state[2u] = state[2u+1] = 0 (report counts).
-// The hubs (perm[0 .. nhub)) get zeros here; jp_init_hub_chunks splits their
-// rows and adds their counts.
+// (1 for degree 0, else 0), npred and state[u] = npred << mb. The hubs
+// (perm[0 .. nhub)) get zeros here; jp_init_hub_chunks splits their rows and
+// adds their counts.
 constexpr int kQuad = 4;
 constexpr int kQuadSteps = 4; // 16 edges in flight per vertex
 
@@ -178,8 +173,7 @@ __global__ __launch_bounds__(kBlock) void jp_init_quads(uint32_t N, const uint32
                                                         uint32_t *__restrict__ npred,
                                                         uint32_t *__restrict__ esplit,
                                                         uint64_t *__restrict__ state,
-                                                        int *__restrict__ color, int mb,
-                                                        int ordered)
+                                                        int *__restrict__ color, int mb)
 {
     const uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     const uint32_t pi = (uint32_t)(t / kQuad);
@@ -193,10 +187,7 @@ __global__ __launch_bounds__(kBlock) void jp_init_quads(uint32_t N, const uint32
         if (gl == 0) {
             color[u] = 0;
             npred[u] = 0;
-            if (ordered)
-                reinterpret_cast<ulonglong2 *>(state)[u] = make_ulonglong2(0ull, 0ull);
-            else
-                state[u] = 0ull;
+            state[u] = 0ull;
         }
         return;
     }
@@ -234,10 +225,7 @@ __global__ __launch_bounds__(kBlock) void jp_init_quads(uint32_t N, const uint32
     if (gl == 0) {
         color[u] = end == beg ? 1 : 0;
         npred[u] = np;
-        if (ordered)
-            reinterpret_cast<ulonglong2 *>(state)[u] = make_ulonglong2(0ull, 0ull);
-        else
-            state[u] = (uint64_t)np << mb;
+        state[u] = (uint64_t)np << mb;
     }
 }
 
@@ -303,8 +291,7 @@ __global__ __launch_bounds__(kBlock) void jp_init_hub_chunks(const uint32_t *__r
     }
     if (lane == 0 && npb) {
         atomicAdd(&npred[u], npb);
-        if (state) // rounds layout only
-            atomicAdd((unsigned long long *)&state[u], (unsigned long long)npb << mb);
+        atomicAdd((unsigned long long *)&state[u], (unsigned long long)npb << mb);
     }
 }
 
@@ -458,17 +445,23 @@ __global__ __launch_bounds__(kBlock) void jp_expand(uint32_t N, const uint32_t *
         ctl->total_items = end;
 }
 
-// Two state words per vertex (st[2v], st[2v+1]): each is (reports << mb) |
-// colour bits, word 0 for colours 1..mb, word 1 for mb+1..2mb. A predecessor
-// with colour c reports into the word that holds c (OR, then add 1 << mb, on
-// the same location); colours above 2mb go to the overflow words, waited for,
-// then reported in word 0. u is ready when the two report counts sum to
-// npred[u]: counts only grow and each report sits in the word of its bit, so
-// two separate loads that sum to npred have seen every bit.
+// Waits poll with a short exponential backoff (64 .. 512 clocks between
+// loads): thousands of waiting waves otherwise load the memory system enough
+// to slow the critical chain.
+__device__ __forceinline__ void poll_sleep(uint32_t step)
+{
+    switch (step) {
+    case 0: __builtin_amdgcn_s_sleep(1); break;
+    case 1: __builtin_amdgcn_s_sleep(2); break;
+    case 2: __builtin_amdgcn_s_sleep(4); break;
+    default: __builtin_amdgcn_s_sleep(8); break;
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void jp_ordered_kernel(const uint32_t *__restrict__ V,
                                                             const uint32_t *__restrict__ npred,
                                                             const uint32_t *__restrict__ esplit,
-                                                            uint64_t *__restrict__ st,
+                                                            uint64_t *__restrict__ state,
                                                             int *__restrict__ color,
                                                             uint32_t *__restrict__ ovf,
                                                             const uint2 *__restrict__ items,
@@ -477,26 +470,24 @@ __global__ __launch_bounds__(kBlock) void jp_ordered_kernel(const uint32_t *__re
 {
     const uint32_t n = ctl->total_items;
     const int lane = lane_id();
-    const uint64_t one = 1ull << mb, low = one - 1ull;
-    const uint32_t cw = 2u * (uint32_t)mb;
+    const uint64_t one = 1ull << mb;
     const uint32_t w0 = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
     const uint32_t nw = gridDim.x * kWavesPerBlock;
     for (uint32_t j = w0; j < n; j += nw) {
         const uint2 it = items[j];
         const uint32_t u = it.x, k = it.y;
-        uint64_t s0 = load_state(&st[2 * (size_t)u]), s1 = load_state(&st[2 * (size_t)u + 1]);
+        uint64_t s = load_state(&state[u]);
         const uint32_t Vu = V[u], Vu1 = V[u + 1];
-        const uint64_t np = npred[u];
-        const uint32_t i = Vu + (uint32_t)np + k * kChunk + (uint32_t)lane;
+        const uint32_t np = npred[u];
+        const uint32_t i = Vu + np + k * kChunk + (uint32_t)lane;
         const bool has = i < Vu1;
         const uint32_t v = has ? esplit[i] : 0u;
-        if ((s0 >> mb) + (s1 >> mb) != np) {
+        if (s >= one) {
             const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
             uint32_t polls = 0;
             do {
-                __builtin_amdgcn_s_sleep(1);
-                s0 = load_state(&st[2 * (size_t)u]);
-                s1 = load_state(&st[2 * (size_t)u + 1]);
+                poll_sleep(polls);
+                s = load_state(&state[u]);
                 if ((++polls & 255u) == 0u &&
                     (__builtin_amdgcn_s_memrealtime() - t0 > spin_ticks || load_word(&ctl->err))) {
                     if (lane == 0)
@@ -504,32 +495,22 @@ __global__ __launch_bounds__(kBlock) void jp_ordered_kernel(const uint32_t *__re
                                            __HIP_MEMORY_SCOPE_AGENT);
                     return;
                 }
-            } while ((s0 >> mb) + (s1 >> mb) != np);
+            } while (s >= one);
         }
-        uint32_t c;
-        const uint64_t f0 = ~s0 & low, f1 = ~s1 & low;
-        if (f0)
-            c = (uint32_t)__builtin_ctzll(f0) + 1u;
-        else if (f1)
-            c = (uint32_t)mb + (uint32_t)__builtin_ctzll(f1) + 1u;
-        else
-            c = mex_overflow(cw, u, Vu, Vu1 - Vu, ovf);
+        const uint32_t c = mex_of(s, mb, u, Vu, Vu1 - Vu, ovf);
         if (k == 0 && lane == 0)
             color[u] = (int)c;
-        if (c <= cw) {
+        if (c <= (uint32_t)mb) {
             if (has) {
-                const uint32_t wsel = c > (uint32_t)mb ? 1u : 0u;
-                const uint32_t bit = c - 1u - wsel * (uint32_t)mb;
-                unsigned long long *w = (unsigned long long *)&st[2 * (size_t)v + wsel];
-                atomicOr(w, 1ull << bit);
-                atomicAdd(w, (unsigned long long)one);
+                atomicOr((unsigned long long *)&state[v], 1ull << (c - 1u));
+                atomicAdd((unsigned long long *)&state[v], 0ull - one);
             }
         } else {
             if (has)
-                or_overflow(V, ovf, v, c, cw);
+                or_overflow(V, ovf, v, c, (uint32_t)mb);
             __builtin_amdgcn_s_waitcnt(0);
             if (has)
-                atomicAdd((unsigned long long *)&st[2 * (size_t)v], (unsigned long long)one);
+                atomicAdd((unsigned long long *)&state[v], 0ull - one);
         }
     }
 }
@@ -681,8 +662,9 @@ __global__ __launch_bounds__(kBlock) void mex_check_hubs(const uint32_t *__restr
         atomicAdd((unsigned long long *)&ctl->viol, 1ull);
 }
 
-// Workgroups of the ordered kernel that fit on the device at once.
-uint32_t g_ordered_grid = 0;
+// Workgroups of the ordered kernel: default, and how many fit on the device
+// at once.
+uint32_t g_ordered_grid = 0, g_ordered_cap = 0;
 
 // Byte offset of the sort / scan temporary storage inside the scratch:
 // after two key arrays and two id arrays, rounded up to 256 bytes.
@@ -748,27 +730,24 @@ extern "C" int gcxk_jp_init(uint32_t N, const uint32_t *dV, const uint32_t *dE, 
                             int key_bits, int mb, const uint32_t *perm, uint32_t *npred,
                             uint32_t *esplit, uint64_t *state, int *color, const uint32_t *hubs,
                             uint32_t nhub, const uint32_t *hchunk, uint32_t nhchunk,
-                            uint32_t *hpos, int ordered, void *stream)
+                            uint32_t *hpos, void *stream)
 {
     if (N == 0)
         return GCX_OK;
     hipStream_t s = as_stream(stream);
     const uint32_t grid = div_up((uint64_t)N * kQuad, kBlock);
-    uint64_t *hstate = ordered ? nullptr : state;
     if (key_bits == 32) {
         jp_init_quads<uint32_t><<<grid, kBlock, 0, s>>>(N, dV, dE, (const uint32_t *)key, perm,
-                                                        nhub, npred, esplit, state, color, mb,
-                                                        ordered);
+                                                        nhub, npred, esplit, state, color, mb);
         if (nhub)
             jp_init_hub_chunks<uint32_t><<<nhchunk, kBlock, 0, s>>>(
-                dV, dE, (const uint32_t *)key, npred, hstate, esplit, hubs, hchunk, nhub, hpos, mb);
+                dV, dE, (const uint32_t *)key, npred, state, esplit, hubs, hchunk, nhub, hpos, mb);
     } else {
         jp_init_quads<uint64_t><<<grid, kBlock, 0, s>>>(N, dV, dE, (const uint64_t *)key, perm,
-                                                        nhub, npred, esplit, state, color, mb,
-                                                        ordered);
+                                                        nhub, npred, esplit, state, color, mb);
         if (nhub)
             jp_init_hub_chunks<uint64_t><<<nhchunk, kBlock, 0, s>>>(
-                dV, dE, (const uint64_t *)key, npred, hstate, esplit, hubs, hchunk, nhub, hpos, mb);
+                dV, dE, (const uint64_t *)key, npred, state, esplit, hubs, hchunk, nhub, hpos, mb);
     }
     GCXK_LAUNCH_CHECK();
     return GCX_OK;
@@ -861,7 +840,12 @@ extern "C" uint32_t gcxk_jp_ordered_grid(void)
                 hipSuccess ||
             per_cu <= 0)
             return 0;
-        g_ordered_grid = (uint32_t)per_cu * (uint32_t)prop.multiProcessorCount;
+        // half the resident capacity (4 workgroups per CU at full
+        // occupancy): fewer pollers, the same throughput on the critical chain
+        const uint32_t cap = (uint32_t)per_cu * (uint32_t)prop.multiProcessorCount;
+        const uint32_t want = 4u * (uint32_t)prop.multiProcessorCount;
+        g_ordered_grid = want < cap ? want : cap;
+        g_ordered_cap = cap;
     }
     return g_ordered_grid;
 }
@@ -871,11 +855,13 @@ extern "C" int gcxk_jp_ordered(const uint32_t *dV, const uint32_t *npred, const 
                                const gcxk_item *items, gcxk_ctl *ctl, uint32_t grid,
                                uint64_t spin_ticks, void *stream)
 {
-    const uint32_t cap = gcxk_jp_ordered_grid();
-    if (!cap)
+    const uint32_t dflt = gcxk_jp_ordered_grid();
+    if (!dflt)
         return GCX_ERR_HIP;
-    if (grid == 0 || grid > cap)
-        grid = cap;
+    if (grid == 0)
+        grid = dflt;
+    if (grid > g_ordered_cap)
+        grid = g_ordered_cap;
     const uint2 *it = reinterpret_cast<const uint2 *>(items);
     void *args[] = {(void *)&dV,  (void *)&npred, (void *)&esplit, (void *)&state,
                     (void *)&color, (void *)&ovf, (void *)&it,     (void *)&ctl,
