@@ -6,7 +6,9 @@ Jones-Plassmann colouring of an R-MAT scale-22 graph (4,194,304 vertices,
 edge factor 16, a/b/c/d = .57/.19/.19/.05, seed 1, symmetrised, self loops
 and repeats removed, int32 CSR), with the CSR already resident in HBM. This
 is the FP32 path, bit-exact with the reference. The metric is colored edges/s
-= CSR entries (V[N]) / seconds per step.
+= CSR entries (V[N]) / seconds per step. A second leg times the BF16 GNN
+(bf16 rows, MFMA layer products, FP32 accumulation) on the same graph and
+reports it beside the FP32 value with its colour count (must be within +1).
 
 Prints ONE JSON line. `--gpus N` with N > 1 exits: this build runs on one GPU.
 """
@@ -28,8 +30,7 @@ HBM_PEAK_GBS = 8000.0
 
 # Algorithmic bytes per launch (model: every neighbour-row gather counted once,
 # 64-byte FP32 rows, 8-byte priorities; N vertices, M CSR entries).
-def algorithmic_bytes(stage, N, M, launches=1):
-    R = 64
+def algorithmic_bytes(stage, N, M, launches=1, R=64):
     per_stage = {
         "gnn_in": 4 * N + 12 * M + R * N,
         "gnn_hid": 4 * N + (4 + R) * M + 2 * R * N,
@@ -70,6 +71,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--scale", type=int, default=22)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-bf16", action="store_true")
     ap.add_argument("--schedule", choices=["auto", "rounds", "ordered"], default="auto")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -161,9 +163,49 @@ def main():
     per_launch = algorithmic_bytes(dominant, N, M, launches if dominant == "jp_rounds" else 1)
     achieved = per_launch / (avg_ms * 1e-3) / 1e9
     prof_path = os.path.join(ROOT, "profiles", "traffic.json")
-    traffic = None
-    if os.path.exists(prof_path):
-        traffic = json.load(open(prof_path)).get(f"rmat{args.scale}_{dominant}")
+    tmap = json.load(open(prof_path)) if os.path.exists(prof_path) else {}
+    traffic = tmap.get(f"rmat{args.scale}_{dominant}")
+    # every stage against the same model (ms per launch from the HIP events)
+    stage_roof = {}
+    for k in stage_ms:
+        if stage_ms[k] <= 0 or k not in ("gnn_in", "gnn_hid", "gnn_out", "jp_init", "jp_rounds"):
+            continue
+        nl = launches * args.steps if k == "jp_rounds" else counts[k]
+        per = algorithmic_bytes(k, N, M, launches if k == "jp_rounds" else 1)
+        ms_l = stage_ms[k] / max(nl, 1)
+        t = tmap.get(f"rmat{args.scale}_{k}")
+        stage_roof[k] = {"ms_per_launch": ms_l, "algorithmic_gbs": per / (ms_l * 1e-3) / 1e9,
+                         "frac": per / (ms_l * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                         "traffic_bytes": t, "traffic_gbs": (t / (ms_l * 1e-3) / 1e9) if t else None}
+
+    # BF16 leg: same graph and schedule, GNN rows in bf16 and layer products
+    # on MFMA; colours must stay within +1 of FP32
+    bf16 = None
+    if not args.no_bf16:
+        def step16():
+            ws.order(g, gcx.GNN, model, gcx.BF16, stream)
+            ws.color(g, stream)
+        for _ in range(max(args.warmup, 1)):
+            step16()
+        stream.sync()
+        gcx.profile_enable(True)
+        gcx.profile_read()
+        w0 = time.perf_counter()
+        for _ in range(args.steps):
+            step16()
+        stream.sync()
+        wall16 = time.perf_counter() - w0
+        prof16 = gcx.profile_read()
+        gcx.profile_enable(False)
+        c16 = ws.validate(g, stream)
+        v16 = ws.mex_check(g, stream)
+        bf16 = {"value": M / (wall16 / args.steps), "ms_per_step": 1000.0 * wall16 / args.steps,
+                "colors": c16, "mex_violations": v16, "colors_within_1_of_fp32": 0 < c16 <= colors + 1,
+                "speedup_vs_fp32": ms_per_step / (1000.0 * wall16 / args.steps),
+                "stage_ms_per_step": {k: v[0] / args.steps for k, v in prof16.items()}}
+        if c16 == 0 or v16 != 0:
+            print(f"[bench] BF16 VALIDATION FAILED colors={c16} mex_violations={v16}", file=sys.stderr)
+            sys.exit(1)
 
     out = {
         "metric": METRIC,
@@ -194,6 +236,8 @@ def main():
         "jp_launches_per_step": launches,
         "jp_ordered_fell_back": fell_back,
         "stage_ms_per_step": {k: v / args.steps for k, v in stage_ms.items()},
+        "stage_rooflines": stage_roof,
+        "bf16": bf16,
         "event_ms_per_step": ev_ms / args.steps,
         "bit_exact_vs_reference": golden,
     }
