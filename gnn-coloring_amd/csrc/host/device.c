@@ -5,6 +5,7 @@
  * launches and polls.
  */
 #include <hip/hip_runtime_api.h>
+#include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -71,6 +72,14 @@ int gcx__ensure_init(void)
     }
     if (hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking) != hipSuccess)
         return g_init_rc = GCX_ERR_HIP;
+    /* Graphs, workspaces and temporaries come from the device's stream-ordered
+     * pool; the pool keeps freed memory for reuse instead of returning it at
+     * every synchronisation. */
+    hipMemPool_t pool;
+    uint64_t keep = UINT64_MAX;
+    if (hipDeviceGetDefaultMemPool(&pool, 0) != hipSuccess ||
+        hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep) != hipSuccess)
+        return g_init_rc = GCX_ERR_HIP;
     g_round_grid = gcxk_round_grid();
     return g_init_rc = GCX_OK;
 }
@@ -80,6 +89,20 @@ int gcx_init(void) { return gcx__ensure_init(); }
 void *gcx__default_stream(void) { return (void *)g_stream; }
 
 void *gcx__pick(void *stream) { return stream ? stream : (void *)g_stream; }
+
+int gcx__dalloc(void **p, size_t bytes, void *stream)
+{
+    *p = NULL;
+    return hipMallocAsync(p, bytes ? bytes : 16, (hipStream_t)gcx__pick(stream)) == hipSuccess
+               ? GCX_OK
+               : GCX_ERR_NOMEM;
+}
+
+void gcx__dfree(void *p, void *stream)
+{
+    if (p)
+        (void)hipFreeAsync(p, (hipStream_t)gcx__pick(stream));
+}
 
 /* ------------------------------------------------------------ streams/events */
 
@@ -303,10 +326,10 @@ static int graph_stats(gcx_graph *g, hipStream_t s, uint32_t *tmp)
 {
     TRY(gcxk_graph_stats(g->N, g->V, g->hubs, tmp, &g->nhub, &g->maxdeg, s));
     if (!g->perm)
-        HIPC(hipMalloc((void **)&g->perm, 4ull * (g->N ? g->N : 1)));
+        TRY(gcx__dalloc((void **)&g->perm, 4ull * (g->N ? g->N : 1), s));
     TRY(gcxk_degree_order(g->N, g->V, g->maxdeg, g->perm, s));
     if (g->hchunk) {
-        (void)hipFree(g->hchunk);
+        gcx__dfree(g->hchunk, s);
         g->hchunk = NULL;
     }
     g->nhchunk = 0;
@@ -326,7 +349,7 @@ static int graph_stats(gcx_graph *g, hipStream_t s, uint32_t *tmp)
         goto done;
     qsort(ids, n, sizeof(uint32_t), cmp_u32);
     if (hipMemcpyAsync(g->hubs, ids, 4ull * n, hipMemcpyHostToDevice, s) != hipSuccess ||
-        hipMalloc((void **)&ddeg, 4ull * n) != hipSuccess)
+        gcx__dalloc((void **)&ddeg, 4ull * n, s) != GCX_OK)
         goto done;
     if ((rc = gcxk_gather_degrees(g->V, g->hubs, n, ddeg, s)) != GCX_OK)
         goto done;
@@ -337,50 +360,52 @@ static int graph_stats(gcx_graph *g, hipStream_t s, uint32_t *tmp)
     chunk[0] = 0;
     for (uint32_t h = 0; h < n; h++)
         chunk[h + 1] = chunk[h] + (deg[h] + GCXK_HUB_CHUNK_EDGES - 1) / GCXK_HUB_CHUNK_EDGES;
-    if (hipMalloc((void **)&g->hchunk, 4ull * (n + 1)) != hipSuccess ||
+    if (gcx__dalloc((void **)&g->hchunk, 4ull * (n + 1), s) != GCX_OK ||
         hipMemcpyAsync(g->hchunk, chunk, 4ull * (n + 1), hipMemcpyHostToDevice, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
         goto done;
     g->nhchunk = chunk[n];
     rc = GCX_OK;
 done:
-    if (ddeg)
-        (void)hipFree(ddeg);
+    gcx__dfree(ddeg, s);
     free(ids);
     free(deg);
     free(chunk);
     return rc;
 }
 
-static int graph_finish(gcx_graph *g, void *stream)
+static int graph_finish(gcx_graph *g, hipStream_t s)
 {
     uint32_t *tmp = NULL;
-    HIPC(hipMalloc((void **)&g->hubs, 4ull * (g->N ? g->N : 1)));
-    HIPC(hipMalloc((void **)&tmp, 16));
-    int rc = graph_stats(g, (hipStream_t)stream, tmp);
-    (void)hipFree(tmp);
+    TRY(gcx__dalloc((void **)&g->hubs, 4ull * (g->N ? g->N : 1), s));
+    TRY(gcx__dalloc((void **)&tmp, 16, s));
+    int rc = graph_stats(g, s, tmp);
+    gcx__dfree(tmp, s);
     return rc;
 }
 
-int gcx_graph_free(gcx_graph *g)
+int gcx_graph_free_on(gcx_graph *g, void *stream)
 {
     if (!g)
         return GCX_OK;
-    if (g->V)
-        (void)hipFree(g->V);
-    if (g->E)
-        (void)hipFree(g->E);
-    if (g->hubs)
-        (void)hipFree(g->hubs);
-    if (g->hchunk)
-        (void)hipFree(g->hchunk);
-    if (g->perm)
-        (void)hipFree(g->perm);
+    void *bufs[] = {g->V, g->E, g->hubs, g->hchunk, g->perm};
+    for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); i++)
+        gcx__dfree(bufs[i], stream);
     free(g);
     return GCX_OK;
 }
 
-int gcx__graph_alloc(uint32_t N, uint64_t M, gcx_graph **out)
+int gcx_graph_free(gcx_graph *g)
+{
+    /* any stream may still read the graph: finish all work first */
+    if (g && hipDeviceSynchronize() != hipSuccess) {
+        (void)gcx_graph_free_on(g, NULL);
+        return GCX_ERR_HIP;
+    }
+    return gcx_graph_free_on(g, NULL);
+}
+
+int gcx__graph_alloc(uint32_t N, uint64_t M, void *stream, gcx_graph **out)
 {
     TRY(gcx__ensure_init());
     *out = NULL;
@@ -389,11 +414,11 @@ int gcx__graph_alloc(uint32_t N, uint64_t M, gcx_graph **out)
         return GCX_ERR_NOMEM;
     g->N = N;
     g->M = M;
-    if (hipMalloc((void **)&g->V, 4ull * (N + 1ull)) != hipSuccess ||
-        hipMalloc((void **)&g->E, 4ull * (M ? M : 1)) != hipSuccess ||
-        hipMalloc((void **)&g->hubs, 4ull * (N ? N : 1)) != hipSuccess ||
-        hipMalloc((void **)&g->perm, 4ull * (N ? N : 1)) != hipSuccess) {
-        gcx_graph_free(g);
+    if (gcx__dalloc((void **)&g->V, 4ull * (N + 1ull), stream) != GCX_OK ||
+        gcx__dalloc((void **)&g->E, 4ull * (M ? M : 1), stream) != GCX_OK ||
+        gcx__dalloc((void **)&g->hubs, 4ull * (N ? N : 1), stream) != GCX_OK ||
+        gcx__dalloc((void **)&g->perm, 4ull * (N ? N : 1), stream) != GCX_OK) {
+        gcx_graph_free_on(g, stream);
         return GCX_ERR_NOMEM;
     }
     *out = g;
@@ -413,14 +438,14 @@ int gcx__graph_fill(gcx_graph *g, uint32_t N, const uint32_t *V, const uint32_t 
         HIPC(hipMemcpyAsync(g->E, E, 4ull * V[N], hipMemcpyHostToDevice, s));
     g->N = N;
     g->M = V[N];
-    HIPC(hipMalloc((void **)&tmp, 16));
+    TRY(gcx__dalloc((void **)&tmp, 16, s));
     int ok = 0;
     int rc = gcxk_csr_bounds(g->N, g->M, g->V, g->E, tmp, &ok, s);
     if (rc == GCX_OK && !ok)
         rc = GCX_ERR_ARG; /* indices out of range: refuse before any kernel reads them */
     if (rc == GCX_OK)
         rc = graph_stats(g, s, tmp);
-    (void)hipFree(tmp);
+    gcx__dfree(tmp, s);
     return rc;
 }
 
@@ -432,10 +457,10 @@ int gcx_graph_from_csr(uint32_t N, const uint32_t *V, const uint32_t *E, void *s
     if (N == 0 || !V || (V[N] && !E))
         return GCX_ERR_ARG;
     gcx_graph *g = NULL;
-    TRY(gcx__graph_alloc(N, V[N], &g));
+    TRY(gcx__graph_alloc(N, V[N], stream, &g));
     int rc = gcx__graph_fill(g, N, V, E, stream);
     if (rc != GCX_OK) {
-        gcx_graph_free(g);
+        gcx_graph_free_on(g, stream);
         return rc;
     }
     *out = g;
@@ -449,8 +474,8 @@ static int graph_from_device_pairs(uint32_t N, uint64_t L, uint32_t *dI, uint32_
     if (!g)
         return GCX_ERR_NOMEM;
     g->N = N;
-    int rc = GCX_ERR_HIP;
-    if (hipMalloc((void **)&g->V, 4ull * (N + 1ull)) != hipSuccess)
+    int rc = GCX_ERR_NOMEM;
+    if (gcx__dalloc((void **)&g->V, 4ull * (N + 1ull), s) != GCX_OK)
         goto fail;
     if ((rc = gcxk_csr_from_pairs(N, L, dI, dJ, g->V, &g->E, &g->M, s)) != GCX_OK)
         goto fail;
@@ -459,7 +484,8 @@ static int graph_from_device_pairs(uint32_t N, uint64_t L, uint32_t *dI, uint32_
     *out = g;
     return GCX_OK;
 fail:
-    gcx_graph_free(g);
+    (void)hipStreamSynchronize(s);
+    gcx_graph_free_on(g, s);
     return rc;
 }
 
@@ -476,19 +502,16 @@ int gcx_graph_from_pairs(uint32_t N, uint64_t L, const uint32_t *I, const uint32
     hipStream_t s = (hipStream_t)gcx__pick(stream);
     uint32_t *dI = NULL, *dJ = NULL;
     int rc = GCX_ERR_HIP;
-    if (hipMalloc((void **)&dI, 4ull * (L ? L : 1)) != hipSuccess ||
-        hipMalloc((void **)&dJ, 4ull * (L ? L : 1)) != hipSuccess)
+    if (gcx__dalloc((void **)&dI, 4ull * (L ? L : 1), s) != GCX_OK ||
+        gcx__dalloc((void **)&dJ, 4ull * (L ? L : 1), s) != GCX_OK)
         goto done;
     if (L && (hipMemcpyAsync(dI, I, 4ull * L, hipMemcpyHostToDevice, s) != hipSuccess ||
               hipMemcpyAsync(dJ, J, 4ull * L, hipMemcpyHostToDevice, s) != hipSuccess))
         goto done;
     rc = graph_from_device_pairs(N, L, dI, dJ, s, out);
 done:
-    (void)hipStreamSynchronize(s);
-    if (dI)
-        (void)hipFree(dI);
-    if (dJ)
-        (void)hipFree(dJ);
+    gcx__dfree(dI, s);
+    gcx__dfree(dJ, s);
     return rc;
 }
 
@@ -505,18 +528,15 @@ int gcx_graph_rmat(int scale, uint32_t edge_factor, double a, double b, double c
     gcx_rmat_thresholds(a, b, c, thr);
     uint32_t *dI = NULL, *dJ = NULL;
     int rc = GCX_ERR_HIP;
-    if (hipMalloc((void **)&dI, 4ull * L) != hipSuccess ||
-        hipMalloc((void **)&dJ, 4ull * L) != hipSuccess)
+    if (gcx__dalloc((void **)&dI, 4ull * L, s) != GCX_OK ||
+        gcx__dalloc((void **)&dJ, 4ull * L, s) != GCX_OK)
         goto done;
     if ((rc = gcxk_gen_rmat(scale, L, thr[0], thr[1], thr[2], seed, dI, dJ, s)) != GCX_OK)
         goto done;
     rc = graph_from_device_pairs(1u << scale, L, dI, dJ, s, out);
 done:
-    (void)hipStreamSynchronize(s);
-    if (dI)
-        (void)hipFree(dI);
-    if (dJ)
-        (void)hipFree(dJ);
+    gcx__dfree(dI, s);
+    gcx__dfree(dJ, s);
     return rc;
 }
 
@@ -529,18 +549,15 @@ int gcx_graph_er(uint32_t n, uint64_t m, uint64_t seed, void *stream, gcx_graph 
     hipStream_t s = (hipStream_t)gcx__pick(stream);
     uint32_t *dI = NULL, *dJ = NULL;
     int rc = GCX_ERR_HIP;
-    if (hipMalloc((void **)&dI, 4ull * (m ? m : 1)) != hipSuccess ||
-        hipMalloc((void **)&dJ, 4ull * (m ? m : 1)) != hipSuccess)
+    if (gcx__dalloc((void **)&dI, 4ull * (m ? m : 1), s) != GCX_OK ||
+        gcx__dalloc((void **)&dJ, 4ull * (m ? m : 1), s) != GCX_OK)
         goto done;
     if ((rc = gcxk_gen_er(n, m, seed, dI, dJ, s)) != GCX_OK)
         goto done;
     rc = graph_from_device_pairs(n, m, dI, dJ, s, out);
 done:
-    (void)hipStreamSynchronize(s);
-    if (dI)
-        (void)hipFree(dI);
-    if (dJ)
-        (void)hipFree(dJ);
+    gcx__dfree(dI, s);
+    gcx__dfree(dJ, s);
     return rc;
 }
 
@@ -556,11 +573,8 @@ int gcx_graph_rgg(uint32_t n, double radius, uint64_t seed, void *stream, gcx_gr
     int rc = gcxk_gen_rgg(n, gcx_rgg_cells(radius), gcx_rgg_r2(radius), seed, &dI, &dJ, &L, s);
     if (rc == GCX_OK)
         rc = graph_from_device_pairs(n, L, dI, dJ, s, out);
-    (void)hipStreamSynchronize(s);
-    if (dI)
-        (void)hipFree(dI);
-    if (dJ)
-        (void)hipFree(dJ);
+    gcx__dfree(dI, s);
+    gcx__dfree(dJ, s);
     return rc;
 }
 
@@ -587,9 +601,9 @@ int gcx_graph_check(const gcx_graph *g, void *stream, int *valid)
     if (!g || !valid)
         return GCX_ERR_ARG;
     uint32_t *tmp = NULL;
-    HIPC(hipMalloc((void **)&tmp, 16));
+    TRY(gcx__dalloc((void **)&tmp, 16, stream));
     int rc = gcxk_graph_check(g->N, g->V, g->E, tmp, valid, gcx__pick(stream));
-    (void)hipFree(tmp);
+    gcx__dfree(tmp, stream);
     return rc;
 }
 
@@ -699,7 +713,39 @@ uint64_t gcx_workspace_footprint(uint32_t N, uint64_t M)
            order_extra(N);
 }
 
-int gcx_workspace_free(gcx_workspace *ws)
+/* Pinned control-block mirrors are recycled: freeing pinned host memory
+ * synchronises the whole device, which would stall other streams' jobs. */
+static pthread_mutex_t g_pin_lock = PTHREAD_MUTEX_INITIALIZER;
+static void *g_pin_free[64];
+static int g_pin_n = 0;
+
+static void *pinned_ctl_get(void)
+{
+    void *p = NULL;
+    pthread_mutex_lock(&g_pin_lock);
+    if (g_pin_n > 0)
+        p = g_pin_free[--g_pin_n];
+    pthread_mutex_unlock(&g_pin_lock);
+    if (!p && hipHostMalloc(&p, 4 * sizeof(gcxk_ctl), hipHostMallocDefault) != hipSuccess)
+        p = NULL;
+    return p;
+}
+
+static void pinned_ctl_put(void *p)
+{
+    if (!p)
+        return;
+    pthread_mutex_lock(&g_pin_lock);
+    if (g_pin_n < 64) {
+        g_pin_free[g_pin_n++] = p;
+        p = NULL;
+    }
+    pthread_mutex_unlock(&g_pin_lock);
+    if (p)
+        (void)hipHostFree(p);
+}
+
+int gcx_workspace_free_on(gcx_workspace *ws, void *stream)
 {
     if (!ws)
         return GCX_OK;
@@ -708,10 +754,8 @@ int gcx_workspace_free(gcx_workspace *ws)
                     ws->npred,  ws->state, ws->ovf,      ws->esplit,   ws->items[0],
                     ws->items[1], ws->hpos, ws->order_buf};
     for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); i++)
-        if (bufs[i])
-            (void)hipFree(bufs[i]);
-    if (ws->hctl)
-        (void)hipHostFree(ws->hctl);
+        gcx__dfree(bufs[i], stream);
+    pinned_ctl_put(ws->hctl);
     if (ws->aux)
         (void)hipStreamDestroy((hipStream_t)ws->aux);
     void *evs[] = {ws->ev_fork, ws->ev_join, ws->ev_poll[0], ws->ev_poll[1]};
@@ -722,7 +766,17 @@ int gcx_workspace_free(gcx_workspace *ws)
     return GCX_OK;
 }
 
-int gcx_workspace_create(uint32_t N, uint64_t M, gcx_workspace **out)
+int gcx_workspace_free(gcx_workspace *ws)
+{
+    /* any stream may still use the workspace: finish all work first */
+    if (ws && hipDeviceSynchronize() != hipSuccess) {
+        (void)gcx_workspace_free_on(ws, NULL);
+        return GCX_ERR_HIP;
+    }
+    return gcx_workspace_free_on(ws, NULL);
+}
+
+int gcx_workspace_create_on(uint32_t N, uint64_t M, void *stream, gcx_workspace **out)
 {
     TRY(gcx__ensure_init());
     *out = NULL;
@@ -734,32 +788,32 @@ int gcx_workspace_create(uint32_t N, uint64_t M, gcx_workspace **out)
     ws->capN = N;
     ws->capM = M;
     const size_t n = (size_t)N + 1;
-    int ok = hipMalloc(&ws->key, 8 * n) == hipSuccess &&
-             hipMalloc((void **)&ws->count, 4 * n) == hipSuccess &&
-             hipMalloc((void **)&ws->color, 4 * n) == hipSuccess &&
-             hipMalloc((void **)&ws->q[0], 4 * n) == hipSuccess &&
-             hipMalloc((void **)&ws->q[1], 4 * n) == hipSuccess &&
-             hipMalloc((void **)&ws->h[0], 4 * n) == hipSuccess &&
-             hipMalloc((void **)&ws->h[1], 4 * n) == hipSuccess &&
-             hipMalloc((void **)&ws->x, 64 * n) == hipSuccess &&
-             hipMalloc((void **)&ws->y, 64 * n) == hipSuccess &&
-             hipMalloc((void **)&ws->ctl, 2 * sizeof(gcxk_ctl)) == hipSuccess &&
-             hipMalloc((void **)&ws->npred, 4 * n) == hipSuccess &&
-             hipMalloc((void **)&ws->state, 8 * n) == hipSuccess &&
-             hipMalloc((void **)&ws->hpos, 8 * n) == hipSuccess &&
-             hipMalloc((void **)&ws->ovf, 4 * gcxk_jp_ovf_words(N, M)) == hipSuccess &&
-             hipMalloc((void **)&ws->esplit, 4 * (M ? M : 1)) == hipSuccess &&
-             hipMalloc((void **)&ws->items[0], sizeof(gcxk_item) * gcxk_jp_item_capacity(N, M)) ==
-                 hipSuccess &&
-             hipMalloc((void **)&ws->items[1], sizeof(gcxk_item) * gcxk_jp_item_capacity(N, M)) ==
-                 hipSuccess &&
-             hipHostMalloc((void **)&ws->hctl, 4 * sizeof(gcxk_ctl), hipHostMallocDefault) ==
-                 hipSuccess;
+    void *st = stream;
+    int ok = gcx__dalloc(&ws->key, 8 * n, st) == GCX_OK &&
+             gcx__dalloc((void **)&ws->count, 4 * n, st) == GCX_OK &&
+             gcx__dalloc((void **)&ws->color, 4 * n, st) == GCX_OK &&
+             gcx__dalloc((void **)&ws->q[0], 4 * n, st) == GCX_OK &&
+             gcx__dalloc((void **)&ws->q[1], 4 * n, st) == GCX_OK &&
+             gcx__dalloc((void **)&ws->h[0], 4 * n, st) == GCX_OK &&
+             gcx__dalloc((void **)&ws->h[1], 4 * n, st) == GCX_OK &&
+             gcx__dalloc((void **)&ws->x, 64 * n, st) == GCX_OK &&
+             gcx__dalloc((void **)&ws->y, 64 * n, st) == GCX_OK &&
+             gcx__dalloc((void **)&ws->ctl, 2 * sizeof(gcxk_ctl), st) == GCX_OK &&
+             gcx__dalloc((void **)&ws->npred, 4 * n, st) == GCX_OK &&
+             gcx__dalloc((void **)&ws->state, 8 * n, st) == GCX_OK &&
+             gcx__dalloc((void **)&ws->hpos, 8 * n, st) == GCX_OK &&
+             gcx__dalloc((void **)&ws->ovf, 4 * gcxk_jp_ovf_words(N, M), st) == GCX_OK &&
+             gcx__dalloc((void **)&ws->esplit, 4 * (M ? M : 1), st) == GCX_OK &&
+             gcx__dalloc((void **)&ws->items[0],
+                         sizeof(gcxk_item) * gcxk_jp_item_capacity(N, M), st) == GCX_OK &&
+             gcx__dalloc((void **)&ws->items[1],
+                         sizeof(gcxk_item) * gcxk_jp_item_capacity(N, M), st) == GCX_OK &&
+             (ws->hctl = pinned_ctl_get()) != NULL;
     if (ok) {
         const size_t extra = order_extra(N);
         ws->order_bytes = extra ? extra : 64 * n;
         if (extra)
-            ok = hipMalloc(&ws->order_buf, extra) == hipSuccess;
+            ok = gcx__dalloc(&ws->order_buf, extra, st) == GCX_OK;
     }
     if (ok) {
         hipStream_t aux;
@@ -780,11 +834,19 @@ int gcx_workspace_create(uint32_t N, uint64_t M, gcx_workspace **out)
         }
     }
     if (!ok) {
-        gcx_workspace_free(ws);
+        gcx_workspace_free_on(ws, stream);
         return GCX_ERR_NOMEM;
     }
     memset(ws->hctl, 0, 4 * sizeof(gcxk_ctl));
     *out = ws;
+    return GCX_OK;
+}
+
+int gcx_workspace_create(uint32_t N, uint64_t M, gcx_workspace **out)
+{
+    TRY(gcx_workspace_create_on(N, M, NULL, out));
+    /* usable on any stream once the pool allocations are complete */
+    HIPC(hipStreamSynchronize((hipStream_t)gcx__default_stream()));
     return GCX_OK;
 }
 
@@ -1001,14 +1063,14 @@ static int color_rounds(gcx_workspace *ws, const gcx_graph *g, int mb, hipStream
 }
 
 /* Ordered launches never overlap on the device: each waits for the previous
- * one (a persistent launch needs all its workgroups resident). */
+ * one (a persistent launch needs all its workgroups resident). The lock keeps
+ * wait + launch + record atomic across host threads. */
 static hipEvent_t g_ordered_done;
+static pthread_mutex_t g_ordered_lock = PTHREAD_MUTEX_INITIALIZER;
 
 static int color_ordered(gcx_workspace *ws, const gcx_graph *g, int mb, hipStream_t s)
 {
     void *tok = NULL;
-    if (!g_ordered_done)
-        HIPC(hipEventCreateWithFlags(&g_ordered_done, hipEventDisableTiming));
     TRY(gcx__prof_begin(GCX_ST_JP_INIT, s, &tok));
     TRY(jp_prepare(ws, g, mb, s));
     void *scratch = ws->order_buf ? ws->order_buf : (void *)ws->x;
@@ -1017,11 +1079,21 @@ static int color_ordered(gcx_workspace *ws, const gcx_graph *g, int mb, hipStrea
                             ws->order_bytes, cnt, incl, ws->items[0], ws->ctl, s));
     TRY(gcx__prof_end(tok, s));
     TRY(gcx__prof_begin(GCX_ST_JP_ROUNDS, s, &tok));
-    HIPC(hipStreamWaitEvent(s, g_ordered_done, 0));
+    pthread_mutex_lock(&g_ordered_lock);
+    int rc = GCX_OK;
+    if (!g_ordered_done &&
+        hipEventCreateWithFlags(&g_ordered_done, hipEventDisableTiming) != hipSuccess)
+        rc = GCX_ERR_HIP;
+    if (rc == GCX_OK && hipStreamWaitEvent(s, g_ordered_done, 0) != hipSuccess)
+        rc = GCX_ERR_HIP;
     /* 2 s per wait: far beyond any legitimate wait, and it still ends the kernel */
-    TRY(gcxk_jp_ordered(g->V, ws->npred, ws->esplit, ws->state, ws->color, ws->ovf, mb,
-                        ws->items[0], ws->ctl, ws->ordered_grid, 200000000ull, s));
-    HIPC(hipEventRecord(g_ordered_done, s));
+    if (rc == GCX_OK)
+        rc = gcxk_jp_ordered(g->V, ws->npred, ws->esplit, ws->state, ws->color, ws->ovf, mb,
+                             ws->items[0], ws->ctl, ws->ordered_grid, 200000000ull, s);
+    if (rc == GCX_OK && hipEventRecord(g_ordered_done, s) != hipSuccess)
+        rc = GCX_ERR_HIP;
+    pthread_mutex_unlock(&g_ordered_lock);
+    TRY(rc);
     TRY(gcx__prof_end(tok, s));
     HIPC(hipMemcpyAsync(ws->hctl + 2, ws->ctl, sizeof(gcxk_ctl), hipMemcpyDeviceToHost, s));
     HIPC(hipStreamSynchronize(s));
@@ -1109,6 +1181,21 @@ int gcx_validate(gcx_workspace *ws, const gcx_graph *g, void *stream, int *max_c
     HIPC(hipMemcpyAsync(ws->hctl + 1, vc, sizeof(gcxk_ctl), hipMemcpyDeviceToHost, s));
     HIPC(hipStreamSynchronize(s));
     *max_color = ws->hctl[1].bad ? 0 : (ws->hctl[1].maxcolor > 0 ? ws->hctl[1].maxcolor : 1);
+    return GCX_OK;
+}
+
+int gcx_color_digest(gcx_workspace *ws, const gcx_graph *g, void *stream, uint64_t *digest)
+{
+    TRY(gcx__ensure_init());
+    if (!ws_fits(ws, g) || !digest)
+        return GCX_ERR_ARG;
+    hipStream_t s = (hipStream_t)gcx__pick(stream);
+    gcxk_ctl *vc = ws->ctl + 1;
+    HIPC(hipMemsetAsync(vc, 0, sizeof(gcxk_ctl), s));
+    TRY(gcxk_color_digest(g->N, ws->color, &vc->viol, s));
+    HIPC(hipMemcpyAsync(ws->hctl + 1, vc, sizeof(gcxk_ctl), hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    *digest = ws->hctl[1].viol;
     return GCX_OK;
 }
 

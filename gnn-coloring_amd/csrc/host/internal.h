@@ -63,7 +63,11 @@ struct gcx_workspace {
 };
 
 /* device.c */
-int gcx__graph_alloc(uint32_t N, uint64_t M, gcx_graph **out);
+int gcx__graph_alloc(uint32_t N, uint64_t M, void *stream, gcx_graph **out);
+/* Stream-ordered device memory from the device pool (stream NULL = the
+ * library's stream); freeing never synchronises the device. */
+int gcx__dalloc(void **p, size_t bytes, void *stream);
+void gcx__dfree(void *p, void *stream);
 int gcx__graph_fill(gcx_graph *g, uint32_t N, const uint32_t *V, const uint32_t *E, void *stream);
 /* FF keys (N - id) for a workspace without a graph (first_fit.c:8-19). */
 int gcx__order_ff_only(gcx_workspace *ws, uint32_t N, void *stream);

@@ -49,7 +49,7 @@ static ref_ctx *ctx_create(uint32_t N, uint64_t M)
         die("allocation", GCX_ERR_NOMEM);
     c->capN = N ? N : 1;
     c->capM = M;
-    CHECK("graph allocation", gcx__graph_alloc(c->capN, c->capM, &c->g));
+    CHECK("graph allocation", gcx__graph_alloc(c->capN, c->capM, NULL, &c->g));
     CHECK("workspace allocation", gcx_workspace_create(c->capN, c->capM, &c->ws));
     return c;
 }

@@ -1,0 +1,328 @@
+/*
+ * serve.c -- request loop over the device-resident path (gcx_serve.h).
+ *
+ * The submitting thread queues each request at its arrival time; worker
+ * threads take the queue head when its footprint fits the remaining budget
+ * (first come, first served: a large head job waits for memory rather than
+ * being overtaken) and run generate -> order -> colour -> validate -> digest
+ * on their own stream, then free the job's memory in stream order so no call
+ * synchronises the whole device. Jobs on different workers overlap on the
+ * GPU; ordered JP launches are serialised by the library itself.
+ */
+#define _GNU_SOURCE
+#include <hip/hip_runtime_api.h>
+#include <math.h>
+#include <pthread.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "gcx_serve.h"
+#include "internal.h"
+
+typedef struct worker {
+    struct gcx_server *srv;
+    int id;
+    hipStream_t stream;
+    pthread_t thread;
+} worker;
+
+struct gcx_server {
+    int nworkers;
+    worker *w;
+    uint64_t budget;
+    gcx_model *model[5]; /* by layer count 2..4 */
+    pthread_mutex_t lock;
+    pthread_cond_t cv;      /* queue / memory changed */
+    pthread_cond_t done_cv; /* a job finished */
+    int stop;
+    /* current run */
+    const gcx_request *req;
+    gcx_reply *rep;
+    size_t *queue, head, tail, done, n;
+    uint64_t in_use, peak;
+    struct timespec t0;
+};
+
+static double since(const struct timespec *t0)
+{
+    struct timespec t;
+    clock_gettime(CLOCK_MONOTONIC, &t);
+    return (double)(t.tv_sec - t0->tv_sec) + 1e-9 * (double)(t.tv_nsec - t0->tv_nsec);
+}
+
+static int rmat_scale(uint32_t n)
+{
+    int s = 1;
+    while (s < 31 && (1u << s) < n)
+        s++;
+    return s;
+}
+
+static double rgg_radius(const gcx_request *r)
+{
+    return r->param > 0.0 ? r->param : sqrt(12.0 / (M_PI * (double)(r->n ? r->n : 1)));
+}
+
+/* Generated pairs (upper bound for RGG) and vertex count. */
+static uint64_t request_pairs(const gcx_request *r, uint32_t *N)
+{
+    switch (r->generator) {
+    case GCX_GEN_RMAT: {
+        const int sc = rmat_scale(r->n);
+        *N = 1u << sc;
+        const uint64_t ef = r->param >= 1.0 ? (uint64_t)r->param : 16u;
+        return ef << sc;
+    }
+    case GCX_GEN_ER:
+        *N = r->n;
+        return (uint64_t)llround((double)r->n * (r->param > 0.0 ? r->param : 16.0) / 2.0);
+    default: {
+        *N = r->n;
+        const double rad = rgg_radius(r);
+        const double pairs = 0.5 * (double)r->n * (double)r->n * M_PI * rad * rad;
+        return (uint64_t)(1.25 * pairs) + 1024u;
+    }
+    }
+}
+
+uint64_t gcx_request_footprint(const gcx_request *r)
+{
+    uint32_t N = 0;
+    const uint64_t L = request_pairs(r, &N);
+    const uint64_t M = 2 * L;
+    /* build: pairs (8L), two key arrays (32L), flags + positions (16L), the
+     * radix-sort scratch and E (4M) */
+    const uint64_t build = 8 * L + 32 * L + 16 * L + 8 * L + 4 * M + (64u << 20);
+    const uint64_t ws = gcx_workspace_footprint(N, M);
+    return gcx_graph_footprint(N, M) + (build > ws ? build : ws);
+}
+
+static int make_graph(const gcx_request *r, hipStream_t s, gcx_graph **g)
+{
+    switch (r->generator) {
+    case GCX_GEN_RMAT:
+        return gcx_graph_rmat(rmat_scale(r->n), r->param >= 1.0 ? (uint32_t)r->param : 16u, 0.57,
+                              0.19, 0.19, r->seed, s, g);
+    case GCX_GEN_ER: {
+        uint32_t N;
+        const uint64_t m = request_pairs(r, &N);
+        return gcx_graph_er(r->n, m, r->seed, s, g);
+    }
+    case GCX_GEN_RGG:
+        return gcx_graph_rgg(r->n, rgg_radius(r), r->seed, s, g);
+    default:
+        return GCX_ERR_ARG;
+    }
+}
+
+static void run_job(worker *w, const gcx_request *r, gcx_reply *out)
+{
+    gcx_server *S = w->srv;
+    hipStream_t s = w->stream;
+    gcx_graph *g = NULL;
+    gcx_workspace *ws = NULL;
+    int rc = make_graph(r, s, &g);
+    if (rc == GCX_OK)
+        rc = gcx_workspace_create_on(g->N, g->M, s, &ws);
+    if (rc == GCX_OK)
+        rc = gcx_set_schedule(ws, r->schedule, 0);
+    if (rc == GCX_OK) {
+        const gcx_model *m = NULL;
+        if (r->heuristic == GCX_GNN)
+            m = (r->layers >= 2 && r->layers <= 4) ? S->model[r->layers] : NULL;
+        rc = (r->heuristic == GCX_GNN && !m) ? GCX_ERR_ARG
+                                             : gcx_order(ws, g, r->heuristic, m, r->precision, s);
+    }
+    if (rc == GCX_OK)
+        rc = gcx_color(ws, g, s);
+    int colors = 0;
+    uint64_t digest = 0;
+    if (rc == GCX_OK)
+        rc = gcx_validate(ws, g, s, &colors);
+    if (rc == GCX_OK)
+        rc = gcx_color_digest(ws, g, s, &digest);
+    out->status = rc;
+    out->N = g ? g->N : 0;
+    out->M = g ? g->M : 0;
+    out->colors = colors;
+    out->digest = digest;
+    out->worker = w->id;
+    (void)gcx_workspace_free_on(ws, s);
+    (void)gcx_graph_free_on(g, s);
+}
+
+static int head_fits(const gcx_server *S)
+{
+    if (S->head >= S->tail)
+        return 0;
+    return S->in_use + S->rep[S->queue[S->head]].footprint <= S->budget;
+}
+
+static void *worker_main(void *arg)
+{
+    worker *w = (worker *)arg;
+    gcx_server *S = w->srv;
+    pthread_mutex_lock(&S->lock);
+    for (;;) {
+        while (!S->stop && !head_fits(S))
+            pthread_cond_wait(&S->cv, &S->lock);
+        if (S->stop)
+            break;
+        const size_t i = S->queue[S->head++];
+        gcx_reply *rp = &S->rep[i];
+        S->in_use += rp->footprint;
+        if (S->in_use > S->peak)
+            S->peak = S->in_use;
+        rp->t_admit = since(&S->t0);
+        pthread_mutex_unlock(&S->lock);
+        run_job(w, &S->req[i], rp);
+        /* the job's frees are ordered on this stream; wait for them so the
+         * budget never counts memory the pool has not got back */
+        (void)hipStreamSynchronize(w->stream);
+        pthread_mutex_lock(&S->lock);
+        S->in_use -= rp->footprint;
+        rp->t_done = since(&S->t0);
+        S->done++;
+        pthread_cond_broadcast(&S->cv);
+        pthread_cond_broadcast(&S->done_cv);
+    }
+    pthread_mutex_unlock(&S->lock);
+    return NULL;
+}
+
+int gcx_server_create(int workers, uint64_t hbm_budget, const char *model_dir, gcx_server **out)
+{
+    *out = NULL;
+    int rc = gcx_init();
+    if (rc != GCX_OK)
+        return rc;
+    if (workers < 1 || workers > 64)
+        return GCX_ERR_ARG;
+    gcx_server *S = calloc(1, sizeof(*S));
+    if (!S)
+        return GCX_ERR_NOMEM;
+    S->w = calloc((size_t)workers, sizeof(worker));
+    if (!S->w) {
+        free(S);
+        return GCX_ERR_NOMEM;
+    }
+    pthread_mutex_init(&S->lock, NULL);
+    pthread_cond_init(&S->cv, NULL);
+    pthread_cond_init(&S->done_cv, NULL);
+    if (!hbm_budget) {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+            gcx_server_destroy(S);
+            return GCX_ERR_HIP;
+        }
+        hbm_budget = (uint64_t)fr / 4 * 3;
+    }
+    S->budget = hbm_budget;
+    for (int l = 2; l <= 4; l++) {
+        char path[4096];
+        snprintf(path, sizeof(path), "%s/m%d.gnn", model_dir ? model_dir : "models", l);
+        if ((rc = gcx_model_load(path, &S->model[l])) != GCX_OK) {
+            gcx_server_destroy(S);
+            return rc;
+        }
+    }
+    for (int i = 0; i < workers; i++) {
+        worker *w = &S->w[i];
+        w->srv = S;
+        w->id = i;
+        if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess ||
+            pthread_create(&w->thread, NULL, worker_main, w) != 0) {
+            w->stream = NULL;
+            gcx_server_destroy(S);
+            return GCX_ERR_HIP;
+        }
+        S->nworkers = i + 1;
+    }
+    *out = S;
+    return GCX_OK;
+}
+
+int gcx_server_run(gcx_server *S, const gcx_request *reqs, size_t n, gcx_reply *replies)
+{
+    if (!S || (n && (!reqs || !replies)))
+        return GCX_ERR_ARG;
+    for (size_t i = 1; i < n; i++)
+        if (reqs[i].arrival < reqs[i - 1].arrival)
+            return GCX_ERR_ARG;
+    size_t *queue = malloc(sizeof(size_t) * (n ? n : 1));
+    if (!queue)
+        return GCX_ERR_NOMEM;
+    memset(replies, 0, sizeof(gcx_reply) * n);
+    for (size_t i = 0; i < n; i++)
+        replies[i].footprint = gcx_request_footprint(&reqs[i]);
+    pthread_mutex_lock(&S->lock);
+    S->req = reqs;
+    S->rep = replies;
+    S->queue = queue;
+    S->head = S->tail = S->done = 0;
+    S->n = n;
+    S->in_use = S->peak = 0;
+    clock_gettime(CLOCK_MONOTONIC, &S->t0);
+    pthread_mutex_unlock(&S->lock);
+    for (size_t i = 0; i < n; i++) {
+        /* sleep to within 200 us of the arrival, then spin */
+        for (;;) {
+            const double wait = reqs[i].arrival - since(&S->t0);
+            if (wait <= 0.0)
+                break;
+            if (wait > 2e-4) {
+                struct timespec ts = {0, (long)((wait - 1e-4) * 1e9)};
+                nanosleep(&ts, NULL);
+            }
+        }
+        pthread_mutex_lock(&S->lock);
+        gcx_reply *rp = &replies[i];
+        rp->t_arrival = since(&S->t0);
+        if (rp->footprint > S->budget) {
+            rp->status = GCX_ERR_BUDGET;
+            rp->t_admit = rp->t_done = rp->t_arrival;
+            rp->worker = -1;
+            S->done++;
+        } else {
+            S->queue[S->tail++] = i;
+            pthread_cond_broadcast(&S->cv);
+        }
+        pthread_mutex_unlock(&S->lock);
+    }
+    pthread_mutex_lock(&S->lock);
+    while (S->done < n)
+        pthread_cond_wait(&S->done_cv, &S->lock);
+    S->queue = NULL;
+    S->req = NULL;
+    S->rep = NULL;
+    pthread_mutex_unlock(&S->lock);
+    free(queue);
+    return GCX_OK;
+}
+
+uint64_t gcx_server_peak_reserved(const gcx_server *S) { return S ? S->peak : 0; }
+
+int gcx_server_destroy(gcx_server *S)
+{
+    if (!S)
+        return GCX_OK;
+    pthread_mutex_lock(&S->lock);
+    S->stop = 1;
+    pthread_cond_broadcast(&S->cv);
+    pthread_mutex_unlock(&S->lock);
+    for (int i = 0; i < S->nworkers; i++)
+        pthread_join(S->w[i].thread, NULL);
+    for (int i = 0; i < S->nworkers; i++)
+        if (S->w[i].stream)
+            (void)hipStreamDestroy(S->w[i].stream);
+    for (int l = 2; l <= 4; l++)
+        gcx_model_free(S->model[l]);
+    pthread_cond_destroy(&S->cv);
+    pthread_cond_destroy(&S->done_cv);
+    pthread_mutex_destroy(&S->lock);
+    free(S->w);
+    free(S);
+    return GCX_OK;
+}

@@ -37,32 +37,47 @@ __global__ __launch_bounds__(kBlock) void expand_pairs(uint64_t L, const uint32_
     }
 }
 
+// Keep the first of each run of equal keys; self loops carry the all-ones
+// key, which sorts last and is dropped.
 __global__ __launch_bounds__(kBlock) void mark_unique(uint64_t n, const uint64_t *__restrict__ keys,
-                                                      int b, uint32_t *__restrict__ flag,
-                                                      uint32_t *__restrict__ deg)
+                                                      int b, uint32_t *__restrict__ flag)
 {
     const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= n)
         return;
     const uint64_t none = (b == 32) ? ~0ull : ((1ull << (2 * b)) - 1ull);
     const uint64_t k = keys[i];
-    const bool keep = k != none && (i == 0 || keys[i - 1] != k);
-    flag[i] = keep ? 1u : 0u;
-    if (keep)
-        atomicAdd(deg + (uint32_t)(k >> b), 1u);
+    flag[i] = (k != none && (i == 0 || keys[i - 1] != k)) ? 1u : 0u;
 }
 
-__global__ __launch_bounds__(kBlock) void scatter_unique(uint64_t n, const uint64_t *__restrict__ keys,
-                                                         const uint32_t *__restrict__ flag,
-                                                         const uint32_t *__restrict__ pos, int b,
-                                                         uint32_t *__restrict__ E)
+// Columns and row offsets of the kept entries (keys sorted, pos = exclusive
+// scan of flag, total = kept count). Kept entry i lands at pos[i] and opens
+// every row after the previous distinct key's row up to its own; the last
+// one also closes the rows after it. No atomics: each V[u] has one writer.
+__global__ __launch_bounds__(kBlock) void scatter_rows(uint64_t n, const uint64_t *__restrict__ keys,
+                                                       const uint32_t *__restrict__ flag,
+                                                       const uint32_t *__restrict__ pos, int b,
+                                                       uint32_t N, uint32_t total,
+                                                       uint32_t *__restrict__ V,
+                                                       uint32_t *__restrict__ E)
 {
     const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= n || !flag[i])
         return;
     const uint64_t mask = (b == 32) ? 0xFFFFFFFFull : ((1ull << b) - 1ull);
-    E[pos[i]] = (uint32_t)(keys[i] & mask);
+    const uint64_t k = keys[i];
+    const uint32_t r = (uint32_t)(k >> b), j = pos[i];
+    E[j] = (uint32_t)(k & mask);
+    // keys[i-1] < keys[i] (i is kept), so it is a real entry of an earlier
+    // or the same row
+    const uint32_t r0 = i == 0 ? 0u : (uint32_t)(keys[i - 1] >> b) + 1u;
+    for (uint32_t u = r0; u <= r; u++)
+        V[u] = j;
+    if (j + 1u == total)
+        for (uint32_t u = r + 1u; u <= N; u++)
+            V[u] = total;
 }
+
 
 __global__ __launch_bounds__(kBlock) void degree_stats(uint32_t N, const uint32_t *__restrict__ V,
                                                        uint32_t *__restrict__ hubs,
@@ -310,14 +325,21 @@ int bit_length(uint32_t x)
     return b;
 }
 
+// Stream-ordered temporary from the device memory pool: freeing it never
+// synchronises the device, so work on other streams keeps running.
 struct DevBuf {
     void *p = nullptr;
+    hipStream_t s = nullptr;
     ~DevBuf()
     {
         if (p)
-            (void)hipFree(p);
+            (void)hipFreeAsync(p, s);
     }
-    hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 8); }
+    hipError_t alloc(size_t bytes, hipStream_t st)
+    {
+        s = st;
+        return hipMallocAsync(&p, bytes ? bytes : 8, st);
+    }
 };
 
 // Sort keys for the degree order: maxdeg - deg (so ascending = degree
@@ -345,58 +367,54 @@ extern "C" int gcxk_csr_from_pairs(uint32_t N, uint64_t L, const uint32_t *dI, c
         return GCX_ERR_ARG;
     const int b = bit_length(N - 1) ? bit_length(N - 1) : 1;
     const uint64_t n = 2 * L;
-    DevBuf keys, keys2, flag, pos, deg, temp;
-    GCXK_CHECK(keys.alloc(n * sizeof(uint64_t)));
-    GCXK_CHECK(keys2.alloc(n * sizeof(uint64_t)));
-    GCXK_CHECK(deg.alloc(((size_t)N + 1) * sizeof(uint32_t)));
-    GCXK_CHECK(hipMemsetAsync(deg.p, 0, ((size_t)N + 1) * sizeof(uint32_t), s));
-    uint64_t *k_in = (uint64_t *)keys.p, *k_out = (uint64_t *)keys2.p;
+    DevBuf keys, keys2, flag, pos, temp, temp2;
+    uint64_t *k_out = nullptr;
+    uint32_t total = 0;
     if (L) {
+        GCXK_CHECK(keys.alloc(n * sizeof(uint64_t), s));
+        GCXK_CHECK(keys2.alloc(n * sizeof(uint64_t), s));
+        uint64_t *k_in = (uint64_t *)keys.p;
+        k_out = (uint64_t *)keys2.p;
         expand_pairs<<<div_up(L, kBlock), kBlock, 0, s>>>(L, dI, dJ, b, k_in);
         GCXK_LAUNCH_CHECK();
         size_t tb = 0;
         GCXK_CHECK(rocprim::radix_sort_keys(nullptr, tb, k_in, k_out, (size_t)n, 0, 2 * b, s));
-        GCXK_CHECK(temp.alloc(tb));
+        GCXK_CHECK(temp.alloc(tb, s));
         GCXK_CHECK(rocprim::radix_sort_keys(temp.p, tb, k_in, k_out, (size_t)n, 0, 2 * b, s));
-        GCXK_CHECK(flag.alloc(n * sizeof(uint32_t)));
-        GCXK_CHECK(pos.alloc(n * sizeof(uint32_t)));
-        mark_unique<<<div_up(n, kBlock), kBlock, 0, s>>>(n, k_out, b, (uint32_t *)flag.p,
-                                                          (uint32_t *)deg.p);
+        GCXK_CHECK(flag.alloc(n * sizeof(uint32_t), s));
+        GCXK_CHECK(pos.alloc(n * sizeof(uint32_t), s));
+        mark_unique<<<div_up(n, kBlock), kBlock, 0, s>>>(n, k_out, b, (uint32_t *)flag.p);
         GCXK_LAUNCH_CHECK();
         size_t tb2 = 0;
         GCXK_CHECK(rocprim::exclusive_scan(nullptr, tb2, (uint32_t *)flag.p, (uint32_t *)pos.p, 0u,
                                            (size_t)n, rocprim::plus<uint32_t>(), s));
-        DevBuf temp2;
-        GCXK_CHECK(temp2.alloc(tb2));
+        GCXK_CHECK(temp2.alloc(tb2, s));
         GCXK_CHECK(rocprim::exclusive_scan(temp2.p, tb2, (uint32_t *)flag.p, (uint32_t *)pos.p, 0u,
                                            (size_t)n, rocprim::plus<uint32_t>(), s));
+        uint32_t last[2] = {0, 0};
+        GCXK_CHECK(hipMemcpyAsync(&last[0], (uint32_t *)pos.p + (n - 1), 4, hipMemcpyDeviceToHost, s));
+        GCXK_CHECK(hipMemcpyAsync(&last[1], (uint32_t *)flag.p + (n - 1), 4, hipMemcpyDeviceToHost, s));
+        GCXK_CHECK(hipStreamSynchronize(s));
+        total = last[0] + last[1];
     }
-    // V = exclusive scan of the row counts (N+1 entries; last = M).
-    size_t tb3 = 0;
-    GCXK_CHECK(rocprim::exclusive_scan(nullptr, tb3, (uint32_t *)deg.p, dV, 0u, (size_t)N + 1,
-                                       rocprim::plus<uint32_t>(), s));
-    DevBuf temp3;
-    GCXK_CHECK(temp3.alloc(tb3));
-    GCXK_CHECK(rocprim::exclusive_scan(temp3.p, tb3, (uint32_t *)deg.p, dV, 0u, (size_t)N + 1,
-                                       rocprim::plus<uint32_t>(), s));
-    uint32_t m32 = 0;
-    GCXK_CHECK(hipMemcpyAsync(&m32, dV + N, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    GCXK_CHECK(hipStreamSynchronize(s));
     uint32_t *E = nullptr;
-    GCXK_CHECK(hipMalloc((void **)&E, (m32 ? m32 : 1) * sizeof(uint32_t)));
-    if (L && m32) {
-        scatter_unique<<<div_up(n, kBlock), kBlock, 0, s>>>(n, k_out, (uint32_t *)flag.p,
-                                                             (uint32_t *)pos.p, b, E);
-        hipError_t e = hipGetLastError();
-        if (e == hipSuccess)
-            e = hipStreamSynchronize(s);
-        if (e != hipSuccess) {
-            (void)hipFree(E);
-            return GCX_ERR_HIP;
-        }
+    GCXK_CHECK(hipMallocAsync((void **)&E, (total ? total : 1) * sizeof(uint32_t), s));
+    hipError_t e = hipSuccess;
+    if (total) {
+        scatter_rows<<<div_up(n, kBlock), kBlock, 0, s>>>(n, k_out, (uint32_t *)flag.p,
+                                                           (uint32_t *)pos.p, b, N, total, dV, E);
+        e = hipGetLastError();
+    } else {
+        e = hipMemsetAsync(dV, 0, ((size_t)N + 1) * sizeof(uint32_t), s);
+    }
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        (void)hipFreeAsync(E, s);
+        return GCX_ERR_HIP;
     }
     *dE = E;
-    *M = m32;
+    *M = total;
     return GCX_OK;
 }
 
@@ -411,27 +429,27 @@ extern "C" int gcxk_degree_order(uint32_t N, const uint32_t *dV, uint32_t maxdeg
     void *tmp = nullptr;
     size_t tb = 0;
     int rc = GCX_ERR_HIP;
-    if (hipMalloc((void **)&k0, 4ull * N) != hipSuccess ||
-        hipMalloc((void **)&k1, 4ull * N) != hipSuccess ||
-        hipMalloc((void **)&id, 4ull * N) != hipSuccess)
+    if (hipMallocAsync((void **)&k0, 4ull * N, s) != hipSuccess ||
+        hipMallocAsync((void **)&k1, 4ull * N, s) != hipSuccess ||
+        hipMallocAsync((void **)&id, 4ull * N, s) != hipSuccess)
         goto done;
     degree_keys<<<div_up(N, kBlock), kBlock, 0, s>>>(N, dV, maxdeg, k0, id);
     if (hipGetLastError() != hipSuccess ||
         rocprim::radix_sort_pairs(nullptr, tb, k0, k1, id, perm, N, 0u, bits, s) != hipSuccess ||
-        hipMalloc(&tmp, tb ? tb : 1) != hipSuccess ||
+        hipMallocAsync(&tmp, tb ? tb : 1, s) != hipSuccess ||
         rocprim::radix_sort_pairs(tmp, tb, k0, k1, id, perm, N, 0u, bits, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
         goto done;
     rc = GCX_OK;
 done:
     if (k0)
-        (void)hipFree(k0);
+        (void)hipFreeAsync(k0, s);
     if (k1)
-        (void)hipFree(k1);
+        (void)hipFreeAsync(k1, s);
     if (id)
-        (void)hipFree(id);
+        (void)hipFreeAsync(id, s);
     if (tmp)
-        (void)hipFree(tmp);
+        (void)hipFreeAsync(tmp, s);
     return rc;
 }
 
@@ -511,15 +529,15 @@ extern "C" int gcxk_sort_rows(uint32_t N, const uint32_t *dV, uint32_t *dE, uint
         return GCX_OK;
     const int b = bit_length(N - 1) ? bit_length(N - 1) : 1;
     DevBuf keys, keys2, temp;
-    GCXK_CHECK(keys.alloc(M * sizeof(uint64_t)));
-    GCXK_CHECK(keys2.alloc(M * sizeof(uint64_t)));
+    GCXK_CHECK(keys.alloc(M * sizeof(uint64_t), s));
+    GCXK_CHECK(keys2.alloc(M * sizeof(uint64_t), s));
     row_keys<<<div_up((uint64_t)N * kWave, kBlock), kBlock, 0, s>>>(N, dV, dE, b,
                                                                     (uint64_t *)keys.p);
     GCXK_LAUNCH_CHECK();
     size_t tb = 0;
     GCXK_CHECK(rocprim::radix_sort_keys(nullptr, tb, (uint64_t *)keys.p, (uint64_t *)keys2.p,
                                         (size_t)M, 0, 2 * b, s));
-    GCXK_CHECK(temp.alloc(tb));
+    GCXK_CHECK(temp.alloc(tb, s));
     GCXK_CHECK(rocprim::radix_sort_keys(temp.p, tb, (uint64_t *)keys.p, (uint64_t *)keys2.p,
                                         (size_t)M, 0, 2 * b, s));
     keys_low<<<div_up(M, kBlock), kBlock, 0, s>>>(M, (uint64_t *)keys2.p, b, dE);
@@ -533,9 +551,9 @@ extern "C" int gcxk_make_simple(uint32_t N, uint32_t *dV, const uint32_t *dE, ui
 {
     hipStream_t s = as_stream(stream);
     DevBuf flag, pos, deg, temp, temp2;
-    GCXK_CHECK(flag.alloc(M * sizeof(uint32_t)));
-    GCXK_CHECK(pos.alloc(M * sizeof(uint32_t)));
-    GCXK_CHECK(deg.alloc(((size_t)N + 1) * sizeof(uint32_t)));
+    GCXK_CHECK(flag.alloc(M * sizeof(uint32_t), s));
+    GCXK_CHECK(pos.alloc(M * sizeof(uint32_t), s));
+    GCXK_CHECK(deg.alloc(((size_t)N + 1) * sizeof(uint32_t), s));
     GCXK_CHECK(hipMemsetAsync(deg.p, 0, ((size_t)N + 1) * sizeof(uint32_t), s));
     if (N)
         simple_flags<<<div_up((uint64_t)N * kWave, kBlock), kBlock, 0, s>>>(
@@ -545,7 +563,7 @@ extern "C" int gcxk_make_simple(uint32_t N, uint32_t *dV, const uint32_t *dE, ui
         size_t tb = 0;
         GCXK_CHECK(rocprim::exclusive_scan(nullptr, tb, (uint32_t *)flag.p, (uint32_t *)pos.p, 0u,
                                            (size_t)M, rocprim::plus<uint32_t>(), s));
-        GCXK_CHECK(temp.alloc(tb));
+        GCXK_CHECK(temp.alloc(tb, s));
         GCXK_CHECK(rocprim::exclusive_scan(temp.p, tb, (uint32_t *)flag.p, (uint32_t *)pos.p, 0u,
                                            (size_t)M, rocprim::plus<uint32_t>(), s));
         compact_entries<<<div_up(M, kBlock), kBlock, 0, s>>>(M, dE, (uint32_t *)flag.p,
@@ -555,7 +573,7 @@ extern "C" int gcxk_make_simple(uint32_t N, uint32_t *dV, const uint32_t *dE, ui
     size_t tb2 = 0;
     GCXK_CHECK(rocprim::exclusive_scan(nullptr, tb2, (uint32_t *)deg.p, dV, 0u, (size_t)N + 1,
                                        rocprim::plus<uint32_t>(), s));
-    GCXK_CHECK(temp2.alloc(tb2));
+    GCXK_CHECK(temp2.alloc(tb2, s));
     GCXK_CHECK(rocprim::exclusive_scan(temp2.p, tb2, (uint32_t *)deg.p, dV, 0u, (size_t)N + 1,
                                        rocprim::plus<uint32_t>(), s));
     uint32_t m32 = 0;
@@ -587,15 +605,15 @@ extern "C" int gcxk_gen_rgg(uint32_t n, uint32_t cells, uint64_t r2, uint64_t se
     if (nc >= UINT32_MAX)
         return GCX_ERR_ARG;
     DevBuf px, py, cellof, cellcnt, cellstart, fill, order, cnt, off, t1, t2;
-    GCXK_CHECK(px.alloc(4ull * n));
-    GCXK_CHECK(py.alloc(4ull * n));
-    GCXK_CHECK(cellof.alloc(4ull * n));
-    GCXK_CHECK(order.alloc(4ull * n));
-    GCXK_CHECK(cellcnt.alloc(4ull * (nc + 1)));
-    GCXK_CHECK(cellstart.alloc(4ull * (nc + 1)));
-    GCXK_CHECK(fill.alloc(4ull * (nc + 1)));
-    GCXK_CHECK(cnt.alloc(4ull * (n + 1ull)));
-    GCXK_CHECK(off.alloc(8ull * (n + 1ull)));
+    GCXK_CHECK(px.alloc(4ull * n, s));
+    GCXK_CHECK(py.alloc(4ull * n, s));
+    GCXK_CHECK(cellof.alloc(4ull * n, s));
+    GCXK_CHECK(order.alloc(4ull * n, s));
+    GCXK_CHECK(cellcnt.alloc(4ull * (nc + 1), s));
+    GCXK_CHECK(cellstart.alloc(4ull * (nc + 1), s));
+    GCXK_CHECK(fill.alloc(4ull * (nc + 1), s));
+    GCXK_CHECK(cnt.alloc(4ull * (n + 1ull), s));
+    GCXK_CHECK(off.alloc(8ull * (n + 1ull), s));
     GCXK_CHECK(hipMemsetAsync(cellcnt.p, 0, 4ull * (nc + 1), s));
     GCXK_CHECK(hipMemsetAsync(cnt.p, 0, 4ull * (n + 1ull), s));
     uint32_t *P = (uint32_t *)px.p, *Q = (uint32_t *)py.p, *CO = (uint32_t *)cellof.p;
@@ -604,7 +622,7 @@ extern "C" int gcxk_gen_rgg(uint32_t n, uint32_t cells, uint64_t r2, uint64_t se
     size_t tb = 0;
     GCXK_CHECK(rocprim::exclusive_scan(nullptr, tb, (uint32_t *)cellcnt.p, (uint32_t *)cellstart.p,
                                        0u, (size_t)nc + 1, rocprim::plus<uint32_t>(), s));
-    GCXK_CHECK(t1.alloc(tb));
+    GCXK_CHECK(t1.alloc(tb, s));
     GCXK_CHECK(rocprim::exclusive_scan(t1.p, tb, (uint32_t *)cellcnt.p, (uint32_t *)cellstart.p, 0u,
                                        (size_t)nc + 1, rocprim::plus<uint32_t>(), s));
     GCXK_CHECK(hipMemcpyAsync(fill.p, cellstart.p, 4ull * (nc + 1), hipMemcpyDeviceToDevice, s));
@@ -617,17 +635,17 @@ extern "C" int gcxk_gen_rgg(uint32_t n, uint32_t cells, uint64_t r2, uint64_t se
     size_t tb2 = 0;
     GCXK_CHECK(rocprim::exclusive_scan(nullptr, tb2, (uint32_t *)cnt.p, (uint64_t *)off.p, 0ull,
                                        (size_t)n + 1, rocprim::plus<uint64_t>(), s));
-    GCXK_CHECK(t2.alloc(tb2));
+    GCXK_CHECK(t2.alloc(tb2, s));
     GCXK_CHECK(rocprim::exclusive_scan(t2.p, tb2, (uint32_t *)cnt.p, (uint64_t *)off.p, 0ull,
                                        (size_t)n + 1, rocprim::plus<uint64_t>(), s));
     uint64_t total = 0;
     GCXK_CHECK(hipMemcpyAsync(&total, (uint64_t *)off.p + n, 8, hipMemcpyDeviceToHost, s));
     GCXK_CHECK(hipStreamSynchronize(s));
     uint32_t *I = nullptr, *J = nullptr;
-    if (hipMalloc((void **)&I, 4ull * (total ? total : 1)) != hipSuccess ||
-        hipMalloc((void **)&J, 4ull * (total ? total : 1)) != hipSuccess) {
+    if (hipMallocAsync((void **)&I, 4ull * (total ? total : 1), s) != hipSuccess ||
+        hipMallocAsync((void **)&J, 4ull * (total ? total : 1), s) != hipSuccess) {
         if (I)
-            (void)hipFree(I);
+            (void)hipFreeAsync(I, s);
         return GCX_ERR_NOMEM;
     }
     rgg_pairs<1><<<div_up(n, kBlock), kBlock, 0, s>>>(n, cells, r2, P, Q, CO,
@@ -638,8 +656,8 @@ extern "C" int gcxk_gen_rgg(uint32_t n, uint32_t cells, uint64_t r2, uint64_t se
     if (e == hipSuccess)
         e = hipStreamSynchronize(s);
     if (e != hipSuccess) {
-        (void)hipFree(I);
-        (void)hipFree(J);
+        (void)hipFreeAsync(I, s);
+        (void)hipFreeAsync(J, s);
         return GCX_ERR_HIP;
     }
     *dI = I;

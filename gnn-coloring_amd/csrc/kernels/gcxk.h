@@ -193,6 +193,9 @@ int gcxk_mex_check(uint32_t N, const uint32_t *dV, const uint32_t *dE, const voi
                    int key_bits, const int *color, const uint32_t *hubs, uint32_t nhub,
                    gcxk_ctl *ctl, void *stream);
 
+/* Order-independent digest of color[0..N): adds sum_u mix(u, color[u]) to
+ * *out (device, zeroed by the caller). */
+int gcxk_color_digest(uint32_t N, const int *color, uint64_t *out, void *stream);
 /* Number of workgroups the round kernel should use (fills the GPU). */
 uint32_t gcxk_round_grid(void);
 

@@ -701,9 +701,54 @@ int sort_order(uint32_t N, const void *key, void *scratch, size_t scratch_bytes,
     return GCX_OK;
 }
 
+// Order-independent digest of a colouring: sum over u of mix(u, colour[u]).
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(kBlock) void color_digest_kernel(uint32_t N, const int *__restrict__ color,
+                                                              unsigned long long *out)
+{
+    __shared__ unsigned long long part[kWavesPerBlock];
+    unsigned long long h = 0;
+    for (uint64_t u = (uint64_t)blockIdx.x * kBlock + threadIdx.x; u < N;
+         u += (uint64_t)gridDim.x * kBlock)
+        h += mix64((u << 32) | (uint32_t)color[u]);
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        const uint32_t lo = __shfl_xor((uint32_t)h, off), hi = __shfl_xor((uint32_t)(h >> 32), off);
+        h += ((uint64_t)hi << 32) | lo;
+    }
+    if (lane_id() == 0)
+        part[threadIdx.x / kWave] = h;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int i = 0; i < kWavesPerBlock; i++)
+            t += part[i];
+        atomicAdd(out, t);
+    }
+}
+
 } // namespace
 
 extern "C" uint32_t gcxk_round_grid(void) { return 1024u; }
+
+extern "C" int gcxk_color_digest(uint32_t N, const int *color, uint64_t *out, void *stream)
+{
+    if (N == 0)
+        return GCX_OK;
+    uint32_t grid = div_up(N, kBlock);
+    grid = grid < 1024u ? grid : 1024u;
+    color_digest_kernel<<<grid, kBlock, 0, as_stream(stream)>>>(
+        N, color, reinterpret_cast<unsigned long long *>(out));
+    GCXK_LAUNCH_CHECK();
+    return GCX_OK;
+}
 
 extern "C" int gcxk_jp_mask_bits(uint32_t maxdeg)
 {

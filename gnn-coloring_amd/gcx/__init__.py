@@ -85,6 +85,12 @@ def _load():
         "gcx_set_schedule": (C.c_int, [vp, C.c_int, C.c_uint32]),
         "gcx_validate": (C.c_int, [vp, vp, vp, i32p]),
         "gcx_mex_check": (C.c_int, [vp, vp, vp, C.POINTER(C.c_uint64)]),
+        "gcx_color_digest": (C.c_int, [vp, vp, vp, C.POINTER(C.c_uint64)]),
+        "gcx_server_create": (C.c_int, [C.c_int, C.c_uint64, C.c_char_p, C.POINTER(vp)]),
+        "gcx_server_run": (C.c_int, [vp, vp, C.c_size_t, vp]),
+        "gcx_server_destroy": (C.c_int, [vp]),
+        "gcx_request_footprint": (C.c_uint64, [vp]),
+        "gcx_server_peak_reserved": (C.c_uint64, [vp]),
         "gcx_download_colors": (C.c_int, [vp, C.c_uint32, i32p, vp]),
         "gcx_upload_colors": (C.c_int, [vp, C.c_uint32, i32p, vp]),
         "gcx_download_priorities": (C.c_int, [vp, C.c_uint32, f64p, vp]),
@@ -406,6 +412,12 @@ class Workspace:
         _chk(_lib.gcx_mex_check(self.handle, g.handle, _sh(stream), C.byref(c)), "mex check")
         return int(c.value)
 
+    def digest(self, g: Graph, stream=None) -> int:
+        """Order-independent 64-bit digest of the colours (gcx_color_digest)."""
+        c = C.c_uint64()
+        _chk(_lib.gcx_color_digest(self.handle, g.handle, _sh(stream), C.byref(c)), "digest")
+        return int(c.value)
+
     def colors(self, N, stream=None):
         out = np.empty(N, np.int32)
         _chk(_lib.gcx_download_colors(self.handle, N, out.ctypes.data_as(i32p), _sh(stream)), "colors")
@@ -609,3 +621,72 @@ class ref:
         V, Vp = _u32(V)
         E, Ep = _u32(E)
         return int(_lib.graph_validate_par(len(V) - 1, Vp, Ep))
+
+
+# ---------------------------------------------------------------- serving
+
+GEN_RMAT, GEN_ER, GEN_RGG = 0, 1, 2
+
+
+class Request(C.Structure):
+    """gcx_request (gcx_serve.h)."""
+    _fields_ = [("generator", C.c_int), ("n", C.c_uint32), ("param", C.c_double),
+                ("seed", C.c_uint64), ("heuristic", C.c_int), ("layers", C.c_int),
+                ("precision", C.c_int), ("schedule", C.c_int), ("arrival", C.c_double)]
+
+
+class Reply(C.Structure):
+    """gcx_reply (gcx_serve.h)."""
+    _fields_ = [("status", C.c_int), ("N", C.c_uint32), ("M", C.c_uint64), ("colors", C.c_int),
+                ("digest", C.c_uint64), ("footprint", C.c_uint64), ("worker", C.c_int),
+                ("t_arrival", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def make_request(generator, n, param=0.0, seed=1, heuristic=FF, layers=3, precision=FP32,
+                 schedule=SCHED_ORDERED, arrival=0.0) -> Request:
+    if isinstance(heuristic, str):
+        if heuristic.startswith("GNN"):
+            layers, heuristic = int(heuristic[3:]), GNN
+        else:
+            heuristic = HEURISTICS[heuristic]
+    return Request(generator, n, param, seed, heuristic, layers, precision, schedule, arrival)
+
+
+def request_footprint(req: Request) -> int:
+    return int(_lib.gcx_request_footprint(C.byref(req)))
+
+
+class Server:
+    """Request loop: `workers` threads/streams, HBM admission budget in bytes
+    (0 = 3/4 of free HBM), models m2/m3/m4 loaded once from model_dir."""
+
+    def __init__(self, workers=4, budget=0, model_dir=None):
+        h = vp()
+        md = (model_dir or MODEL_DIR).encode()
+        _chk(_lib.gcx_server_create(workers, budget, md, C.byref(h)), "server create")
+        self.handle = h
+
+    def run(self, requests):
+        """Run requests (sorted by arrival); returns the list of Reply."""
+        n = len(requests)
+        arr = (Request * max(n, 1))(*requests)
+        rep = (Reply * max(n, 1))()
+        _chk(_lib.gcx_server_run(self.handle, arr, n, rep), "server run")
+        return [rep[i] for i in range(n)]
+
+    def peak_reserved(self) -> int:
+        return int(_lib.gcx_server_peak_reserved(self.handle))
+
+    def close(self):
+        if self.handle:
+            _lib.gcx_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

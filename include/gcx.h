@@ -71,7 +71,8 @@ int gcx_event_elapsed_ms(void *start, void *stop, float *ms);
 /* Free and total HBM in bytes. */
 int gcx_mem_info(size_t *free_bytes, size_t *total_bytes);
 
-/* ---- device-resident graphs (canonical CSR: symmetric, sorted, simple) */
+/* ---- device-resident graphs (canonical CSR: symmetric, sorted, simple) ----
+ * Graphs and workspaces live in the device's stream-ordered memory pool. */
 int gcx_graph_from_csr(uint32_t N, const uint32_t *V, const uint32_t *E, void *stream,
                        gcx_graph **out);
 /* Host pair list (0-based) -> canonical CSR built on the device. */
@@ -84,7 +85,12 @@ int gcx_graph_er(uint32_t n, uint64_t m, uint64_t seed, void *stream, gcx_graph 
 /* 2D random geometric graph: n points in the unit square, an edge for every
  * pair closer than `radius` (same pairs as gcx_gen_rgg2d_pairs). */
 int gcx_graph_rgg(uint32_t n, double radius, uint64_t seed, void *stream, gcx_graph **out);
+/* Free after finishing all device work (like hipFree). */
 int gcx_graph_free(gcx_graph *g);
+/* Stream-ordered free: the memory returns to the device pool once the work
+ * already enqueued on `stream` is done; every other use of g must be ordered
+ * before it. Never synchronises the device. */
+int gcx_graph_free_on(gcx_graph *g, void *stream);
 uint32_t gcx_graph_num_vertices(const gcx_graph *g);
 uint64_t gcx_graph_num_entries(const gcx_graph *g);
 uint32_t gcx_graph_max_degree(const gcx_graph *g);
@@ -102,7 +108,11 @@ int gcx_model_layers(const gcx_model *m);
 
 /* ---- workspace: scratch for ordering + colouring of graphs up to (N, M) */
 int gcx_workspace_create(uint32_t N, uint64_t M, gcx_workspace **out);
+/* Allocation ordered on `stream` (usable there at once, elsewhere after it). */
+int gcx_workspace_create_on(uint32_t N, uint64_t M, void *stream, gcx_workspace **out);
 int gcx_workspace_free(gcx_workspace *ws);
+/* Stream-ordered free, as gcx_graph_free_on. */
+int gcx_workspace_free_on(gcx_workspace *ws, void *stream);
 uint64_t gcx_workspace_footprint(uint32_t N, uint64_t M);
 
 /* Compute priorities for g into the workspace (model/precision used by
@@ -117,11 +127,12 @@ int gcx_color(gcx_workspace *ws, const gcx_graph *g, void *stream);
 /* JP schedule; both give the same colours.
  *   ROUNDS:  one launch per DAG level over a compacted frontier.
  *   ORDERED: work items sorted in greedy order, then one persistent
- *            (cooperative) launch of `param` workgroups (0 = as many as are
- *            resident at once) in which each wave takes a fixed stride of
- *            the items and waits on their predecessor counts. Only one
- *            ordered launch runs on the device at a time. If a wait exceeds
- *            its bound the call redoes the colouring with ROUNDS. */
+ *            (cooperative) launch of `param` workgroups (0 = 4 per CU; at
+ *            most as many as are resident at once) in which each wave takes
+ *            a fixed stride of the items and waits on their predecessor
+ *            counts. Only one ordered launch runs on the device at a time,
+ *            across all streams and host threads. If a wait exceeds its
+ *            bound the call redoes the colouring with ROUNDS. */
 enum gcx_schedule { GCX_SCHED_ROUNDS = 0, GCX_SCHED_ORDERED = 1 };
 int gcx_set_schedule(gcx_workspace *ws, int schedule, uint32_t param);
 /* 1 when the last gcx_color fell back from ORDERED to ROUNDS. */
@@ -130,6 +141,9 @@ int gcx_last_fallback(const gcx_workspace *ws, uint32_t *fell_back);
 int gcx_recolor(gcx_workspace *ws, const gcx_graph *g, int iterations, void *stream);
 /* Proper-colouring check of the workspace colours: *max_color = C or 0. */
 int gcx_validate(gcx_workspace *ws, const gcx_graph *g, void *stream, int *max_color);
+/* Order-independent 64-bit digest of the workspace colours: the sum over u
+ * of a hash of (u, colour[u]); equal colourings give equal digests. */
+int gcx_color_digest(gcx_workspace *ws, const gcx_graph *g, void *stream, uint64_t *digest);
 /* Greedy-rule certificate: number of vertices whose colour is not the mex of
  * their predecessors' colours under the workspace priorities. */
 int gcx_mex_check(gcx_workspace *ws, const gcx_graph *g, void *stream, uint64_t *violations);

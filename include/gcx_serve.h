@@ -1,0 +1,73 @@
+/*
+ * gcx_serve.h -- a request loop for colouring jobs on one MI355X.
+ *
+ * Jobs of different graph sizes and heuristics arrive over time; a fixed set
+ * of worker threads, each with its own HIP stream, runs them concurrently on
+ * the device. A job is admitted when it reaches the head of the arrival-order
+ * queue and its HBM footprint (gcx_request_footprint) fits in what the
+ * budget has left; memory freed by finished jobs admits the next ones. Graph
+ * construction, priorities, colouring and validation all stay on the device;
+ * only the colour count and a digest of the colouring come back. Models are
+ * loaded once and shared.
+ *
+ * No counterpart in the reference (its runner colours one host graph at a
+ * time); this is the serving extension of gcx.h.
+ */
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+#include "gcx.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum gcx_generator {
+    GCX_GEN_RMAT = 0, /* R-MAT, a/b/c = .57/.19/.19, n rounded up to 2^scale */
+    GCX_GEN_ER = 1,   /* Erdos-Renyi G(n, m), m = n * degree / 2 pairs */
+    GCX_GEN_RGG = 2   /* 2D random geometric graph in the unit square */
+};
+
+typedef struct {
+    int generator;    /* GCX_GEN_* */
+    uint32_t n;       /* vertices */
+    double param;     /* RMAT: edge factor; ER: average degree; RGG: radius
+                         (0 = sqrt(12 / (pi n)), about 12 neighbours) */
+    uint64_t seed;
+    int heuristic;    /* GCX_FF .. GCX_GNN */
+    int layers;       /* GNN layers 2..4 (models m2/m3/m4) */
+    int precision;    /* GCX_FP32 / GCX_BF16 (GNN only) */
+    int schedule;     /* GCX_SCHED_ROUNDS / GCX_SCHED_ORDERED */
+    double arrival;   /* seconds after the start of gcx_server_run */
+} gcx_request;
+
+typedef struct {
+    int status;         /* GCX_OK, GCX_ERR_BUDGET (never fits), or an error */
+    uint32_t N;
+    uint64_t M;         /* CSR entries of the generated graph */
+    int colors;         /* validated colour count (0 = invalid) */
+    uint64_t digest;    /* gcx_color_digest of the colouring */
+    uint64_t footprint; /* HBM bytes reserved at admission */
+    int worker;
+    double t_arrival, t_admit, t_done; /* seconds since the start of the run */
+} gcx_reply;
+
+typedef struct gcx_server gcx_server;
+
+/* workers >= 1 threads/streams; hbm_budget in bytes (0 = 3/4 of free HBM);
+ * model_dir holds m2.gnn, m3.gnn, m4.gnn (NULL = "models"). */
+int gcx_server_create(int workers, uint64_t hbm_budget, const char *model_dir, gcx_server **out);
+/* Runs n requests sorted by arrival time: each is queued at its arrival
+ * time and replies[i] is filled when request i is done. Blocks until all
+ * are done. */
+int gcx_server_run(gcx_server *s, const gcx_request *reqs, size_t n, gcx_reply *replies);
+int gcx_server_destroy(gcx_server *s);
+/* HBM bytes a request may occupy at its peak (graph build or colouring). */
+uint64_t gcx_request_footprint(const gcx_request *r);
+/* Peak HBM bytes reserved by admitted jobs during the last run. */
+uint64_t gcx_server_peak_reserved(const gcx_server *s);
+
+#ifdef __cplusplus
+}
+#endif

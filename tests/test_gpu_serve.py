@@ -1,0 +1,97 @@
+"""Serving loop (gcx_serve.h): concurrent jobs give the same colourings as
+the same jobs alone and as the direct device API; admission respects the
+HBM budget (needs gfx950)."""
+import numpy as np
+import pytest
+
+import gcx
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    gcx.init()
+    yield
+
+
+def trace(k, seed=7, nmax=60000):
+    """A small mixed trace: R-MAT / ER / RGG, FF/LF/SL/GNN2/GNN3, BF16 GNN,
+    both JP schedules, arrivals spread over ~20 ms."""
+    rng = np.random.default_rng(seed)
+    gens = [gcx.GEN_RMAT, gcx.GEN_ER, gcx.GEN_RGG]
+    heur = ["FF", "LF", "SL", "GNN2", "GNN3"]
+    reqs, t = [], 0.0
+    for i in range(k):
+        g = gens[i % 3]
+        n = int(np.exp(rng.uniform(np.log(1000), np.log(nmax))))
+        param = {gcx.GEN_RMAT: 16.0, gcx.GEN_ER: float(rng.uniform(8, 64)), gcx.GEN_RGG: 0.0}[g]
+        h = heur[int(rng.integers(0, len(heur)))]
+        sched = gcx.SCHED_ORDERED if i % 2 else gcx.SCHED_ROUNDS
+        reqs.append(gcx.make_request(g, n, param, seed=100 + i, heuristic=h,
+                                     precision=gcx.BF16, schedule=sched, arrival=t))
+        t += float(rng.exponential(0.001))
+    return reqs
+
+
+def direct(req):
+    """The same job through the plain device API."""
+    if req.generator == gcx.GEN_RMAT:
+        scale = max(1, int(np.ceil(np.log2(req.n))))
+        g = gcx.Graph.rmat(scale, int(req.param), seed=req.seed)
+    elif req.generator == gcx.GEN_ER:
+        g = gcx.Graph.er(req.n, int(np.floor(req.n * req.param / 2 + 0.5)), seed=req.seed)
+    else:
+        g = gcx.Graph.rgg(req.n, float(np.sqrt(12.0 / (np.pi * req.n))), seed=req.seed)
+    ws = gcx.Workspace(g.N, g.M)
+    ws.set_schedule(req.schedule)
+    if req.heuristic == gcx.GNN:
+        ws.order(g, gcx.GNN, gcx.Model.layers_file(req.layers), req.precision)
+    else:
+        ws.order(g, req.heuristic)
+    ws.color(g)
+    assert ws.mex_check(g) == 0
+    return ws.validate(g), ws.digest(g), g.M
+
+
+def test_serve_matches_alone_and_direct():
+    reqs = trace(24)
+    srv = gcx.Server(workers=4, budget=8 << 30)
+    mix = srv.run(reqs)
+    assert all(r.status == 0 for r in mix), [r.status for r in mix]
+    assert len({r.worker for r in mix}) > 1  # jobs really ran on several workers
+    for i, q in enumerate(reqs):
+        alone = srv.run([gcx.make_request(q.generator, q.n, q.param, q.seed, q.heuristic,
+                                          q.layers, q.precision, q.schedule, 0.0)])[0]
+        assert alone.status == 0
+        assert (alone.colors, alone.digest, alone.M) == (mix[i].colors, mix[i].digest, mix[i].M), i
+    for i in (0, 1, 2, 5):
+        colors, digest, M = direct(reqs[i])
+        assert (colors, digest, M) == (mix[i].colors, mix[i].digest, mix[i].M), i
+    for r in mix:
+        assert r.t_arrival <= r.t_admit <= r.t_done
+    srv.close()
+
+
+def test_serve_budget_admission():
+    reqs = trace(10, seed=3, nmax=20000)
+    fps = [gcx.request_footprint(q) for q in reqs]
+    budget = max(fps) + min(fps)  # room for the largest job plus at most a small one
+    srv = gcx.Server(workers=4, budget=budget)
+    tight = srv.run(reqs)
+    assert srv.peak_reserved() <= budget
+    srv.close()
+    srv = gcx.Server(workers=4, budget=64 << 30)
+    loose = srv.run(reqs)
+    srv.close()
+    assert [(r.colors, r.digest) for r in tight] == [(r.colors, r.digest) for r in loose]
+
+
+def test_serve_rejects_oversized():
+    big = gcx.make_request(gcx.GEN_RMAT, 1 << 20, 16.0, heuristic="FF")
+    small = gcx.make_request(gcx.GEN_ER, 2000, 8.0, heuristic="LF", arrival=0.0)
+    srv = gcx.Server(workers=2, budget=gcx.request_footprint(small) * 2)
+    out = srv.run([small, big])
+    assert out[0].status == 0 and out[0].colors > 0
+    assert out[1].status == gcx.ERR_BUDGET
+    srv.close()
