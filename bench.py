@@ -64,6 +64,57 @@ def cpu_baseline(g, scale):
             "colors": int(col.max(initial=0))}
 
 
+def serving(args, gcx):
+    """Serving leg: the same seeded request mix (R-MAT / ER / RGG, FF LF SL
+    GNN2 GNN3) run three ways: BF16 GNN on `workers` streams with all
+    requests queued at t = 0 (capacity), BF16 with Poisson arrivals at 70% of
+    that capacity (latency), and the plain FP32 path one graph at a time
+    (1 worker, all queued)."""
+    import numpy as np
+    k = args.serving_requests
+    base = gcx.serving_trace(k, seed=11, nmax=args.serving_nmax)
+
+    def summary(rep, wall):
+        ok = [r for r in rep if r.status == 0]
+        lat = np.array([1000.0 * (r.t_done - r.t_arrival) for r in ok])
+        edges = float(sum(r.M for r in ok))
+        return {"requests": len(rep), "ok": len(ok), "seconds": wall,
+                "graphs_per_s": len(ok) / wall, "colored_edges_per_s": edges / wall,
+                "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                "mean_ms": float(lat.mean())}
+
+    def run(srv, reqs):
+        t0 = time.perf_counter()
+        rep = srv.run(reqs)
+        return rep, time.perf_counter() - t0
+
+    srv = gcx.Server(workers=args.serving_workers, budget=int(args.serving_budget_gb * (1 << 30)))
+    srv.run(gcx.with_arrivals(base[:min(k, 16)], None))  # warm the pool and the kernels
+    rep, wall = run(srv, gcx.with_arrivals(base, None))
+    cap = summary(rep, wall)
+    rate = 0.7 * cap["graphs_per_s"]
+    rep2, wall2 = run(srv, gcx.with_arrivals(base, rate, seed=12))
+    lat = summary(rep2, wall2)
+    peak = srv.peak_reserved()
+    srv.close()
+    plain = gcx.Server(workers=1, budget=int(args.serving_budget_gb * (1 << 30)))
+    plain.run(gcx.with_arrivals(base[:min(k, 16)], None, precision=gcx.FP32))
+    rep3, wall3 = run(plain, gcx.with_arrivals(base, None, precision=gcx.FP32))
+    fp32 = summary(rep3, wall3)
+    plain.close()
+    same = sum(1 for a, b in zip(rep, rep2) if (a.colors, a.digest) == (b.colors, b.digest))
+    worse = [b.colors - a.colors for a, b in zip(rep3, rep) if a.status == 0 and b.status == 0]
+    return {"requests": k, "workers": args.serving_workers,
+            "budget_gb": args.serving_budget_gb, "peak_reserved_gb": peak / (1 << 30),
+            "trace": f"seed 11, n log-uniform [1e3, {args.serving_nmax:.0e}], 40% R-MAT / 30% ER "
+                     "(degree 8-64) / 30% RGG (degree ~12), FF/LF/SL/GNN2/GNN3, GNN in BF16",
+            "bf16_capacity": cap, "bf16_poisson": dict(lat, rate_per_s=rate),
+            "plain_fp32_one_at_a_time": fp32,
+            "capacity_speedup_vs_plain_fp32": cap["colored_edges_per_s"] / fp32["colored_edges_per_s"],
+            "same_result_queued_vs_poisson": f"{same}/{k}",
+            "bf16_minus_fp32_colors_max": max(worse) if worse else None}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +123,10 @@ def main():
     ap.add_argument("--scale", type=int, default=22)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-bf16", action="store_true")
+    ap.add_argument("--serving-requests", type=int, default=1000)
+    ap.add_argument("--serving-nmax", type=float, default=4e6)
+    ap.add_argument("--serving-workers", type=int, default=8)
+    ap.add_argument("--serving-budget-gb", type=float, default=48.0)
     ap.add_argument("--schedule", choices=["auto", "rounds", "ordered"], default="auto")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -241,6 +296,9 @@ def main():
         "event_ms_per_step": ev_ms / args.steps,
         "bit_exact_vs_reference": golden,
     }
+    if args.serving_requests > 0:
+        print(f"[bench] serving {args.serving_requests} requests ...", file=sys.stderr, flush=True)
+        out["serving"] = serving(args, gcx)
     if not args.no_cpu_baseline:
         print("[bench] timing the reference CPU path on the host ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(g, args.scale)

@@ -659,6 +659,46 @@ def request_footprint(req: Request) -> int:
     return int(_lib.gcx_request_footprint(C.byref(req)))
 
 
+def serving_trace(k, seed=1, nmin=1000, nmax=4_000_000, rate=None, precision=BF16,
+                  ordered_from=1 << 15):
+    """Seeded request mix: 40% R-MAT (edge factor 16), 30% Erdos-Renyi
+    (average degree uniform in [8, 64]), 30% 2D random geometric (about 12
+    neighbours); n log-uniform in [nmin, nmax]; heuristic uniform over FF, LF,
+    SL, GNN2, GNN3; GNN in `precision`. Arrivals are Poisson at `rate` per
+    second (None = all at t = 0). Graphs with n >= ordered_from use the
+    ordered JP schedule, smaller ones the round schedule."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    heur = ["FF", "LF", "SL", "GNN2", "GNN3"]
+    reqs, t = [], 0.0
+    for i in range(k):
+        u = rng.random()
+        gen = GEN_RMAT if u < 0.4 else (GEN_ER if u < 0.7 else GEN_RGG)
+        n = int(np.exp(rng.uniform(np.log(nmin), np.log(nmax))))
+        param = {GEN_RMAT: 16.0, GEN_ER: float(rng.uniform(8, 64)), GEN_RGG: 0.0}[gen]
+        h = heur[int(rng.integers(0, len(heur)))]
+        if rate:
+            t += float(rng.exponential(1.0 / rate))
+        sched = SCHED_ORDERED if n >= ordered_from else SCHED_ROUNDS
+        reqs.append(make_request(gen, n, param, seed=int(rng.integers(1, 1 << 62)), heuristic=h,
+                                 precision=precision, schedule=sched, arrival=t))
+    return reqs
+
+
+def with_arrivals(reqs, rate, seed=2, precision=None):
+    """Copy of reqs with Poisson arrivals at `rate` per second (None = all
+    at t = 0), optionally with another GNN precision."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out, t = [], 0.0
+    for q in reqs:
+        if rate:
+            t += float(rng.exponential(1.0 / rate))
+        out.append(Request(q.generator, q.n, q.param, q.seed, q.heuristic, q.layers,
+                           q.precision if precision is None else precision, q.schedule, t))
+    return out
+
+
 class Server:
     """Request loop: `workers` threads/streams, HBM admission budget in bytes
     (0 = 3/4 of free HBM), models m2/m3/m4 loaded once from model_dir."""
