@@ -40,7 +40,8 @@ struct gcx_workspace {
     uint32_t *q[2];   /* SL alive lists */
     uint32_t *h[2];   /* SL removed list */
     uint32_t *npred;  /* JP: predecessors per vertex */
-    uint64_t *state;  /* JP: pending predecessors << mb | colours 1..mb taken */
+    uint64_t *state;  /* JP state words (gcxk_jp_state_words; layouts in jp.hip) */
+    uint32_t *wof;    /* JP (ordered): first state word of each vertex, N + 1 */
     uint32_t *ovf;    /* JP: colour-set overflow words (colours > mb) */
     uint32_t *esplit; /* JP: rows split into predecessors | successors (capM) */
     gcxk_item *items[2]; /* JP: round queues; items[0] = greedy-order items */

@@ -131,6 +131,13 @@ static inline uint64_t gcxk_jp_ovf_words(uint32_t N, uint64_t M)
 {
     return M / 32u + (uint64_t)N + 2u;
 }
+/* State words: the round schedule uses one per vertex; the ordered schedule
+ * gives vertex v W(v) = min(8, ceil((npred(v)+1)/mb)) <= 1 + npred(v)/32, and
+ * the predecessor counts sum to M/2. */
+static inline uint64_t gcxk_jp_state_words(uint32_t N, uint64_t M)
+{
+    return (uint64_t)N + 1u + M / 64u + 8u;
+}
 /* Item queue capacity: every vertex contributes max(1, ceil(nsucc / 64)). */
 static inline uint64_t gcxk_jp_item_capacity(uint32_t N, uint64_t M)
 {
@@ -164,14 +171,17 @@ int gcxk_jp_roots(uint32_t N, const uint32_t *dV, const uint32_t *npred, gcxk_it
 int gcxk_jp_round(const uint32_t *dV, const uint32_t *npred, const uint32_t *esplit,
                   uint64_t *state, int *color, uint32_t *ovf, int mb, gcxk_item *const q[2],
                   gcxk_ctl *ctl, uint32_t r, uint32_t grid, void *stream);
-/* Ordered schedule, step 1: items of every vertex in greedy order (p
- * descending, id ascending) into `items`, their number into
- * ctl->total_items. scratch: gcxk_jp_order_scratch(N, key_bits) bytes; cnt,
- * incl: N words each. */
+/* Ordered schedule, step 1 (after gcxk_jp_init): the state word offsets
+ * wof (N + 1 entries; vertex v owns words[wof[v] .. wof[v+1]), zeroed,
+ * words_cap = gcxk_jp_state_words) and the items of every vertex in greedy
+ * order (p descending, id ascending) into `items`, their number into
+ * ctl->total_items. scratch: gcxk_jp_order_scratch(N, key_bits) bytes; cnt:
+ * N words, incl: N + 1 words. */
 size_t gcxk_jp_order_scratch(uint32_t N, int key_bits);
 int gcxk_jp_order_items(uint32_t N, const uint32_t *dV, const uint32_t *npred, const void *key,
-                        int key_bits, void *scratch, size_t scratch_bytes, uint32_t *cnt,
-                        uint32_t *incl, gcxk_item *items, gcxk_ctl *ctl, void *stream);
+                        int key_bits, int mb, void *scratch, size_t scratch_bytes, uint32_t *cnt,
+                        uint32_t *incl, uint32_t *wof, uint64_t *words, uint64_t words_cap,
+                        gcxk_item *items, gcxk_ctl *ctl, void *stream);
 /* Default workgroups of the ordered kernel: 4 per CU, at most what is
  * resident at once (0 on error). */
 uint32_t gcxk_jp_ordered_grid(void);
@@ -180,8 +190,9 @@ uint32_t gcxk_jp_ordered_grid(void);
  * wave w colours items w, w + W, ... waiting on each item's state word. spin_ticks bounds every wait (100 MHz ticks); exceeding
  * it sets ctl->err and ends the kernel. */
 int gcxk_jp_ordered(const uint32_t *dV, const uint32_t *npred, const uint32_t *esplit,
-                    uint64_t *state, int *color, uint32_t *ovf, int mb, const gcxk_item *items,
-                    gcxk_ctl *ctl, uint32_t grid, uint64_t spin_ticks, void *stream);
+                    uint64_t *words, const uint32_t *wof, int *color, uint32_t *ovf, int mb,
+                    const gcxk_item *items, gcxk_ctl *ctl, uint32_t grid, uint64_t spin_ticks,
+                    void *stream);
 /* Degrees of the listed vertices (for the hub chunk table). */
 int gcxk_gather_degrees(const uint32_t *dV, const uint32_t *list, uint32_t n, uint32_t *out,
                         void *stream);

@@ -29,13 +29,18 @@
 // colour 1 at init and have no items.
 //
 // Two schedules, same colours:
-//   rounds  -- one launch per DAG level; the frontier is an item queue.
+//   rounds  -- one launch per DAG level; the frontier is an item queue; the
+//              state word above (its returned old value tells a producer that
+//              it released a successor).
 //   ordered -- items sorted in greedy order; one persistent launch whose
 //              wave w takes items w, w + W, w + 2W, ... and waits on each
-//              item's state word. The earliest unfinished item is always
-//              ready and its wave is at it, so the launch completes when all
-//              W waves are resident; a wall-clock bound on every wait ends
-//              the kernel with ctl->err set if they are not.
+//              item's state. The earliest unfinished item is always ready
+//              and its wave is at it, so the launch completes when all W
+//              waves are resident; a wall-clock bound on every wait ends the
+//              kernel with ctl->err set if they are not. Here each vertex has
+//              up to 8 state words, each with its own report counter (see
+//              jp_ordered_kernel), so colours up to 8*mb need no overflow
+//              words and no wait between a report's OR and its count.
 #include "common.hip.h"
 
 #include <cstring>
@@ -445,6 +450,45 @@ __global__ __launch_bounds__(kBlock) void jp_expand(uint32_t N, const uint32_t *
         ctl->total_items = end;
 }
 
+// Word counts of the ordered schedule's state: W(v) = min(8, ceil((npred+1)/mb))
+// words hold every colour v can take (mex <= npred + 1), capped at 8 words;
+// wn[N] = 0 closes the scan.
+constexpr uint32_t kMaxWords = 8;
+
+__global__ __launch_bounds__(kBlock) void jp_word_counts(uint32_t N, const uint32_t *__restrict__ npred,
+                                                         int mb, uint32_t *__restrict__ wn)
+{
+    const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+    if (v < N) {
+        const uint32_t w = (npred[v] + 1u + (uint32_t)mb - 1u) / (uint32_t)mb;
+        wn[v] = w < kMaxWords ? w : kMaxWords;
+    } else if (v == N) {
+        wn[v] = 0u;
+    }
+}
+
+// Zero the wof[N] state words (grid-stride; the count is read on the device).
+__global__ __launch_bounds__(kBlock) void jp_zero_words(const uint32_t *__restrict__ wof, uint32_t N,
+                                                        uint64_t cap, uint64_t *__restrict__ st)
+{
+    uint64_t n = wof[N];
+    n = n < cap ? n : cap;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * kBlock)
+        st[i] = 0ull;
+}
+
+// Ordered schedule. Vertex v owns state words st[wof[v] .. wof[v+1]); word i
+// is (reports << mb) | colour bits for colours i*mb+1 .. (i+1)*mb. A
+// predecessor with colour c reports into the word that holds c (OR, then
+// add 1 << mb, on one location: no wait); a colour past v's words can never
+// be v's mex, so it is only counted (add to word 0), except for vertices
+// with the full 8 words, whose higher colours go to overflow words (waited
+// for, then counted in word 0). v is ready when its words' reports sum to
+// npred[v]: reports only grow and each one sits in the word of its bit, so
+// loads whose reports sum to npred have seen every bit. Lanes 0..W-1 poll
+// the words in one load instruction.
+//
 // Waits poll with a short exponential backoff (64 .. 512 clocks between
 // loads): thousands of waiting waves otherwise load the memory system enough
 // to slow the critical chain.
@@ -461,7 +505,8 @@ __device__ __forceinline__ void poll_sleep(uint32_t step)
 __global__ __launch_bounds__(kBlock) void jp_ordered_kernel(const uint32_t *__restrict__ V,
                                                             const uint32_t *__restrict__ npred,
                                                             const uint32_t *__restrict__ esplit,
-                                                            uint64_t *__restrict__ state,
+                                                            uint64_t *__restrict__ st,
+                                                            const uint32_t *__restrict__ wof,
                                                             int *__restrict__ color,
                                                             uint32_t *__restrict__ ovf,
                                                             const uint2 *__restrict__ items,
@@ -470,47 +515,74 @@ __global__ __launch_bounds__(kBlock) void jp_ordered_kernel(const uint32_t *__re
 {
     const uint32_t n = ctl->total_items;
     const int lane = lane_id();
-    const uint64_t one = 1ull << mb;
+    const uint64_t one = 1ull << mb, low = one - 1ull;
+    const uint32_t cap = kMaxWords * (uint32_t)mb;
     const uint32_t w0 = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
     const uint32_t nw = gridDim.x * kWavesPerBlock;
     for (uint32_t j = w0; j < n; j += nw) {
         const uint2 it = items[j];
         const uint32_t u = it.x, k = it.y;
-        uint64_t s = load_state(&state[u]);
         const uint32_t Vu = V[u], Vu1 = V[u + 1];
         const uint32_t np = npred[u];
+        const uint32_t wo = wof[u], wn = wof[u + 1] - wo;
         const uint32_t i = Vu + np + k * kChunk + (uint32_t)lane;
         const bool has = i < Vu1;
         const uint32_t v = has ? esplit[i] : 0u;
-        if (s >= one) {
-            const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
-            uint32_t polls = 0;
-            do {
-                poll_sleep(polls);
-                s = load_state(&state[u]);
-                if ((++polls & 255u) == 0u &&
-                    (__builtin_amdgcn_s_memrealtime() - t0 > spin_ticks || load_word(&ctl->err))) {
-                    if (lane == 0)
-                        __hip_atomic_store(&ctl->err, 1u, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT);
-                    return;
-                }
-            } while (s >= one);
+        const uint32_t vo = has ? wof[v] : 0u, vn = has ? wof[v + 1] - vo : 0u;
+        const bool mine = (uint32_t)lane < wn;
+        uint64_t w;
+        uint32_t polls = 0;
+        uint64_t t0 = 0;
+        for (;;) {
+            w = mine ? load_state(&st[wo + lane]) : ~0ull;
+            uint32_t r = mine ? (uint32_t)(w >> mb) : 0u;
+            r += __shfl_xor(r, 1);
+            r += __shfl_xor(r, 2);
+            r += __shfl_xor(r, 4);
+            if (__shfl(r, 0) == np)
+                break;
+            if (polls == 0)
+                t0 = __builtin_amdgcn_s_memrealtime();
+            poll_sleep(polls);
+            if ((++polls & 255u) == 0u &&
+                (__builtin_amdgcn_s_memrealtime() - t0 > spin_ticks || load_word(&ctl->err))) {
+                if (lane == 0)
+                    __hip_atomic_store(&ctl->err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                return;
+            }
         }
-        const uint32_t c = mex_of(s, mb, u, Vu, Vu1 - Vu, ovf);
+        // mex: the first word with a free colour bit (all 8 full: overflow)
+        const uint64_t fr = mine ? (~w & low) : 0ull;
+        const uint64_t bw = __ballot(fr != 0ull);
+        uint32_t c;
+        if (bw) {
+            const int fl = __ffsll((long long)bw) - 1;
+            const uint32_t lo = __shfl((uint32_t)fr, fl), hi = __shfl((uint32_t)(fr >> 32), fl);
+            c = (uint32_t)fl * (uint32_t)mb +
+                (uint32_t)__builtin_ctzll(((uint64_t)hi << 32) | lo) + 1u;
+        } else {
+            c = mex_overflow(cap, u, Vu, Vu1 - Vu, ovf);
+        }
         if (k == 0 && lane == 0)
             color[u] = (int)c;
-        if (c <= (uint32_t)mb) {
-            if (has) {
-                atomicOr((unsigned long long *)&state[v], 1ull << (c - 1u));
-                atomicAdd((unsigned long long *)&state[v], 0ull - one);
+        const uint32_t wi = (c - 1u) / (uint32_t)mb;
+        bool late = false;
+        if (has) {
+            if (wi < vn) {
+                unsigned long long *p = (unsigned long long *)&st[vo + wi];
+                atomicOr(p, 1ull << (c - 1u - wi * (uint32_t)mb));
+                atomicAdd(p, (unsigned long long)one);
+            } else if (vn == kMaxWords) {
+                or_overflow(V, ovf, v, c, cap);
+                late = true;
+            } else {
+                atomicAdd((unsigned long long *)&st[vo], (unsigned long long)one);
             }
-        } else {
-            if (has)
-                or_overflow(V, ovf, v, c, (uint32_t)mb);
+        }
+        if (__ballot(late)) {
             __builtin_amdgcn_s_waitcnt(0);
-            if (has)
-                atomicAdd((unsigned long long *)&state[v], 0ull - one);
+            if (late)
+                atomicAdd((unsigned long long *)&st[vo], (unsigned long long)one);
         }
     }
 }
@@ -838,22 +910,45 @@ extern "C" size_t gcxk_jp_order_scratch(uint32_t N, int key_bits)
                 return 0;
         }
     }
-    size_t scan = 0;
-    if (N && rocprim::inclusive_scan(nullptr, scan, (const uint32_t *)nullptr, (uint32_t *)nullptr,
-                                     N, rocprim::plus<uint32_t>()) != hipSuccess)
+    size_t scan = 0, scan2 = 0;
+    if (N && (rocprim::inclusive_scan(nullptr, scan, (const uint32_t *)nullptr, (uint32_t *)nullptr,
+                                      N, rocprim::plus<uint32_t>()) != hipSuccess ||
+              rocprim::exclusive_scan(nullptr, scan2, (const uint32_t *)nullptr,
+                                      (uint32_t *)nullptr, 0u, (size_t)N + 1,
+                                      rocprim::plus<uint32_t>()) != hipSuccess))
         return 0;
     need = need > scan ? need : scan;
+    need = need > scan2 ? need : scan2;
     return order_tmp_offset(N, ks) + need;
 }
 
 extern "C" int gcxk_jp_order_items(uint32_t N, const uint32_t *dV, const uint32_t *npred,
-                                   const void *key, int key_bits, void *scratch,
+                                   const void *key, int key_bits, int mb, void *scratch,
                                    size_t scratch_bytes, uint32_t *cnt, uint32_t *incl,
+                                   uint32_t *wof, uint64_t *words, uint64_t words_cap,
                                    gcxk_item *items, gcxk_ctl *ctl, void *stream)
 {
     if (N == 0)
         return GCX_OK;
     hipStream_t s = as_stream(stream);
+    {
+        // state words: counts into incl (N + 1), exclusive scan -> wof, zero
+        const size_t ks = key_bits == 32 ? 4 : 8;
+        char *tmp = static_cast<char *>(scratch) + order_tmp_offset(N, ks);
+        size_t tmp_bytes = scratch_bytes - order_tmp_offset(N, ks), need = 0;
+        jp_word_counts<<<div_up((uint64_t)N + 1, kBlock), kBlock, 0, s>>>(N, npred, mb, incl);
+        GCXK_LAUNCH_CHECK();
+        GCXK_CHECK(rocprim::exclusive_scan(nullptr, need, incl, wof, 0u, (size_t)N + 1,
+                                           rocprim::plus<uint32_t>(), s));
+        if (need > tmp_bytes)
+            return GCX_ERR_CAPACITY;
+        GCXK_CHECK(rocprim::exclusive_scan(tmp, need, incl, wof, 0u, (size_t)N + 1,
+                                           rocprim::plus<uint32_t>(), s));
+        uint32_t zg = div_up((uint64_t)N + 1, kBlock);
+        zg = zg < 2048u ? zg : 2048u;
+        jp_zero_words<<<zg, kBlock, 0, s>>>(wof, N, words_cap, words);
+        GCXK_LAUNCH_CHECK();
+    }
     uint32_t *ids = nullptr;
     const int rc = key_bits == 32 ? sort_order<uint32_t>(N, key, scratch, scratch_bytes, &ids, s)
                                   : sort_order<uint64_t>(N, key, scratch, scratch_bytes, &ids, s);
@@ -896,8 +991,8 @@ extern "C" uint32_t gcxk_jp_ordered_grid(void)
 }
 
 extern "C" int gcxk_jp_ordered(const uint32_t *dV, const uint32_t *npred, const uint32_t *esplit,
-                               uint64_t *state, int *color, uint32_t *ovf, int mb,
-                               const gcxk_item *items, gcxk_ctl *ctl, uint32_t grid,
+                               uint64_t *words, const uint32_t *wof, int *color, uint32_t *ovf,
+                               int mb, const gcxk_item *items, gcxk_ctl *ctl, uint32_t grid,
                                uint64_t spin_ticks, void *stream)
 {
     const uint32_t dflt = gcxk_jp_ordered_grid();
@@ -908,9 +1003,9 @@ extern "C" int gcxk_jp_ordered(const uint32_t *dV, const uint32_t *npred, const 
     if (grid > g_ordered_cap)
         grid = g_ordered_cap;
     const uint2 *it = reinterpret_cast<const uint2 *>(items);
-    void *args[] = {(void *)&dV,  (void *)&npred, (void *)&esplit, (void *)&state,
-                    (void *)&color, (void *)&ovf, (void *)&it,     (void *)&ctl,
-                    (void *)&mb,  (void *)&spin_ticks};
+    void *args[] = {(void *)&dV,  (void *)&npred, (void *)&esplit, (void *)&words,
+                    (void *)&wof, (void *)&color, (void *)&ovf,    (void *)&it,
+                    (void *)&ctl, (void *)&mb,    (void *)&spin_ticks};
     GCXK_CHECK(hipLaunchCooperativeKernel((const void *)jp_ordered_kernel, dim3(grid), dim3(kBlock),
                                           args, 0, as_stream(stream)));
     return GCX_OK;
