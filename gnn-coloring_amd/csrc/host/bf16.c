@@ -82,7 +82,7 @@ int gcx__order_gnn_bf16(gcx_workspace *ws, const gcx_graph *g, const gcx_model *
         }
         rc = gcxk_gnn_bf16_layer(kind, g->N, g->V, g->E, m->dWpack + 512 * (size_t)l,
                                  m->dWb + 512 * (size_t)l, b, m->md, in, out, (uint32_t *)ws->key,
-                                 g->hubs, g->nhub, g->perm, s, hs);
+                                 g->hubs, g->nhub, g->perm, g->fidv, s, hs);
         if (rc != GCX_OK)
             return rc;
         if (g->nhub) {

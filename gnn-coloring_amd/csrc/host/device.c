@@ -311,84 +311,38 @@ int gcx_profile_read(double ms[GCX_ST_COUNT], uint64_t launches[GCX_ST_COUNT])
 
 uint64_t gcx_graph_footprint(uint32_t N, uint64_t M)
 {
-    return 4ull * ((uint64_t)N + 1) + 4ull * M + 8ull * N;
-}
-
-static int cmp_u32(const void *a, const void *b)
-{
-    const uint32_t x = *(const uint32_t *)a, y = *(const uint32_t *)b;
-    return (x > y) - (x < y);
+    /* V and the hub chunk table (N + 1 each), E, hubs and the degree order */
+    return 8ull * ((uint64_t)N + 1) + 4ull * M + 8ull * N;
 }
 
 /* Degree statistics: max degree, hub list (ascending ids) and the hubs'
  * 4096-edge chunk table used by the JP orientation pass. */
-static int graph_stats(gcx_graph *g, hipStream_t s, uint32_t *tmp)
+/* Degree statistics, hub list (ascending ids) with its 4096-edge chunk table
+ * for the JP orientation pass, and the degree order. */
+static int graph_stats(gcx_graph *g, hipStream_t s)
 {
-    TRY(gcxk_graph_stats(g->N, g->V, g->hubs, tmp, &g->nhub, &g->maxdeg, s));
+    if (!g->hchunk)
+        TRY(gcx__dalloc((void **)&g->hchunk, 4ull * (g->N + 1ull), s));
+    TRY(gcxk_graph_stats(g->N, g->V, g->hubs, g->hchunk, &g->nhub, &g->maxdeg, &g->nhchunk, s));
     if (!g->perm)
         TRY(gcx__dalloc((void **)&g->perm, 4ull * (g->N ? g->N : 1), s));
     TRY(gcxk_degree_order(g->N, g->V, g->maxdeg, g->perm, s));
-    if (g->hchunk) {
-        gcx__dfree(g->hchunk, s);
-        g->hchunk = NULL;
-    }
-    g->nhchunk = 0;
-    if (g->nhub == 0)
-        return GCX_OK;
-    const uint32_t n = g->nhub;
-    uint32_t *ids = malloc(sizeof(uint32_t) * n), *deg = malloc(sizeof(uint32_t) * n);
-    uint32_t *chunk = malloc(sizeof(uint32_t) * (n + 1ull));
-    uint32_t *ddeg = NULL;
-    int rc = GCX_ERR_HIP;
-    if (!ids || !deg || !chunk) {
-        rc = GCX_ERR_NOMEM;
-        goto done;
-    }
-    if (hipMemcpyAsync(ids, g->hubs, 4ull * n, hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess)
-        goto done;
-    qsort(ids, n, sizeof(uint32_t), cmp_u32);
-    if (hipMemcpyAsync(g->hubs, ids, 4ull * n, hipMemcpyHostToDevice, s) != hipSuccess ||
-        gcx__dalloc((void **)&ddeg, 4ull * n, s) != GCX_OK)
-        goto done;
-    if ((rc = gcxk_gather_degrees(g->V, g->hubs, n, ddeg, s)) != GCX_OK)
-        goto done;
-    rc = GCX_ERR_HIP;
-    if (hipMemcpyAsync(deg, ddeg, 4ull * n, hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess)
-        goto done;
-    chunk[0] = 0;
-    for (uint32_t h = 0; h < n; h++)
-        chunk[h + 1] = chunk[h] + (deg[h] + GCXK_HUB_CHUNK_EDGES - 1) / GCXK_HUB_CHUNK_EDGES;
-    if (gcx__dalloc((void **)&g->hchunk, 4ull * (n + 1), s) != GCX_OK ||
-        hipMemcpyAsync(g->hchunk, chunk, 4ull * (n + 1), hipMemcpyHostToDevice, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess)
-        goto done;
-    g->nhchunk = chunk[n];
-    rc = GCX_OK;
-done:
-    gcx__dfree(ddeg, s);
-    free(ids);
-    free(deg);
-    free(chunk);
-    return rc;
+    /* creation returns a graph usable on any stream */
+    HIPC(hipStreamSynchronize(s));
+    return GCX_OK;
 }
 
 static int graph_finish(gcx_graph *g, hipStream_t s)
 {
-    uint32_t *tmp = NULL;
     TRY(gcx__dalloc((void **)&g->hubs, 4ull * (g->N ? g->N : 1), s));
-    TRY(gcx__dalloc((void **)&tmp, 16, s));
-    int rc = graph_stats(g, s, tmp);
-    gcx__dfree(tmp, s);
-    return rc;
+    return graph_stats(g, s);
 }
 
 int gcx_graph_free_on(gcx_graph *g, void *stream)
 {
     if (!g)
         return GCX_OK;
-    void *bufs[] = {g->V, g->E, g->hubs, g->hchunk, g->perm};
+    void *bufs[] = {g->V, g->E, g->hubs, g->hchunk, g->perm, g->fidv, g->goff};
     for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); i++)
         gcx__dfree(bufs[i], stream);
     free(g);
@@ -417,7 +371,8 @@ int gcx__graph_alloc(uint32_t N, uint64_t M, void *stream, gcx_graph **out)
     if (gcx__dalloc((void **)&g->V, 4ull * (N + 1ull), stream) != GCX_OK ||
         gcx__dalloc((void **)&g->E, 4ull * (M ? M : 1), stream) != GCX_OK ||
         gcx__dalloc((void **)&g->hubs, 4ull * (N ? N : 1), stream) != GCX_OK ||
-        gcx__dalloc((void **)&g->perm, 4ull * (N ? N : 1), stream) != GCX_OK) {
+        gcx__dalloc((void **)&g->perm, 4ull * (N ? N : 1), stream) != GCX_OK ||
+        gcx__dalloc((void **)&g->hchunk, 4ull * (N + 1ull), stream) != GCX_OK) {
         gcx_graph_free_on(g, stream);
         return GCX_ERR_NOMEM;
     }
@@ -443,9 +398,9 @@ int gcx__graph_fill(gcx_graph *g, uint32_t N, const uint32_t *V, const uint32_t 
     int rc = gcxk_csr_bounds(g->N, g->M, g->V, g->E, tmp, &ok, s);
     if (rc == GCX_OK && !ok)
         rc = GCX_ERR_ARG; /* indices out of range: refuse before any kernel reads them */
-    if (rc == GCX_OK)
-        rc = graph_stats(g, s, tmp);
     gcx__dfree(tmp, s);
+    if (rc == GCX_OK)
+        rc = graph_stats(g, s);
     return rc;
 }
 
@@ -575,6 +530,91 @@ int gcx_graph_rgg(uint32_t n, double radius, uint64_t seed, void *stream, gcx_gr
         rc = graph_from_device_pairs(n, L, dI, dJ, s, out);
     gcx__dfree(dI, s);
     gcx__dfree(dJ, s);
+    return rc;
+}
+
+static int ws_fits(const gcx_workspace *ws, const gcx_graph *g);
+
+int gcx_graph_pack(const gcx_graph *const *graphs, uint32_t count, void *stream, gcx_graph **out)
+{
+    TRY(gcx__ensure_init());
+    *out = NULL;
+    if (!graphs || count == 0)
+        return GCX_ERR_ARG;
+    hipStream_t s = (hipStream_t)gcx__pick(stream);
+    uint64_t N = 0, M = 0;
+    uint32_t *off = malloc(sizeof(uint32_t) * ((size_t)count + 1));
+    if (!off)
+        return GCX_ERR_NOMEM;
+    for (uint32_t i = 0; i < count; i++) {
+        if (!graphs[i]) {
+            free(off);
+            return GCX_ERR_ARG;
+        }
+        off[i] = (uint32_t)N;
+        N += graphs[i]->N;
+        M += graphs[i]->M;
+    }
+    off[count] = (uint32_t)N;
+    if (N == 0 || N >= UINT32_MAX || M >= UINT32_MAX) {
+        free(off);
+        return GCX_ERR_CAPACITY;
+    }
+    gcx_graph *g = NULL;
+    int rc = gcx__graph_alloc((uint32_t)N, M, s, &g);
+    if (rc == GCX_OK)
+        rc = gcx__dalloc((void **)&g->fidv, 4ull * N, s);
+    if (rc == GCX_OK)
+        rc = gcx__dalloc((void **)&g->goff, 4ull * ((size_t)count + 1), s);
+    if (rc == GCX_OK && hipMemcpyAsync(g->goff, off, 4ull * ((size_t)count + 1),
+                                       hipMemcpyHostToDevice, s) != hipSuccess)
+        rc = GCX_ERR_HIP;
+    uint64_t eoff = 0;
+    for (uint32_t i = 0; rc == GCX_OK && i < count; i++) {
+        const gcx_graph *h = graphs[i];
+        rc = gcxk_pack_graph(h->N, h->M, h->V, h->E, off[i], (uint32_t)eoff, g->V, g->E, g->fidv,
+                             s);
+        eoff += h->M;
+    }
+    if (rc == GCX_OK)
+        rc = gcxk_pack_close((uint32_t)N, (uint32_t)M, g->V, s);
+    if (rc == GCX_OK) {
+        g->count = count;
+        rc = graph_stats(g, s); /* synchronises: off may be freed */
+    } else {
+        (void)hipStreamSynchronize(s);
+    }
+    free(off);
+    if (rc != GCX_OK) {
+        gcx_graph_free_on(g, s);
+        return rc;
+    }
+    *out = g;
+    return GCX_OK;
+}
+
+uint32_t gcx_graph_packed_count(const gcx_graph *g) { return g ? g->count : 0; }
+
+int gcx_packed_colors(gcx_workspace *ws, const gcx_graph *g, int *colors, uint64_t *digests,
+                      void *stream)
+{
+    TRY(gcx__ensure_init());
+    if (!ws_fits(ws, g) || !g->count || !g->goff || !colors || !digests)
+        return GCX_ERR_ARG;
+    hipStream_t s = (hipStream_t)gcx__pick(stream);
+    const size_t n = g->count;
+    void *dbuf = NULL;
+    TRY(gcx__dalloc(&dbuf, 12 * n + 8, s));
+    uint64_t *dd = (uint64_t *)dbuf;
+    int *dc = (int *)(dd + n);
+    int rc = GCX_ERR_HIP;
+    if (hipMemsetAsync(dbuf, 0, 12 * n, s) == hipSuccess &&
+        gcxk_packed_stats(g->N, g->count, g->goff, ws->color, dc, dd, s) == GCX_OK &&
+        hipMemcpyAsync(colors, dc, 4 * n, hipMemcpyDeviceToHost, s) == hipSuccess &&
+        hipMemcpyAsync(digests, dd, 8 * n, hipMemcpyDeviceToHost, s) == hipSuccess &&
+        hipStreamSynchronize(s) == hipSuccess)
+        rc = GCX_OK;
+    gcx__dfree(dbuf, s);
     return rc;
 }
 
@@ -747,6 +787,59 @@ static void pinned_ctl_put(void *p)
         (void)hipHostFree(p);
 }
 
+/* Side streams and events are recycled too: creating and destroying them per
+ * job costs milliseconds in the serving loop. */
+static hipStream_t g_aux_free[64];
+static int g_aux_n = 0;
+static hipEvent_t g_ev_free[256];
+static int g_ev_n = 0;
+
+static int aux_get(hipStream_t *out)
+{
+    pthread_mutex_lock(&g_pin_lock);
+    const int have = g_aux_n > 0;
+    if (have)
+        *out = g_aux_free[--g_aux_n];
+    pthread_mutex_unlock(&g_pin_lock);
+    return have || hipStreamCreateWithFlags(out, hipStreamNonBlocking) == hipSuccess;
+}
+
+static void aux_put(hipStream_t st)
+{
+    if (!st)
+        return;
+    pthread_mutex_lock(&g_pin_lock);
+    const int kept = g_aux_n < 64;
+    if (kept)
+        g_aux_free[g_aux_n++] = st;
+    pthread_mutex_unlock(&g_pin_lock);
+    if (!kept)
+        (void)hipStreamDestroy(st);
+}
+
+static int ev_get(hipEvent_t *out)
+{
+    pthread_mutex_lock(&g_pin_lock);
+    const int have = g_ev_n > 0;
+    if (have)
+        *out = g_ev_free[--g_ev_n];
+    pthread_mutex_unlock(&g_pin_lock);
+    return have || hipEventCreateWithFlags(out, hipEventDisableTiming) == hipSuccess;
+}
+
+static void ev_put(hipEvent_t e)
+{
+    if (!e)
+        return;
+    pthread_mutex_lock(&g_pin_lock);
+    const int kept = g_ev_n < 256;
+    if (kept)
+        g_ev_free[g_ev_n++] = e;
+    pthread_mutex_unlock(&g_pin_lock);
+    if (!kept)
+        (void)hipEventDestroy(e);
+}
+
 int gcx_workspace_free_on(gcx_workspace *ws, void *stream)
 {
     if (!ws)
@@ -758,12 +851,10 @@ int gcx_workspace_free_on(gcx_workspace *ws, void *stream)
     for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); i++)
         gcx__dfree(bufs[i], stream);
     pinned_ctl_put(ws->hctl);
-    if (ws->aux)
-        (void)hipStreamDestroy((hipStream_t)ws->aux);
+    aux_put((hipStream_t)ws->aux);
     void *evs[] = {ws->ev_fork, ws->ev_join, ws->ev_poll[0], ws->ev_poll[1]};
     for (size_t i = 0; i < 4; i++)
-        if (evs[i])
-            (void)hipEventDestroy((hipEvent_t)evs[i]);
+        ev_put((hipEvent_t)evs[i]);
     free(ws);
     return GCX_OK;
 }
@@ -821,11 +912,11 @@ int gcx_workspace_create_on(uint32_t N, uint64_t M, void *stream, gcx_workspace 
     if (ok) {
         hipStream_t aux;
         hipEvent_t e[4];
-        ok = hipStreamCreateWithFlags(&aux, hipStreamNonBlocking) == hipSuccess;
+        ok = aux_get(&aux);
         if (ok)
             ws->aux = (void *)aux;
         for (int i = 0; ok && i < 4; i++) {
-            ok = hipEventCreateWithFlags(&e[i], hipEventDisableTiming) == hipSuccess;
+            ok = ev_get(&e[i]);
             if (ok) {
                 if (i == 0)
                     ws->ev_fork = e[i];
@@ -940,7 +1031,8 @@ static int order_gnn_fp32(gcx_workspace *ws, const gcx_graph *g, const gcx_model
             hs = (hipStream_t)ws->aux;
         }
         TRY(gcxk_gnn_fp32_layer(kind, g->N, g->V, g->E, W, b, m->md, in, out,
-                                (uint32_t *)ws->key, g->hubs, g->nhub, g->perm, s, hs));
+                                (uint32_t *)ws->key, g->hubs, g->nhub, g->perm, g->fidv, s,
+                                hs));
         if (g->nhub) {
             HIPC(hipEventRecord((hipEvent_t)ws->ev_join, hs));
             HIPC(hipStreamWaitEvent(s, (hipEvent_t)ws->ev_join, 0));
@@ -1065,11 +1157,67 @@ static int color_rounds(gcx_workspace *ws, const gcx_graph *g, int mb, hipStream
     return gcx__prof_end(tok, s);
 }
 
-/* Ordered launches never overlap on the device: each waits for the previous
- * one (a persistent launch needs all its workgroups resident). The lock keeps
- * wait + launch + record atomic across host threads. */
-static hipEvent_t g_ordered_done;
+/* Ordered launches. g_ordered_slots = 0: cooperative, one at a time (the
+ * runtime checks residency and runs the launch alone). k >= 1: plain
+ * launches that share the device with other streams' kernels, at most k at
+ * once, each with at most 1/k of the resident workgroups, so their grids fit
+ * together. Launch i goes to slot i mod k and waits for the previous launch
+ * in that slot; after a change of k the next launch waits for all slots. The
+ * lock keeps pick + wait + launch + record atomic across host threads. */
+#define ORDERED_MAX_SLOTS 8
+static hipEvent_t g_slot_done[ORDERED_MAX_SLOTS];
+static int g_ordered_slots = 0;
+static int g_ordered_drain = 0;
+static uint32_t g_next_slot = 0;
 static pthread_mutex_t g_ordered_lock = PTHREAD_MUTEX_INITIALIZER;
+
+int gcx_set_ordered_launch(int slots)
+{
+    TRY(gcx__ensure_init());
+    if (slots < 0 || slots > ORDERED_MAX_SLOTS)
+        return GCX_ERR_ARG;
+    pthread_mutex_lock(&g_ordered_lock);
+    if (slots != g_ordered_slots)
+        g_ordered_drain = 1;
+    g_ordered_slots = slots;
+    pthread_mutex_unlock(&g_ordered_lock);
+    return GCX_OK;
+}
+
+int gcx_get_ordered_launch(void) { return g_ordered_slots; }
+
+/* Enqueues the ordered kernel on s under the launch policy above. */
+static int launch_ordered(gcx_workspace *ws, const gcx_graph *g, int mb, hipStream_t s)
+{
+    pthread_mutex_lock(&g_ordered_lock);
+    int rc = GCX_OK;
+    for (int i = 0; rc == GCX_OK && i < ORDERED_MAX_SLOTS; i++)
+        if (!g_slot_done[i] &&
+            hipEventCreateWithFlags(&g_slot_done[i], hipEventDisableTiming) != hipSuccess)
+            rc = GCX_ERR_HIP;
+    const int k = g_ordered_slots;
+    const uint32_t slot = k ? g_next_slot++ % (uint32_t)k : 0u;
+    for (int i = 0; rc == GCX_OK && i < ORDERED_MAX_SLOTS; i++)
+        if ((g_ordered_drain || i == (int)slot) && hipStreamWaitEvent(s, g_slot_done[i], 0) != hipSuccess)
+            rc = GCX_ERR_HIP;
+    g_ordered_drain = 0;
+    uint32_t grid = ws->ordered_grid ? ws->ordered_grid : gcxk_jp_ordered_grid();
+    if (k) {
+        const uint32_t share = gcxk_jp_ordered_cap() / (uint32_t)k;
+        if (grid > share)
+            grid = share;
+        if (grid == 0)
+            rc = GCX_ERR_HIP;
+    }
+    /* 2 s per wait: far beyond any legitimate wait, and it still ends the kernel */
+    if (rc == GCX_OK)
+        rc = gcxk_jp_ordered(g->V, ws->npred, ws->esplit, ws->state, ws->wof, ws->color, ws->ovf,
+                             mb, ws->items[0], ws->ctl, grid, 200000000ull, k == 0, s);
+    if (rc == GCX_OK && hipEventRecord(g_slot_done[slot], s) != hipSuccess)
+        rc = GCX_ERR_HIP;
+    pthread_mutex_unlock(&g_ordered_lock);
+    return rc;
+}
 
 static int color_ordered(gcx_workspace *ws, const gcx_graph *g, int mb, hipStream_t s)
 {
@@ -1083,21 +1231,7 @@ static int color_ordered(gcx_workspace *ws, const gcx_graph *g, int mb, hipStrea
                             gcxk_jp_state_words(ws->capN, ws->capM), ws->items[0], ws->ctl, s));
     TRY(gcx__prof_end(tok, s));
     TRY(gcx__prof_begin(GCX_ST_JP_ROUNDS, s, &tok));
-    pthread_mutex_lock(&g_ordered_lock);
-    int rc = GCX_OK;
-    if (!g_ordered_done &&
-        hipEventCreateWithFlags(&g_ordered_done, hipEventDisableTiming) != hipSuccess)
-        rc = GCX_ERR_HIP;
-    if (rc == GCX_OK && hipStreamWaitEvent(s, g_ordered_done, 0) != hipSuccess)
-        rc = GCX_ERR_HIP;
-    /* 2 s per wait: far beyond any legitimate wait, and it still ends the kernel */
-    if (rc == GCX_OK)
-        rc = gcxk_jp_ordered(g->V, ws->npred, ws->esplit, ws->state, ws->wof, ws->color,
-                             ws->ovf, mb, ws->items[0], ws->ctl, ws->ordered_grid, 200000000ull,
-                             s);
-    if (rc == GCX_OK && hipEventRecord(g_ordered_done, s) != hipSuccess)
-        rc = GCX_ERR_HIP;
-    pthread_mutex_unlock(&g_ordered_lock);
+    const int rc = launch_ordered(ws, g, mb, s);
     TRY(rc);
     TRY(gcx__prof_end(tok, s));
     HIPC(hipMemcpyAsync(ws->hctl + 2, ws->ctl, sizeof(gcxk_ctl), hipMemcpyDeviceToHost, s));
@@ -1158,19 +1292,93 @@ int gcx_last_launches(const gcx_workspace *ws, uint32_t *launches)
     return GCX_OK;
 }
 
-int gcx_recolor(gcx_workspace *ws, const gcx_graph *g, int iterations, void *stream)
+/* One Culberson pass on the round schedule, enqueued on s: keys from the
+ * colours, orientation, roots and `rounds` round launches. No host reads
+ * and no memsets, so it can be captured. */
+static int recolor_pass(gcx_workspace *ws, const gcx_graph *g, int mb, uint32_t rounds,
+                        hipStream_t s)
 {
-    /* main.c:36-43: p[j] = color[j], then JP again; repeated. */
-    TRY(gcx__ensure_init());
-    if (!ws_fits(ws, g) || iterations < 0)
-        return GCX_ERR_ARG;
-    hipStream_t s = (hipStream_t)gcx__pick(stream);
+    TRY(gcxk_keys_from_colors(g->N, ws->color, (uint32_t *)ws->key, s));
+    TRY(jp_prepare(ws, g, mb, s));
+    TRY(gcxk_jp_roots(g->N, g->V, ws->npred, ws->items[0], ws->ctl, s));
+    jp_ctx c = {ws, g, g_round_grid, mb};
+    for (uint32_t r = 0; r < rounds; r++)
+        TRY(jp_round(&c, r, s));
+    return GCX_OK;
+}
+
+static int recolor_eager(gcx_workspace *ws, const gcx_graph *g, int iterations, hipStream_t s)
+{
     for (int it = 0; it < iterations; it++) {
         TRY(gcxk_keys_from_colors(g->N, ws->color, (uint32_t *)ws->key, s));
         ws->key_bits = 32;
         ws->key_kind = 0;
         TRY(gcx_color(ws, g, s));
     }
+    return GCX_OK;
+}
+
+/* Longest captured pass: beyond this many colour classes the eager loop
+ * (which polls for completion) is used. */
+#define RECOLOR_GRAPH_MAX_ROUNDS 4096u
+
+int gcx_recolor(gcx_workspace *ws, const gcx_graph *g, int iterations, void *stream)
+{
+    /* main.c:34-43: p[j] = color[j], then JP again; repeated.
+     *
+     * With p = the previous colours, a vertex's predecessors all have larger
+     * colours, so its depth in the JP DAG is at most the number of colour
+     * classes C of the input colouring, and a pass never uses more classes
+     * than it was given (each vertex's class is a valid choice again). So
+     * C + 1 round launches finish every pass: the passes are captured once
+     * as a HIP graph and replayed, with no host round trip between rounds
+     * or passes. */
+    TRY(gcx__ensure_init());
+    if (!ws_fits(ws, g) || iterations < 0 || ws->curN != g->N)
+        return GCX_ERR_ARG;
+    if (iterations == 0 || g->N == 0)
+        return GCX_OK;
+    hipStream_t s = (hipStream_t)gcx__pick(stream);
+    int C = 0;
+    TRY(gcx_validate(ws, g, s, &C));
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    HIPC(hipStreamIsCapturing(s, &cap));
+    if (C <= 0 || (uint32_t)C >= RECOLOR_GRAPH_MAX_ROUNDS || cap != hipStreamCaptureStatusNone)
+        return recolor_eager(ws, g, iterations, s);
+    const int mb = gcxk_jp_mask_bits(g->maxdeg);
+    ws->key_bits = 32;
+    ws->key_kind = 0;
+    void *tok = NULL;
+    TRY(gcx__prof_begin(GCX_ST_JP_ROUNDS, s, &tok));
+    hipGraph_t graph = NULL;
+    hipGraphExec_t exec = NULL;
+    HIPC(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int rc = recolor_pass(ws, g, mb, (uint32_t)C + 1, s);
+    const hipError_t ec = hipStreamEndCapture(s, &graph);
+    if (rc == GCX_OK && ec != hipSuccess)
+        rc = GCX_ERR_HIP;
+    if (rc == GCX_OK && hipGraphInstantiate(&exec, graph, NULL, NULL, 0) != hipSuccess)
+        rc = GCX_ERR_HIP;
+    for (int it = 0; rc == GCX_OK && it < iterations; it++)
+        if (hipGraphLaunch(exec, s) != hipSuccess)
+            rc = GCX_ERR_HIP;
+    if (rc == GCX_OK)
+        rc = gcx__prof_end(tok, s);
+    if (rc == GCX_OK &&
+        hipMemcpyAsync(ws->hctl + 2, ws->ctl, sizeof(gcxk_ctl), hipMemcpyDeviceToHost, s) != hipSuccess)
+        rc = GCX_ERR_HIP;
+    if (hipStreamSynchronize(s) != hipSuccess && rc == GCX_OK)
+        rc = GCX_ERR_HIP;
+    if (exec)
+        (void)hipGraphExecDestroy(exec);
+    if (graph)
+        (void)hipGraphDestroy(graph);
+    TRY(rc);
+    /* the last pass drained its queue within the captured rounds */
+    if (!ws->hctl[2].done)
+        return GCX_ERR_INVALID;
+    ws->last_launches = (uint32_t)C + 1;
+    ws->last_fallback = 0;
     return GCX_OK;
 }
 
@@ -1182,7 +1390,7 @@ int gcx_validate(gcx_workspace *ws, const gcx_graph *g, void *stream, int *max_c
     hipStream_t s = (hipStream_t)gcx__pick(stream);
     gcxk_ctl *vc = ws->ctl + 1;
     HIPC(hipMemsetAsync(vc, 0, sizeof(gcxk_ctl), s));
-    TRY(gcxk_validate(g->N, g->V, g->E, ws->color, g->hubs, g->nhub, vc, s));
+    TRY(gcxk_validate(g->N, g->V, g->E, ws->color, g->hubs, g->nhub, g->perm, vc, s));
     HIPC(hipMemcpyAsync(ws->hctl + 1, vc, sizeof(gcxk_ctl), hipMemcpyDeviceToHost, s));
     HIPC(hipStreamSynchronize(s));
     *max_color = ws->hctl[1].bad ? 0 : (ws->hctl[1].maxcolor > 0 ? ws->hctl[1].maxcolor : 1);

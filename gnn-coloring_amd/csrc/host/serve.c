@@ -33,6 +33,8 @@ struct gcx_server {
     worker *w;
     uint64_t budget;
     gcx_model *model[5]; /* by layer count 2..4 */
+    const gcx_graph **resident;
+    uint32_t nresident, resident_cap;
     pthread_mutex_t lock;
     pthread_cond_t cv;      /* queue / memory changed */
     pthread_cond_t done_cv; /* a job finished */
@@ -89,6 +91,8 @@ static uint64_t request_pairs(const gcx_request *r, uint32_t *N)
 
 uint64_t gcx_request_footprint(const gcx_request *r)
 {
+    if (r->generator == GCX_GEN_RESIDENT)
+        return 0;
     uint32_t N = 0;
     const uint64_t L = request_pairs(r, &N);
     const uint64_t M = 2 * L;
@@ -117,13 +121,27 @@ static int make_graph(const gcx_request *r, hipStream_t s, gcx_graph **g)
     }
 }
 
+static const gcx_graph *resident_graph(const gcx_server *S, const gcx_request *r)
+{
+    return (r->generator == GCX_GEN_RESIDENT && r->n < S->nresident) ? S->resident[r->n] : NULL;
+}
+
+static uint64_t job_footprint(const gcx_server *S, const gcx_request *r)
+{
+    if (r->generator != GCX_GEN_RESIDENT)
+        return gcx_request_footprint(r);
+    const gcx_graph *g = resident_graph(S, r);
+    return g ? gcx_workspace_footprint(g->N, g->M) : 0;
+}
+
 static void run_job(worker *w, const gcx_request *r, gcx_reply *out)
 {
     gcx_server *S = w->srv;
     hipStream_t s = w->stream;
-    gcx_graph *g = NULL;
+    const gcx_graph *res = resident_graph(S, r);
+    gcx_graph *g = (gcx_graph *)res;
     gcx_workspace *ws = NULL;
-    int rc = make_graph(r, s, &g);
+    int rc = res ? GCX_OK : r->generator == GCX_GEN_RESIDENT ? GCX_ERR_ARG : make_graph(r, s, &g);
     if (rc == GCX_OK)
         rc = gcx_workspace_create_on(g->N, g->M, s, &ws);
     if (rc == GCX_OK)
@@ -150,7 +168,8 @@ static void run_job(worker *w, const gcx_request *r, gcx_reply *out)
     out->digest = digest;
     out->worker = w->id;
     (void)gcx_workspace_free_on(ws, s);
-    (void)gcx_graph_free_on(g, s);
+    if (!res)
+        (void)gcx_graph_free_on(g, s);
 }
 
 static int head_fits(const gcx_server *S)
@@ -256,7 +275,7 @@ int gcx_server_run(gcx_server *S, const gcx_request *reqs, size_t n, gcx_reply *
         return GCX_ERR_NOMEM;
     memset(replies, 0, sizeof(gcx_reply) * n);
     for (size_t i = 0; i < n; i++)
-        replies[i].footprint = gcx_request_footprint(&reqs[i]);
+        replies[i].footprint = job_footprint(S, &reqs[i]);
     pthread_mutex_lock(&S->lock);
     S->req = reqs;
     S->rep = replies;
@@ -302,6 +321,36 @@ int gcx_server_run(gcx_server *S, const gcx_request *reqs, size_t n, gcx_reply *
     return GCX_OK;
 }
 
+int gcx_server_add_graph(gcx_server *S, const gcx_graph *g, uint32_t *id)
+{
+    if (!S || !g || !id)
+        return GCX_ERR_ARG;
+    const uint64_t bytes = gcx_graph_footprint(g->N, g->M);
+    pthread_mutex_lock(&S->lock);
+    int rc = GCX_OK;
+    if (S->req)
+        rc = GCX_ERR_ARG; /* a run is active */
+    else if (bytes > S->budget)
+        rc = GCX_ERR_BUDGET;
+    else if (S->nresident == S->resident_cap) {
+        const uint32_t cap = S->resident_cap ? 2 * S->resident_cap : 8;
+        const gcx_graph **p = realloc(S->resident, sizeof(*p) * cap);
+        if (!p)
+            rc = GCX_ERR_NOMEM;
+        else {
+            S->resident = p;
+            S->resident_cap = cap;
+        }
+    }
+    if (rc == GCX_OK) {
+        S->budget -= bytes;
+        *id = S->nresident;
+        S->resident[S->nresident++] = g;
+    }
+    pthread_mutex_unlock(&S->lock);
+    return rc;
+}
+
 uint64_t gcx_server_peak_reserved(const gcx_server *S) { return S ? S->peak : 0; }
 
 int gcx_server_destroy(gcx_server *S)
@@ -322,6 +371,7 @@ int gcx_server_destroy(gcx_server *S)
     pthread_cond_destroy(&S->cv);
     pthread_cond_destroy(&S->done_cv);
     pthread_mutex_destroy(&S->lock);
+    free(S->resident);
     free(S->w);
     free(S);
     return GCX_OK;

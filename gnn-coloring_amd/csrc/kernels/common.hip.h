@@ -103,3 +103,12 @@ __device__ __forceinline__ bool precedes(K kv, uint32_t v, K ku, uint32_t u)
 {
     return kv > ku || (kv == ku && v < u);
 }
+
+// SplitMix64 finaliser: the per-vertex term of the colouring digest.
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}

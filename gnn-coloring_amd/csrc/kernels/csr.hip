@@ -79,20 +79,52 @@ __global__ __launch_bounds__(kBlock) void scatter_rows(uint64_t n, const uint64_
 }
 
 
-__global__ __launch_bounds__(kBlock) void degree_stats(uint32_t N, const uint32_t *__restrict__ V,
-                                                       uint32_t *__restrict__ hubs,
-                                                       uint32_t *__restrict__ tmp)
+// Hub flags and hub chunk counts per vertex (entry N = 0 closes the scans),
+// and the maximum degree (one atomic per workgroup).
+__global__ __launch_bounds__(kBlock) void hub_flags(uint32_t N, const uint32_t *__restrict__ V,
+                                                    uint32_t *__restrict__ flag,
+                                                    uint32_t *__restrict__ chunks,
+                                                    uint32_t *__restrict__ maxdeg)
 {
+    __shared__ uint32_t wmax[kBlock / kWave];
     const uint32_t u = blockIdx.x * kBlock + threadIdx.x;
     uint32_t d = 0;
     if (u < N) {
         d = V[u + 1] - V[u];
-        if (d >= GCXK_HUB_DEG)
-            hubs[atomicAdd(tmp + 0, 1u)] = u;
+        const bool hub = d >= GCXK_HUB_DEG;
+        flag[u] = hub ? 1u : 0u;
+        chunks[u] = hub ? (d + GCXK_HUB_CHUNK_EDGES - 1u) / GCXK_HUB_CHUNK_EDGES : 0u;
+    } else if (u == N) {
+        flag[u] = 0u;
+        chunks[u] = 0u;
     }
     d = group_max32<kWave>(d);
     if (lane_id() == 0)
-        atomicMax(tmp + 1, d);
+        wmax[threadIdx.x / kWave] = d;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t m = 0;
+        for (int i = 0; i < kBlock / kWave; i++)
+            m = wmax[i] > m ? wmax[i] : m;
+        if (m)
+            atomicMax(maxdeg, m);
+    }
+}
+
+// Hub list in ascending id order and its chunk prefix (hchunk[nhub] = total).
+__global__ __launch_bounds__(kBlock) void hub_scatter(uint32_t N, const uint32_t *__restrict__ flag,
+                                                      const uint32_t *__restrict__ pos,
+                                                      const uint32_t *__restrict__ cpre,
+                                                      uint32_t *__restrict__ hubs,
+                                                      uint32_t *__restrict__ hchunk)
+{
+    const uint32_t u = blockIdx.x * kBlock + threadIdx.x;
+    if (u < N && flag[u]) {
+        hubs[pos[u]] = u;
+        hchunk[pos[u]] = cpre[u];
+    } else if (u == N) {
+        hchunk[pos[N]] = cpre[N];
+    }
 }
 
 // Binary search for x in row `row`; a malformed row (decreasing or past the
@@ -437,8 +469,7 @@ extern "C" int gcxk_degree_order(uint32_t N, const uint32_t *dV, uint32_t maxdeg
     if (hipGetLastError() != hipSuccess ||
         rocprim::radix_sort_pairs(nullptr, tb, k0, k1, id, perm, N, 0u, bits, s) != hipSuccess ||
         hipMallocAsync(&tmp, tb ? tb : 1, s) != hipSuccess ||
-        rocprim::radix_sort_pairs(tmp, tb, k0, k1, id, perm, N, 0u, bits, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess)
+        rocprim::radix_sort_pairs(tmp, tb, k0, k1, id, perm, N, 0u, bits, s) != hipSuccess)
         goto done;
     rc = GCX_OK;
 done:
@@ -453,19 +484,38 @@ done:
     return rc;
 }
 
-extern "C" int gcxk_graph_stats(uint32_t N, const uint32_t *dV, uint32_t *dhub, uint32_t *dtmp,
-                                uint32_t *nhub, uint32_t *maxdeg, void *stream)
+extern "C" int gcxk_graph_stats(uint32_t N, const uint32_t *dV, uint32_t *dhub, uint32_t *dchunk,
+                                uint32_t *nhub, uint32_t *maxdeg, uint32_t *nchunk, void *stream)
 {
     hipStream_t s = as_stream(stream);
-    GCXK_CHECK(hipMemsetAsync(dtmp, 0, 2 * sizeof(uint32_t), s));
-    if (N)
-        degree_stats<<<div_up(N, kBlock), kBlock, 0, s>>>(N, dV, dhub, dtmp);
+    *nhub = *maxdeg = *nchunk = 0;
+    DevBuf flag, chunks, pos, cpre, mx, temp;
+    const size_t n1 = (size_t)N + 1;
+    GCXK_CHECK(flag.alloc(4 * n1, s));
+    GCXK_CHECK(chunks.alloc(4 * n1, s));
+    GCXK_CHECK(pos.alloc(4 * n1, s));
+    GCXK_CHECK(cpre.alloc(4 * n1, s));
+    GCXK_CHECK(mx.alloc(4, s));
+    GCXK_CHECK(hipMemsetAsync(mx.p, 0, 4, s));
+    uint32_t *f = (uint32_t *)flag.p, *c = (uint32_t *)chunks.p, *ps = (uint32_t *)pos.p,
+             *cp = (uint32_t *)cpre.p;
+    hub_flags<<<div_up(n1, kBlock), kBlock, 0, s>>>(N, dV, f, c, (uint32_t *)mx.p);
     GCXK_LAUNCH_CHECK();
-    uint32_t h[2];
-    GCXK_CHECK(hipMemcpyAsync(h, dtmp, sizeof(h), hipMemcpyDeviceToHost, s));
+    size_t tb = 0;
+    GCXK_CHECK(rocprim::exclusive_scan(nullptr, tb, f, ps, 0u, n1, rocprim::plus<uint32_t>(), s));
+    GCXK_CHECK(temp.alloc(tb, s));
+    GCXK_CHECK(rocprim::exclusive_scan(temp.p, tb, f, ps, 0u, n1, rocprim::plus<uint32_t>(), s));
+    GCXK_CHECK(rocprim::exclusive_scan(temp.p, tb, c, cp, 0u, n1, rocprim::plus<uint32_t>(), s));
+    hub_scatter<<<div_up(n1, kBlock), kBlock, 0, s>>>(N, f, ps, cp, dhub, dchunk);
+    GCXK_LAUNCH_CHECK();
+    uint32_t h[3] = {0, 0, 0};
+    GCXK_CHECK(hipMemcpyAsync(&h[0], ps + N, 4, hipMemcpyDeviceToHost, s));
+    GCXK_CHECK(hipMemcpyAsync(&h[1], mx.p, 4, hipMemcpyDeviceToHost, s));
+    GCXK_CHECK(hipMemcpyAsync(&h[2], cp + N, 4, hipMemcpyDeviceToHost, s));
     GCXK_CHECK(hipStreamSynchronize(s));
     *nhub = h[0];
     *maxdeg = h[1];
+    *nchunk = h[2];
     return GCX_OK;
 }
 

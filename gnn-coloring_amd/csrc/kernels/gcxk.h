@@ -40,14 +40,14 @@ typedef struct {
  * caller frees) and fills dV (N+1 entries). Temporaries are freed inside. */
 int gcxk_csr_from_pairs(uint32_t N, uint64_t L, const uint32_t *dI, const uint32_t *dJ,
                         uint32_t *dV, uint32_t **dE, uint64_t *M, void *stream);
-/* Hub list: vertices with degree >= GCXK_HUB_DEG, ascending id. Writes the
- * count to *nhub (host) and the ids to dhub (capacity N). Also returns max
- * degree. Uses dtmp (>= 8 bytes device scratch). */
-int gcxk_graph_stats(uint32_t N, const uint32_t *dV, uint32_t *dhub, uint32_t *dtmp,
-                     uint32_t *nhub, uint32_t *maxdeg, void *stream);
+/* Hub list: vertices with degree >= GCXK_HUB_DEG in ascending id order into
+ * dhub (capacity N), their 4096-edge chunk prefix into dchunk (capacity N+1,
+ * dchunk[nhub] = total chunks); counts and the maximum degree to the host.
+ * One host synchronisation. */
+int gcxk_graph_stats(uint32_t N, const uint32_t *dV, uint32_t *dhub, uint32_t *dchunk,
+                     uint32_t *nhub, uint32_t *maxdeg, uint32_t *nchunk, void *stream);
 /* Degree order: perm = vertex ids by degree descending, id ascending (so the
- * nhub hubs come first). Temporaries are allocated and freed inside;
- * synchronises. */
+ * nhub hubs come first). Stream-ordered temporaries; no synchronisation. */
 int gcxk_degree_order(uint32_t N, const uint32_t *dV, uint32_t maxdeg, uint32_t *perm,
                       void *stream);
 /* CSR validity (sorted, simple, symmetric): *valid = 1 or 0. */
@@ -70,6 +70,20 @@ int gcxk_gen_er(uint32_t n, uint64_t m, uint64_t seed, uint32_t *dI, uint32_t *d
 /* 2D random geometric graph: allocates *dI, *dJ (caller frees), *L pairs. */
 int gcxk_gen_rgg(uint32_t n, uint32_t cells, uint64_t r2, uint64_t seed, uint32_t **dI,
                  uint32_t **dJ, uint64_t *L, void *stream);
+
+/* ---- packed graphs (batch.hip) ---- */
+/* Copy one graph (n vertices, m entries) into a pack at vertex offset off and
+ * entry offset eoff (Vout rows, Eout columns shifted by off) and write its
+ * graph-local id features fid[off + i] = (float)((double)i / n). */
+int gcxk_pack_graph(uint32_t n, uint64_t m, const uint32_t *dV, const uint32_t *dE, uint32_t off,
+                    uint32_t eoff, uint32_t *Vout, uint32_t *Eout, float *fid, void *stream);
+/* Vout[N] = M. */
+int gcxk_pack_close(uint32_t N, uint32_t M, uint32_t *Vout, void *stream);
+/* Per packed graph (goff: count + 1 first vertices): largest colour into
+ * maxc and the colouring digest over graph-local ids into digest (both
+ * zeroed by the caller). */
+int gcxk_packed_stats(uint32_t N, uint32_t count, const uint32_t *goff, const int *color, int *maxc,
+                      uint64_t *digest, void *stream);
 
 /* ---- ordering keys (keys.hip); key32: larger = earlier ---- */
 int gcxk_keys_ff(uint32_t N, uint32_t *key, void *stream);
@@ -106,18 +120,20 @@ int gcxk_sl_dec(const uint32_t *dV, const uint32_t *dE, uint32_t *deg, const uin
  * unused), 1 = hidden (x -> out), 2 = output (x -> key, float order keys).
  * Rows are 16 floats. Hub rows run on hub_stream, the rest on stream in the
  * degree order perm (gcxk_degree_order; non-hubs are perm[nhub .. N)); the
- * caller joins the two streams before the next layer. */
+ * caller joins the two streams before the next layer. fidv (input layer
+ * only): per-vertex id feature, NULL = (float)(v / N). */
 int gcxk_gnn_fp32_layer(int kind, uint32_t N, const uint32_t *dV, const uint32_t *dE,
                         const float *W, const float *b, int md, const float *x, float *out,
                         uint32_t *key, const uint32_t *hubs, uint32_t nhub, const uint32_t *perm,
-                        void *stream, void *hub_stream);
+                        const float *fidv, void *stream, void *hub_stream);
 /* BF16 rows (16 x bf16 = 32 B), FP32 aggregation, MFMA layer transform.
  * Wpack: the layer's two MFMA B operands in lane order (512 bf16); Wb: the
  * layer's weights W[k][j] in bf16 (512); b: FP32 biases (16). */
 int gcxk_gnn_bf16_layer(int kind, uint32_t N, const uint32_t *dV, const uint32_t *dE,
                         const uint16_t *Wpack, const uint16_t *Wb, const float *b, int md,
                         const uint16_t *x, uint16_t *out, uint32_t *key, const uint32_t *hubs,
-                        uint32_t nhub, const uint32_t *perm, void *stream, void *hub_stream);
+                        uint32_t nhub, const uint32_t *perm, const float *fidv, void *stream,
+                        void *hub_stream);
 
 /* ---- Jones-Plassmann (jp.hip) ---- */
 /* Work item: (vertex, chunk of 64 successor edges). Stored as two uint32. */
@@ -185,20 +201,28 @@ int gcxk_jp_order_items(uint32_t N, const uint32_t *dV, const uint32_t *npred, c
 /* Default workgroups of the ordered kernel: 4 per CU, at most what is
  * resident at once (0 on error). */
 uint32_t gcxk_jp_ordered_grid(void);
-/* Ordered schedule, step 2: one cooperative launch of `grid` workgroups (0 =
+/* Ordered schedule, step 2: one launch of `grid` workgroups (0 =
  * gcxk_jp_ordered_grid(); larger values are capped at the resident maximum);
- * wave w colours items w, w + W, ... waiting on each item's state word. spin_ticks bounds every wait (100 MHz ticks); exceeding
- * it sets ctl->err and ends the kernel. */
+ * wave w colours items w, w + W, ... waiting on each item's state word.
+ * spin_ticks bounds every wait (100 MHz ticks); exceeding it sets ctl->err
+ * and ends the kernel. cooperative = 1 launches cooperatively (the runtime
+ * checks residency and runs it alone); 0 launches it as a plain kernel that
+ * shares the device with other streams (the caller keeps the grids of
+ * concurrent ordered launches within the resident maximum). */
 int gcxk_jp_ordered(const uint32_t *dV, const uint32_t *npred, const uint32_t *esplit,
                     uint64_t *words, const uint32_t *wof, int *color, uint32_t *ovf, int mb,
                     const gcxk_item *items, gcxk_ctl *ctl, uint32_t grid, uint64_t spin_ticks,
-                    void *stream);
+                    int cooperative, void *stream);
+/* Workgroups of the ordered kernel resident on the device at once (0 on
+ * error). */
+uint32_t gcxk_jp_ordered_cap(void);
 /* Degrees of the listed vertices (for the hub chunk table). */
 int gcxk_gather_degrees(const uint32_t *dV, const uint32_t *list, uint32_t n, uint32_t *out,
                         void *stream);
 /* Proper-colouring check: ctl->bad, ctl->maxcolor (ctl fields zeroed by caller). */
 int gcxk_validate(uint32_t N, const uint32_t *dV, const uint32_t *dE, const int *color,
-                  const uint32_t *hubs, uint32_t nhub, gcxk_ctl *ctl, void *stream);
+                  const uint32_t *hubs, uint32_t nhub, const uint32_t *perm, gcxk_ctl *ctl,
+                  void *stream);
 /* Greedy-rule certificate: ctl->viol. */
 int gcxk_mex_check(uint32_t N, const uint32_t *dV, const uint32_t *dE, const void *key,
                    int key_bits, const int *color, const uint32_t *hubs, uint32_t nhub,

@@ -40,6 +40,13 @@ __device__ __forceinline__ float fid(uint32_t v, uint32_t N)
     return (float)((double)v / (double)N);
 }
 
+// Id feature of v: v / N, or the caller's per-vertex value (a batch of graphs
+// packed into one uses each vertex's id and size within its own graph).
+__device__ __forceinline__ float fid_of(uint32_t v, uint32_t N, const float *__restrict__ fidv)
+{
+    return fidv ? fidv[v] : fid(v, N);
+}
+
 // Dense step for lane j of a 16-lane group; A[i] for i < 16 is a_lo of lane
 // i, for i >= 16 a_hi of lane i - 16.
 __device__ __forceinline__ float dense_group(const float (&w)[2 * F], float bias, float a_lo,
@@ -111,7 +118,7 @@ __global__ __launch_bounds__(kBlock) void sage_quads(uint32_t N, const uint32_t 
                                                      float *__restrict__ out,
                                                      uint32_t *__restrict__ key,
                                                      const uint32_t *__restrict__ perm, uint32_t p0,
-                                                     uint32_t p1)
+                                                     uint32_t p1, const float *__restrict__ fidv)
 {
     __shared__ float4 Ws[2 * F][4];
     __shared__ float4 bs[4];
@@ -144,7 +151,7 @@ __global__ __launch_bounds__(kBlock) void sage_quads(uint32_t N, const uint32_t 
                     if (i < end) {
                         const uint32_t v = E[i];
                         fd[t] = (float)(V[v + 1] - V[v]) / fmd;
-                        fi[t] = fid(v, N);
+                        fi[t] = fid_of(v, N, fidv);
                     }
                 }
                 const uint32_t nb = end - i0 < 16u ? end - i0 : 16u;
@@ -163,7 +170,7 @@ __global__ __launch_bounds__(kBlock) void sage_quads(uint32_t N, const uint32_t 
                 sd = sd / fdeg_f;
                 si = si / fdeg_f;
             }
-            const float selfd = fdeg_f / fmd, selfi = fid(u, N);
+            const float selfd = fdeg_f / fmd, selfi = fid_of(u, N, fidv);
 #pragma unroll
             for (int i = 0; i < 2 * F; i++) {
                 const float a = i < 8 ? sd : i < 16 ? si : i < 24 ? selfd : selfi;
@@ -258,7 +265,8 @@ __global__ __launch_bounds__(kHubBlock) void sage_hubs(uint32_t N, const uint32_
                                                        const float *__restrict__ x,
                                                        float *__restrict__ out,
                                                        uint32_t *__restrict__ key,
-                                                       const uint32_t *__restrict__ hubs)
+                                                       const uint32_t *__restrict__ hubs,
+                                                       const float *__restrict__ fidv)
 {
     // Feature-major staging: buf[s][f][r] = feature f of row r of step s.
     __shared__ __attribute__((aligned(16))) float buf[2][F][kHubChunk + kHubPad];
@@ -276,7 +284,7 @@ __global__ __launch_bounds__(kHubBlock) void sage_hubs(uint32_t N, const uint32_
             const uint32_t v = E[i];
             if (kind == kInput) {
                 buf[s][0][tid] = (float)(V[v + 1] - V[v]) / fmd;
-                buf[s][1][tid] = fid(v, N);
+                buf[s][1][tid] = fid_of(v, N, fidv);
             } else {
                 const float4 *row = reinterpret_cast<const float4 *>(x + (size_t)v * F);
 #pragma unroll
@@ -340,7 +348,7 @@ __global__ __launch_bounds__(kHubBlock) void sage_hubs(uint32_t N, const uint32_
             sd = sd / fdeg_f;
             si = si / fdeg_f;
         }
-        h = dense_input(w, bias, sd, si, fdeg_f / fmd, fid(u, N));
+        h = dense_input(w, bias, sd, si, fdeg_f / fmd, fid_of(u, N, fidv));
     } else {
         float a = acc;
         if (fdeg_f > 0.0f)
@@ -365,20 +373,21 @@ constexpr uint32_t kQuadGrid = 2048;
 template <int kind>
 int launch_layer(uint32_t N, const uint32_t *V, const uint32_t *E, const float *W, const float *b,
                  float fmd, const float *x, float *out, uint32_t *key, const uint32_t *hubs,
-                 uint32_t nhub, const uint32_t *perm, void *stream, void *hub_stream)
+                 uint32_t nhub, const uint32_t *perm, const float *fidv, void *stream,
+                 void *hub_stream)
 {
     if (N == 0)
         return GCX_OK;
     // Hubs go first on their own stream so that they overlap the quads.
     if (nhub)
         sage_hubs<kind><<<nhub, kHubBlock, 0, as_stream(hub_stream)>>>(N, V, E, W, b, fmd, x, out,
-                                                                       key, hubs);
+                                                                       key, hubs, fidv);
     const uint32_t rest = N - nhub;
     if (rest) {
         uint32_t grid = div_up(rest, kQuads);
         grid = grid < kQuadGrid ? grid : kQuadGrid;
         sage_quads<kind><<<grid, kBlock, 0, as_stream(stream)>>>(N, V, E, W, b, fmd, x, out, key,
-                                                                 perm, nhub, N);
+                                                                 perm, nhub, N, fidv);
     }
     GCXK_LAUNCH_CHECK();
     return GCX_OK;
@@ -389,18 +398,19 @@ int launch_layer(uint32_t N, const uint32_t *V, const uint32_t *E, const float *
 extern "C" int gcxk_gnn_fp32_layer(int kind, uint32_t N, const uint32_t *dV, const uint32_t *dE,
                                    const float *W, const float *b, int md, const float *x,
                                    float *out, uint32_t *key, const uint32_t *hubs, uint32_t nhub,
-                                   const uint32_t *perm, void *stream, void *hub_stream)
+                                   const uint32_t *perm, const float *fidv, void *stream,
+                                   void *hub_stream)
 {
     switch (kind) {
     case 0:
         return launch_layer<kInput>(N, dV, dE, W, b, (float)md, nullptr, out, nullptr, hubs, nhub,
-                                    perm, stream, hub_stream);
+                                    perm, fidv, stream, hub_stream);
     case 1:
         return launch_layer<kHidden>(N, dV, dE, W, b, 0.0f, x, out, nullptr, hubs, nhub, perm,
-                                     stream, hub_stream);
+                                     nullptr, stream, hub_stream);
     case 2:
         return launch_layer<kOutput>(N, dV, dE, W, b, 0.0f, x, nullptr, key, hubs, nhub, perm,
-                                     stream, hub_stream);
+                                     nullptr, stream, hub_stream);
     default:
         return GCX_ERR_ARG;
     }

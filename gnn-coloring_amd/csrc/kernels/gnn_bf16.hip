@@ -47,6 +47,12 @@ __device__ __forceinline__ float from_bf16(uint32_t h) { return __builtin_bit_ca
 
 __device__ __forceinline__ float fid(uint32_t v, uint32_t N) { return (float)((double)v / (double)N); }
 
+// Id feature: v / N, or the caller's per-vertex value (graph-local for batches).
+__device__ __forceinline__ float fid_of(uint32_t v, uint32_t N, const float *__restrict__ fidv)
+{
+    return fidv ? fidv[v] : fid(v, N);
+}
+
 // Accumulate the 16 bf16 features of row r (two 16-byte loads).
 __device__ __forceinline__ void add_row(const uint16_t *__restrict__ x, uint32_t r, float (&s)[F])
 {
@@ -69,7 +75,7 @@ __global__ __launch_bounds__(kBlock) void sage_bf16_tiles(uint32_t N, const uint
                                                           uint16_t *__restrict__ out,
                                                           uint32_t *__restrict__ key,
                                                           const uint32_t *__restrict__ perm,
-                                                          uint32_t p0)
+                                                          uint32_t p0, const float *__restrict__ fidv)
 {
     // Operand tile: 64 vertices x 32 bf16, padded to 36 to spread LDS banks.
     __shared__ uint16_t A[kTile][36];
@@ -94,7 +100,7 @@ __global__ __launch_bounds__(kBlock) void sage_bf16_tiles(uint32_t N, const uint
             for (uint32_t i = beg + sub; i < end; i += kSub) {
                 const uint32_t v = E[i];
                 sd += (float)(V[v + 1] - V[v]) / fmd;
-                si += fid(v, N);
+                si += fid_of(v, N, fidv);
             }
         sd += __shfl_xor(sd, 1);
         si += __shfl_xor(si, 1);
@@ -106,7 +112,8 @@ __global__ __launch_bounds__(kBlock) void sage_bf16_tiles(uint32_t N, const uint
         }
         // k = 4*sub .. 4*sub+3 of [sd x8, si x8] and of [fdeg x8, fid x8]
         const uint16_t lo = to_bf16(sub < 2 ? sd : si);
-        const uint16_t hi = to_bf16(sub < 2 ? (skip ? 0.0f : fdeg / fmd) : (skip ? 0.0f : fid(u, N)));
+        const uint16_t hi =
+            to_bf16(sub < 2 ? (skip ? 0.0f : fdeg / fmd) : (skip ? 0.0f : fid_of(u, N, fidv)));
 #pragma unroll
         for (int e = 0; e < 4; e++) {
             A[vi][4 * sub + e] = lo;
@@ -183,7 +190,8 @@ __global__ __launch_bounds__(kHubBlock) void sage_bf16_hubs(uint32_t N, const ui
                                                             const uint16_t *__restrict__ x,
                                                             uint16_t *__restrict__ out,
                                                             uint32_t *__restrict__ key,
-                                                            const uint32_t *__restrict__ hubs)
+                                                            const uint32_t *__restrict__ hubs,
+                                                            const float *__restrict__ fidv)
 {
     __shared__ float part[kHubBlock][F + 1];
     __shared__ float mean[F];
@@ -199,7 +207,7 @@ __global__ __launch_bounds__(kHubBlock) void sage_bf16_hubs(uint32_t N, const ui
         for (uint32_t i = beg + t; i < end; i += kHubBlock) {
             const uint32_t v = E[i];
             s[0] += (float)(V[v + 1] - V[v]) / fmd;
-            s[1] += fid(v, N);
+            s[1] += fid_of(v, N, fidv);
         }
     } else {
         for (uint32_t i = beg + t; i < end; i += kHubBlock)
@@ -221,7 +229,8 @@ __global__ __launch_bounds__(kHubBlock) void sage_bf16_hubs(uint32_t N, const ui
     float a[2 * F];
     if (kind == kInput) {
         const float sd = from_bf16(to_bf16(mean[0])), si = from_bf16(to_bf16(mean[1]));
-        const float d = from_bf16(to_bf16(fdeg / fmd)), id = from_bf16(to_bf16(fid(u, N)));
+        const float d = from_bf16(to_bf16(fdeg / fmd)),
+                    id = from_bf16(to_bf16(fid_of(u, N, fidv)));
         for (int k = 0; k < 8; k++) {
             a[k] = sd;
             a[8 + k] = si;
@@ -255,16 +264,16 @@ template <int kind>
 int launch(uint32_t N, const uint32_t *V, const uint32_t *E, const uint16_t *Wpack,
            const uint16_t *Wb, const float *b, float fmd, const uint16_t *x, uint16_t *out,
            uint32_t *key, const uint32_t *hubs, uint32_t nhub, const uint32_t *perm,
-           void *stream, void *hub_stream)
+           const float *fidv, void *stream, void *hub_stream)
 {
     if (N == 0)
         return GCX_OK;
     if (nhub)
         sage_bf16_hubs<kind><<<nhub, kHubBlock, 0, as_stream(hub_stream)>>>(N, V, E, Wb, b, fmd, x,
-                                                                            out, key, hubs);
+                                                                            out, key, hubs, fidv);
     if (N > nhub)
         sage_bf16_tiles<kind><<<div_up(N - nhub, kTile), kBlock, 0, as_stream(stream)>>>(
-            N, V, E, Wpack, b, fmd, x, out, key, perm, nhub);
+            N, V, E, Wpack, b, fmd, x, out, key, perm, nhub, fidv);
     GCXK_LAUNCH_CHECK();
     return GCX_OK;
 }
@@ -275,19 +284,19 @@ extern "C" int gcxk_gnn_bf16_layer(int kind, uint32_t N, const uint32_t *dV, con
                                    const uint16_t *Wpack, const uint16_t *Wb, const float *b,
                                    int md, const uint16_t *x, uint16_t *out, uint32_t *key,
                                    const uint32_t *hubs, uint32_t nhub, const uint32_t *perm,
-                                   void *stream, void *hub_stream)
+                                   const float *fidv, void *stream, void *hub_stream)
 {
     const float fmd = (float)md;
     switch (kind) {
     case 0:
         return launch<kInput>(N, dV, dE, Wpack, Wb, b, fmd, nullptr, out, nullptr, hubs, nhub,
-                              perm, stream, hub_stream);
+                              perm, fidv, stream, hub_stream);
     case 1:
         return launch<kHidden>(N, dV, dE, Wpack, Wb, b, fmd, x, out, nullptr, hubs, nhub, perm,
-                               stream, hub_stream);
+                               nullptr, stream, hub_stream);
     case 2:
         return launch<kOutput>(N, dV, dE, Wpack, Wb, b, fmd, x, nullptr, key, hubs, nhub, perm,
-                               stream, hub_stream);
+                               nullptr, stream, hub_stream);
     default:
         return GCX_ERR_ARG;
     }

@@ -589,22 +589,26 @@ __global__ __launch_bounds__(kBlock) void jp_ordered_kernel(const uint32_t *__re
 
 // ------------------------------------------------------------- validation
 
-// Grid-stride over vertices, 16 lanes each; one atomic per workgroup.
-__global__ __launch_bounds__(kBlock) void validate_groups(uint32_t N, const uint32_t *__restrict__ V,
-                                                          const uint32_t *__restrict__ E,
-                                                          const int *__restrict__ color,
-                                                          gcxk_ctl *ctl)
+// Vertices in degree order (perm), 4 lanes each, grid-stride; one atomic per
+// workgroup. Hub rows (perm[0 .. nhub)) only get their own colour checked
+// here; their edges are validate_hubs' work.
+__global__ __launch_bounds__(kBlock) void validate_quads(uint32_t N, const uint32_t *__restrict__ V,
+                                                         const uint32_t *__restrict__ E,
+                                                         const int *__restrict__ color,
+                                                         const uint32_t *__restrict__ perm,
+                                                         uint32_t nhub, gcxk_ctl *ctl)
 {
     __shared__ uint32_t sbad[kWavesPerBlock], stop[kWavesPerBlock];
-    const int gl = threadIdx.x % kGroup;
-    const uint32_t gpb = kBlock / kGroup;
+    const uint32_t gl = threadIdx.x % kQuad;
+    const uint32_t qpb = kBlock / kQuad;
     uint32_t bad = 0, top = 0;
-    for (uint64_t u = (uint64_t)blockIdx.x * gpb + threadIdx.x / kGroup; u < N;
-         u += (uint64_t)gridDim.x * gpb) {
-        const uint32_t beg = V[u], end = V[u + 1];
+    for (uint64_t p = (uint64_t)blockIdx.x * qpb + threadIdx.x / kQuad; p < N;
+         p += (uint64_t)gridDim.x * qpb) {
+        const uint32_t u = perm[p];
         const int cu = color[u];
-        if (end - beg < GCXK_HUB_DEG) {
-            for (uint32_t i = beg + gl; i < end; i += kGroup)
+        if (p >= nhub) {
+            const uint32_t beg = V[u], end = V[u + 1];
+            for (uint32_t i = beg + gl; i < end; i += kQuad)
                 bad += (color[E[i]] == cu) ? 1u : 0u;
         }
         if (gl == 0) {
@@ -774,14 +778,6 @@ int sort_order(uint32_t N, const void *key, void *scratch, size_t scratch_bytes,
 }
 
 // Order-independent digest of a colouring: sum over u of mix(u, colour[u]).
-__device__ __forceinline__ uint64_t mix64(uint64_t z)
-{
-    z += 0x9E3779B97F4A7C15ull;
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    return z ^ (z >> 31);
-}
-
 __global__ __launch_bounds__(kBlock) void color_digest_kernel(uint32_t N, const int *__restrict__ color,
                                                               unsigned long long *out)
 {
@@ -990,10 +986,12 @@ extern "C" uint32_t gcxk_jp_ordered_grid(void)
     return g_ordered_grid;
 }
 
+extern "C" uint32_t gcxk_jp_ordered_cap(void) { return gcxk_jp_ordered_grid() ? g_ordered_cap : 0; }
+
 extern "C" int gcxk_jp_ordered(const uint32_t *dV, const uint32_t *npred, const uint32_t *esplit,
                                uint64_t *words, const uint32_t *wof, int *color, uint32_t *ovf,
                                int mb, const gcxk_item *items, gcxk_ctl *ctl, uint32_t grid,
-                               uint64_t spin_ticks, void *stream)
+                               uint64_t spin_ticks, int cooperative, void *stream)
 {
     const uint32_t dflt = gcxk_jp_ordered_grid();
     if (!dflt)
@@ -1006,20 +1004,28 @@ extern "C" int gcxk_jp_ordered(const uint32_t *dV, const uint32_t *npred, const 
     void *args[] = {(void *)&dV,  (void *)&npred, (void *)&esplit, (void *)&words,
                     (void *)&wof, (void *)&color, (void *)&ovf,    (void *)&it,
                     (void *)&ctl, (void *)&mb,    (void *)&spin_ticks};
-    GCXK_CHECK(hipLaunchCooperativeKernel((const void *)jp_ordered_kernel, dim3(grid), dim3(kBlock),
-                                          args, 0, as_stream(stream)));
+    if (cooperative) {
+        GCXK_CHECK(hipLaunchCooperativeKernel((const void *)jp_ordered_kernel, dim3(grid),
+                                              dim3(kBlock), args, 0, as_stream(stream)));
+    } else {
+        // grid <= the resident capacity: once the kernels of other streams
+        // (none of which wait on anything) drain, every workgroup is resident
+        GCXK_CHECK(hipLaunchKernel((const void *)jp_ordered_kernel, dim3(grid), dim3(kBlock), args,
+                                   0, as_stream(stream)));
+    }
     return GCX_OK;
 }
 
 extern "C" int gcxk_validate(uint32_t N, const uint32_t *dV, const uint32_t *dE, const int *color,
-                             const uint32_t *hubs, uint32_t nhub, gcxk_ctl *ctl, void *stream)
+                             const uint32_t *hubs, uint32_t nhub, const uint32_t *perm,
+                             gcxk_ctl *ctl, void *stream)
 {
     if (N == 0)
         return GCX_OK;
-    uint32_t grid = div_up((uint64_t)N * kGroup, kBlock);
+    uint32_t grid = div_up((uint64_t)N * kQuad, kBlock);
     if (grid > 2048)
         grid = 2048;
-    validate_groups<<<grid, kBlock, 0, as_stream(stream)>>>(N, dV, dE, color, ctl);
+    validate_quads<<<grid, kBlock, 0, as_stream(stream)>>>(N, dV, dE, color, perm, nhub, ctl);
     if (nhub)
         validate_hubs<<<nhub, kBlock, 0, as_stream(stream)>>>(dV, dE, color, hubs, ctl);
     GCXK_LAUNCH_CHECK();

@@ -71,6 +71,9 @@ def _load():
         "gcx_graph_download": (C.c_int, [vp, u32p, u32p]),
         "gcx_graph_check": (C.c_int, [vp, vp, i32p]),
         "gcx_graph_footprint": (C.c_uint64, [C.c_uint32, C.c_uint64]),
+        "gcx_graph_pack": (C.c_int, [C.POINTER(vp), C.c_uint32, vp, C.POINTER(vp)]),
+        "gcx_graph_packed_count": (C.c_uint32, [vp]),
+        "gcx_packed_colors": (C.c_int, [vp, vp, i32p, C.POINTER(C.c_uint64), vp]),
         "gcx_model_load": (C.c_int, [C.c_char_p, C.POINTER(vp)]),
         "gcx_model_from_params": (C.c_int, [C.c_int, f32p, C.c_int, C.POINTER(vp)]),
         "gcx_model_free": (C.c_int, [vp]),
@@ -91,6 +94,9 @@ def _load():
         "gcx_server_destroy": (C.c_int, [vp]),
         "gcx_request_footprint": (C.c_uint64, [vp]),
         "gcx_server_peak_reserved": (C.c_uint64, [vp]),
+        "gcx_server_add_graph": (C.c_int, [vp, vp, u32p]),
+        "gcx_set_ordered_launch": (C.c_int, [C.c_int]),
+        "gcx_get_ordered_launch": (C.c_int, []),
         "gcx_download_colors": (C.c_int, [vp, C.c_uint32, i32p, vp]),
         "gcx_upload_colors": (C.c_int, [vp, C.c_uint32, i32p, vp]),
         "gcx_download_priorities": (C.c_int, [vp, C.c_uint32, f64p, vp]),
@@ -232,6 +238,16 @@ def _sh(stream):
     return stream.handle if stream is not None else None
 
 
+def set_ordered_launch(slots: int):
+    """0: cooperative ORDERED launches, one at a time; k >= 1: plain
+    launches sharing the device, at most k at once (gcx_set_ordered_launch)."""
+    _chk(_lib.gcx_set_ordered_launch(slots), "set ordered launch")
+
+
+def get_ordered_launch() -> int:
+    return int(_lib.gcx_get_ordered_launch())
+
+
 def mem_info():
     f, t = C.c_size_t(), C.c_size_t()
     _chk(_lib.gcx_mem_info(C.byref(f), C.byref(t)), "mem info")
@@ -283,6 +299,18 @@ class Graph:
         h = vp()
         _chk(_lib.gcx_graph_rgg(n, radius, seed, _sh(stream), C.byref(h)), "RGG graph")
         return cls(h)
+
+    @classmethod
+    def pack(cls, graphs, stream=None):
+        """One block-diagonal graph holding `graphs` in order (gcx_graph_pack)."""
+        arr = (vp * len(graphs))(*[g.handle for g in graphs])
+        h = vp()
+        _chk(_lib.gcx_graph_pack(arr, len(graphs), _sh(stream), C.byref(h)), "graph pack")
+        return cls(h)
+
+    @property
+    def packed_count(self) -> int:
+        return int(_lib.gcx_graph_packed_count(self.handle))
 
     @property
     def N(self) -> int:
@@ -417,6 +445,16 @@ class Workspace:
         c = C.c_uint64()
         _chk(_lib.gcx_color_digest(self.handle, g.handle, _sh(stream), C.byref(c)), "digest")
         return int(c.value)
+
+    def packed_colors(self, g: Graph, stream=None):
+        """Per packed graph: (colour count, digest of the graph alone)."""
+        n = g.packed_count
+        c = np.zeros(n, np.int32)
+        d = np.zeros(n, np.uint64)
+        _chk(_lib.gcx_packed_colors(self.handle, g.handle, c.ctypes.data_as(i32p),
+                                    d.ctypes.data_as(C.POINTER(C.c_uint64)), _sh(stream)),
+             "packed colours")
+        return c, d
 
     def colors(self, N, stream=None):
         out = np.empty(N, np.int32)
@@ -625,7 +663,7 @@ class ref:
 
 # ---------------------------------------------------------------- serving
 
-GEN_RMAT, GEN_ER, GEN_RGG = 0, 1, 2
+GEN_RMAT, GEN_ER, GEN_RGG, GEN_RESIDENT = 0, 1, 2, 3
 
 
 class Request(C.Structure):
@@ -708,6 +746,15 @@ class Server:
         md = (model_dir or MODEL_DIR).encode()
         _chk(_lib.gcx_server_create(workers, budget, md, C.byref(h)), "server create")
         self.handle = h
+        self._resident = []
+
+    def add_graph(self, g: Graph) -> int:
+        """Register a resident graph; requests name it as
+        make_request(GEN_RESIDENT, id, ...). Keep g alive while the server is."""
+        gid = C.c_uint32()
+        _chk(_lib.gcx_server_add_graph(self.handle, g.handle, C.byref(gid)), "server add graph")
+        self._resident.append(g)
+        return int(gid.value)
 
     def run(self, requests):
         """Run requests (sorted by arrival); returns the list of Reply."""

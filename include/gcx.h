@@ -97,6 +97,14 @@ uint32_t gcx_graph_max_degree(const gcx_graph *g);
 int gcx_graph_download(const gcx_graph *g, uint32_t *V, uint32_t *E);
 /* Device CSR check (sorted, simple, symmetric, V[N] == M): 1 valid, 0 not. */
 int gcx_graph_check(const gcx_graph *g, void *stream, int *valid);
+/* Pack graphs into one block-diagonal graph (graph i's vertices follow graph
+ * i-1's). Ordering + colouring the pack gives every graph exactly the
+ * priorities and colours it gets alone (the GNN keeps graph-local id
+ * features), in one pass over the device: many small graphs become one
+ * launch sequence and one persistent JP kernel. */
+int gcx_graph_pack(const gcx_graph *const *graphs, uint32_t count, void *stream, gcx_graph **out);
+/* Number of graphs packed into g (0 for a plain graph). */
+uint32_t gcx_graph_packed_count(const gcx_graph *g);
 /* Bytes of HBM a graph with N vertices and M entries occupies. */
 uint64_t gcx_graph_footprint(uint32_t N, uint64_t M);
 
@@ -135,12 +143,23 @@ int gcx_color(gcx_workspace *ws, const gcx_graph *g, void *stream);
  *            bound the call redoes the colouring with ROUNDS. */
 enum gcx_schedule { GCX_SCHED_ROUNDS = 0, GCX_SCHED_ORDERED = 1 };
 int gcx_set_schedule(gcx_workspace *ws, int schedule, uint32_t param);
+/* How ORDERED launches share the device (process-wide). 0: cooperative
+ * launches, one at a time, each running alone on the device. k = 1..8:
+ * plain launches that overlap other streams' kernels, at most k at once,
+ * each with at most 1/k of the resident workgroups. Colours are the same
+ * under every setting. */
+int gcx_set_ordered_launch(int slots);
+int gcx_get_ordered_launch(void);
 /* 1 when the last gcx_color fell back from ORDERED to ROUNDS. */
 int gcx_last_fallback(const gcx_workspace *ws, uint32_t *fell_back);
 /* Culberson iterated greedy: `iterations` more passes, each with p = colour. */
 int gcx_recolor(gcx_workspace *ws, const gcx_graph *g, int iterations, void *stream);
 /* Proper-colouring check of the workspace colours: *max_color = C or 0. */
 int gcx_validate(gcx_workspace *ws, const gcx_graph *g, void *stream, int *max_color);
+/* Per packed graph of g: its colour count (largest colour) and the digest
+ * gcx_color_digest gives for that graph coloured alone (count entries). */
+int gcx_packed_colors(gcx_workspace *ws, const gcx_graph *g, int *colors, uint64_t *digests,
+                      void *stream);
 /* Order-independent 64-bit digest of the workspace colours: the sum over u
  * of a hash of (u, colour[u]); equal colourings give equal digests. */
 int gcx_color_digest(gcx_workspace *ws, const gcx_graph *g, void *stream, uint64_t *digest);

@@ -26,7 +26,9 @@ extern "C" {
 enum gcx_generator {
     GCX_GEN_RMAT = 0, /* R-MAT, a/b/c = .57/.19/.19, n rounded up to 2^scale */
     GCX_GEN_ER = 1,   /* Erdos-Renyi G(n, m), m = n * degree / 2 pairs */
-    GCX_GEN_RGG = 2   /* 2D random geometric graph in the unit square */
+    GCX_GEN_RGG = 2,  /* 2D random geometric graph in the unit square */
+    GCX_GEN_RESIDENT = 3 /* a graph registered with gcx_server_add_graph;
+                            n is its id, param and seed are unused */
 };
 
 typedef struct {
@@ -63,7 +65,14 @@ int gcx_server_create(int workers, uint64_t hbm_budget, const char *model_dir, g
  * are done. */
 int gcx_server_run(gcx_server *s, const gcx_request *reqs, size_t n, gcx_reply *replies);
 int gcx_server_destroy(gcx_server *s);
-/* HBM bytes a request may occupy at its peak (graph build or colouring). */
+/* Registers a device graph that requests name by id (GCX_GEN_RESIDENT):
+ * jobs on it read the resident CSR and only allocate a workspace. The caller
+ * keeps ownership and frees the graph after the server. Its bytes are taken
+ * from the budget for the server's lifetime. Not while a run is active. */
+int gcx_server_add_graph(gcx_server *s, const gcx_graph *g, uint32_t *id);
+/* HBM bytes a request may occupy at its peak (graph build or colouring);
+ * for GCX_GEN_RESIDENT the server counts the graph's workspace instead and
+ * this returns 0. */
 uint64_t gcx_request_footprint(const gcx_request *r);
 /* Peak HBM bytes reserved by admitted jobs during the last run. */
 uint64_t gcx_server_peak_reserved(const gcx_server *s);

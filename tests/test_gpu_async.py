@@ -133,3 +133,32 @@ def test_rgg_device_equals_host():
         Vd, Ed = gcx.Graph.rgg(n, r, seed=seed).download()
         np.testing.assert_array_equal(Vh, Vd)
         np.testing.assert_array_equal(Eh, Ed)
+
+
+@pytest.mark.parametrize("slots", [1, 2, 4])
+def test_shared_ordered_launch(slots, digests, gmodels):
+    """Plain (non-cooperative) ordered launches, up to `slots` at once, give
+    the reference digests; concurrent streams give the same colourings."""
+    rec = digests["rmat16"]
+    gd = rec["graph"]
+    g = gcx.Graph.rmat(gd["scale"], gd["edge_factor"], *gd["abc"], seed=gd["seed"])
+    gcx.set_ordered_launch(slots)
+    try:
+        ws = gcx.Workspace(g.N, g.M)
+        ws.set_schedule(gcx.SCHED_ORDERED)
+        for h in HEUR:
+            order(ws, g, h, gmodels)
+            ws.color(g)
+            assert not ws.fell_back(), h
+            assert sha(ws.colors(g.N)) == rec[h]["sha_color"], h
+        srv = gcx.Server(workers=4, budget=8 << 30)
+        gid = srv.add_graph(g)
+        reqs = [gcx.make_request(gcx.GEN_RESIDENT, gid, heuristic=h, schedule=gcx.SCHED_ORDERED,
+                                 arrival=0.0002 * i) for i, h in enumerate(HEUR * 2)]
+        out = srv.run(reqs)
+        srv.close()
+        for i, r in enumerate(out):
+            h = (HEUR * 2)[i]
+            assert r.status == 0 and r.colors == rec[h]["ncolors"], h
+    finally:
+        gcx.set_ordered_launch(0)

@@ -95,3 +95,45 @@ def test_serve_rejects_oversized():
     assert out[0].status == 0 and out[0].colors > 0
     assert out[1].status == gcx.ERR_BUDGET
     srv.close()
+
+
+def test_serve_resident_graphs():
+    """Requests on registered device graphs equal direct runs on them, alone
+    and mixed with generated jobs; the graph is only read."""
+    models = {l: gcx.Model.layers_file(l) for l in (2, 3)}
+    gs = [gcx.Graph.rmat(14, 16, seed=11), gcx.Graph.er(20000, 320000, seed=12)]
+    srv = gcx.Server(workers=4, budget=8 << 30)
+    ids = [srv.add_graph(g) for g in gs]
+    reqs, t = [], 0.0
+    for i in range(16):
+        h = ["FF", "SL", "GNN2", "GNN3"][i % 4]
+        prec = gcx.BF16 if i % 3 == 0 else gcx.FP32
+        sched = gcx.SCHED_ORDERED if i % 2 else gcx.SCHED_ROUNDS
+        if i % 5 == 4:
+            reqs.append(gcx.make_request(gcx.GEN_ER, 5000, 16.0, seed=i, heuristic=h,
+                                         precision=prec, schedule=sched, arrival=t))
+        else:
+            reqs.append(gcx.make_request(gcx.GEN_RESIDENT, ids[i % 2], heuristic=h,
+                                         precision=prec, schedule=sched, arrival=t))
+        t += 0.0005
+    out = srv.run(reqs)
+    assert all(r.status == 0 for r in out), [r.status for r in out]
+    for q, r in zip(reqs, out):
+        if q.generator != gcx.GEN_RESIDENT:
+            continue
+        g = gs[q.n]
+        assert (r.N, r.M) == (g.N, g.M)
+        assert r.footprint == gcx.workspace_footprint(g.N, g.M)
+        ws = gcx.Workspace(g.N, g.M)
+        ws.set_schedule(q.schedule)
+        if q.heuristic == gcx.GNN:
+            ws.order(g, gcx.GNN, models[q.layers], q.precision)
+        else:
+            ws.order(g, q.heuristic)
+        ws.color(g)
+        assert (ws.validate(g), ws.digest(g)) == (r.colors, r.digest)
+    bad = srv.run([gcx.make_request(gcx.GEN_RESIDENT, 99, heuristic="FF")])[0]
+    assert bad.status == gcx.ERR_ARG
+    srv.close()
+    for g in gs:
+        assert g.check()
