@@ -187,6 +187,113 @@ def serving_rmat(args, gcx):
                                               if (a.colors, a.digest) == (b.colors, b.digest))}
 
 
+def batching(args, gcx):
+    """Batching leg: `batch_graphs` seeded Erdos-Renyi graphs (n log-uniform
+    in [1e3, 2e5], average degree uniform in [4, 64]) coloured with GNN3 FP32
+    priorities one graph at a time and packed into one block-diagonal graph
+    (one launch sequence, one persistent JP launch), schedules and hub
+    degrees from the autotuner; plus `culberson_passes` iterated-greedy
+    passes (mixed class orders, seed per pass) of an R-MAT scale-20 graph as
+    captured graph replays vs one gcx_color per pass."""
+    import numpy as np
+    rng = np.random.default_rng(21)
+    k = args.batch_graphs
+    model = gcx.Model.layers_file(3)
+    specs = []
+    for i in range(k):
+        n = int(np.exp(rng.uniform(np.log(1e3), np.log(2e5))))
+        d = float(rng.uniform(4, 64))
+        specs.append((n, int(round(n * d / 2)), 1000 + i))
+    graphs = [gcx.Graph.er(n, m, seed=sd) for n, m, sd in specs]
+    # autotuner: one measurement per new degree signature
+    t0 = time.perf_counter()
+    sigs = {}
+    for g in graphs:
+        if g.signature not in sigs:
+            sigs[g.signature] = gcx.tune(g, model, gcx.FP32).as_dict()
+    tune_s = time.perf_counter() - t0
+    for g in graphs:  # graphs made before their signature was tuned
+        g.set_hub_degree(0)
+    maxN = max(g.N for g in graphs)
+    maxM = max(g.M for g in graphs)
+    ws = gcx.Workspace(maxN, maxM)
+    ws.set_schedule(gcx.SCHED_AUTO)
+    stream = gcx.Stream()
+
+    def one_at_a_time():
+        for g in graphs:
+            ws.order(g, gcx.GNN, model, gcx.FP32, stream)
+            ws.color(g, stream)
+        stream.sync()
+
+    one_at_a_time()
+    t0 = time.perf_counter()
+    one_at_a_time()
+    single_s = time.perf_counter() - t0
+    alone = []
+    for g in graphs:
+        ws.order(g, gcx.GNN, model, gcx.FP32, stream)
+        ws.color(g, stream)
+        alone.append((ws.validate(g, stream), ws.digest(g, stream)))
+    t0 = time.perf_counter()
+    pack = gcx.Graph.pack(graphs, stream)
+    pack_s = time.perf_counter() - t0
+    tp = gcx.tune(pack, model, gcx.FP32).as_dict()
+    wsp = gcx.Workspace(pack.N, pack.M)
+    wsp.set_schedule(gcx.SCHED_AUTO)
+
+    def packed():
+        wsp.order(pack, gcx.GNN, model, gcx.FP32, stream)
+        wsp.color(pack, stream)
+        stream.sync()
+
+    packed()
+    t0 = time.perf_counter()
+    packed()
+    packed_s = time.perf_counter() - t0
+    colors, digests = wsp.packed_colors(pack, stream)
+    same = sum(1 for i, a in enumerate(alone) if a == (int(colors[i]), int(digests[i])))
+    M = sum(g.M for g in graphs)
+    out = {"graphs": k, "vertices": int(pack.N), "entries": int(M),
+           "one_at_a_time_s": single_s, "packed_s": packed_s, "pack_build_s": pack_s,
+           "packed_speedup": single_s / packed_s,
+           "packed_colored_edges_per_s": M / packed_s,
+           "one_at_a_time_colored_edges_per_s": M / single_s,
+           "same_colours_as_alone": f"{same}/{k}",
+           "tuned_signatures": len(sigs), "tune_s": tune_s, "pack_tuning": tp,
+           "schedules_chosen": {"rounds": sum(1 for t in sigs.values() if t["schedule"] == 0),
+                                "ordered": sum(1 for t in sigs.values() if t["schedule"] == 1)}}
+    pack.free()
+    for g in graphs:
+        g.free()
+    # Culberson chain on R-MAT 20
+    g = gcx.Graph.rmat(20, 16, seed=1)
+    w = gcx.Workspace(g.N, g.M)
+    w.order(g, gcx.GNN, model, gcx.FP32, stream)
+    w.color(g, stream)
+    c0 = w.colors(g.N)
+    k0 = w.validate(g, stream)
+    res = {}
+    for name, flags in (("captured", 0), ("one_call_per_pass", gcx.CULB_NO_GRAPH)):
+        w.upload_colors(c0)
+        w.recolor_ex(g, 2, gcx.CULB_MIXED | flags, seed=7, stream=stream)  # warm
+        w.upload_colors(c0)
+        stream.sync()
+        t0 = time.perf_counter()
+        w.recolor_ex(g, args.culberson_passes, gcx.CULB_MIXED | flags, seed=7, stream=stream)
+        stream.sync()
+        dt = time.perf_counter() - t0
+        res[name] = {"ms_per_pass": 1000.0 * dt / args.culberson_passes,
+                     "colors": w.validate(g, stream), "digest": w.digest(g, stream)}
+    out["culberson"] = {"graph": "R-MAT scale 20, GNN3 FP32 start", "passes": args.culberson_passes,
+                        "strategy": "mixed (reverse / largest first / random, seed 7 + pass)",
+                        "start_colors": k0, **res,
+                        "same_result": res["captured"]["digest"] == res["one_call_per_pass"]["digest"],
+                        "captured_speedup": res["one_call_per_pass"]["ms_per_pass"]
+                        / res["captured"]["ms_per_pass"]}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -197,6 +304,8 @@ def main():
     ap.add_argument("--no-bf16", action="store_true")
     ap.add_argument("--serving-requests", type=int, default=1000)
     ap.add_argument("--rmat-jobs", type=int, default=32)
+    ap.add_argument("--batch-graphs", type=int, default=512)
+    ap.add_argument("--culberson-passes", type=int, default=64)
     ap.add_argument("--ordered-slots", type=int, nargs="+", default=[0, 1, 2, 4])
     ap.add_argument("--serving-nmax", type=float, default=4e6)
     ap.add_argument("--serving-workers", type=int, default=8)
@@ -307,12 +416,12 @@ def main():
                          "frac": per / (ms_l * 1e-3) / 1e9 / HBM_PEAK_GBS,
                          "traffic_bytes": t, "traffic_gbs": (t / (ms_l * 1e-3) / 1e9) if t else None}
 
-    # BF16 leg: same graph and schedule, GNN rows in bf16 and layer products
-    # on MFMA; colours must stay within +1 of FP32
-    bf16 = None
-    if not args.no_bf16:
+    # BF16 legs: same graph and schedule, GNN layer products on MFMA. BF16:
+    # bf16 rows and operands; BF16X3: FP32 rows, operands split into bf16
+    # hi + lo (three MFMAs per product). Colours are compared with FP32.
+    def mfma_leg(precision):
         def step16():
-            ws.order(g, gcx.GNN, model, gcx.BF16, stream)
+            ws.order(g, gcx.GNN, model, precision, stream)
             ws.color(g, stream)
         for _ in range(max(args.warmup, 1)):
             step16()
@@ -328,13 +437,20 @@ def main():
         gcx.profile_enable(False)
         c16 = ws.validate(g, stream)
         v16 = ws.mex_check(g, stream)
-        bf16 = {"value": M / (wall16 / args.steps), "ms_per_step": 1000.0 * wall16 / args.steps,
-                "colors": c16, "mex_violations": v16, "colors_within_1_of_fp32": 0 < c16 <= colors + 1,
+        if c16 == 0 or v16 != 0:
+            print(f"[bench] MFMA LEG VALIDATION FAILED precision={precision} colors={c16} "
+                  f"mex_violations={v16}", file=sys.stderr)
+            sys.exit(1)
+        return {"value": M / (wall16 / args.steps), "ms_per_step": 1000.0 * wall16 / args.steps,
+                "colors": c16, "mex_violations": v16, "colors_minus_fp32": c16 - colors,
+                "colors_within_1_of_fp32": 0 < c16 <= colors + 1,
                 "speedup_vs_fp32": ms_per_step / (1000.0 * wall16 / args.steps),
                 "stage_ms_per_step": {k: v[0] / args.steps for k, v in prof16.items()}}
-        if c16 == 0 or v16 != 0:
-            print(f"[bench] BF16 VALIDATION FAILED colors={c16} mex_violations={v16}", file=sys.stderr)
-            sys.exit(1)
+
+    bf16 = bf16x3 = None
+    if not args.no_bf16:
+        bf16 = mfma_leg(gcx.BF16)
+        bf16x3 = mfma_leg(gcx.BF16X3)
 
     out = {
         "metric": METRIC,
@@ -367,9 +483,13 @@ def main():
         "stage_ms_per_step": {k: v / args.steps for k, v in stage_ms.items()},
         "stage_rooflines": stage_roof,
         "bf16": bf16,
+        "bf16x3": bf16x3,
         "event_ms_per_step": ev_ms / args.steps,
         "bit_exact_vs_reference": golden,
     }
+    if args.batch_graphs > 0:
+        print(f"[bench] batching {args.batch_graphs} graphs ...", file=sys.stderr, flush=True)
+        out["batching"] = batching(args, gcx)
     if args.rmat_jobs > 0:
         print(f"[bench] serving {args.rmat_jobs} GNN3 jobs on resident R-MAT graphs ...",
               file=sys.stderr, flush=True)

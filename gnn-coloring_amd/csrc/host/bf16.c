@@ -21,14 +21,25 @@ static uint16_t to_bf16(float f)
     return (uint16_t)(b >> 16);
 }
 
-/* Per layer: Wb = W[k][j] in bf16 (512 values, k-major), and the two MFMA
- * B operands in lane order: pack[m*256 + 4*lane + e] = W[16m + 4*(lane/16) + e][lane%16]
- * (v_mfma_f32_16x16x16_bf16: B[k][j] lives in lane j + 16*(k/4), element k%4). */
+static float from_bf16(uint16_t h)
+{
+    const uint32_t b = (uint32_t)h << 16;
+    float f;
+    memcpy(&f, &b, 4);
+    return f;
+}
+
+/* Per layer (1024 values each): Wb = W[k][j] in bf16, k-major, hi parts
+ * (512) then lo parts (512, bf16(W - hi)); and the two MFMA B operands in
+ * lane order, hi then lo: pack[p*512 + m*256 + 4*lane + e] = part p of
+ * W[16m + 4*(lane/16) + e][lane%16] (v_mfma_f32_16x16x16_bf16: B[k][j] lives
+ * in lane j + 16*(k/4), element k%4). */
 int gcx__model_pack_bf16(gcx_model *m)
 {
     const int L = m->layers;
-    uint16_t *wb = malloc(sizeof(uint16_t) * 512 * (size_t)L);
-    uint16_t *pk = malloc(sizeof(uint16_t) * 512 * (size_t)L);
+    const size_t per = 1024;
+    uint16_t *wb = malloc(sizeof(uint16_t) * per * (size_t)L);
+    uint16_t *pk = malloc(sizeof(uint16_t) * per * (size_t)L);
     if (!wb || !pk) {
         free(wb);
         free(pk);
@@ -36,38 +47,42 @@ int gcx__model_pack_bf16(gcx_model *m)
     }
     for (int l = 0; l < L; l++) {
         const float *W = m->hparam + (size_t)l * LAYER_PARAMS;
-        for (int k = 0; k < 512; k++)
-            wb[512 * l + k] = to_bf16(W[k]);
-        for (int mm = 0; mm < 2; mm++)
-            for (int lane = 0; lane < 64; lane++)
-                for (int e = 0; e < 4; e++) {
-                    const int k = 16 * mm + 4 * (lane / 16) + e, j = lane % 16;
-                    pk[512 * l + 256 * mm + 4 * lane + e] = wb[512 * l + 16 * k + j];
-                }
+        uint16_t *hi = wb + per * l, *lo = hi + 512;
+        for (int k = 0; k < 512; k++) {
+            hi[k] = to_bf16(W[k]);
+            lo[k] = to_bf16(W[k] - from_bf16(hi[k]));
+        }
+        for (int p = 0; p < 2; p++)
+            for (int mm = 0; mm < 2; mm++)
+                for (int lane = 0; lane < 64; lane++)
+                    for (int e = 0; e < 4; e++) {
+                        const int k = 16 * mm + 4 * (lane / 16) + e, j = lane % 16;
+                        pk[per * l + 512 * p + 256 * mm + 4 * lane + e] = hi[512 * p + 16 * k + j];
+                    }
     }
     int rc = GCX_OK;
-    if (hipMalloc((void **)&m->dWb, sizeof(uint16_t) * 512 * (size_t)L) != hipSuccess ||
-        hipMalloc((void **)&m->dWpack, sizeof(uint16_t) * 512 * (size_t)L) != hipSuccess ||
-        hipMemcpy(m->dWb, wb, sizeof(uint16_t) * 512 * (size_t)L, hipMemcpyHostToDevice) !=
-            hipSuccess ||
-        hipMemcpy(m->dWpack, pk, sizeof(uint16_t) * 512 * (size_t)L, hipMemcpyHostToDevice) !=
-            hipSuccess)
+    const size_t bytes = sizeof(uint16_t) * per * (size_t)L;
+    if (hipMalloc((void **)&m->dWb, bytes) != hipSuccess ||
+        hipMalloc((void **)&m->dWpack, bytes) != hipSuccess ||
+        hipMemcpy(m->dWb, wb, bytes, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(m->dWpack, pk, bytes, hipMemcpyHostToDevice) != hipSuccess)
         rc = GCX_ERR_HIP;
     free(wb);
     free(pk);
     return rc;
 }
 
-int gcx__order_gnn_bf16(gcx_workspace *ws, const gcx_graph *g, const gcx_model *m, void *stream)
+int gcx__order_gnn_bf16(gcx_workspace *ws, const gcx_graph *g, const gcx_model *m, int split,
+                        void *stream)
 {
     hipStream_t s = (hipStream_t)stream;
     const int L = m->layers;
-    uint16_t *bufs[2] = {(uint16_t *)ws->x, (uint16_t *)ws->y};
-    const uint16_t *in = NULL;
+    void *bufs[2] = {ws->x, ws->y};
+    const void *in = NULL;
     for (int l = 0; l < L; l++) {
         const int kind = l == 0 ? 0 : (l == L - 1 ? 2 : 1);
         const float *b = m->dparam + (size_t)l * LAYER_PARAMS + 512;
-        uint16_t *out = bufs[l & 1];
+        void *out = bufs[l & 1];
         void *tok;
         int rc = gcx__prof_begin(kind == 0 ? GCX_ST_GNN_IN : kind == 1 ? GCX_ST_GNN_HID : GCX_ST_GNN_OUT,
                                  s, &tok);
@@ -80,8 +95,8 @@ int gcx__order_gnn_bf16(gcx_workspace *ws, const gcx_graph *g, const gcx_model *
                 return GCX_ERR_HIP;
             hs = (hipStream_t)ws->aux;
         }
-        rc = gcxk_gnn_bf16_layer(kind, g->N, g->V, g->E, m->dWpack + 512 * (size_t)l,
-                                 m->dWb + 512 * (size_t)l, b, m->md, in, out, (uint32_t *)ws->key,
+        rc = gcxk_gnn_bf16_layer(kind, split, g->N, g->V, g->E, m->dWpack + 1024 * (size_t)l,
+                                 m->dWb + 1024 * (size_t)l, b, m->md, in, out, (uint32_t *)ws->key,
                                  g->hubs, g->nhub, g->perm, g->fidv, s, hs);
         if (rc != GCX_OK)
             return rc;

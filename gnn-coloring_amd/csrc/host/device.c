@@ -13,19 +13,6 @@
 #include "gen.h"
 #include "internal.h"
 
-#define HIPC(expr)                                                              \
-    do {                                                                        \
-        if ((expr) != hipSuccess)                                               \
-            return GCX_ERR_HIP;                                                 \
-    } while (0)
-
-#define TRY(expr)                                                               \
-    do {                                                                        \
-        int rc_ = (expr);                                                       \
-        if (rc_ != GCX_OK)                                                      \
-            return rc_;                                                         \
-    } while (0)
-
 /* Rounds enqueued between two looks at the device's "done" flag. */
 #define ROUND_BATCH 32u
 
@@ -315,15 +302,24 @@ uint64_t gcx_graph_footprint(uint32_t N, uint64_t M)
     return 8ull * ((uint64_t)N + 1) + 4ull * M + 8ull * N;
 }
 
-/* Degree statistics: max degree, hub list (ascending ids) and the hubs'
- * 4096-edge chunk table used by the JP orientation pass. */
 /* Degree statistics, hub list (ascending ids) with its 4096-edge chunk table
- * for the JP orientation pass, and the degree order. */
+ * for the JP orientation pass, and the degree order. The hub degree is the
+ * caller's, else the tuner's for the graph's degree signature, else the
+ * default. */
 static int graph_stats(gcx_graph *g, hipStream_t s)
 {
     if (!g->hchunk)
         TRY(gcx__dalloc((void **)&g->hchunk, 4ull * (g->N + 1ull), s));
-    TRY(gcxk_graph_stats(g->N, g->V, g->hubs, g->hchunk, &g->nhub, &g->maxdeg, &g->nhchunk, s));
+    if (!g->hub_deg)
+        g->hub_deg = GCXK_HUB_DEG;
+    TRY(gcxk_graph_stats(g->N, g->V, g->hub_deg, g->hubs, g->hchunk, &g->nhub, &g->maxdeg,
+                         &g->nhchunk, s));
+    const uint32_t tuned = g->hub_fixed ? 0u : gcx__tuned_hub_degree(g->N, g->M, g->maxdeg);
+    if (tuned && tuned != g->hub_deg) {
+        g->hub_deg = tuned;
+        TRY(gcxk_graph_stats(g->N, g->V, g->hub_deg, g->hubs, g->hchunk, &g->nhub, &g->maxdeg,
+                             &g->nhchunk, s));
+    }
     if (!g->perm)
         TRY(gcx__dalloc((void **)&g->perm, 4ull * (g->N ? g->N : 1), s));
     TRY(gcxk_degree_order(g->N, g->V, g->maxdeg, g->perm, s));
@@ -594,6 +590,25 @@ int gcx_graph_pack(const gcx_graph *const *graphs, uint32_t count, void *stream,
 }
 
 uint32_t gcx_graph_packed_count(const gcx_graph *g) { return g ? g->count : 0; }
+
+int gcx_graph_set_hub_degree(gcx_graph *g, uint32_t hub_degree, void *stream)
+{
+    TRY(gcx__ensure_init());
+    if (!g)
+        return GCX_ERR_ARG;
+    hipStream_t s = (hipStream_t)gcx__pick(stream);
+    g->hub_fixed = hub_degree != 0;
+    g->hub_deg = hub_degree ? hub_degree
+                            : (gcx__tuned_hub_degree(g->N, g->M, g->maxdeg)
+                                   ? gcx__tuned_hub_degree(g->N, g->M, g->maxdeg)
+                                   : GCXK_HUB_DEG);
+    /* the degree order does not depend on the cut: only the hub tables */
+    TRY(gcxk_graph_stats(g->N, g->V, g->hub_deg, g->hubs, g->hchunk, &g->nhub, &g->maxdeg,
+                         &g->nhchunk, s));
+    return GCX_OK;
+}
+
+uint32_t gcx_graph_hub_degree(const gcx_graph *g) { return g ? g->hub_deg : 0; }
 
 int gcx_packed_colors(gcx_workspace *ws, const gcx_graph *g, int *colors, uint64_t *digests,
                       void *stream)
@@ -1076,8 +1091,8 @@ int gcx_order(gcx_workspace *ws, const gcx_graph *g, int heuristic, const gcx_mo
             return GCX_ERR_ARG;
         if (precision == GCX_FP32)
             rc = order_gnn_fp32(ws, g, model, s);
-        else if (precision == GCX_BF16)
-            rc = gcx__order_gnn_bf16(ws, g, model, s);
+        else if (precision == GCX_BF16 || precision == GCX_BF16X3)
+            rc = gcx__order_gnn_bf16(ws, g, model, precision == GCX_BF16X3, s);
         else
             return GCX_ERR_ARG;
         ws->key_kind = 1;
@@ -1237,6 +1252,7 @@ static int color_ordered(gcx_workspace *ws, const gcx_graph *g, int mb, hipStrea
     HIPC(hipMemcpyAsync(ws->hctl + 2, ws->ctl, sizeof(gcxk_ctl), hipMemcpyDeviceToHost, s));
     HIPC(hipStreamSynchronize(s));
     ws->last_launches = 1;
+    ws->last_ordered = 1;
     return ws->hctl[2].err ? GCX_ERR_INVALID : GCX_OK;
 }
 
@@ -1248,21 +1264,24 @@ int gcx_color(gcx_workspace *ws, const gcx_graph *g, void *stream)
     hipStream_t s = (hipStream_t)gcx__pick(stream);
     const int mb = gcxk_jp_mask_bits(g->maxdeg);
     ws->last_fallback = 0;
-    if (ws->schedule == GCX_SCHED_ORDERED && g->N) {
+    const int sched = ws->schedule == GCX_SCHED_AUTO ? gcx__tuned_schedule(g) : ws->schedule;
+    ws->last_ordered = 0;
+    if (sched == GCX_SCHED_ORDERED && g->N) {
         const int rc = color_ordered(ws, g, mb, s);
         if (rc != GCX_ERR_INVALID)
             return rc;
         ws->last_fallback = 1; /* a wait hit its bound: redo with rounds */
+        ws->last_ordered = 0;
     }
     return color_rounds(ws, g, mb, s);
 }
 
 int gcx_set_schedule(gcx_workspace *ws, int schedule, uint32_t param)
 {
-    if (!ws || schedule < GCX_SCHED_ROUNDS || schedule > GCX_SCHED_ORDERED)
+    if (!ws || schedule < GCX_SCHED_ROUNDS || schedule > GCX_SCHED_AUTO)
         return GCX_ERR_ARG;
     ws->schedule = schedule;
-    ws->ordered_grid = schedule == GCX_SCHED_ORDERED ? param : 0;
+    ws->ordered_grid = schedule != GCX_SCHED_ROUNDS ? param : 0;
     return GCX_OK;
 }
 
@@ -1280,7 +1299,7 @@ int gcx_last_rounds(const gcx_workspace *ws, uint32_t *rounds)
         return GCX_ERR_ARG;
     gcxk_ctl c;
     HIPC(hipMemcpy(&c, ws->ctl, sizeof(c), hipMemcpyDeviceToHost));
-    *rounds = ws->schedule == GCX_SCHED_ORDERED && !ws->last_fallback ? 0 : c.rounds;
+    *rounds = ws->last_ordered && !ws->last_fallback ? 0 : c.rounds;
     return GCX_OK;
 }
 
@@ -1293,12 +1312,14 @@ int gcx_last_launches(const gcx_workspace *ws, uint32_t *launches)
 }
 
 /* One Culberson pass on the round schedule, enqueued on s: keys from the
- * colours, orientation, roots and `rounds` round launches. No host reads
- * and no memsets, so it can be captured. */
-static int recolor_pass(gcx_workspace *ws, const gcx_graph *g, int mb, uint32_t rounds,
-                        hipStream_t s)
+ * colour classes (in the strategy's class order), orientation, roots and
+ * `rounds` round launches. No host reads and no memsets, so it can be
+ * captured. C bounds the colours of the input colouring. */
+static int recolor_pass(gcx_workspace *ws, const gcx_graph *g, int mb, uint32_t C, int strategy,
+                        uint64_t seed, uint32_t rounds, hipStream_t s)
 {
-    TRY(gcxk_keys_from_colors(g->N, ws->color, (uint32_t *)ws->key, s));
+    TRY(gcxk_culberson_keys(g->N, C, strategy, seed, ws->color, (uint32_t *)ws->y,
+                            (uint32_t *)ws->key, s));
     TRY(jp_prepare(ws, g, mb, s));
     TRY(gcxk_jp_roots(g->N, g->V, ws->npred, ws->items[0], ws->ctl, s));
     jp_ctx c = {ws, g, g_round_grid, mb};
@@ -1307,10 +1328,16 @@ static int recolor_pass(gcx_workspace *ws, const gcx_graph *g, int mb, uint32_t 
     return GCX_OK;
 }
 
-static int recolor_eager(gcx_workspace *ws, const gcx_graph *g, int iterations, hipStream_t s)
+static int recolor_eager(gcx_workspace *ws, const gcx_graph *g, int iterations, int strategy,
+                         uint64_t seed, hipStream_t s)
 {
     for (int it = 0; it < iterations; it++) {
-        TRY(gcxk_keys_from_colors(g->N, ws->color, (uint32_t *)ws->key, s));
+        int C = 0;
+        TRY(gcx_validate(ws, g, s, &C));
+        if (C <= 0)
+            return GCX_ERR_INVALID;
+        TRY(gcxk_culberson_keys(g->N, (uint32_t)C, strategy, seed, ws->color, (uint32_t *)ws->y,
+                                (uint32_t *)ws->key, s));
         ws->key_bits = 32;
         ws->key_kind = 0;
         TRY(gcx_color(ws, g, s));
@@ -1322,29 +1349,37 @@ static int recolor_eager(gcx_workspace *ws, const gcx_graph *g, int iterations, 
  * (which polls for completion) is used. */
 #define RECOLOR_GRAPH_MAX_ROUNDS 4096u
 
-int gcx_recolor(gcx_workspace *ws, const gcx_graph *g, int iterations, void *stream)
+int gcx_recolor_ex(gcx_workspace *ws, const gcx_graph *g, int iterations, int strategy,
+                   uint64_t seed, void *stream)
 {
-    /* main.c:34-43: p[j] = color[j], then JP again; repeated.
+    /* main.c:34-43: p[j] = color[j], then JP again; repeated (strategy
+     * GCX_CULB_REVERSE). Other strategies reorder the colour classes.
      *
-     * With p = the previous colours, a vertex's predecessors all have larger
-     * colours, so its depth in the JP DAG is at most the number of colour
-     * classes C of the input colouring, and a pass never uses more classes
-     * than it was given (each vertex's class is a valid choice again). So
+     * A vertex's predecessors all lie in classes earlier in the order, so
+     * its depth in the JP DAG is at most the number of colour classes C of
+     * the input colouring, and a pass never uses more classes than it was
+     * given (each vertex's class position is a valid colour again). So
      * C + 1 round launches finish every pass: the passes are captured once
      * as a HIP graph and replayed, with no host round trip between rounds
      * or passes. */
     TRY(gcx__ensure_init());
-    if (!ws_fits(ws, g) || iterations < 0 || ws->curN != g->N)
+    const int eager = (strategy & GCX_CULB_NO_GRAPH) != 0;
+    strategy &= ~GCX_CULB_NO_GRAPH;
+    if (!ws_fits(ws, g) || iterations < 0 || ws->curN != g->N || strategy < GCX_CULB_REVERSE ||
+        strategy > GCX_CULB_MIXED)
         return GCX_ERR_ARG;
     if (iterations == 0 || g->N == 0)
         return GCX_OK;
     hipStream_t s = (hipStream_t)gcx__pick(stream);
+    /* the pass counter of the class orders starts at 0 for this call */
+    HIPC(hipMemsetAsync(ws->y, 0, sizeof(uint32_t), s));
     int C = 0;
     TRY(gcx_validate(ws, g, s, &C));
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     HIPC(hipStreamIsCapturing(s, &cap));
-    if (C <= 0 || (uint32_t)C >= RECOLOR_GRAPH_MAX_ROUNDS || cap != hipStreamCaptureStatusNone)
-        return recolor_eager(ws, g, iterations, s);
+    if (eager || C <= 0 || (uint32_t)C >= RECOLOR_GRAPH_MAX_ROUNDS ||
+        cap != hipStreamCaptureStatusNone)
+        return recolor_eager(ws, g, iterations, strategy, seed, s);
     const int mb = gcxk_jp_mask_bits(g->maxdeg);
     ws->key_bits = 32;
     ws->key_kind = 0;
@@ -1353,7 +1388,7 @@ int gcx_recolor(gcx_workspace *ws, const gcx_graph *g, int iterations, void *str
     hipGraph_t graph = NULL;
     hipGraphExec_t exec = NULL;
     HIPC(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    int rc = recolor_pass(ws, g, mb, (uint32_t)C + 1, s);
+    int rc = recolor_pass(ws, g, mb, (uint32_t)C, strategy, seed, (uint32_t)C + 1, s);
     const hipError_t ec = hipStreamEndCapture(s, &graph);
     if (rc == GCX_OK && ec != hipSuccess)
         rc = GCX_ERR_HIP;
@@ -1380,6 +1415,11 @@ int gcx_recolor(gcx_workspace *ws, const gcx_graph *g, int iterations, void *str
     ws->last_launches = (uint32_t)C + 1;
     ws->last_fallback = 0;
     return GCX_OK;
+}
+
+int gcx_recolor(gcx_workspace *ws, const gcx_graph *g, int iterations, void *stream)
+{
+    return gcx_recolor_ex(ws, g, iterations, GCX_CULB_REVERSE, 0, stream);
 }
 
 int gcx_validate(gcx_workspace *ws, const gcx_graph *g, void *stream, int *max_color)
@@ -1420,7 +1460,8 @@ int gcx_mex_check(gcx_workspace *ws, const gcx_graph *g, void *stream, uint64_t 
     hipStream_t s = (hipStream_t)gcx__pick(stream);
     gcxk_ctl *vc = ws->ctl + 1;
     HIPC(hipMemsetAsync(vc, 0, sizeof(gcxk_ctl), s));
-    TRY(gcxk_mex_check(g->N, g->V, g->E, ws->key, ws->key_bits, ws->color, g->hubs, g->nhub, vc,
+    TRY(gcxk_mex_check(g->N, g->V, g->E, ws->key, ws->key_bits, ws->color, g->hubs, g->nhub,
+                       g->hub_deg, vc,
                        s));
     HIPC(hipMemcpyAsync(ws->hctl + 1, vc, sizeof(gcxk_ctl), hipMemcpyDeviceToHost, s));
     HIPC(hipStreamSynchronize(s));

@@ -13,7 +13,9 @@ struct gcx_graph {
     uint64_t M;
     uint32_t *V;      /* device, N+1 */
     uint32_t *E;      /* device, M */
-    uint32_t *hubs;   /* device, nhub entries (degree >= GCXK_HUB_DEG) */
+    uint32_t *hubs;   /* device, nhub entries (degree >= hub_deg) */
+    uint32_t hub_deg; /* block-per-vertex cut (0 until the stats run) */
+    int hub_fixed;    /* hub_deg set by the caller: the tuner leaves it */
     uint32_t nhub;
     uint32_t maxdeg;
     uint32_t *hchunk; /* device, nhub+1: prefix of 4096-edge chunks per hub */
@@ -28,8 +30,8 @@ struct gcx_model {
     int params, md, layers;
     float *dparam; /* device copy of the weights */
     float *hparam; /* host copy */
-    uint16_t *dWb;    /* bf16 weights, 512 per layer, W[k][j] k-major */
-    uint16_t *dWpack; /* bf16 MFMA B operands, 512 per layer, lane order */
+    uint16_t *dWb;    /* bf16 weights, 1024 per layer: W[k][j] k-major, hi then lo */
+    uint16_t *dWpack; /* bf16 MFMA B operands, 1024 per layer, lane order, hi then lo */
 };
 
 struct gcx_workspace {
@@ -61,10 +63,32 @@ struct gcx_workspace {
     int schedule;           /* GCX_SCHED_ROUNDS or GCX_SCHED_ORDERED */
     uint32_t ordered_grid;  /* workgroups of the ordered launch (0 = all resident) */
     uint32_t last_fallback; /* 1 when the last ordered launch hit its wait bound */
+    int last_ordered;       /* the last gcx_color ran the ordered schedule */
     void *order_buf;        /* sort scratch when it does not fit in x */
     size_t order_bytes;     /* bytes of sort scratch available */
     uint32_t curN;    /* vertex count of the graph the contents belong to */
 };
+
+/* Return GCX_ERR_HIP when a HIP call fails / the status of a failing call. */
+#define HIPC(expr)                                                              \
+    do {                                                                        \
+        if ((expr) != hipSuccess)                                               \
+            return GCX_ERR_HIP;                                                 \
+    } while (0)
+
+#define TRY(expr)                                                               \
+    do {                                                                        \
+        int rc_ = (expr);                                                       \
+        if (rc_ != GCX_OK)                                                      \
+            return rc_;                                                         \
+    } while (0)
+
+/* tune.c */
+uint32_t gcx__signature(uint32_t N, uint64_t M, uint32_t maxdeg);
+/* Recorded hub degree for this degree signature, 0 if none. */
+uint32_t gcx__tuned_hub_degree(uint32_t N, uint64_t M, uint32_t maxdeg);
+/* Recorded schedule for the graph's signature, else the size rule. */
+int gcx__tuned_schedule(const gcx_graph *g);
 
 /* device.c */
 int gcx__graph_alloc(uint32_t N, uint64_t M, void *stream, gcx_graph **out);
@@ -77,7 +101,9 @@ int gcx__graph_fill(gcx_graph *g, uint32_t N, const uint32_t *V, const uint32_t 
 int gcx__order_ff_only(gcx_workspace *ws, uint32_t N, void *stream);
 /* bf16.c */
 int gcx__model_pack_bf16(gcx_model *m);
-int gcx__order_gnn_bf16(gcx_workspace *ws, const gcx_graph *g, const gcx_model *m, void *stream);
+/* split = 0: GCX_BF16, 1: GCX_BF16X3. */
+int gcx__order_gnn_bf16(gcx_workspace *ws, const gcx_graph *g, const gcx_model *m, int split,
+                        void *stream);
 int gcx__ensure_init(void);
 void *gcx__default_stream(void);
 void *gcx__pick(void *stream);

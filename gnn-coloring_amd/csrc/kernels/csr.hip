@@ -82,7 +82,7 @@ __global__ __launch_bounds__(kBlock) void scatter_rows(uint64_t n, const uint64_
 // Hub flags and hub chunk counts per vertex (entry N = 0 closes the scans),
 // and the maximum degree (one atomic per workgroup).
 __global__ __launch_bounds__(kBlock) void hub_flags(uint32_t N, const uint32_t *__restrict__ V,
-                                                    uint32_t *__restrict__ flag,
+                                                    uint32_t hub_deg, uint32_t *__restrict__ flag,
                                                     uint32_t *__restrict__ chunks,
                                                     uint32_t *__restrict__ maxdeg)
 {
@@ -91,7 +91,7 @@ __global__ __launch_bounds__(kBlock) void hub_flags(uint32_t N, const uint32_t *
     uint32_t d = 0;
     if (u < N) {
         d = V[u + 1] - V[u];
-        const bool hub = d >= GCXK_HUB_DEG;
+        const bool hub = d >= hub_deg;
         flag[u] = hub ? 1u : 0u;
         chunks[u] = hub ? (d + GCXK_HUB_CHUNK_EDGES - 1u) / GCXK_HUB_CHUNK_EDGES : 0u;
     } else if (u == N) {
@@ -484,8 +484,9 @@ done:
     return rc;
 }
 
-extern "C" int gcxk_graph_stats(uint32_t N, const uint32_t *dV, uint32_t *dhub, uint32_t *dchunk,
-                                uint32_t *nhub, uint32_t *maxdeg, uint32_t *nchunk, void *stream)
+extern "C" int gcxk_graph_stats(uint32_t N, const uint32_t *dV, uint32_t hub_deg, uint32_t *dhub,
+                                uint32_t *dchunk, uint32_t *nhub, uint32_t *maxdeg, uint32_t *nchunk,
+                                void *stream)
 {
     hipStream_t s = as_stream(stream);
     *nhub = *maxdeg = *nchunk = 0;
@@ -499,7 +500,7 @@ extern "C" int gcxk_graph_stats(uint32_t N, const uint32_t *dV, uint32_t *dhub, 
     GCXK_CHECK(hipMemsetAsync(mx.p, 0, 4, s));
     uint32_t *f = (uint32_t *)flag.p, *c = (uint32_t *)chunks.p, *ps = (uint32_t *)pos.p,
              *cp = (uint32_t *)cpre.p;
-    hub_flags<<<div_up(n1, kBlock), kBlock, 0, s>>>(N, dV, f, c, (uint32_t *)mx.p);
+    hub_flags<<<div_up(n1, kBlock), kBlock, 0, s>>>(N, dV, hub_deg, f, c, (uint32_t *)mx.p);
     GCXK_LAUNCH_CHECK();
     size_t tb = 0;
     GCXK_CHECK(rocprim::exclusive_scan(nullptr, tb, f, ps, 0u, n1, rocprim::plus<uint32_t>(), s));

@@ -11,7 +11,9 @@
 extern "C" {
 #endif
 
-/* Vertices with degree >= GCXK_HUB_DEG are processed one per workgroup. */
+/* Default hub degree: vertices with degree >= a graph's hub degree are
+ * processed one per workgroup (block-per-vertex), the others four lanes per
+ * vertex (16 vertices per wave). Each graph carries its own value. */
 #define GCXK_HUB_DEG 1024u
 
 /* Control block shared by the JP / SL / validate kernels (device memory). */
@@ -40,12 +42,13 @@ typedef struct {
  * caller frees) and fills dV (N+1 entries). Temporaries are freed inside. */
 int gcxk_csr_from_pairs(uint32_t N, uint64_t L, const uint32_t *dI, const uint32_t *dJ,
                         uint32_t *dV, uint32_t **dE, uint64_t *M, void *stream);
-/* Hub list: vertices with degree >= GCXK_HUB_DEG in ascending id order into
+/* Hub list: vertices with degree >= hub_deg in ascending id order into
  * dhub (capacity N), their 4096-edge chunk prefix into dchunk (capacity N+1,
  * dchunk[nhub] = total chunks); counts and the maximum degree to the host.
  * One host synchronisation. */
-int gcxk_graph_stats(uint32_t N, const uint32_t *dV, uint32_t *dhub, uint32_t *dchunk,
-                     uint32_t *nhub, uint32_t *maxdeg, uint32_t *nchunk, void *stream);
+int gcxk_graph_stats(uint32_t N, const uint32_t *dV, uint32_t hub_deg, uint32_t *dhub,
+                     uint32_t *dchunk, uint32_t *nhub, uint32_t *maxdeg, uint32_t *nchunk,
+                     void *stream);
 /* Degree order: perm = vertex ids by degree descending, id ascending (so the
  * nhub hubs come first). Stream-ordered temporaries; no synchronisation. */
 int gcxk_degree_order(uint32_t N, const uint32_t *dV, uint32_t maxdeg, uint32_t *perm,
@@ -90,6 +93,13 @@ int gcxk_keys_ff(uint32_t N, uint32_t *key, void *stream);
 int gcxk_keys_lf(uint32_t N, const uint32_t *dV, uint32_t *key, void *stream);
 int gcxk_keys_llf(uint32_t N, const uint32_t *dV, uint32_t *key, void *stream);
 int gcxk_keys_from_colors(uint32_t N, const int *color, uint32_t *key, void *stream);
+/* Culberson pass keys for strategy 0 reverse (key = colour), 1 largest class
+ * first, 2 seeded random class order, 3 mixed (pass i: i mod 3). C = number
+ * of colour classes (colours 1..C); scratch holds 2C + 3 words: the pass
+ * counter (word 0, advanced once per call on the device), class sizes and
+ * class keys. */
+int gcxk_culberson_keys(uint32_t N, uint32_t C, int strategy, uint64_t seed, const int *color,
+                        uint32_t *scratch, uint32_t *key, void *stream);
 /* Host doubles (already on device) -> key32 when every value is exactly a
  * float, else key64; counts NaNs and non-floats in ctl. */
 int gcxk_keys_scan_doubles(uint32_t N, const double *p, gcxk_ctl *ctl, void *stream);
@@ -126,14 +136,18 @@ int gcxk_gnn_fp32_layer(int kind, uint32_t N, const uint32_t *dV, const uint32_t
                         const float *W, const float *b, int md, const float *x, float *out,
                         uint32_t *key, const uint32_t *hubs, uint32_t nhub, const uint32_t *perm,
                         const float *fidv, void *stream, void *hub_stream);
-/* BF16 rows (16 x bf16 = 32 B), FP32 aggregation, MFMA layer transform.
- * Wpack: the layer's two MFMA B operands in lane order (512 bf16); Wb: the
- * layer's weights W[k][j] in bf16 (512); b: FP32 biases (16). */
-int gcxk_gnn_bf16_layer(int kind, uint32_t N, const uint32_t *dV, const uint32_t *dE,
-                        const uint16_t *Wpack, const uint16_t *Wb, const float *b, int md,
-                        const uint16_t *x, uint16_t *out, uint32_t *key, const uint32_t *hubs,
-                        uint32_t nhub, const uint32_t *perm, const float *fidv, void *stream,
-                        void *hub_stream);
+/* MFMA layer transform with BF16 operands and FP32 accumulation.
+ * split_mode 0: bf16 rows (16 x bf16 = 32 B), operands rounded to bf16.
+ * split_mode 1: FP32 rows, operands and weights split into bf16 hi + lo,
+ * three MFMAs per product (near-FP32 results).
+ * Wpack: the layer's two MFMA B operands in lane order, hi parts (512 bf16)
+ * then lo parts (512); Wb: the layer's weights W[k][j] in bf16, hi (512) then
+ * lo (512); b: FP32 biases (16). */
+int gcxk_gnn_bf16_layer(int kind, int split_mode, uint32_t N, const uint32_t *dV,
+                        const uint32_t *dE, const uint16_t *Wpack, const uint16_t *Wb,
+                        const float *b, int md, const void *x, void *out, uint32_t *key,
+                        const uint32_t *hubs, uint32_t nhub, const uint32_t *perm,
+                        const float *fidv, void *stream, void *hub_stream);
 
 /* ---- Jones-Plassmann (jp.hip) ---- */
 /* Work item: (vertex, chunk of 64 successor edges). Stored as two uint32. */
@@ -226,7 +240,7 @@ int gcxk_validate(uint32_t N, const uint32_t *dV, const uint32_t *dE, const int 
 /* Greedy-rule certificate: ctl->viol. */
 int gcxk_mex_check(uint32_t N, const uint32_t *dV, const uint32_t *dE, const void *key,
                    int key_bits, const int *color, const uint32_t *hubs, uint32_t nhub,
-                   gcxk_ctl *ctl, void *stream);
+                   uint32_t hub_deg, gcxk_ctl *ctl, void *stream);
 
 /* Order-independent digest of color[0..N): adds sum_u mix(u, color[u]) to
  * *out (device, zeroed by the caller). */

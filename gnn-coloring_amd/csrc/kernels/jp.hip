@@ -663,7 +663,7 @@ __global__ __launch_bounds__(kBlock) void mex_check_groups(uint32_t N, const uin
                                                            const uint32_t *__restrict__ E,
                                                            const K *__restrict__ key,
                                                            const int *__restrict__ color,
-                                                           gcxk_ctl *ctl)
+                                                           uint32_t hub_deg, gcxk_ctl *ctl)
 {
     const uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     const uint32_t u = (uint32_t)(t / kGroup);
@@ -671,7 +671,7 @@ __global__ __launch_bounds__(kBlock) void mex_check_groups(uint32_t N, const uin
     if (u >= N) // whole groups leave together
         return;
     const uint32_t beg = V[u], end = V[u + 1];
-    if (end - beg >= GCXK_HUB_DEG)
+    if (end - beg >= hub_deg)
         return;
     const K ku = key[u];
     uint32_t base = 0;
@@ -1034,7 +1034,7 @@ extern "C" int gcxk_validate(uint32_t N, const uint32_t *dV, const uint32_t *dE,
 
 extern "C" int gcxk_mex_check(uint32_t N, const uint32_t *dV, const uint32_t *dE, const void *key,
                               int key_bits, const int *color, const uint32_t *hubs, uint32_t nhub,
-                              gcxk_ctl *ctl, void *stream)
+                              uint32_t hub_deg, gcxk_ctl *ctl, void *stream)
 {
     if (N == 0)
         return GCX_OK;
@@ -1042,13 +1042,13 @@ extern "C" int gcxk_mex_check(uint32_t N, const uint32_t *dV, const uint32_t *dE
     const uint32_t grid = div_up((uint64_t)N * kGroup, kBlock);
     if (key_bits == 32) {
         mex_check_groups<uint32_t><<<grid, kBlock, 0, s>>>(N, dV, dE, (const uint32_t *)key, color,
-                                                           ctl);
+                                                           hub_deg, ctl);
         if (nhub)
             mex_check_hubs<uint32_t><<<nhub, kBlock, 0, s>>>(dV, dE, (const uint32_t *)key, color,
                                                              hubs, ctl);
     } else {
         mex_check_groups<uint64_t><<<grid, kBlock, 0, s>>>(N, dV, dE, (const uint64_t *)key, color,
-                                                           ctl);
+                                                           hub_deg, ctl);
         if (nhub)
             mex_check_hubs<uint64_t><<<nhub, kBlock, 0, s>>>(dV, dE, (const uint64_t *)key, color,
                                                              hubs, ctl);

@@ -22,8 +22,10 @@ OK, ERR_ARG, ERR_NOMEM, ERR_HIP, ERR_CAPACITY, ERR_IO, ERR_FORMAT, ERR_NODEV, ER
     0, -1, -2, -3, -4, -5, -6, -7, -8, -9)
 
 FF, LF, LLF, SL, SLL, GNN = range(6)
-FP32, BF16 = 0, 1
-SCHED_ROUNDS, SCHED_ORDERED = 0, 1
+FP32, BF16, BF16X3 = 0, 1, 2
+CULB_REVERSE, CULB_LARGEST_FIRST, CULB_RANDOM, CULB_MIXED = 0, 1, 2, 3
+CULB_NO_GRAPH = 0x100
+SCHED_ROUNDS, SCHED_ORDERED, SCHED_AUTO = 0, 1, 2
 HEURISTICS = {"FF": FF, "LF": LF, "LLF": LLF, "SL": SL, "SLL": SLL}
 STAGES = ["keys", "sl", "gnn_in", "gnn_hid", "gnn_out", "jp_init", "jp_rounds"]
 
@@ -85,6 +87,7 @@ def _load():
         "gcx_set_priorities": (C.c_int, [vp, vp, f64p, vp]),
         "gcx_color": (C.c_int, [vp, vp, vp]),
         "gcx_recolor": (C.c_int, [vp, vp, C.c_int, vp]),
+        "gcx_recolor_ex": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_uint64, vp]),
         "gcx_set_schedule": (C.c_int, [vp, C.c_int, C.c_uint32]),
         "gcx_validate": (C.c_int, [vp, vp, vp, i32p]),
         "gcx_mex_check": (C.c_int, [vp, vp, vp, C.POINTER(C.c_uint64)]),
@@ -96,6 +99,14 @@ def _load():
         "gcx_server_peak_reserved": (C.c_uint64, [vp]),
         "gcx_server_add_graph": (C.c_int, [vp, vp, u32p]),
         "gcx_set_ordered_launch": (C.c_int, [C.c_int]),
+        "gcx_degree_signature": (C.c_uint32, [vp]),
+        "gcx_tune": (C.c_int, [vp, vp, C.c_int, vp, vp]),
+        "gcx_tuning_lookup": (C.c_int, [vp, vp]),
+        "gcx_tuner_reset": (None, []),
+        "gcx_tuner_save": (C.c_int, [C.c_char_p]),
+        "gcx_tuner_load": (C.c_int, [C.c_char_p]),
+        "gcx_graph_set_hub_degree": (C.c_int, [vp, C.c_uint32, vp]),
+        "gcx_graph_hub_degree": (C.c_uint32, [vp]),
         "gcx_get_ordered_launch": (C.c_int, []),
         "gcx_download_colors": (C.c_int, [vp, C.c_uint32, i32p, vp]),
         "gcx_upload_colors": (C.c_int, [vp, C.c_uint32, i32p, vp]),
@@ -238,6 +249,40 @@ def _sh(stream):
     return stream.handle if stream is not None else None
 
 
+class Tuning(C.Structure):
+    """gcx_tuning (gcx.h)."""
+    _fields_ = [("signature", C.c_uint32), ("hub_degree", C.c_uint32), ("schedule", C.c_int),
+                ("order_ms", C.c_float), ("color_ms", C.c_float)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def tune(g, model=None, precision=FP32, stream=None) -> Tuning:
+    """Autotune hub degree and JP schedule for g's degree signature."""
+    t = Tuning()
+    _chk(_lib.gcx_tune(g.handle, model.handle if model else None, precision, _sh(stream),
+                       C.byref(t)), "tune")
+    return t
+
+
+def tuning_lookup(g):
+    t = Tuning()
+    return t if _lib.gcx_tuning_lookup(g.handle, C.byref(t)) else None
+
+
+def tuner_reset():
+    _lib.gcx_tuner_reset()
+
+
+def tuner_save(path):
+    _chk(_lib.gcx_tuner_save(str(path).encode()), "tuner save")
+
+
+def tuner_load(path):
+    _chk(_lib.gcx_tuner_load(str(path).encode()), "tuner load")
+
+
 def set_ordered_launch(slots: int):
     """0: cooperative ORDERED launches, one at a time; k >= 1: plain
     launches sharing the device, at most k at once (gcx_set_ordered_launch)."""
@@ -307,6 +352,18 @@ class Graph:
         h = vp()
         _chk(_lib.gcx_graph_pack(arr, len(graphs), _sh(stream), C.byref(h)), "graph pack")
         return cls(h)
+
+    @property
+    def hub_degree(self) -> int:
+        return int(_lib.gcx_graph_hub_degree(self.handle))
+
+    def set_hub_degree(self, d, stream=None):
+        """Block-per-vertex from degree d up (0 = tuned value or default)."""
+        _chk(_lib.gcx_graph_set_hub_degree(self.handle, d, _sh(stream)), "set hub degree")
+
+    @property
+    def signature(self) -> int:
+        return int(_lib.gcx_degree_signature(self.handle))
 
     @property
     def packed_count(self) -> int:
@@ -426,6 +483,12 @@ class Workspace:
         (greedy-order items, one persistent launch of `grid` workgroups, 0 =
         all that are resident at once)."""
         _chk(_lib.gcx_set_schedule(self.handle, schedule, grid), "schedule")
+
+    def recolor_ex(self, g: Graph, iterations, strategy, seed=0, stream=None):
+        """Iterated greedy with a class order (CULB_REVERSE / LARGEST_FIRST /
+        RANDOM / MIXED), each pass a captured HIP graph replay."""
+        _chk(_lib.gcx_recolor_ex(self.handle, g.handle, iterations, strategy, seed, _sh(stream)),
+             "recolor")
 
     def recolor(self, g: Graph, iterations, stream=None):
         _chk(_lib.gcx_recolor(self.handle, g.handle, iterations, _sh(stream)), "recolor")

@@ -43,8 +43,10 @@ enum gcx_heuristic {
 };
 
 enum gcx_precision {
-    GCX_FP32 = 0, /* bit-exact with the reference's FP32 GNN */
-    GCX_BF16 = 1  /* BF16 rows + MFMA layer transforms, FP32 accumulate */
+    GCX_FP32 = 0,  /* bit-exact with the reference's FP32 GNN */
+    GCX_BF16 = 1,  /* BF16 rows + MFMA layer transforms, FP32 accumulate */
+    GCX_BF16X3 = 2 /* FP32 rows; operands and weights split into bf16 hi + lo,
+                      three MFMAs per product: within a few FP32 ulps of FP32 */
 };
 
 typedef struct gcx_graph gcx_graph;
@@ -97,6 +99,36 @@ uint32_t gcx_graph_max_degree(const gcx_graph *g);
 int gcx_graph_download(const gcx_graph *g, uint32_t *V, uint32_t *E);
 /* Device CSR check (sorted, simple, symmetric, V[N] == M): 1 valid, 0 not. */
 int gcx_graph_check(const gcx_graph *g, void *stream, int *valid);
+/* ---- autotuner (no reference counterpart) ----
+ * Per degree distribution: the hub degree, i.e. the cut between four lanes
+ * per vertex (16 vertices per wave) and one workgroup per vertex in the GNN,
+ * JP orientation and validation kernels, and the JP schedule. Neither changes
+ * results. A graph's degree signature = log2 buckets of N, the average and
+ * the maximum degree. */
+typedef struct {
+    uint32_t signature;
+    uint32_t hub_degree; /* block-per-vertex from this degree up */
+    int schedule;        /* GCX_SCHED_ROUNDS / GCX_SCHED_ORDERED */
+    float order_ms;      /* measured ordering time at the chosen hub degree */
+    float color_ms;      /* measured colouring time under the chosen schedule */
+} gcx_tuning;
+uint32_t gcx_degree_signature(const gcx_graph *g);
+/* Measures hub degrees 256..16384 (order + colour) and both schedules on g
+ * (GNN with model m in `precision`, or LF keys when m is NULL), records the
+ * fastest under g's signature and applies the hub degree to g. Graphs
+ * created later with the same signature take the recorded hub degree. */
+int gcx_tune(gcx_graph *g, const gcx_model *m, int precision, void *stream, gcx_tuning *out);
+/* 1 and *out filled when g's signature has a record, else 0. */
+int gcx_tuning_lookup(const gcx_graph *g, gcx_tuning *out);
+void gcx_tuner_reset(void);
+/* Text file of records: "signature hub_degree schedule order_ms color_ms". */
+int gcx_tuner_save(const char *path);
+int gcx_tuner_load(const char *path);
+/* Fix g's hub degree (0 = the tuned value or the default); rebuilds the hub
+ * tables. */
+int gcx_graph_set_hub_degree(gcx_graph *g, uint32_t hub_degree, void *stream);
+uint32_t gcx_graph_hub_degree(const gcx_graph *g);
+
 /* Pack graphs into one block-diagonal graph (graph i's vertices follow graph
  * i-1's). Ordering + colouring the pack gives every graph exactly the
  * priorities and colours it gets alone (the GNN keeps graph-local id
@@ -132,16 +164,17 @@ int gcx_set_priorities(gcx_workspace *ws, const gcx_graph *g, const double *p, v
 /* Jones-Plassmann colouring with the workspace's priorities (p descending,
  * id ascending); colours 1..C land in the workspace. */
 int gcx_color(gcx_workspace *ws, const gcx_graph *g, void *stream);
-/* JP schedule; both give the same colours.
+/* JP schedule; all give the same colours.
  *   ROUNDS:  one launch per DAG level over a compacted frontier.
- *   ORDERED: work items sorted in greedy order, then one persistent
- *            (cooperative) launch of `param` workgroups (0 = 4 per CU; at
- *            most as many as are resident at once) in which each wave takes
- *            a fixed stride of the items and waits on their predecessor
- *            counts. Only one ordered launch runs on the device at a time,
- *            across all streams and host threads. If a wait exceeds its
- *            bound the call redoes the colouring with ROUNDS. */
-enum gcx_schedule { GCX_SCHED_ROUNDS = 0, GCX_SCHED_ORDERED = 1 };
+ *   ORDERED: work items sorted in greedy order, then one persistent launch
+ *            of `param` workgroups (0 = 4 per CU; at most as many as are
+ *            resident at once) in which each wave takes a fixed stride of
+ *            the items and waits on their predecessor counts. Concurrent
+ *            ordered launches follow gcx_set_ordered_launch. If a wait
+ *            exceeds its bound the call redoes the colouring with ROUNDS.
+ *   AUTO:    the schedule the autotuner recorded for the graph's degree
+ *            signature (gcx_tune), else ORDERED from 2^15 vertices up. */
+enum gcx_schedule { GCX_SCHED_ROUNDS = 0, GCX_SCHED_ORDERED = 1, GCX_SCHED_AUTO = 2 };
 int gcx_set_schedule(gcx_workspace *ws, int schedule, uint32_t param);
 /* How ORDERED launches share the device (process-wide). 0: cooperative
  * launches, one at a time, each running alone on the device. k = 1..8:
@@ -152,8 +185,28 @@ int gcx_set_ordered_launch(int slots);
 int gcx_get_ordered_launch(void);
 /* 1 when the last gcx_color fell back from ORDERED to ROUNDS. */
 int gcx_last_fallback(const gcx_workspace *ws, uint32_t *fell_back);
-/* Culberson iterated greedy: `iterations` more passes, each with p = colour. */
+/* Culberson iterated greedy: `iterations` more passes, each with p = colour
+ * (main.c:34-43). Each pass is a captured HIP graph replay. */
 int gcx_recolor(gcx_workspace *ws, const gcx_graph *g, int iterations, void *stream);
+/* Iterated greedy with other colour-class orders (no reference counterpart;
+ * every order keeps the colour count non-increasing):
+ *   REVERSE        higher colours first (= gcx_recolor)
+ *   LARGEST_FIRST  classes by size descending, ties by colour descending
+ *   RANDOM         seeded random class order, a new order every pass
+ *                  (pass i of a call uses seed + i)
+ *   MIXED          pass i uses REVERSE, LARGEST_FIRST, RANDOM for i mod 3
+ * Uses the workspace's GNN row buffer as scratch. OR-ing GCX_CULB_NO_GRAPH
+ * into the strategy runs the passes one gcx_color call at a time (the
+ * workspace's schedule, a host poll per pass) instead of graph replays. */
+enum gcx_culberson {
+    GCX_CULB_REVERSE = 0,
+    GCX_CULB_LARGEST_FIRST = 1,
+    GCX_CULB_RANDOM = 2,
+    GCX_CULB_MIXED = 3,
+    GCX_CULB_NO_GRAPH = 0x100
+};
+int gcx_recolor_ex(gcx_workspace *ws, const gcx_graph *g, int iterations, int strategy,
+                   uint64_t seed, void *stream);
 /* Proper-colouring check of the workspace colours: *max_color = C or 0. */
 int gcx_validate(gcx_workspace *ws, const gcx_graph *g, void *stream, int *max_color);
 /* Per packed graph of g: its colour count (largest colour) and the digest

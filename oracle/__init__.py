@@ -180,6 +180,50 @@ def culberson(V, E, p, K):
     return color[:N], counts
 
 
+def mix64(z):
+    """SplitMix64 finaliser on uint64 arrays (wrapping arithmetic)."""
+    z = np.asarray(z, np.uint64) + np.uint64(0x9E3779B97F4A7C15)
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def class_keys(color, strategy, seed, it):
+    """Culberson class orders (no reference counterpart beyond strategy 0,
+    main.c:38-40): key per vertex, larger = coloured earlier.
+    0 reverse: key = colour; 1 largest class first (ties: higher colour
+    first); 2 random: (hi32(mix64((seed + it) ^ c)) << b) | c, b = 12 for
+    C < 4096; 3 mixed: strategy it mod 3."""
+    color = np.asarray(color, np.int64)
+    C = int(color.max(initial=0))
+    st = it % 3 if strategy == 3 else strategy
+    c = np.arange(C + 1, dtype=np.int64)
+    if st == 0:
+        ck = c
+    elif st == 1:
+        size = np.bincount(color, minlength=C + 1)
+        order = sorted(range(1, C + 1), key=lambda d: (-size[d], -d))
+        ck = np.zeros(C + 1, np.int64)
+        for r, d in enumerate(order):
+            ck[d] = C - r
+    else:
+        b = 12 if C < 4096 else int(C).bit_length()
+        h = mix64((np.uint64(seed + it) ^ c.astype(np.uint64)).astype(np.uint64))
+        hi = (h >> np.uint64(32)).astype(np.uint64)
+        ck = (((hi << np.uint64(b)) & np.uint64(0xFFFFFFFF)) | c.astype(np.uint64)).astype(np.int64)
+    return ck[color].astype(np.float64)
+
+
+def culberson_classes(V, E, color, K, strategy, seed=0):
+    """K iterated-greedy passes from `color` with a class-order strategy;
+    returns (final colours, colour count after each pass)."""
+    counts = []
+    for it in range(K):
+        color, _ = greedy(V, E, class_keys(color, strategy, seed, it))
+        counts.append(int(color.max(initial=0)))
+    return color, counts
+
+
 def bf16_round(x):
     """Round float32 values to bfloat16 (nearest even), returned as float32."""
     b = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
@@ -187,24 +231,33 @@ def bf16_round(x):
     return b.astype(np.uint32).view(np.float32)
 
 
-def gnn_bf16_emulated(param, V, E, md=100, rounding=True):
+def gnn_bf16_emulated(param, V, E, md=100, rounding=True, split=False):
     """The BF16 GraphSAGE contract evaluated on the CPU: operands of each
     dense step ([mean | self] and W) rounded to bf16, products accumulated
     exactly (float64 here), FP32 bias, ReLU, next-layer rows rounded to bf16,
-    output = sum of the 16 features. The reference has no BF16 path, so this
-    is pinned by `rounding=False`, which must equal the FP64 network."""
+    output = sum of the 16 features. With split=True the BF16X3 contract:
+    FP32 rows, every operand a and weight w split into bf16 parts
+    hi = bf16(a), lo = bf16(a - hi), products hi*w_hi + hi*w_lo + lo*w_hi.
+    The reference has no BF16 path, so this is pinned by `rounding=False`,
+    which must equal the FP64 network."""
     import scipy.sparse as sp
 
-    if rounding:
-        def rnd(a):  # float32 value rounded to bf16, as float64
-            return bf16_round(np.asarray(a, np.float32)).astype(np.float64)
+    def f32(a):
+        return np.asarray(a, np.float32).astype(np.float64)
 
-        def f32(a):
-            return np.asarray(a, np.float32).astype(np.float64)
-    else:
-        def rnd(a):
+    def rnd(a):  # float32 value rounded to bf16, as float64
+        return bf16_round(np.asarray(a, np.float32)).astype(np.float64)
+
+    def parts(a):  # (hi, lo) of the FP32 value of a
+        a32 = np.asarray(a, np.float32)
+        hi = bf16_round(a32)
+        lo = bf16_round((a32 - hi).astype(np.float32))
+        return hi.astype(np.float64), lo.astype(np.float64)
+
+    if not rounding:
+        def f32(a):  # noqa: F811
             return np.asarray(a, np.float64)
-        f32 = rnd
+        rnd = f32
     V = np.asarray(V, np.int64)
     E = np.asarray(E, np.int64)
     N = len(V) - 1
@@ -216,17 +269,22 @@ def gnn_bf16_emulated(param, V, E, md=100, rounding=True):
     fidv = f32(np.arange(N, dtype=np.float64) / N)
     x = None
     for l in range(w.shape[0]):
-        W = rnd(w[l, :512]).reshape(32, 16)
         b = w[l, 512:].astype(np.float64)
         if l == 0:
-            sd = rnd((A @ fdeg) * inv)
-            si = rnd((A @ fidv) * inv)
-            Aop = np.stack([sd] * 8 + [si] * 8 + [rnd(fdeg)] * 8 + [rnd(fidv)] * 8, 1)
+            sd = f32((A @ fdeg) * inv)
+            si = f32((A @ fidv) * inv)
+            Aop = np.stack([sd] * 8 + [si] * 8 + [fdeg] * 8 + [fidv] * 8, 1)
         else:
-            Aop = np.concatenate([rnd((A @ x) * inv[:, None]), x], 1)
-        H = np.maximum(Aop @ W + b, 0.0)
+            Aop = np.concatenate([f32((A @ x) * inv[:, None]), x], 1)
+        if split and rounding:
+            ah, al = parts(Aop)
+            wh, wl = parts(w[l, :512])
+            wh, wl = wh.reshape(32, 16), wl.reshape(32, 16)
+            H = np.maximum(ah @ wh + ah @ wl + al @ wh + b, 0.0)
+        else:
+            H = np.maximum(rnd(Aop) @ rnd(w[l, :512]).reshape(32, 16) + b, 0.0)
         if l < w.shape[0] - 1:
-            x = rnd(H)
+            x = f32(H) if split else rnd(H)
         else:
             return H.sum(1)
     return None

@@ -278,3 +278,44 @@ def test_bf16_emulation_structure(name, layers, models):
     rounded = oracle.gnn_bf16_emulated(param, c["V"], c["E"], md)
     rel = np.abs(rounded - exact) / np.maximum(np.abs(exact), 1e-3)
     assert rel.mean() < 2e-2 and not np.array_equal(rounded, exact)
+
+
+@pytest.mark.parametrize("name", ["er300", "rmat10", "mixed10"])
+@pytest.mark.parametrize("layers", [2, 3, 4])
+def test_bf16x3_emulation_structure(name, layers, models):
+    """The split (BF16X3) contract: each operand and weight is carried to
+    about 2^-18 relative by its two bf16 parts and only the lo*lo product
+    (about 2^-18 relative) is dropped, so priorities sit about 2^-17 from
+    FP64 -- hundreds of times closer than the one-term BF16 contract."""
+    c = load_case(name)
+    param, md = models[layers]
+    exact = oracle.gnn_fp64(param, c["V"], c["E"], md)
+    x3 = oracle.gnn_bf16_emulated(param, c["V"], c["E"], md, split=True)
+    one = oracle.gnn_bf16_emulated(param, c["V"], c["E"], md)
+    scale = np.maximum(np.abs(exact), 1e-3)
+    err3, err1 = np.abs(x3 - exact) / scale, np.abs(one - exact) / scale
+    assert err3.max() < 5e-3 and err3.mean() < 3e-5, (err3.max(), err3.mean())
+    assert err3.mean() < err1.mean() / 50, (err3.mean(), err1.mean())
+
+
+
+def test_class_orders_keep_colour_count():
+    """Every class order gives a proper colouring with at most as many
+    colours (Culberson); reverse is the reference's p = colour."""
+    rng = np.random.default_rng(5)
+    pairs = {(int(a), int(b)) for a, b in rng.integers(0, 400, (3000, 2)) if a != b}
+    V, E = oracle.csr_from_pairs(400, *zip(*pairs))
+    c0, _ = oracle.greedy(V, E, rng.random(400))
+    # orc_culberson's first pass colours with p = c0 itself: 5 reverse passes
+    ref, ref_counts = oracle.culberson(V, E, c0.astype(np.float64), 5)
+    for strategy in range(4):
+        col, counts = oracle.culberson_classes(V, E, c0, 5, strategy, seed=9)
+        assert oracle.validate(V, E, col) == counts[-1]
+        assert all(a >= b for a, b in zip([c0.max()] + counts, counts))
+        if strategy == 0:  # the reference chain
+            np.testing.assert_array_equal(col, ref)
+            assert counts == ref_counts.tolist()
+    # largest first puts the biggest class first: it keeps colour 1
+    key = oracle.class_keys(c0, 1, 0, 0)
+    sizes = np.bincount(c0)
+    assert key[c0 == np.argmax(sizes[1:]) + 1].max() == c0.max()
