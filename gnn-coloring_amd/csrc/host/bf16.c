@@ -95,8 +95,8 @@ int gcx__order_gnn_bf16(gcx_workspace *ws, const gcx_graph *g, const gcx_model *
                 return GCX_ERR_HIP;
             hs = (hipStream_t)ws->aux;
         }
-        rc = gcxk_gnn_bf16_layer(kind, split, g->N, g->V, g->E, m->dWpack + 1024 * (size_t)l,
-                                 m->dWb + 1024 * (size_t)l, b, m->md, in, out, (uint32_t *)ws->key,
+        rc = gcxk_gnn_bf16_layer(kind, split, g->N, g->V, g->E, g->Er, g->rank,
+                                 m->dWpack + 1024 * (size_t)l, b, m->md, in, out, (uint32_t *)ws->key,
                                  g->hubs, g->nhub, g->perm, g->fidv, s, hs);
         if (rc != GCX_OK)
             return rc;

@@ -298,8 +298,9 @@ int gcx_profile_read(double ms[GCX_ST_COUNT], uint64_t launches[GCX_ST_COUNT])
 
 uint64_t gcx_graph_footprint(uint32_t N, uint64_t M)
 {
-    /* V and the hub chunk table (N + 1 each), E, hubs and the degree order */
-    return 8ull * ((uint64_t)N + 1) + 4ull * M + 8ull * N;
+    /* V and the hub chunk table (N + 1 each), E and Er, hubs, the degree
+     * order and its inverse */
+    return 8ull * ((uint64_t)N + 1) + 8ull * M + 12ull * N;
 }
 
 /* Degree statistics, hub list (ascending ids) with its 4096-edge chunk table
@@ -323,6 +324,11 @@ static int graph_stats(gcx_graph *g, hipStream_t s)
     if (!g->perm)
         TRY(gcx__dalloc((void **)&g->perm, 4ull * (g->N ? g->N : 1), s));
     TRY(gcxk_degree_order(g->N, g->V, g->maxdeg, g->perm, s));
+    if (!g->rank)
+        TRY(gcx__dalloc((void **)&g->rank, 4ull * (g->N ? g->N : 1), s));
+    if (!g->Er)
+        TRY(gcx__dalloc((void **)&g->Er, 4ull * (g->M ? g->M : 1), s));
+    TRY(gcxk_rank_columns(g->N, g->M, g->perm, g->E, g->rank, g->Er, s));
     /* creation returns a graph usable on any stream */
     HIPC(hipStreamSynchronize(s));
     return GCX_OK;
@@ -338,7 +344,7 @@ int gcx_graph_free_on(gcx_graph *g, void *stream)
 {
     if (!g)
         return GCX_OK;
-    void *bufs[] = {g->V, g->E, g->hubs, g->hchunk, g->perm, g->fidv, g->goff};
+    void *bufs[] = {g->V, g->E, g->hubs, g->hchunk, g->perm, g->rank, g->Er, g->fidv, g->goff};
     for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); i++)
         gcx__dfree(bufs[i], stream);
     free(g);
@@ -368,6 +374,8 @@ int gcx__graph_alloc(uint32_t N, uint64_t M, void *stream, gcx_graph **out)
         gcx__dalloc((void **)&g->E, 4ull * (M ? M : 1), stream) != GCX_OK ||
         gcx__dalloc((void **)&g->hubs, 4ull * (N ? N : 1), stream) != GCX_OK ||
         gcx__dalloc((void **)&g->perm, 4ull * (N ? N : 1), stream) != GCX_OK ||
+        gcx__dalloc((void **)&g->rank, 4ull * (N ? N : 1), stream) != GCX_OK ||
+        gcx__dalloc((void **)&g->Er, 4ull * (M ? M : 1), stream) != GCX_OK ||
         gcx__dalloc((void **)&g->hchunk, 4ull * (N + 1ull), stream) != GCX_OK) {
         gcx_graph_free_on(g, stream);
         return GCX_ERR_NOMEM;
@@ -1045,7 +1053,7 @@ static int order_gnn_fp32(gcx_workspace *ws, const gcx_graph *g, const gcx_model
             HIPC(hipStreamWaitEvent((hipStream_t)ws->aux, (hipEvent_t)ws->ev_fork, 0));
             hs = (hipStream_t)ws->aux;
         }
-        TRY(gcxk_gnn_fp32_layer(kind, g->N, g->V, g->E, W, b, m->md, in, out,
+        TRY(gcxk_gnn_fp32_layer(kind, g->N, g->V, g->E, g->Er, g->rank, W, b, m->md, in, out,
                                 (uint32_t *)ws->key, g->hubs, g->nhub, g->perm, g->fidv, s,
                                 hs));
         if (g->nhub) {

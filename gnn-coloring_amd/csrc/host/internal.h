@@ -21,6 +21,8 @@ struct gcx_graph {
     uint32_t *hchunk; /* device, nhub+1: prefix of 4096-edge chunks per hub */
     uint32_t nhchunk;
     uint32_t *perm;   /* device, N: ids by degree descending (hubs first) */
+    uint32_t *rank;   /* device, N: rank[perm[i]] = i */
+    uint32_t *Er;     /* device, M: rank[E[j]] (GNN rows live at degree-order positions) */
     float *fidv;      /* device, N, or NULL: GNN id feature per vertex (batches) */
     uint32_t *goff;   /* device, count + 1, or NULL: first vertex of each packed graph */
     uint32_t count;   /* packed graphs (0 = a single graph) */

@@ -450,6 +450,37 @@ extern "C" int gcxk_csr_from_pairs(uint32_t N, uint64_t L, const uint32_t *dI, c
     return GCX_OK;
 }
 
+// rank[perm[i]] = i: position of each vertex in the degree order.
+__global__ __launch_bounds__(kBlock) void rank_of(uint32_t N, const uint32_t *__restrict__ perm,
+                                                  uint32_t *__restrict__ rank)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < N)
+        rank[perm[i]] = i;
+}
+
+// Er[j] = rank[E[j]]: the CSR's columns as degree-order positions.
+__global__ __launch_bounds__(kBlock) void rank_cols(uint64_t M, const uint32_t *__restrict__ E,
+                                                    const uint32_t *__restrict__ rank,
+                                                    uint32_t *__restrict__ Er)
+{
+    const uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (j < M)
+        Er[j] = rank[E[j]];
+}
+
+extern "C" int gcxk_rank_columns(uint32_t N, uint64_t M, const uint32_t *perm, const uint32_t *dE,
+                                 uint32_t *rank, uint32_t *Er, void *stream)
+{
+    hipStream_t s = as_stream(stream);
+    if (N)
+        rank_of<<<div_up(N, kBlock), kBlock, 0, s>>>(N, perm, rank);
+    if (M)
+        rank_cols<<<(unsigned)div_up(M, (uint64_t)kBlock), kBlock, 0, s>>>(M, dE, rank, Er);
+    GCXK_LAUNCH_CHECK();
+    return GCX_OK;
+}
+
 extern "C" int gcxk_degree_order(uint32_t N, const uint32_t *dV, uint32_t maxdeg, uint32_t *perm,
                                  void *stream)
 {

@@ -51,6 +51,11 @@ int gcxk_graph_stats(uint32_t N, const uint32_t *dV, uint32_t hub_deg, uint32_t 
                      void *stream);
 /* Degree order: perm = vertex ids by degree descending, id ascending (so the
  * nhub hubs come first). Stream-ordered temporaries; no synchronisation. */
+/* rank[perm[i]] = i and Er[j] = rank[E[j]]: the columns as positions in the
+ * degree order, so that rows stored in degree order put the hubs' rows
+ * together (they take most neighbour reads on skewed graphs). */
+int gcxk_rank_columns(uint32_t N, uint64_t M, const uint32_t *perm, const uint32_t *dE,
+                      uint32_t *rank, uint32_t *Er, void *stream);
 int gcxk_degree_order(uint32_t N, const uint32_t *dV, uint32_t maxdeg, uint32_t *perm,
                       void *stream);
 /* CSR validity (sorted, simple, symmetric): *valid = 1 or 0. */
@@ -128,25 +133,28 @@ int gcxk_sl_dec(const uint32_t *dV, const uint32_t *dE, uint32_t *deg, const uin
 /* ---- GraphSAGE-16 (gnn.hip) ---- */
 /* FP32, bit-exact with the reference op order. kind 0 = input layer (x
  * unused), 1 = hidden (x -> out), 2 = output (x -> key, float order keys).
- * Rows are 16 floats. Hub rows run on hub_stream, the rest on stream in the
- * degree order perm (gcxk_degree_order; non-hubs are perm[nhub .. N)); the
- * caller joins the two streams before the next layer. fidv (input layer
- * only): per-vertex id feature, NULL = (float)(v / N). */
+ * Rows are 16 floats, stored by degree-order position (row of vertex v =
+ * rank[v]; neighbours read through Er, gcxk_rank_columns). Hub rows run on
+ * hub_stream, the rest on stream in the degree order perm (non-hubs are
+ * perm[nhub .. N)); the caller joins the two streams before the next layer.
+ * Priorities (key) stay indexed by vertex id. fidv (input layer only):
+ * per-vertex id feature, NULL = (float)(v / N). */
 int gcxk_gnn_fp32_layer(int kind, uint32_t N, const uint32_t *dV, const uint32_t *dE,
-                        const float *W, const float *b, int md, const float *x, float *out,
-                        uint32_t *key, const uint32_t *hubs, uint32_t nhub, const uint32_t *perm,
-                        const float *fidv, void *stream, void *hub_stream);
+                        const uint32_t *Er, const uint32_t *rank, const float *W, const float *b,
+                        int md, const float *x, float *out, uint32_t *key, const uint32_t *hubs,
+                        uint32_t nhub, const uint32_t *perm, const float *fidv, void *stream,
+                        void *hub_stream);
 /* MFMA layer transform with BF16 operands and FP32 accumulation.
  * split_mode 0: bf16 rows (16 x bf16 = 32 B), operands rounded to bf16.
  * split_mode 1: FP32 rows, operands and weights split into bf16 hi + lo,
  * three MFMAs per product (near-FP32 results).
- * Wpack: the layer's two MFMA B operands in lane order, hi parts (512 bf16)
- * then lo parts (512); Wb: the layer's weights W[k][j] in bf16, hi (512) then
- * lo (512); b: FP32 biases (16). */
+ * Rows by degree-order position as in the FP32 layer. Wpack: the layer's two
+ * MFMA B operands in lane order, hi parts (512 bf16) then lo parts (512);
+ * b: FP32 biases (16). */
 int gcxk_gnn_bf16_layer(int kind, int split_mode, uint32_t N, const uint32_t *dV,
-                        const uint32_t *dE, const uint16_t *Wpack, const uint16_t *Wb,
-                        const float *b, int md, const void *x, void *out, uint32_t *key,
-                        const uint32_t *hubs, uint32_t nhub, const uint32_t *perm,
+                        const uint32_t *dE, const uint32_t *Er, const uint32_t *rank,
+                        const uint16_t *Wpack, const float *b, int md, const void *x, void *out,
+                        uint32_t *key, const uint32_t *hubs, uint32_t nhub, const uint32_t *perm,
                         const float *fidv, void *stream, void *hub_stream);
 
 /* ---- Jones-Plassmann (jp.hip) ---- */

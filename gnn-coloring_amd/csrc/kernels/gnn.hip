@@ -9,6 +9,10 @@
 // Neighbour sums run left to right in CSR order from +0 and are divided by
 // the degree when it is positive.
 //
+// Rows live at degree-order positions (row of v = rank[v], neighbours read
+// through the graph's Er = rank[E]): the hubs' rows, which take most
+// neighbour reads on skewed graphs, sit together in a few cache lines.
+//
 // Mapping: vertices below the hub degree run in degree order (the graph's
 // perm), 4 lanes per vertex and 16 vertices per wave (see sage_quads); the
 // sequential dependency is only on the ordered adds, the row loads are
@@ -112,6 +116,7 @@ constexpr int kUnroll = 8;         // neighbour rows in flight per lane
 template <int kind>
 __global__ __launch_bounds__(kBlock) void sage_quads(uint32_t N, const uint32_t *__restrict__ V,
                                                      const uint32_t *__restrict__ E,
+                                                     const uint32_t *__restrict__ Er,
                                                      const float *__restrict__ W,
                                                      const float *__restrict__ b, float fmd,
                                                      const float *__restrict__ x,
@@ -187,7 +192,7 @@ __global__ __launch_bounds__(kBlock) void sage_quads(uint32_t N, const uint32_t 
                 uint32_t v[kUnroll];
 #pragma unroll
                 for (int t = 0; t < kUnroll; t++)
-                    v[t] = E[i + t];
+                    v[t] = Er[i + t];
                 float4 r[kUnroll];
 #pragma unroll
                 for (int t = 0; t < kUnroll; t++)
@@ -201,7 +206,7 @@ __global__ __launch_bounds__(kBlock) void sage_quads(uint32_t N, const uint32_t 
                 }
             }
             for (; i < end; i++) {
-                const float4 r = X[(size_t)E[i] * 4 + q];
+                const float4 r = X[(size_t)Er[i] * 4 + q];
                 s0 = s0 + r.x;
                 s1 = s1 + r.y;
                 s2 = s2 + r.z;
@@ -213,7 +218,7 @@ __global__ __launch_bounds__(kBlock) void sage_quads(uint32_t N, const uint32_t 
                 s2 = s2 / fdeg_f;
                 s3 = s3 / fdeg_f;
             }
-            const float4 self = X[(size_t)u * 4 + q];
+            const float4 self = X[(size_t)g * 4 + q];
             const float ag[4] = {s0, s1, s2, s3};
             const float sf[4] = {self.x, self.y, self.z, self.w};
 #pragma unroll
@@ -250,7 +255,7 @@ __global__ __launch_bounds__(kBlock) void sage_quads(uint32_t N, const uint32_t 
             if (q == 0)
                 key[u] = float_order_key(p);
         } else {
-            reinterpret_cast<float4 *>(out)[(size_t)u * 4 + q] = make_float4(h[0], h[1], h[2], h[3]);
+            reinterpret_cast<float4 *>(out)[(size_t)g * 4 + q] = make_float4(h[0], h[1], h[2], h[3]);
         }
     }
 }
@@ -260,6 +265,8 @@ __global__ __launch_bounds__(kBlock) void sage_quads(uint32_t N, const uint32_t 
 template <int kind>
 __global__ __launch_bounds__(kHubBlock) void sage_hubs(uint32_t N, const uint32_t *__restrict__ V,
                                                        const uint32_t *__restrict__ E,
+                                                       const uint32_t *__restrict__ Er,
+                                                       const uint32_t *__restrict__ rank,
                                                        const float *__restrict__ W,
                                                        const float *__restrict__ b, float fmd,
                                                        const float *__restrict__ x,
@@ -281,12 +288,12 @@ __global__ __launch_bounds__(kHubBlock) void sage_hubs(uint32_t N, const uint32_
     auto stage = [&](uint32_t c, int s) {
         const uint32_t i = beg + c * kHubChunk + tid;
         if (i < end) {
-            const uint32_t v = E[i];
             if (kind == kInput) {
+                const uint32_t v = E[i];
                 buf[s][0][tid] = (float)(V[v + 1] - V[v]) / fmd;
                 buf[s][1][tid] = fid_of(v, N, fidv);
             } else {
-                const float4 *row = reinterpret_cast<const float4 *>(x + (size_t)v * F);
+                const float4 *row = reinterpret_cast<const float4 *>(x + (size_t)Er[i] * F);
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
                     const float4 r4 = row[q];
@@ -353,7 +360,7 @@ __global__ __launch_bounds__(kHubBlock) void sage_hubs(uint32_t N, const uint32_
         float a = acc;
         if (fdeg_f > 0.0f)
             a = a / fdeg_f;
-        h = dense_group(w, bias, a, x[(size_t)u * F + j]);
+        h = dense_group(w, bias, a, x[(size_t)rank[u] * F + j]);
     }
     if (tid < F) {
         if (kind == kOutput) {
@@ -361,7 +368,7 @@ __global__ __launch_bounds__(kHubBlock) void sage_hubs(uint32_t N, const uint32_
             if (tid == 0)
                 key[u] = float_order_key(p);
         } else {
-            out[(size_t)u * F + j] = h;
+            out[(size_t)rank[u] * F + j] = h;
         }
     }
 }
@@ -371,23 +378,23 @@ __global__ __launch_bounds__(kHubBlock) void sage_hubs(uint32_t N, const uint32_
 constexpr uint32_t kQuadGrid = 2048;
 
 template <int kind>
-int launch_layer(uint32_t N, const uint32_t *V, const uint32_t *E, const float *W, const float *b,
-                 float fmd, const float *x, float *out, uint32_t *key, const uint32_t *hubs,
-                 uint32_t nhub, const uint32_t *perm, const float *fidv, void *stream,
-                 void *hub_stream)
+int launch_layer(uint32_t N, const uint32_t *V, const uint32_t *E, const uint32_t *Er,
+                 const uint32_t *rank, const float *W, const float *b, float fmd, const float *x,
+                 float *out, uint32_t *key, const uint32_t *hubs, uint32_t nhub,
+                 const uint32_t *perm, const float *fidv, void *stream, void *hub_stream)
 {
     if (N == 0)
         return GCX_OK;
     // Hubs go first on their own stream so that they overlap the quads.
     if (nhub)
-        sage_hubs<kind><<<nhub, kHubBlock, 0, as_stream(hub_stream)>>>(N, V, E, W, b, fmd, x, out,
-                                                                       key, hubs, fidv);
+        sage_hubs<kind><<<nhub, kHubBlock, 0, as_stream(hub_stream)>>>(N, V, E, Er, rank, W, b, fmd,
+                                                                       x, out, key, hubs, fidv);
     const uint32_t rest = N - nhub;
     if (rest) {
         uint32_t grid = div_up(rest, kQuads);
         grid = grid < kQuadGrid ? grid : kQuadGrid;
-        sage_quads<kind><<<grid, kBlock, 0, as_stream(stream)>>>(N, V, E, W, b, fmd, x, out, key,
-                                                                 perm, nhub, N, fidv);
+        sage_quads<kind><<<grid, kBlock, 0, as_stream(stream)>>>(N, V, E, Er, W, b, fmd, x, out,
+                                                                 key, perm, nhub, N, fidv);
     }
     GCXK_LAUNCH_CHECK();
     return GCX_OK;
@@ -396,21 +403,22 @@ int launch_layer(uint32_t N, const uint32_t *V, const uint32_t *E, const float *
 } // namespace
 
 extern "C" int gcxk_gnn_fp32_layer(int kind, uint32_t N, const uint32_t *dV, const uint32_t *dE,
-                                   const float *W, const float *b, int md, const float *x,
-                                   float *out, uint32_t *key, const uint32_t *hubs, uint32_t nhub,
+                                   const uint32_t *Er, const uint32_t *rank, const float *W,
+                                   const float *b, int md, const float *x, float *out,
+                                   uint32_t *key, const uint32_t *hubs, uint32_t nhub,
                                    const uint32_t *perm, const float *fidv, void *stream,
                                    void *hub_stream)
 {
     switch (kind) {
     case 0:
-        return launch_layer<kInput>(N, dV, dE, W, b, (float)md, nullptr, out, nullptr, hubs, nhub,
-                                    perm, fidv, stream, hub_stream);
+        return launch_layer<kInput>(N, dV, dE, Er, rank, W, b, (float)md, nullptr, out, nullptr,
+                                    hubs, nhub, perm, fidv, stream, hub_stream);
     case 1:
-        return launch_layer<kHidden>(N, dV, dE, W, b, 0.0f, x, out, nullptr, hubs, nhub, perm,
-                                     nullptr, stream, hub_stream);
+        return launch_layer<kHidden>(N, dV, dE, Er, rank, W, b, 0.0f, x, out, nullptr, hubs, nhub,
+                                     perm, nullptr, stream, hub_stream);
     case 2:
-        return launch_layer<kOutput>(N, dV, dE, W, b, 0.0f, x, nullptr, key, hubs, nhub, perm,
-                                     nullptr, stream, hub_stream);
+        return launch_layer<kOutput>(N, dV, dE, Er, rank, W, b, 0.0f, x, nullptr, key, hubs, nhub,
+                                     perm, nullptr, stream, hub_stream);
     default:
         return GCX_ERR_ARG;
     }

@@ -14,6 +14,9 @@
 //          term is below 2^-16 relative). Priorities are within a few FP32
 //          ulps of the FP32 path's; no step needs FP64.
 //
+// Rows live at degree-order positions (row of v = rank[v]; neighbours read
+// through Er = rank[E]), as in the FP32 kernel.
+//
 // Mapping (as in the FP32 kernel): vertices below the hub degree in the
 // graph's degree order, 4 lanes per vertex, 16 vertices per wave; lane q of a
 // vertex loads slice q (features 4q .. 4q+3) of every neighbour row, 8 rows
@@ -82,6 +85,7 @@ __device__ __forceinline__ void split(float a, short &hi, short &lo)
 template <int kind, int mode>
 __global__ __launch_bounds__(kBlock) void sage_mfma_quads(uint32_t N, const uint32_t *__restrict__ V,
                                                           const uint32_t *__restrict__ E,
+                                                          const uint32_t *__restrict__ Er,
                                                           const uint16_t *__restrict__ Wpack,
                                                           const float *__restrict__ bias, float fmd,
                                                           const void *__restrict__ x,
@@ -167,7 +171,7 @@ __global__ __launch_bounds__(kBlock) void sage_mfma_quads(uint32_t N, const uint
                 uint32_t v[kUnroll];
 #pragma unroll
                 for (int t = 0; t < kUnroll; t++)
-                    v[t] = E[i + t];
+                    v[t] = Er[i + t];
                 float4 r[kUnroll];
 #pragma unroll
                 for (int t = 0; t < kUnroll; t++)
@@ -181,7 +185,7 @@ __global__ __launch_bounds__(kBlock) void sage_mfma_quads(uint32_t N, const uint
                 }
             }
             for (; i < end; i++) {
-                const float4 r = load_slice<mode>(x, E[i], q);
+                const float4 r = load_slice<mode>(x, Er[i], q);
                 s0 = s0 + r.x;
                 s1 = s1 + r.y;
                 s2 = s2 + r.z;
@@ -197,7 +201,7 @@ __global__ __launch_bounds__(kBlock) void sage_mfma_quads(uint32_t N, const uint
             m[1] = s1;
             m[2] = s2;
             m[3] = s3;
-            const float4 self = valid ? load_slice<mode>(x, u, q) : make_float4(0.f, 0.f, 0.f, 0.f);
+            const float4 self = valid ? load_slice<mode>(x, pos, q) : make_float4(0.f, 0.f, 0.f, 0.f);
             sf[0] = self.x;
             sf[1] = self.y;
             sf[2] = self.z;
@@ -245,10 +249,11 @@ __global__ __launch_bounds__(kBlock) void sage_mfma_quads(uint32_t N, const uint
                 if (mi == 0 && ok)
                     key[v] = float_order_key(h);
             } else if (ok) {
+                const size_t row = base + (uint32_t)r; // degree-order position
                 if (mode == kBf16)
-                    reinterpret_cast<uint16_t *>(out)[(size_t)v * F + mi] = to_bf16(h);
+                    reinterpret_cast<uint16_t *>(out)[row * F + mi] = to_bf16(h);
                 else
-                    reinterpret_cast<float *>(out)[(size_t)v * F + mi] = h;
+                    reinterpret_cast<float *>(out)[row * F + mi] = h;
             }
         }
     }
@@ -267,6 +272,8 @@ constexpr int kHubPad = 4;
 template <int kind, int mode>
 __global__ __launch_bounds__(kHubChunk) void sage_mfma_hubs(uint32_t N, const uint32_t *__restrict__ V,
                                                             const uint32_t *__restrict__ E,
+                                                            const uint32_t *__restrict__ Er,
+                                                            const uint32_t *__restrict__ rank,
                                                             const uint16_t *__restrict__ Wpack,
                                                             const float *__restrict__ bias, float fmd,
                                                             const void *__restrict__ x,
@@ -286,11 +293,12 @@ __global__ __launch_bounds__(kHubChunk) void sage_mfma_hubs(uint32_t N, const ui
     auto stage = [&](uint32_t c, int s) {
         const uint32_t i = beg + c * kHubChunk + tid;
         if (i < end) {
-            const uint32_t v = E[i];
             if (kind == kInput) {
+                const uint32_t v = E[i];
                 buf[s][0][tid] = (float)(V[v + 1] - V[v]) / fmd;
                 buf[s][1][tid] = fid_of(v, N, fidv);
             } else {
+                const uint32_t v = Er[i];
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
                     const float4 r4 = load_slice<mode>(x, v, q);
@@ -339,6 +347,7 @@ __global__ __launch_bounds__(kHubChunk) void sage_mfma_hubs(uint32_t N, const ui
     const int lane = tid;
     const int mi = lane & 15, mq = lane >> 4;
     const float fdeg = (float)deg;
+    const uint32_t self_row = rank[u];
     // mean features (lane j holds feature j; layer 0: lanes 0, 1)
     float mean = acc;
     if (fdeg > 0.0f)
@@ -353,7 +362,7 @@ __global__ __launch_bounds__(kHubChunk) void sage_mfma_hubs(uint32_t N, const ui
             b[e] = mq < 2 ? fdeg / fmd : fid_of(u, N, fidv);
         } else {
             a[e] = __shfl(mean, k);
-            const float4 r4 = load_slice<mode>(x, u, mq);
+            const float4 r4 = load_slice<mode>(x, self_row, mq);
             b[e] = e == 0 ? r4.x : e == 1 ? r4.y : e == 2 ? r4.z : r4.w;
         }
     }
@@ -400,50 +409,50 @@ __global__ __launch_bounds__(kHubChunk) void sage_mfma_hubs(uint32_t N, const ui
             key[u] = float_order_key(h);
     } else if (lane < F) {
         if (mode == kBf16)
-            reinterpret_cast<uint16_t *>(out)[(size_t)u * F + mi] = to_bf16(h);
+            reinterpret_cast<uint16_t *>(out)[(size_t)self_row * F + mi] = to_bf16(h);
         else
-            reinterpret_cast<float *>(out)[(size_t)u * F + mi] = h;
+            reinterpret_cast<float *>(out)[(size_t)self_row * F + mi] = h;
     }
 }
 
 template <int kind, int mode>
-int launch(uint32_t N, const uint32_t *V, const uint32_t *E, const uint16_t *Wpack,
-           const uint16_t *Wb, const float *b, float fmd, const void *x, void *out, uint32_t *key,
-           const uint32_t *hubs, uint32_t nhub, const uint32_t *perm, const float *fidv,
-           void *stream, void *hub_stream)
+int launch(uint32_t N, const uint32_t *V, const uint32_t *E, const uint32_t *Er, const uint32_t *rank,
+           const uint16_t *Wpack, const float *b, float fmd, const void *x, void *out,
+           uint32_t *key, const uint32_t *hubs, uint32_t nhub, const uint32_t *perm,
+           const float *fidv, void *stream, void *hub_stream)
 {
     if (N == 0)
         return GCX_OK;
     if (nhub)
         sage_mfma_hubs<kind, mode><<<nhub, kHubChunk, 0, as_stream(hub_stream)>>>(
-            N, V, E, Wpack, b, fmd, x, out, key, hubs, fidv);
+            N, V, E, Er, rank, Wpack, b, fmd, x, out, key, hubs, fidv);
     if (N > nhub) {
         const uint32_t waves = div_up(N - nhub, 16u);
         const uint32_t need = div_up(waves, (uint32_t)(kBlock / kWave));
         sage_mfma_quads<kind, mode><<<need < (uint32_t)kGrid ? need : (uint32_t)kGrid, kBlock, 0,
-                                      as_stream(stream)>>>(N, V, E, Wpack, b, fmd, x, out, key,
-                                                           perm, nhub, N, fidv);
+                                      as_stream(stream)>>>(N, V, E, Er, Wpack, b, fmd, x, out,
+                                                           key, perm, nhub, N, fidv);
     }
     GCXK_LAUNCH_CHECK();
     return GCX_OK;
 }
 
 template <int mode>
-int layer(int kind, uint32_t N, const uint32_t *dV, const uint32_t *dE, const uint16_t *Wpack,
-          const uint16_t *Wb, const float *b, float fmd, const void *x, void *out, uint32_t *key,
-          const uint32_t *hubs, uint32_t nhub, const uint32_t *perm, const float *fidv,
-          void *stream, void *hub_stream)
+int layer(int kind, uint32_t N, const uint32_t *dV, const uint32_t *dE, const uint32_t *Er,
+          const uint32_t *rank, const uint16_t *Wpack, const float *b, float fmd, const void *x,
+          void *out, uint32_t *key, const uint32_t *hubs, uint32_t nhub, const uint32_t *perm,
+          const float *fidv, void *stream, void *hub_stream)
 {
     switch (kind) {
     case 0:
-        return launch<kInput, mode>(N, dV, dE, Wpack, Wb, b, fmd, nullptr, out, nullptr, hubs,
-                                    nhub, perm, fidv, stream, hub_stream);
+        return launch<kInput, mode>(N, dV, dE, Er, rank, Wpack, b, fmd, nullptr, out, nullptr,
+                                    hubs, nhub, perm, fidv, stream, hub_stream);
     case 1:
-        return launch<kHidden, mode>(N, dV, dE, Wpack, Wb, b, fmd, x, out, nullptr, hubs, nhub,
-                                     perm, nullptr, stream, hub_stream);
+        return launch<kHidden, mode>(N, dV, dE, Er, rank, Wpack, b, fmd, x, out, nullptr, hubs,
+                                     nhub, perm, nullptr, stream, hub_stream);
     case 2:
-        return launch<kOutput, mode>(N, dV, dE, Wpack, Wb, b, fmd, x, nullptr, key, hubs, nhub,
-                                     perm, nullptr, stream, hub_stream);
+        return launch<kOutput, mode>(N, dV, dE, Er, rank, Wpack, b, fmd, x, nullptr, key, hubs,
+                                     nhub, perm, nullptr, stream, hub_stream);
     default:
         return GCX_ERR_ARG;
     }
@@ -452,14 +461,15 @@ int layer(int kind, uint32_t N, const uint32_t *dV, const uint32_t *dE, const ui
 } // namespace
 
 extern "C" int gcxk_gnn_bf16_layer(int kind, int split_mode, uint32_t N, const uint32_t *dV,
-                                   const uint32_t *dE, const uint16_t *Wpack, const uint16_t *Wb,
-                                   const float *b, int md, const void *x, void *out, uint32_t *key,
-                                   const uint32_t *hubs, uint32_t nhub, const uint32_t *perm,
-                                   const float *fidv, void *stream, void *hub_stream)
+                                   const uint32_t *dE, const uint32_t *Er, const uint32_t *rank,
+                                   const uint16_t *Wpack, const float *b, int md, const void *x,
+                                   void *out, uint32_t *key, const uint32_t *hubs, uint32_t nhub,
+                                   const uint32_t *perm, const float *fidv, void *stream,
+                                   void *hub_stream)
 {
     const float fmd = (float)md;
-    return split_mode ? layer<kSplit>(kind, N, dV, dE, Wpack, Wb, b, fmd, x, out, key, hubs, nhub,
-                                      perm, fidv, stream, hub_stream)
-                      : layer<kBf16>(kind, N, dV, dE, Wpack, Wb, b, fmd, x, out, key, hubs, nhub,
-                                     perm, fidv, stream, hub_stream);
+    return split_mode ? layer<kSplit>(kind, N, dV, dE, Er, rank, Wpack, b, fmd, x, out, key, hubs,
+                                      nhub, perm, fidv, stream, hub_stream)
+                      : layer<kBf16>(kind, N, dV, dE, Er, rank, Wpack, b, fmd, x, out, key, hubs,
+                                     nhub, perm, fidv, stream, hub_stream);
 }
