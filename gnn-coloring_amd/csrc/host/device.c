@@ -1385,7 +1385,11 @@ int gcx_recolor_ex(gcx_workspace *ws, const gcx_graph *g, int iterations, int st
     TRY(gcx_validate(ws, g, s, &C));
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     HIPC(hipStreamIsCapturing(s, &cap));
-    if (eager || C <= 0 || (uint32_t)C >= RECOLOR_GRAPH_MAX_ROUNDS ||
+    /* The ordered schedule needs a sort per pass (not capturable) but runs a
+     * pass as one persistent launch: with ORDERED (or AUTO choosing it) the
+     * passes run one gcx_color at a time. */
+    const int sched = ws->schedule == GCX_SCHED_AUTO ? gcx__tuned_schedule(g) : ws->schedule;
+    if (eager || sched == GCX_SCHED_ORDERED || C <= 0 || (uint32_t)C >= RECOLOR_GRAPH_MAX_ROUNDS ||
         cap != hipStreamCaptureStatusNone)
         return recolor_eager(ws, g, iterations, strategy, seed, s);
     const int mb = gcxk_jp_mask_bits(g->maxdeg);

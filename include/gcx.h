@@ -186,7 +186,7 @@ int gcx_get_ordered_launch(void);
 /* 1 when the last gcx_color fell back from ORDERED to ROUNDS. */
 int gcx_last_fallback(const gcx_workspace *ws, uint32_t *fell_back);
 /* Culberson iterated greedy: `iterations` more passes, each with p = colour
- * (main.c:34-43). Each pass is a captured HIP graph replay. */
+ * (main.c:34-43); see gcx_recolor_ex for how passes run. */
 int gcx_recolor(gcx_workspace *ws, const gcx_graph *g, int iterations, void *stream);
 /* Iterated greedy with other colour-class orders (no reference counterpart;
  * every order keeps the colour count non-increasing):
@@ -195,9 +195,11 @@ int gcx_recolor(gcx_workspace *ws, const gcx_graph *g, int iterations, void *str
  *   RANDOM         seeded random class order, a new order every pass
  *                  (pass i of a call uses seed + i)
  *   MIXED          pass i uses REVERSE, LARGEST_FIRST, RANDOM for i mod 3
- * Uses the workspace's GNN row buffer as scratch. OR-ing GCX_CULB_NO_GRAPH
- * into the strategy runs the passes one gcx_color call at a time (the
- * workspace's schedule, a host poll per pass) instead of graph replays. */
+ * Uses the workspace's GNN row buffer as scratch. Under the ROUNDS schedule
+ * each pass is a captured graph replay; under ORDERED (or AUTO choosing it)
+ * each pass is one gcx_color call (a sort plus one persistent launch).
+ * OR-ing GCX_CULB_NO_GRAPH into the strategy forces one gcx_color call per
+ * pass under any schedule. */
 enum gcx_culberson {
     GCX_CULB_REVERSE = 0,
     GCX_CULB_LARGEST_FIRST = 1,

@@ -158,9 +158,11 @@ def test_recolor_strategies_match_oracle(strategy, kind):
     col, counts = oracle.culberson_classes(V, E, c0, 6, strategy, seed=77)
     np.testing.assert_array_equal(ws.colors(g.N), col)
     assert ws.validate(g) == counts[-1]
-    # a second call restarts the pass counter: same seed, same result; the
-    # eager loop (one gcx_color per pass) gives the same colours
-    for flags in (0, gcx.CULB_NO_GRAPH):
+    # a second call restarts the pass counter: same seed, same result; one
+    # gcx_color per pass (rounds or ordered) gives the same colours
+    for sched, flags in ((gcx.SCHED_ROUNDS, 0), (gcx.SCHED_ROUNDS, gcx.CULB_NO_GRAPH),
+                         (gcx.SCHED_ORDERED, 0)):
+        ws.set_schedule(sched)
         ws.upload_colors(c0)
         ws.recolor_ex(g, 6, strategy | flags, seed=77)
         np.testing.assert_array_equal(ws.colors(g.N), col)
