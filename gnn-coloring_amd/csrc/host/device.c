@@ -774,7 +774,7 @@ uint64_t gcx_workspace_footprint(uint32_t N, uint64_t M)
     const uint64_t n = (uint64_t)N + 1;
     return 176ull * n + 8ull * gcxk_jp_state_words(N, M) + 4ull * M +
            4ull * gcxk_jp_ovf_words(N, M) +
-           2ull * sizeof(gcxk_item) * gcxk_jp_item_capacity(N, M) + 2 * sizeof(gcxk_ctl) +
+           2ull * sizeof(gcxk_item) * gcxk_jp_item_capacity(N, M) + 2 * sizeof(gcxk_ctl) + 256 +
            order_extra(N);
 }
 
@@ -870,7 +870,7 @@ int gcx_workspace_free_on(gcx_workspace *ws, void *stream)
     void *bufs[] = {ws->key,    ws->count, ws->color,    ws->q[0],     ws->q[1],
                     ws->h[0],   ws->h[1],  ws->x,        ws->y,        ws->ctl,
                     ws->npred,  ws->state, ws->ovf,      ws->esplit,   ws->items[0],
-                    ws->items[1], ws->hpos, ws->order_buf, ws->wof};
+                    ws->items[1], ws->hpos, ws->order_buf, ws->wof, ws->pass};
     for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); i++)
         gcx__dfree(bufs[i], stream);
     pinned_ctl_put(ws->hctl);
@@ -914,6 +914,7 @@ int gcx_workspace_create_on(uint32_t N, uint64_t M, void *stream, gcx_workspace 
              gcx__dalloc((void **)&ws->h[1], 4 * n, st) == GCX_OK &&
              gcx__dalloc((void **)&ws->x, 64 * n, st) == GCX_OK &&
              gcx__dalloc((void **)&ws->y, 64 * n, st) == GCX_OK &&
+             gcx__dalloc((void **)&ws->pass, 256, st) == GCX_OK &&
              gcx__dalloc((void **)&ws->ctl, 2 * sizeof(gcxk_ctl), st) == GCX_OK &&
              gcx__dalloc((void **)&ws->npred, 4 * n, st) == GCX_OK &&
              gcx__dalloc((void **)&ws->state, 8 * gcxk_jp_state_words(N, M), st) == GCX_OK &&
@@ -1326,7 +1327,7 @@ int gcx_last_launches(const gcx_workspace *ws, uint32_t *launches)
 static int recolor_pass(gcx_workspace *ws, const gcx_graph *g, int mb, uint32_t C, int strategy,
                         uint64_t seed, uint32_t rounds, hipStream_t s)
 {
-    TRY(gcxk_culberson_keys(g->N, C, strategy, seed, ws->color, (uint32_t *)ws->y,
+    TRY(gcxk_culberson_keys(g->N, C, strategy, seed, ws->color, ws->pass, (uint32_t *)ws->y,
                             (uint32_t *)ws->key, s));
     TRY(jp_prepare(ws, g, mb, s));
     TRY(gcxk_jp_roots(g->N, g->V, ws->npred, ws->items[0], ws->ctl, s));
@@ -1344,8 +1345,8 @@ static int recolor_eager(gcx_workspace *ws, const gcx_graph *g, int iterations, 
         TRY(gcx_validate(ws, g, s, &C));
         if (C <= 0)
             return GCX_ERR_INVALID;
-        TRY(gcxk_culberson_keys(g->N, (uint32_t)C, strategy, seed, ws->color, (uint32_t *)ws->y,
-                                (uint32_t *)ws->key, s));
+        TRY(gcxk_culberson_keys(g->N, (uint32_t)C, strategy, seed, ws->color, ws->pass,
+                                (uint32_t *)ws->y, (uint32_t *)ws->key, s));
         ws->key_bits = 32;
         ws->key_kind = 0;
         TRY(gcx_color(ws, g, s));
@@ -1380,7 +1381,7 @@ int gcx_recolor_ex(gcx_workspace *ws, const gcx_graph *g, int iterations, int st
         return GCX_OK;
     hipStream_t s = (hipStream_t)gcx__pick(stream);
     /* the pass counter of the class orders starts at 0 for this call */
-    HIPC(hipMemsetAsync(ws->y, 0, sizeof(uint32_t), s));
+    HIPC(hipMemsetAsync(ws->pass, 0, sizeof(uint32_t), s));
     int C = 0;
     TRY(gcx_validate(ws, g, s, &C));
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;

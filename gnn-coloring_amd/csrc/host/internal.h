@@ -66,6 +66,7 @@ struct gcx_workspace {
     uint32_t ordered_grid;  /* workgroups of the ordered launch (0 = all resident) */
     uint32_t last_fallback; /* 1 when the last ordered launch hit its wait bound */
     int last_ordered;       /* the last gcx_color ran the ordered schedule */
+    uint32_t *pass;         /* Culberson pass counter (device word; survives gcx_color) */
     void *order_buf;        /* sort scratch when it does not fit in x */
     size_t order_bytes;     /* bytes of sort scratch available */
     uint32_t curN;    /* vertex count of the graph the contents belong to */

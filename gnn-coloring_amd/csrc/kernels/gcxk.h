@@ -100,11 +100,11 @@ int gcxk_keys_llf(uint32_t N, const uint32_t *dV, uint32_t *key, void *stream);
 int gcxk_keys_from_colors(uint32_t N, const int *color, uint32_t *key, void *stream);
 /* Culberson pass keys for strategy 0 reverse (key = colour), 1 largest class
  * first, 2 seeded random class order, 3 mixed (pass i: i mod 3). C = number
- * of colour classes (colours 1..C); scratch holds 2C + 3 words: the pass
- * counter (word 0, advanced once per call on the device), class sizes and
- * class keys. */
+ * of colour classes (colours 1..C); pass = the pass counter (one device word,
+ * advanced once per call on the device); scratch holds 2C + 2 words (class
+ * sizes and class keys). */
 int gcxk_culberson_keys(uint32_t N, uint32_t C, int strategy, uint64_t seed, const int *color,
-                        uint32_t *scratch, uint32_t *key, void *stream);
+                        uint32_t *pass, uint32_t *scratch, uint32_t *key, void *stream);
 /* Host doubles (already on device) -> key32 when every value is exactly a
  * float, else key64; counts NaNs and non-floats in ctl. */
 int gcxk_keys_scan_doubles(uint32_t N, const double *p, gcxk_ctl *ctl, void *stream);

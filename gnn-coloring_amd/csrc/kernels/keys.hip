@@ -230,7 +230,8 @@ extern "C" int gcxk_keys_from_colors(uint32_t N, const int *color, uint32_t *key
 }
 
 extern "C" int gcxk_culberson_keys(uint32_t N, uint32_t C, int strategy, uint64_t seed,
-                                   const int *color, uint32_t *scratch, uint32_t *key, void *stream)
+                                   const int *color, uint32_t *pass, uint32_t *scratch,
+                                   uint32_t *key, void *stream)
 {
     if (N == 0)
         return GCX_OK;
@@ -239,7 +240,7 @@ extern "C" int gcxk_culberson_keys(uint32_t N, uint32_t C, int strategy, uint64_
     if (strategy < 0 || strategy > 3)
         return GCX_ERR_ARG;
     hipStream_t s = as_stream(stream);
-    uint32_t *pass = scratch, *cnt = scratch + 1, *ckey = cnt + C + 1;
+    uint32_t *cnt = scratch, *ckey = cnt + C + 1;
     class_clear<<<div_up((uint64_t)C + 1, kBlock), kBlock, 0, s>>>(C, cnt);
     const uint32_t grid = div_up(N, kBlock) < 1024u ? div_up(N, kBlock) : 1024u;
     class_count<<<grid, kBlock, 0, s>>>(N, C, color, cnt);

@@ -125,7 +125,8 @@ void gcx_tuner_reset(void);
 int gcx_tuner_save(const char *path);
 int gcx_tuner_load(const char *path);
 /* Fix g's hub degree (0 = the tuned value or the default); rebuilds the hub
- * tables. */
+ * tables on `stream` and waits for it. Not while other streams use g
+ * (gcx_tune calls it too). */
 int gcx_graph_set_hub_degree(gcx_graph *g, uint32_t hub_degree, void *stream);
 uint32_t gcx_graph_hub_degree(const gcx_graph *g);
 
