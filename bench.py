@@ -90,6 +90,15 @@ def serving(args, gcx):
 
     srv = gcx.Server(workers=args.serving_workers, budget=int(args.serving_budget_gb * (1 << 30)))
     srv.run(gcx.with_arrivals(base[:min(k, 16)], None))  # warm the pool and the kernels
+    if args.serving_rate:
+        # one Poisson pass at a given rate (long traces)
+        rep2, wall2 = run(srv, gcx.with_arrivals(base, args.serving_rate, seed=12))
+        srv.close()
+        return {"requests": k, "workers": args.serving_workers,
+                "budget_gb": args.serving_budget_gb,
+                "trace": f"seed 11, n log-uniform [1e3, {args.serving_nmax:.0e}], 40% R-MAT / 30% ER "
+                         "(degree 8-64) / 30% RGG (degree ~12), FF/LF/SL/GNN2/GNN3, GNN in BF16",
+                "bf16_poisson": dict(summary(rep2, wall2), rate_per_s=args.serving_rate)}
     rep, wall = run(srv, gcx.with_arrivals(base, None))
     cap = summary(rep, wall)
     rate = 0.7 * cap["graphs_per_s"]
@@ -313,6 +322,9 @@ def main():
     ap.add_argument("--serving-nmax", type=float, default=4e6)
     ap.add_argument("--serving-workers", type=int, default=8)
     ap.add_argument("--serving-budget-gb", type=float, default=48.0)
+    ap.add_argument("--serving-rate", type=float, default=0.0,
+                    help="run only a Poisson pass at this many requests/s (0 = capacity, "
+                         "70%% Poisson and plain FP32 passes)")
     ap.add_argument("--schedule", choices=["auto", "rounds", "ordered"], default="auto")
     args = ap.parse_args()
     if args.gpus != 1:
