@@ -1160,12 +1160,45 @@ static int jp_round(void *p, uint32_t r, hipStream_t s)
 }
 
 /* Orientation pass shared by both schedules. */
+/* The stream for hub work: the workspace's side stream, forked after what s
+ * has queued, when the graph has hubs and `fork` is set; else s. NULL on a
+ * HIP error. join_hubs makes s wait for it. */
+static hipStream_t fork_hubs(gcx_workspace *ws, const gcx_graph *g, int fork, hipStream_t s)
+{
+    if (!fork || !g->nhub)
+        return s;
+    if (hipEventRecord((hipEvent_t)ws->ev_fork, s) != hipSuccess ||
+        hipStreamWaitEvent((hipStream_t)ws->aux, (hipEvent_t)ws->ev_fork, 0) != hipSuccess)
+        return NULL;
+    return (hipStream_t)ws->aux;
+}
+
+static int join_hubs(gcx_workspace *ws, hipStream_t hs, hipStream_t s)
+{
+    if (hs == s)
+        return GCX_OK;
+    HIPC(hipEventRecord((hipEvent_t)ws->ev_join, hs));
+    HIPC(hipStreamWaitEvent(s, (hipEvent_t)ws->ev_join, 0));
+    return GCX_OK;
+}
+
+/* fork = 0 keeps everything on s (inside a captured graph). */
+static int jp_prepare_ex(gcx_workspace *ws, const gcx_graph *g, int mb, int fork, hipStream_t s)
+{
+    TRY(gcxk_jp_clear(g->N, g->M, ws->ovf, g->nhub, ws->hpos, g->hubs, ws->npred, ws->state,
+                      ws->color, ws->ctl, s));
+    hipStream_t hs = fork_hubs(ws, g, fork, s);
+    if (!hs)
+        return GCX_ERR_HIP;
+    TRY(gcxk_jp_init(g->N, g->V, g->E, ws->key, ws->key_bits, mb, g->perm, ws->npred, ws->esplit,
+                     ws->state, ws->color, g->hubs, g->nhub, g->hchunk, g->nhchunk, ws->hpos, s,
+                     hs));
+    return join_hubs(ws, hs, s);
+}
+
 static int jp_prepare(gcx_workspace *ws, const gcx_graph *g, int mb, hipStream_t s)
 {
-    TRY(gcxk_jp_clear(g->N, g->M, ws->ovf, g->nhub, ws->hpos, ws->ctl, s));
-    return gcxk_jp_init(g->N, g->V, g->E, ws->key, ws->key_bits, mb, g->perm, ws->npred,
-                        ws->esplit, ws->state, ws->color, g->hubs, g->nhub, g->hchunk, g->nhchunk,
-                        ws->hpos, s);
+    return jp_prepare_ex(ws, g, mb, 1, s);
 }
 
 static int color_rounds(gcx_workspace *ws, const gcx_graph *g, int mb, hipStream_t s)
@@ -1329,7 +1362,7 @@ static int recolor_pass(gcx_workspace *ws, const gcx_graph *g, int mb, uint32_t 
 {
     TRY(gcxk_culberson_keys(g->N, C, strategy, seed, ws->color, ws->pass, (uint32_t *)ws->y,
                             (uint32_t *)ws->key, s));
-    TRY(jp_prepare(ws, g, mb, s));
+    TRY(jp_prepare_ex(ws, g, mb, 0, s));
     TRY(gcxk_jp_roots(g->N, g->V, ws->npred, ws->items[0], ws->ctl, s));
     jp_ctx c = {ws, g, g_round_grid, mb};
     for (uint32_t r = 0; r < rounds; r++)
@@ -1443,7 +1476,11 @@ int gcx_validate(gcx_workspace *ws, const gcx_graph *g, void *stream, int *max_c
     hipStream_t s = (hipStream_t)gcx__pick(stream);
     gcxk_ctl *vc = ws->ctl + 1;
     HIPC(hipMemsetAsync(vc, 0, sizeof(gcxk_ctl), s));
-    TRY(gcxk_validate(g->N, g->V, g->E, ws->color, g->hubs, g->nhub, g->perm, vc, s));
+    hipStream_t hs = fork_hubs(ws, g, 1, s);
+    if (!hs)
+        return GCX_ERR_HIP;
+    TRY(gcxk_validate(g->N, g->V, g->E, ws->color, g->hubs, g->nhub, g->perm, vc, s, hs));
+    TRY(join_hubs(ws, hs, s));
     HIPC(hipMemcpyAsync(ws->hctl + 1, vc, sizeof(gcxk_ctl), hipMemcpyDeviceToHost, s));
     HIPC(hipStreamSynchronize(s));
     *max_color = ws->hctl[1].bad ? 0 : (ws->hctl[1].maxcolor > 0 ? ws->hctl[1].maxcolor : 1);

@@ -188,19 +188,21 @@ static inline uint64_t gcxk_jp_item_capacity(uint32_t N, uint64_t M)
  * rest of the word counts pending predecessors. */
 int gcxk_jp_mask_bits(uint32_t maxdeg);
 /* Zero the overflow colour words (gcxk_jp_ovf_words), the per-hub positions
- * (hpos: 2 per hub) and the control block. A plain kernel, so the sequence
- * can be captured in a graph. */
+ * (hpos: 2 per hub), the hubs' colour / npred / state and the control block.
+ * A plain kernel, so the sequence can be captured in a graph. */
 int gcxk_jp_clear(uint32_t N, uint64_t M, uint32_t *ovf, uint32_t nhub, uint32_t *hpos,
+                  const uint32_t *hubs, uint32_t *npred, uint64_t *state, int *color,
                   gcxk_ctl *ctl, void *stream);
 /* Orientation: key_bits = 32 or 64, mb = gcxk_jp_mask_bits(maxdeg), perm =
  * the graph's degree order (hubs first). E_split rows = predecessors then
  * successors; npred = predecessor counts; state = npred << mb; color = 1 for
  * degree-0 vertices, else 0. hchunk: per-hub prefix of 4096-edge chunks
- * (nhub + 1 entries), nhchunk = total chunks. */
+ * (nhub + 1 entries), nhchunk = total chunks. The hub chunks run on
+ * hub_stream (may equal stream), after gcxk_jp_clear; the caller joins. */
 int gcxk_jp_init(uint32_t N, const uint32_t *dV, const uint32_t *dE, const void *key,
                  int key_bits, int mb, const uint32_t *perm, uint32_t *npred, uint32_t *esplit,
                  uint64_t *state, int *color, const uint32_t *hubs, uint32_t nhub,
-                 const uint32_t *hchunk, uint32_t nhchunk, uint32_t *hpos, void *stream);
+                 const uint32_t *hchunk, uint32_t nhchunk, uint32_t *hpos, void *stream, void *hub_stream);
 /* Round schedule: queue the roots (vertices with successors and no
  * predecessors) as round-0 items in q0 / ctl->qn[0]. */
 int gcxk_jp_roots(uint32_t N, const uint32_t *dV, const uint32_t *npred, gcxk_item *q0,
@@ -241,10 +243,11 @@ uint32_t gcxk_jp_ordered_cap(void);
 /* Degrees of the listed vertices (for the hub chunk table). */
 int gcxk_gather_degrees(const uint32_t *dV, const uint32_t *list, uint32_t n, uint32_t *out,
                         void *stream);
-/* Proper-colouring check: ctl->bad, ctl->maxcolor (ctl fields zeroed by caller). */
+/* Proper-colouring check: ctl->bad, ctl->maxcolor (ctl fields zeroed by
+ * caller). Hub rows run on hub_stream (may equal stream); the caller joins. */
 int gcxk_validate(uint32_t N, const uint32_t *dV, const uint32_t *dE, const int *color,
                   const uint32_t *hubs, uint32_t nhub, const uint32_t *perm, gcxk_ctl *ctl,
-                  void *stream);
+                  void *stream, void *hub_stream);
 /* Greedy-rule certificate: ctl->viol. */
 int gcxk_mex_check(uint32_t N, const uint32_t *dV, const uint32_t *dE, const void *key,
                    int key_bits, const int *color, const uint32_t *hubs, uint32_t nhub,

@@ -149,7 +149,10 @@ __device__ __forceinline__ void push_items(bool want, uint32_t v, uint32_t cnt, 
 // A plain kernel, so the sequence can be captured in a graph.
 __global__ __launch_bounds__(kBlock) void jp_clear(uint64_t nwords, uint32_t *__restrict__ ovf,
                                                    uint32_t nhub, uint32_t *__restrict__ hpos,
-                                                   gcxk_ctl *ctl)
+                                                   const uint32_t *__restrict__ hubs,
+                                                   uint32_t *__restrict__ npred,
+                                                   uint64_t *__restrict__ state,
+                                                   int *__restrict__ color, gcxk_ctl *ctl)
 {
     const uint64_t stride = (uint64_t)gridDim.x * kBlock;
     const uint64_t t0 = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -157,15 +160,23 @@ __global__ __launch_bounds__(kBlock) void jp_clear(uint64_t nwords, uint32_t *__
         ovf[i] = 0u;
     for (uint64_t h = t0; h < 2ull * nhub; h += stride)
         hpos[h] = 0u;
+    // the hubs' counts start at zero: jp_init_hub_chunks adds into them,
+    // possibly while jp_init_quads runs on another stream
+    for (uint64_t h = t0; h < nhub; h += stride) {
+        const uint32_t u = hubs[h];
+        color[u] = 0;
+        npred[u] = 0u;
+        state[u] = 0ull;
+    }
     if (t0 < sizeof(gcxk_ctl) / 4)
         reinterpret_cast<uint32_t *>(ctl)[t0] = 0u;
 }
 
 // Orientation in degree order (perm), 4 lanes per vertex so a wave's 16
-// vertices have near-equal degrees. Also writes every vertex's colour slot
-// (1 for degree 0, else 0), npred and state[u] = npred << mb. The hubs
-// (perm[0 .. nhub)) get zeros here; jp_init_hub_chunks splits their rows and
-// adds their counts.
+// vertices have near-equal degrees. Also writes every non-hub vertex's colour
+// slot (1 for degree 0, else 0), npred and state[u] = npred << mb. The hubs
+// (perm[0 .. nhub)) are left alone: jp_clear zeroed them and
+// jp_init_hub_chunks splits their rows and adds their counts.
 constexpr int kQuad = 4;
 constexpr int kQuadSteps = 4; // 16 edges in flight per vertex
 
@@ -186,16 +197,10 @@ __global__ __launch_bounds__(kBlock) void jp_init_quads(uint32_t N, const uint32
     const int gsh = (lane_id() / kQuad) * kQuad; // first lane of the quad
     if (pi >= N) // whole quads leave together
         return;
+    if (pi < nhub)
+        return;
     const uint32_t u = perm[pi];
     const uint32_t beg = V[u], end = V[u + 1];
-    if (pi < nhub) {
-        if (gl == 0) {
-            color[u] = 0;
-            npred[u] = 0;
-            state[u] = 0ull;
-        }
-        return;
-    }
     uint32_t np = 0, ns = 0;
     if (end > beg) {
         const K ku = key[u];
@@ -828,6 +833,7 @@ extern "C" int gcxk_jp_mask_bits(uint32_t maxdeg)
 }
 
 extern "C" int gcxk_jp_clear(uint32_t N, uint64_t M, uint32_t *ovf, uint32_t nhub, uint32_t *hpos,
+                             const uint32_t *hubs, uint32_t *npred, uint64_t *state, int *color,
                              gcxk_ctl *ctl, void *stream)
 {
     const uint64_t nwords = gcxk_jp_ovf_words(N, M);
@@ -837,7 +843,8 @@ extern "C" int gcxk_jp_clear(uint32_t N, uint64_t M, uint32_t *ovf, uint32_t nhu
     uint32_t grid = div_up(n, kBlock);
     if (grid > 4096)
         grid = 4096;
-    jp_clear<<<grid, kBlock, 0, as_stream(stream)>>>(nwords, ovf, nhub, hpos, ctl);
+    jp_clear<<<grid, kBlock, 0, as_stream(stream)>>>(nwords, ovf, nhub, hpos, hubs, npred, state,
+                                                     color, ctl);
     GCXK_LAUNCH_CHECK();
     return GCX_OK;
 }
@@ -846,23 +853,23 @@ extern "C" int gcxk_jp_init(uint32_t N, const uint32_t *dV, const uint32_t *dE, 
                             int key_bits, int mb, const uint32_t *perm, uint32_t *npred,
                             uint32_t *esplit, uint64_t *state, int *color, const uint32_t *hubs,
                             uint32_t nhub, const uint32_t *hchunk, uint32_t nhchunk,
-                            uint32_t *hpos, void *stream)
+                            uint32_t *hpos, void *stream, void *hub_stream)
 {
     if (N == 0)
         return GCX_OK;
-    hipStream_t s = as_stream(stream);
+    hipStream_t s = as_stream(stream), hs = as_stream(hub_stream);
     const uint32_t grid = div_up((uint64_t)N * kQuad, kBlock);
     if (key_bits == 32) {
         jp_init_quads<uint32_t><<<grid, kBlock, 0, s>>>(N, dV, dE, (const uint32_t *)key, perm,
                                                         nhub, npred, esplit, state, color, mb);
         if (nhub)
-            jp_init_hub_chunks<uint32_t><<<nhchunk, kBlock, 0, s>>>(
+            jp_init_hub_chunks<uint32_t><<<nhchunk, kBlock, 0, hs>>>(
                 dV, dE, (const uint32_t *)key, npred, state, esplit, hubs, hchunk, nhub, hpos, mb);
     } else {
         jp_init_quads<uint64_t><<<grid, kBlock, 0, s>>>(N, dV, dE, (const uint64_t *)key, perm,
                                                         nhub, npred, esplit, state, color, mb);
         if (nhub)
-            jp_init_hub_chunks<uint64_t><<<nhchunk, kBlock, 0, s>>>(
+            jp_init_hub_chunks<uint64_t><<<nhchunk, kBlock, 0, hs>>>(
                 dV, dE, (const uint64_t *)key, npred, state, esplit, hubs, hchunk, nhub, hpos, mb);
     }
     GCXK_LAUNCH_CHECK();
@@ -1021,7 +1028,7 @@ extern "C" int gcxk_jp_ordered(const uint32_t *dV, const uint32_t *npred, const 
 
 extern "C" int gcxk_validate(uint32_t N, const uint32_t *dV, const uint32_t *dE, const int *color,
                              const uint32_t *hubs, uint32_t nhub, const uint32_t *perm,
-                             gcxk_ctl *ctl, void *stream)
+                             gcxk_ctl *ctl, void *stream, void *hub_stream)
 {
     if (N == 0)
         return GCX_OK;
@@ -1030,7 +1037,7 @@ extern "C" int gcxk_validate(uint32_t N, const uint32_t *dV, const uint32_t *dE,
         grid = 2048;
     validate_quads<<<grid, kBlock, 0, as_stream(stream)>>>(N, dV, dE, color, perm, nhub, ctl);
     if (nhub)
-        validate_hubs<<<nhub, kBlock, 0, as_stream(stream)>>>(dV, dE, color, hubs, ctl);
+        validate_hubs<<<nhub, kBlock, 0, as_stream(hub_stream)>>>(dV, dE, color, hubs, ctl);
     GCXK_LAUNCH_CHECK();
     return GCX_OK;
 }
