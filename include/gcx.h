@@ -168,7 +168,7 @@ int gcx_color(gcx_workspace *ws, const gcx_graph *g, void *stream);
 /* JP schedule; all give the same colours.
  *   ROUNDS:  one launch per DAG level over a compacted frontier.
  *   ORDERED: work items sorted in greedy order, then one persistent launch
- *            of `param` workgroups (0 = 4 per CU; at most as many as are
+ *            of `param` workgroups (0 = 6 per CU; at most as many as are
  *            resident at once) in which each wave takes a fixed stride of
  *            the items and waits on their predecessor counts. Concurrent
  *            ordered launches follow gcx_set_ordered_launch. If a wait

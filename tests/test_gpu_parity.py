@@ -293,3 +293,32 @@ def test_reference_csr_utilities():
     np.testing.assert_array_equal(E2, c["E"])
     assert gcx.ref.graph_validate_par(V2, E2) == 1
     assert gcx.ref.graph_validate_par(V, Es) == 0
+
+
+@pytest.mark.parametrize("sched", [gcx.SCHED_ROUNDS, gcx.SCHED_ORDERED])
+@pytest.mark.parametrize("name", ["rmat22", "stencil160"])
+def test_full_size_digests(name, sched, digests, gmodels):
+    """The two full-size graphs (R-MAT scale 22; 160^3 27-point stencil with
+    permuted ids): CSR, priorities and colours of all 8 heuristics against
+    the reference's digests, under both JP schedules."""
+    rec = digests[name]
+    gd = rec["graph"]
+    if gd["kind"] == "rmat":
+        g = gcx.Graph.rmat(gd["scale"], gd["edge_factor"], *gd["abc"], seed=gd["seed"])
+    else:
+        N, I, J = gcx.stencil27_pairs(gd["g"], seed=gd["seed"])
+        g = gcx.Graph.from_pairs(N, I, J)
+        del I, J
+    assert (g.N, g.M) == (rec["N"], rec["M"])
+    V, E = g.download()
+    assert sha(V) == rec["sha_V"] and sha(E) == rec["sha_E"]
+    del V, E
+    ws = gcx.Workspace(g.N, g.M)
+    ws.set_schedule(sched)
+    for h in HEUR:
+        order(ws, g, h, gmodels)
+        assert sha(ws.priorities(g.N)) == rec[h]["sha_p"], h
+        ws.color(g)
+        assert not ws.fell_back(), h
+        assert sha(ws.colors(g.N)) == rec[h]["sha_color"], h
+        assert ws.validate(g) == rec[h]["ncolors"], h
