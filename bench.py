@@ -148,9 +148,10 @@ def serving_rmat(args, gcx):
                                         arrival=t))
         return out
 
-    def measure(workers, precision, rate=None):
+    def measure(workers, precision, rate=None, batch=0):
         srv = gcx.Server(workers=workers, budget=int(args.serving_budget_gb * (1 << 30)))
         ids = [srv.add_graph(g) for g in gs]
+        srv.set_batching(batch)
         srv.run(jobs(ids, precision)[:min(k, 2 * workers)])  # warm the pool and the kernels
         reqs = jobs(ids, precision, rate)
         t0 = time.perf_counter()
@@ -159,7 +160,9 @@ def serving_rmat(args, gcx):
         srv.close()
         ok = [r for r in rep if r.status == 0]
         lat = np.array([1000.0 * (r.t_done - r.t_arrival) for r in ok])
-        return rep, {"jobs": k, "ok": len(ok), "workers": workers, "seconds": wall,
+        return rep, {"jobs": k, "ok": len(ok), "workers": workers, "batch": batch,
+                     "mean_batch": float(np.mean([r.batch for r in ok])) if ok else 0.0,
+                     "seconds": wall,
                      "graphs_per_s": len(ok) / wall,
                      "colored_edges_per_s": float(sum(r.M for r in ok)) / wall,
                      "p50_ms": float(np.percentile(lat, 50)),
@@ -177,6 +180,9 @@ def serving_rmat(args, gcx):
     bf_rep, bf = sweep[best]
     lat_rep, lat = measure(args.serving_workers, gcx.BF16, rate=0.7 * bf["graphs_per_s"])
     x3_rep, x3 = measure(args.serving_workers, gcx.BF16X3)
+    batched = {}
+    for b in args.serving_batch:
+        batched[b] = measure(args.serving_workers, gcx.BF16, batch=b)
     gcx.set_ordered_launch(prev)
     fp32_col = {r.digest: r.colors for r in plain_rep}
     per_graph = {}
@@ -186,6 +192,9 @@ def serving_rmat(args, gcx):
     same = all(len(v) == 1 for v in per_graph.values())  # every job on a graph gives one result
     delta = [next(iter(v))[1] - next(iter(v))[0] for v in per_graph.values()]
     delta3 = [next(iter(v))[2] - next(iter(v))[0] for v in per_graph.values()]
+    batched_same = all((a.colors, a.digest) == (b.colors, b.digest)
+                       for rb, _ in batched.values() for a, b in zip(bf_rep, rb))
+    best_b = max(batched, key=lambda b: batched[b][1]["colored_edges_per_s"]) if batched else None
     for g in gs:
         g.free()
     return {"graphs": f"R-MAT scale {args.scale} (edge factor 16, a/b/c .57/.19/.19), seeds 1-4, "
@@ -197,6 +206,10 @@ def serving_rmat(args, gcx):
             "bf16_poisson_70pct": lat,
             "speedup_vs_plain_fp32": bf["colored_edges_per_s"] / plain["colored_edges_per_s"],
             "bf16x3_capacity": x3,
+            "bf16_batched": {str(b): v[1] for b, v in batched.items()},
+            "bf16_batched_same_results": batched_same,
+            "bf16_batched_speedup_vs_plain_fp32": (batched[best_b][1]["colored_edges_per_s"]
+                                                   / plain["colored_edges_per_s"]) if best_b else None,
             "bf16x3_speedup_vs_plain_fp32": x3["colored_edges_per_s"] / plain["colored_edges_per_s"],
             "bf16x3_minus_fp32_colors": delta3,
             "same_result_every_job_on_a_graph": same,
@@ -329,6 +342,8 @@ def main():
     ap.add_argument("--batch-graphs", type=int, default=512)
     ap.add_argument("--culberson-passes", type=int, default=64)
     ap.add_argument("--ordered-slots", type=int, nargs="+", default=[0, 1, 2, 4])
+    ap.add_argument("--serving-batch", type=int, nargs="*", default=[4, 8],
+                    help="continuous-batching sizes for the resident R-MAT serving leg")
     ap.add_argument("--serving-nmax", type=float, default=4e6)
     ap.add_argument("--serving-workers", type=int, default=8)
     ap.add_argument("--serving-budget-gb", type=float, default=48.0)

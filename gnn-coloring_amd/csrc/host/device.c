@@ -347,6 +347,7 @@ int gcx_graph_free_on(gcx_graph *g, void *stream)
     void *bufs[] = {g->V, g->E, g->hubs, g->hchunk, g->perm, g->rank, g->Er, g->fidv, g->goff};
     for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); i++)
         gcx__dfree(bufs[i], stream);
+    free(g->hgoff);
     free(g);
     return GCX_OK;
 }
@@ -573,6 +574,8 @@ int gcx_graph_pack(const gcx_graph *const *graphs, uint32_t count, void *stream,
     if (rc == GCX_OK && hipMemcpyAsync(g->goff, off, 4ull * ((size_t)count + 1),
                                        hipMemcpyHostToDevice, s) != hipSuccess)
         rc = GCX_ERR_HIP;
+    if (rc == GCX_OK)
+        g->hgoff = off; /* the graph keeps the host copy */
     uint64_t eoff = 0;
     for (uint32_t i = 0; rc == GCX_OK && i < count; i++) {
         const gcx_graph *h = graphs[i];
@@ -584,11 +587,12 @@ int gcx_graph_pack(const gcx_graph *const *graphs, uint32_t count, void *stream,
         rc = gcxk_pack_close((uint32_t)N, (uint32_t)M, g->V, s);
     if (rc == GCX_OK) {
         g->count = count;
-        rc = graph_stats(g, s); /* synchronises: off may be freed */
+        rc = graph_stats(g, s); /* synchronises */
     } else {
         (void)hipStreamSynchronize(s);
     }
-    free(off);
+    if (!g || g->hgoff != off)
+        free(off);
     if (rc != GCX_OK) {
         gcx_graph_free_on(g, s);
         return rc;
@@ -626,18 +630,41 @@ int gcx_packed_colors(gcx_workspace *ws, const gcx_graph *g, int *colors, uint64
         return GCX_ERR_ARG;
     hipStream_t s = (hipStream_t)gcx__pick(stream);
     const size_t n = g->count;
+    /* vertex ranges of at most 64K vertices, each inside one graph */
+    const uint32_t span = 1u << 16;
+    size_t nr = 0;
+    for (size_t k = 0; k < n; k++)
+        nr += (g->hgoff[k + 1] - g->hgoff[k] + span - 1) / span;
+    uint32_t *hr = malloc(sizeof(uint32_t) * 3 * (nr ? nr : 1));
+    if (!hr)
+        return GCX_ERR_NOMEM;
+    size_t r = 0;
+    for (size_t k = 0; k < n; k++)
+        for (uint32_t b = g->hgoff[k]; b < g->hgoff[k + 1]; b += span, r++) {
+            hr[r] = b;
+            hr[nr + r] = g->hgoff[k + 1] - b < span ? g->hgoff[k + 1] : b + span;
+            hr[2 * nr + r] = (uint32_t)k;
+        }
     void *dbuf = NULL;
-    TRY(gcx__dalloc(&dbuf, 12 * n + 8, s));
+    int rc = gcx__dalloc(&dbuf, 12 * n + 12 * nr + 16, s);
+    if (rc != GCX_OK) {
+        free(hr);
+        return rc;
+    }
     uint64_t *dd = (uint64_t *)dbuf;
     int *dc = (int *)(dd + n);
-    int rc = GCX_ERR_HIP;
+    uint32_t *dr = (uint32_t *)(dc + n);
+    rc = GCX_ERR_HIP;
     if (hipMemsetAsync(dbuf, 0, 12 * n, s) == hipSuccess &&
-        gcxk_packed_stats(g->N, g->count, g->goff, ws->color, dc, dd, s) == GCX_OK &&
+        (nr == 0 || hipMemcpyAsync(dr, hr, 12 * nr, hipMemcpyHostToDevice, s) == hipSuccess) &&
+        gcxk_packed_stats((uint32_t)nr, dr, dr + nr, dr + 2 * nr, g->goff, ws->color, dc, dd, s) ==
+            GCX_OK &&
         hipMemcpyAsync(colors, dc, 4 * n, hipMemcpyDeviceToHost, s) == hipSuccess &&
         hipMemcpyAsync(digests, dd, 8 * n, hipMemcpyDeviceToHost, s) == hipSuccess &&
         hipStreamSynchronize(s) == hipSuccess)
         rc = GCX_OK;
     gcx__dfree(dbuf, s);
+    free(hr);
     return rc;
 }
 

@@ -25,6 +25,7 @@ struct gcx_graph {
     uint32_t *Er;     /* device, M: rank[E[j]] (GNN rows live at degree-order positions) */
     float *fidv;      /* device, N, or NULL: GNN id feature per vertex (batches) */
     uint32_t *goff;   /* device, count + 1, or NULL: first vertex of each packed graph */
+    uint32_t *hgoff;  /* host copy of goff (malloc'ed), or NULL */
     uint32_t count;   /* packed graphs (0 = a single graph) */
 };
 

@@ -35,6 +35,7 @@ struct gcx_server {
     gcx_model *model[5]; /* by layer count 2..4 */
     const gcx_graph **resident;
     uint32_t nresident, resident_cap;
+    int max_batch; /* continuous batching of resident-graph jobs (<= 1: off) */
     pthread_mutex_t lock;
     pthread_cond_t cv;      /* queue / memory changed */
     pthread_cond_t done_cv; /* a job finished */
@@ -172,6 +173,69 @@ static void run_job(worker *w, const gcx_request *r, gcx_reply *out)
         (void)gcx_graph_free_on(g, s);
 }
 
+/* Jobs that can share one packed pass: resident graphs, same ordering. */
+static int batchable(const gcx_server *S, const gcx_request *a, const gcx_request *b)
+{
+    return resident_graph(S, a) && resident_graph(S, b) && a->heuristic == b->heuristic &&
+           (a->heuristic != GCX_GNN || (a->layers == b->layers && a->precision == b->precision)) &&
+           a->schedule == b->schedule;
+}
+
+/* HBM of one packed pass over graphs with nv vertices and ne entries in
+ * total: the pack and its workspace. */
+static uint64_t pack_footprint(uint64_t nv, uint64_t ne)
+{
+    return gcx_graph_footprint((uint32_t)nv, ne) + gcx_workspace_footprint((uint32_t)nv, ne);
+}
+
+#define MAX_BATCH 64
+
+/* One pass over the packed graphs of reqs[idx[0..n)]. */
+static void run_batch(worker *w, const size_t *idx, int n)
+{
+    gcx_server *S = w->srv;
+    hipStream_t s = w->stream;
+    const gcx_graph *gs[MAX_BATCH];
+    for (int k = 0; k < n; k++)
+        gs[k] = resident_graph(S, &S->req[idx[k]]);
+    const gcx_request *r = &S->req[idx[0]];
+    gcx_graph *pack = NULL;
+    gcx_workspace *ws = NULL;
+    int colors[MAX_BATCH];
+    uint64_t digests[MAX_BATCH];
+    int rc = gcx_graph_pack(gs, (uint32_t)n, s, &pack);
+    if (rc == GCX_OK)
+        rc = gcx_workspace_create_on(pack->N, pack->M, s, &ws);
+    if (rc == GCX_OK)
+        rc = gcx_set_schedule(ws, r->schedule, 0);
+    if (rc == GCX_OK) {
+        const gcx_model *m = NULL;
+        if (r->heuristic == GCX_GNN)
+            m = (r->layers >= 2 && r->layers <= 4) ? S->model[r->layers] : NULL;
+        rc = (r->heuristic == GCX_GNN && !m) ? GCX_ERR_ARG
+                                             : gcx_order(ws, pack, r->heuristic, m, r->precision, s);
+    }
+    if (rc == GCX_OK)
+        rc = gcx_color(ws, pack, s);
+    int proper = 0;
+    if (rc == GCX_OK)
+        rc = gcx_validate(ws, pack, s, &proper);
+    if (rc == GCX_OK)
+        rc = gcx_packed_colors(ws, pack, colors, digests, s);
+    for (int k = 0; k < n; k++) {
+        gcx_reply *out = &S->rep[idx[k]];
+        out->status = rc;
+        out->batch = n;
+        out->N = gs[k]->N;
+        out->M = gs[k]->M;
+        out->colors = rc == GCX_OK && proper ? colors[k] : 0;
+        out->digest = rc == GCX_OK ? digests[k] : 0;
+        out->worker = w->id;
+    }
+    (void)gcx_workspace_free_on(ws, s);
+    (void)gcx_graph_free_on(pack, s);
+}
+
 static int head_fits(const gcx_server *S)
 {
     if (S->head >= S->tail)
@@ -191,19 +255,55 @@ static void *worker_main(void *arg)
             break;
         const size_t i = S->queue[S->head++];
         gcx_reply *rp = &S->rep[i];
-        S->in_use += rp->footprint;
+        size_t idx[MAX_BATCH];
+        int n = 1;
+        idx[0] = i;
+        uint64_t reserve = rp->footprint;
+        if (S->max_batch > 1 && resident_graph(S, &S->req[i])) {
+            /* take the compatible jobs queued right behind it while the
+             * packed pass fits the budget */
+            const gcx_graph *g0 = resident_graph(S, &S->req[i]);
+            uint64_t nv = g0->N, ne = g0->M;
+            while (n < S->max_batch && n < MAX_BATCH && S->head < S->tail) {
+                const size_t j = S->queue[S->head];
+                const gcx_graph *gj = resident_graph(S, &S->req[j]);
+                if (!batchable(S, &S->req[i], &S->req[j]) || nv + gj->N >= UINT32_MAX ||
+                    ne + gj->M >= UINT32_MAX ||
+                    S->in_use + pack_footprint(nv + gj->N, ne + gj->M) > S->budget)
+                    break;
+                nv += gj->N;
+                ne += gj->M;
+                idx[n++] = j;
+                S->head++;
+            }
+            if (n > 1)
+                reserve = pack_footprint(nv, ne);
+        }
+        S->in_use += reserve;
         if (S->in_use > S->peak)
             S->peak = S->in_use;
-        rp->t_admit = since(&S->t0);
+        const double t_admit = since(&S->t0);
+        for (int k = 0; k < n; k++) {
+            S->rep[idx[k]].t_admit = t_admit;
+            if (n > 1)
+                S->rep[idx[k]].footprint = reserve;
+        }
         pthread_mutex_unlock(&S->lock);
-        run_job(w, &S->req[i], rp);
+        if (n > 1) {
+            run_batch(w, idx, n);
+        } else {
+            run_job(w, &S->req[i], rp);
+            rp->batch = 1;
+        }
         /* the job's frees are ordered on this stream; wait for them so the
          * budget never counts memory the pool has not got back */
         (void)hipStreamSynchronize(w->stream);
         pthread_mutex_lock(&S->lock);
-        S->in_use -= rp->footprint;
-        rp->t_done = since(&S->t0);
-        S->done++;
+        S->in_use -= reserve;
+        const double t_done = since(&S->t0);
+        for (int k = 0; k < n; k++)
+            S->rep[idx[k]].t_done = t_done;
+        S->done += (size_t)n;
         pthread_cond_broadcast(&S->cv);
         pthread_cond_broadcast(&S->done_cv);
     }
@@ -352,6 +452,18 @@ int gcx_server_add_graph(gcx_server *S, const gcx_graph *g, uint32_t *id)
 }
 
 uint64_t gcx_server_peak_reserved(const gcx_server *S) { return S ? S->peak : 0; }
+
+int gcx_server_set_batching(gcx_server *S, int max_batch)
+{
+    if (!S || max_batch < 0 || max_batch > MAX_BATCH)
+        return GCX_ERR_ARG;
+    pthread_mutex_lock(&S->lock);
+    const int active = S->req != NULL;
+    if (!active)
+        S->max_batch = max_batch;
+    pthread_mutex_unlock(&S->lock);
+    return active ? GCX_ERR_ARG : GCX_OK;
+}
 
 int gcx_server_destroy(gcx_server *S)
 {

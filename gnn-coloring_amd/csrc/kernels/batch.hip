@@ -40,26 +40,51 @@ __global__ void pack_close(uint32_t N, uint32_t M, uint32_t *__restrict__ Vout)
 
 // Per packed graph: the largest colour and the colouring digest over
 // graph-local ids (the digest gcx_color_digest gives for the graph alone).
-__global__ __launch_bounds__(kBlock) void packed_stats(uint32_t N, uint32_t count,
+// Block b reduces the vertex range [beg[b], end[b]) of graph gid[b] and
+// adds one partial per counter, so a large graph costs one atomic per
+// range, not per vertex.
+__global__ __launch_bounds__(kBlock) void packed_stats(const uint32_t *__restrict__ rbeg,
+                                                       const uint32_t *__restrict__ rend,
+                                                       const uint32_t *__restrict__ rgid,
                                                        const uint32_t *__restrict__ goff,
                                                        const int *__restrict__ color,
                                                        int *__restrict__ maxc,
                                                        unsigned long long *__restrict__ digest)
 {
-    const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
-    if (v >= N)
-        return;
-    uint32_t lo = 0, hi = count; // last g with goff[g] <= v
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) / 2;
-        if (goff[mid] <= v)
-            lo = mid;
-        else
-            hi = mid;
+    __shared__ int smax[kBlock / kWave];
+    __shared__ unsigned long long ssum[kBlock / kWave];
+    const uint32_t beg = rbeg[blockIdx.x], end = rend[blockIdx.x], g = rgid[blockIdx.x];
+    const uint32_t base = goff[g];
+    int m = 0;
+    uint64_t h = 0;
+    for (uint32_t v = beg + threadIdx.x; v < end; v += kBlock) {
+        const int c = color[v];
+        m = c > m ? c : m;
+        h += mix64(((uint64_t)(v - base) << 32) | (uint32_t)c);
     }
-    const int c = color[v];
-    atomicMax(&maxc[lo], c);
-    atomicAdd(&digest[lo], (unsigned long long)mix64(((uint64_t)(v - goff[lo]) << 32) | (uint32_t)c));
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        const int mo = __shfl_xor(m, off);
+        m = mo > m ? mo : m;
+        const uint32_t lo = __shfl_xor((uint32_t)h, off), hi = __shfl_xor((uint32_t)(h >> 32), off);
+        h += ((uint64_t)hi << 32) | lo;
+    }
+    const int w = threadIdx.x / kWave;
+    if (lane_id() == 0) {
+        smax[w] = m;
+        ssum[w] = h;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int bm = 0;
+        unsigned long long bs = 0;
+        for (int i = 0; i < kBlock / kWave; i++) {
+            bm = smax[i] > bm ? smax[i] : bm;
+            bs += ssum[i];
+        }
+        atomicMax(&maxc[g], bm);
+        atomicAdd(&digest[g], bs);
+    }
 }
 
 } // namespace
@@ -84,13 +109,14 @@ extern "C" int gcxk_pack_close(uint32_t N, uint32_t M, uint32_t *Vout, void *str
     return GCX_OK;
 }
 
-extern "C" int gcxk_packed_stats(uint32_t N, uint32_t count, const uint32_t *goff, const int *color,
+extern "C" int gcxk_packed_stats(uint32_t nranges, const uint32_t *rbeg, const uint32_t *rend,
+                                 const uint32_t *rgid, const uint32_t *goff, const int *color,
                                  int *maxc, uint64_t *digest, void *stream)
 {
-    if (N == 0)
+    if (nranges == 0)
         return GCX_OK;
-    packed_stats<<<div_up(N, kBlock), kBlock, 0, as_stream(stream)>>>(
-        N, count, goff, color, maxc, reinterpret_cast<unsigned long long *>(digest));
+    packed_stats<<<nranges, kBlock, 0, as_stream(stream)>>>(
+        rbeg, rend, rgid, goff, color, maxc, reinterpret_cast<unsigned long long *>(digest));
     GCXK_LAUNCH_CHECK();
     return GCX_OK;
 }

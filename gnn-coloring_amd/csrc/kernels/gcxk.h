@@ -89,8 +89,10 @@ int gcxk_pack_graph(uint32_t n, uint64_t m, const uint32_t *dV, const uint32_t *
 int gcxk_pack_close(uint32_t N, uint32_t M, uint32_t *Vout, void *stream);
 /* Per packed graph (goff: count + 1 first vertices): largest colour into
  * maxc and the colouring digest over graph-local ids into digest (both
- * zeroed by the caller). */
-int gcxk_packed_stats(uint32_t N, uint32_t count, const uint32_t *goff, const int *color, int *maxc,
+ * zeroed by the caller). The vertices come as nranges ranges
+ * [rbeg[i], rend[i]) each inside graph rgid[i]; one workgroup per range. */
+int gcxk_packed_stats(uint32_t nranges, const uint32_t *rbeg, const uint32_t *rend,
+                      const uint32_t *rgid, const uint32_t *goff, const int *color, int *maxc,
                       uint64_t *digest, void *stream);
 
 /* ---- ordering keys (keys.hip); key32: larger = earlier ---- */

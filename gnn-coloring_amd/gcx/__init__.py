@@ -98,6 +98,7 @@ def _load():
         "gcx_request_footprint": (C.c_uint64, [vp]),
         "gcx_server_peak_reserved": (C.c_uint64, [vp]),
         "gcx_server_add_graph": (C.c_int, [vp, vp, u32p]),
+        "gcx_server_set_batching": (C.c_int, [vp, C.c_int]),
         "gcx_set_ordered_launch": (C.c_int, [C.c_int]),
         "gcx_degree_signature": (C.c_uint32, [vp]),
         "gcx_tune": (C.c_int, [vp, vp, C.c_int, vp, vp]),
@@ -738,7 +739,8 @@ class Request(C.Structure):
 
 class Reply(C.Structure):
     """gcx_reply (gcx_serve.h)."""
-    _fields_ = [("status", C.c_int), ("N", C.c_uint32), ("M", C.c_uint64), ("colors", C.c_int),
+    _fields_ = [("status", C.c_int), ("batch", C.c_int), ("N", C.c_uint32), ("M", C.c_uint64),
+                ("colors", C.c_int),
                 ("digest", C.c_uint64), ("footprint", C.c_uint64), ("worker", C.c_int),
                 ("t_arrival", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double)]
 
@@ -818,6 +820,11 @@ class Server:
         _chk(_lib.gcx_server_add_graph(self.handle, g.handle, C.byref(gid)), "server add graph")
         self._resident.append(g)
         return int(gid.value)
+
+    def set_batching(self, max_batch: int):
+        """Pack up to max_batch compatible queued resident-graph jobs into one
+        pass (0 or 1 = off)."""
+        _chk(_lib.gcx_server_set_batching(self.handle, max_batch), "server batching")
 
     def run(self, requests):
         """Run requests (sorted by arrival); returns the list of Reply."""

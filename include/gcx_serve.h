@@ -46,6 +46,7 @@ typedef struct {
 
 typedef struct {
     int status;         /* GCX_OK, GCX_ERR_BUDGET (never fits), or an error */
+    int batch;          /* jobs in the pass that served this one (1 = alone) */
     uint32_t N;
     uint64_t M;         /* CSR entries of the generated graph */
     int colors;         /* validated colour count (0 = invalid) */
@@ -76,6 +77,14 @@ int gcx_server_add_graph(gcx_server *s, const gcx_graph *g, uint32_t *id);
 uint64_t gcx_request_footprint(const gcx_request *r);
 /* Peak HBM bytes reserved by admitted jobs during the last run. */
 uint64_t gcx_server_peak_reserved(const gcx_server *s);
+/* Continuous batching (0 or 1 = off, the default): a worker that admits a
+ * request on a resident graph also takes up to max_batch - 1 more queued
+ * right behind it with the same heuristic, layers, precision and schedule
+ * (as budget allows), packs their graphs into one block-diagonal graph
+ * (gcx_graph_pack) and runs them as one ordering + colouring pass. Replies
+ * are those of the jobs alone (per-graph colour count and digest). Not while
+ * a run is active. */
+int gcx_server_set_batching(gcx_server *s, int max_batch);
 
 #ifdef __cplusplus
 }

@@ -137,3 +137,42 @@ def test_serve_resident_graphs():
     srv.close()
     for g in gs:
         assert g.check()
+
+
+def test_serve_batching_matches_alone():
+    """Continuous batching packs compatible queued jobs on resident graphs
+    into one pass; every reply equals the job run alone, and the packed
+    passes stay within the budget."""
+    gs = [gcx.Graph.rmat(13, 16, seed=21), gcx.Graph.er(6000, 60000, seed=22),
+          gcx.Graph.rgg(8000, float(np.sqrt(12.0 / (np.pi * 8000))), seed=23)]
+    reqs = []
+    for i in range(24):
+        h = ["GNN3", "GNN2", "LF", "SL"][(i // 6) % 4]   # runs of 6 compatible jobs
+        prec = gcx.BF16 if i % 12 < 6 else gcx.FP32
+        reqs.append(gcx.make_request(gcx.GEN_RESIDENT, i % 3, heuristic=h, precision=prec,
+                                     schedule=gcx.SCHED_ORDERED, arrival=0.0))
+    results = {}
+    for batching in (0, 4):
+        srv = gcx.Server(workers=2, budget=16 << 30)
+        ids = [srv.add_graph(g) for g in gs]
+        assert ids == [0, 1, 2]
+        srv.set_batching(batching)
+        out = srv.run(reqs)
+        assert all(r.status == 0 for r in out), [r.status for r in out]
+        assert srv.peak_reserved() <= 16 << 30
+        results[batching] = out
+        srv.close()
+    alone, batched = results[0], results[4]
+    assert all(r.batch == 1 for r in alone)
+    assert max(r.batch for r in batched) > 1  # some passes really were packed
+    for a, b in zip(alone, batched):
+        assert (a.N, a.M, a.colors, a.digest) == (b.N, b.M, b.colors, b.digest)
+    # a tight budget still admits every job (smaller or no batches)
+    srv = gcx.Server(workers=2, budget=2 * gcx.workspace_footprint(gs[0].N, gs[0].M)
+                     + 3 * gcx.graph_footprint(gs[0].N, gs[0].M))
+    for g in gs:
+        srv.add_graph(g)
+    srv.set_batching(8)
+    out = srv.run(reqs[:8])
+    assert [(r.colors, r.digest) for r in out] == [(r.colors, r.digest) for r in alone[:8]]
+    srv.close()
