@@ -121,3 +121,26 @@ def test_cli_help_runs_without_gpu():
     assert out.returncode == 0 and "Usage" in out.stdout
     out = subprocess.run([CLI, "-g", "/nonexistent.mtx"], capture_output=True, text=True, timeout=60)
     assert out.returncode == 1 and "Unable to open file" in out.stdout
+
+
+def test_tuner_load_rejects_bad_files(tmp_path):
+    """Tuner records load from text; malformed lines and missing files are
+    errors (no GPU needed)."""
+    import gcx
+    bad = tmp_path / "bad.txt"
+    bad.write_text("# signature hub_degree schedule order_ms color_ms\n12 0 1 1.0 2.0\n")
+    with pytest.raises(gcx.GcxError):
+        gcx.tuner_load(bad)
+    bad.write_text("12 1024 7 1.0 2.0\n")
+    with pytest.raises(gcx.GcxError):
+        gcx.tuner_load(bad)
+    with pytest.raises(gcx.GcxError):
+        gcx.tuner_load(tmp_path / "missing.txt")
+    gcx.tuner_reset()
+    good = tmp_path / "good.txt"
+    good.write_text("# header\n70000 256 1 0.5 0.25\n")
+    gcx.tuner_load(good)
+    out = tmp_path / "out.txt"
+    gcx.tuner_save(out)
+    assert out.read_text().splitlines()[1].split()[:3] == ["70000", "256", "1"]
+    gcx.tuner_reset()

@@ -137,3 +137,25 @@ def test_gr_parse_canonical(tmp_path):
     store("gr", tmp_path / "h.gr", V, E)
     n2, V2, E2, _ = parse("gr", tmp_path / "h.gr")
     assert np.array_equal(V, V2) and np.array_equal(E, E2)
+
+
+@pytest.mark.parametrize("name,text", [("adj", "AdjacencyGraph\n3\n4\n0\n2\n"),
+                                       ("metis", "3 2 0\n2 9\n1\n\n"),
+                                       ("metis", "2 1 0\n2\n")])
+def test_malformed_files_exit(tmp_path, name, text):
+    """Truncated or out-of-range input ends the process with the library's
+    error line (the reference reads past the end of the mapping instead)."""
+    import subprocess
+    import sys
+    p = tmp_path / f"bad.{name}"
+    p.write_text(text)
+    import os
+    tests = os.path.dirname(os.path.abspath(__file__))
+    paths = [tests, os.path.dirname(tests), os.path.join(os.path.dirname(tests), "gnn-coloring_amd")]
+    code = (f"import sys; sys.path[:0] = {paths!r}; "
+            f"import test_formats as t; t.parse({name!r}, {str(p)!r})")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, cwd=tests)
+    if text == "2 1 0\n2\n":  # a valid file: vertex 2 has an empty last line
+        assert r.returncode == 0, r.stderr
+    else:
+        assert r.returncode == 1 and "parse failed" in r.stderr, (r.returncode, r.stderr)
