@@ -5,8 +5,10 @@
  *   coloring -g graph.mtx [-v] [-q] [-f] [-c K] [-t T] tc1 .. tcn
  *
  * -q prints the colour counts "FF LF LLF SL SLL ID SDA SD GNN2 GNN3 GNN4";
- * the ID, SDA and SD orderings are not provided by this library and print 0
- * (the value the reference prints for orderings it skips under -f).
+ * under -f the LLF, SLL, ID, SDA and SD columns print 0, as in the
+ * reference. The ID, SDA and SD orderings are sequential walks on the host
+ * (incidence_degree.h); their priorities are coloured on the GPU like the
+ * others.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -17,6 +19,7 @@
 #include "fast_io.h"
 #include "first_fit.h"
 #include "gnn.h"
+#include "incidence_degree.h"
 #include "largest_degree_first.h"
 #include "smallest_degree_last.h"
 
@@ -97,7 +100,7 @@ int main(int argc, char **argv)
         void *jp = jones_plassmann_setup(N, V[N]);
         double *p = malloc(sizeof(double) * (N + 8ull));
         int *color = malloc(sizeof(int) * (N + 1ull));
-        int ff, lf, llf = 0, sl, sll = 0, gnn2, gnn3, gnn4;
+        int ff, lf, llf = 0, sl, sll = 0, id = 0, sda = 0, sd = 0, gnn2, gnn3, gnn4;
         first_fit_ordering_par(N, V, E, p);
         ff = chain(N, V, E, jp, p, color, culb);
         largest_degree_first_ordering(N, V, E, p);
@@ -115,11 +118,17 @@ int main(int argc, char **argv)
             llf = chain(N, V, E, jp, p, color, culb);
             smallest_log_degree_last_ordering(N, V, E, p);
             sll = chain(N, V, E, jp, p, color, culb);
+            incidence_degree_ordering(N, V, E, p);
+            id = chain(N, V, E, jp, p, color, culb);
+            saturation_degree_ordering_alt(N, V, E, p);
+            sda = chain(N, V, E, jp, p, color, culb);
+            saturation_degree_ordering(N, V, E, p);
+            sd = chain(N, V, E, jp, p, color, culb);
         }
         if (verbose)
             printf("FF LF LLF SL SLL ID SDA SD GNN2 GNN3 GNN4\n");
-        printf("%d %d %d %d %d %d %d %d %d %d %d\n", ff, lf, llf, sl, sll, 0, 0, 0, gnn2, gnn3,
-               gnn4);
+        printf("%d %d %d %d %d %d %d %d %d %d %d\n", ff, lf, llf, sl, sll, id, sda, sd, gnn2,
+               gnn3, gnn4);
         jones_plassmann_cleanup(jp);
         free(p);
         free(color);

@@ -147,6 +147,9 @@ def _load():
         "largest_log_degree_first_ordering": (None, [C.c_uint32, u32p, u32p, f64p]),
         "smallest_degree_last_ordering": (None, [C.c_uint32, u32p, u32p, f64p]),
         "smallest_log_degree_last_ordering": (None, [C.c_uint32, u32p, u32p, f64p]),
+        "incidence_degree_ordering": (None, [C.c_uint32, u32p, u32p, f64p]),
+        "saturation_degree_ordering": (None, [C.c_uint32, u32p, u32p, f64p]),
+        "saturation_degree_ordering_alt": (None, [C.c_uint32, u32p, u32p, f64p]),
         "first_fit_coloring_par": (i32p, [C.c_uint32, u32p, u32p]),
         "largest_degree_first_coloring_par": (i32p, [C.c_uint32, u32p, u32p]),
         "largest_log_degree_first_coloring_par": (i32p, [C.c_uint32, u32p, u32p]),
@@ -679,7 +682,10 @@ class ref:
         pp = p.ctypes.data_as(f64p)
         fn = {"FF": "first_fit_ordering_par", "LF": "largest_degree_first_ordering",
               "LLF": "largest_log_degree_first_ordering", "SL": "smallest_degree_last_ordering",
-              "SLL": "smallest_log_degree_last_ordering"}
+              "SLL": "smallest_log_degree_last_ordering",
+              # sequential heap walks on the host CPU (incidence_degree.h)
+              "ID": "incidence_degree_ordering", "SD": "saturation_degree_ordering",
+              "SDA": "saturation_degree_ordering_alt"}
         if name.startswith("GNN"):
             _lib.gnn_ordering(N, Vp, Ep, pp, int(name[3:]) if layers is None else layers)
         else:

@@ -312,6 +312,9 @@ class Ref:
             "largest_log_degree_first_ordering": (None, [C.c_uint32, u32p, u32p, f64p]),
             "smallest_degree_last_ordering": (None, [C.c_uint32, u32p, u32p, f64p]),
             "smallest_log_degree_last_ordering": (None, [C.c_uint32, u32p, u32p, f64p]),
+            "incidence_degree_ordering": (None, [C.c_uint32, u32p, u32p, f64p]),
+            "saturation_degree_ordering": (None, [C.c_uint32, u32p, u32p, f64p]),
+            "saturation_degree_ordering_alt": (None, [C.c_uint32, u32p, u32p, f64p]),
             "gnn_setup": (C.c_void_p, [C.c_uint32, C.c_uint32]),
             "gnn_cleanup": (None, [C.c_void_p]),
             "graph_sort_edges_par": (None, [C.c_uint32, u32p, u32p]),
@@ -359,7 +362,8 @@ class Ref:
         p = np.zeros(N + 8, np.float64)
         fn = {"FF": "first_fit_ordering_par", "LF": "largest_degree_first_ordering",
               "LLF": "largest_log_degree_first_ordering", "SL": "smallest_degree_last_ordering",
-              "SLL": "smallest_log_degree_last_ordering"}[name]
+              "SLL": "smallest_log_degree_last_ordering", "ID": "incidence_degree_ordering",
+              "SD": "saturation_degree_ordering", "SDA": "saturation_degree_ordering_alt"}[name]
         getattr(self.lib, fn)(N, Vp, Ep, p.ctypes.data_as(f64p))
         return p[:N]
 

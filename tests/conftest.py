@@ -13,8 +13,9 @@ for p in (ROOT, os.path.join(ROOT, "gnn-coloring_amd")):
         sys.path.insert(0, p)
 
 HEUR = ["FF", "LF", "LLF", "SL", "SLL", "GNN2", "GNN3", "GNN4"]
+SEQ = ["ID", "SDA", "SD"]  # sequential heap orderings, incidence_degree.c
 TINY = ["path5", "star6", "clique6", "cycle7", "bip34", "empty5", "mixed10", "jump8",
-        "er300", "stencil6"]
+        "er300", "stencil6", "heapquirk10"]
 
 
 def pytest_configure(config):

@@ -4,12 +4,16 @@ Runs in the development container only: it loads oracle/_ref/libref.so, the
 reference's C sources compiled by `make -C oracle ref` (from /root/reference),
 and the product library's host-side graph generators (no GPU needed).
 
-  python tests/golden/make_golden.py [--big]
+  ulimit -s unlimited; python tests/golden/make_golden.py [--big]
+
+(The reference's ID / SD / SDA orderings label components with a recursive
+DFS, incidence_degree.c:78-90, which needs a large stack at scale 20.)
 
 Outputs
   tiny_<name>.npz   full vectors for hand-made graphs: CSR from the
                     reference's .mtx path, priorities and colours for FF, LF,
-                    LLF, SL, SLL, GNN2-4, and a 5-pass Culberson chain
+                    LLF, SL, SLL, ID, SDA, SD, GNN2-4, and a 5-pass Culberson
+                    chain
   rmat10.npz        the same for R-MAT scale 10
   digests.json      SHA-256 digests (CSR, priority bits, colours) and colour
                     counts for R-MAT scale 16 (via a .mtx file, as the
@@ -39,6 +43,7 @@ import oracle  # noqa: E402
 OUT = os.path.dirname(os.path.abspath(__file__))
 REF_TREE = "/root/reference"
 HEUR = ["FF", "LF", "LLF", "SL", "SLL", "GNN2", "GNN3", "GNN4"]
+SEQ = ["ID", "SDA", "SD"]  # sequential heap orderings (not at scale 22 and up)
 
 
 def sha(a) -> str:
@@ -60,9 +65,9 @@ def ref_csr_via_mtx(ref, N, I, J):
         return ref.mtx_csr(path)
 
 
-def all_heuristics(ref, models, V, E):
+def all_heuristics(ref, models, V, E, seq=True):
     out = {}
-    for h in HEUR:
+    for h in HEUR + (SEQ if seq else []):
         if h.startswith("GNN"):
             param, md = models[int(h[3:])]
             p = ref.gnn(param, V, E, md)
@@ -100,6 +105,11 @@ def tiny_graphs():
     g["er300"] = (n, I, J)
     n, I, J = gcx.stencil27_pairs(6, seed=5)
     g["stencil6"] = (n, I, J)
+    # the reference's heap removal only sifts down (incidence_degree.c:62-76):
+    # on this graph its ID order differs from a textbook priority queue's
+    hq = [(0, 1), (0, 3), (0, 7), (1, 2), (1, 3), (1, 4), (1, 6), (1, 7), (1, 9), (2, 4), (2, 5),
+          (2, 8), (4, 6), (4, 9), (5, 7)]
+    g["heapquirk10"] = (10, [a for a, _ in hq], [b for _, b in hq])
     return g
 
 
@@ -155,7 +165,7 @@ def main():
             V, E = oracle.csr_from_pairs(N, I, J)
         rec = {"graph": desc, "N": int(N), "M": int(V[-1]), "sha_V": sha(V), "sha_E": sha(E),
                "csr_via": "reference mtx_parse_par+sort+make_simple" if via_mtx else "oracle"}
-        res = all_heuristics(ref, models, V, E)
+        res = all_heuristics(ref, models, V, E, seq=not name.startswith(("rmat22", "stencil")))
         for h, (p, c, C) in res.items():
             rec[h] = {"sha_p": sha(p), "sha_color": sha(c), "ncolors": int(C)}
         if name in ("rmat16", "rmat20"):

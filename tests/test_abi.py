@@ -39,7 +39,9 @@ def test_headers_declare_reference_api():
                "validate_coloring_par", "greedy_internal", "cgnn_graph_sage_16_par",
                "gnn_ordering", "first_fit_ordering_par", "largest_degree_first_ordering",
                "smallest_degree_last_ordering", "smallest_log_degree_last_ordering",
-               "runner_test_full", "mtx_parse_par", "graph_make_simple", "graph_validate_par"]:
+               "runner_test_full", "mtx_parse_par", "graph_make_simple", "graph_validate_par",
+               "incidence_degree_ordering", "saturation_degree_ordering",
+               "saturation_degree_ordering_alt"]:
         assert fn in names, fn
 
 

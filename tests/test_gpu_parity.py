@@ -4,13 +4,15 @@ Bit-exact bar: CSR, integer priorities, FP32 GNN priorities (IEEE bits),
 colours and colour counts. Every call goes through the C ABI (libgcx.so).
 """
 import hashlib
+import os
+import subprocess
 
 import numpy as np
 import pytest
 
 import gcx
 import oracle
-from conftest import HEUR, TINY, load_case
+from conftest import HEUR, ROOT, SEQ, TINY, load_case
 
 pytestmark = pytest.mark.gpu
 
@@ -270,6 +272,53 @@ def test_reference_api_parity(models):
     bad = col.copy()
     bad[E[0]] = bad[0]
     assert gcx.ref.validate_coloring_par(V, E, bad) == 0
+
+
+@pytest.mark.parametrize("name", TINY + ["rmat10"])
+def test_seq_orderings_coloured_on_gpu(name):
+    """ID / SDA / SD priorities (host walk) coloured by the GPU JP: the
+    reference's colours, including p = -1 for an isolated last vertex."""
+    c = load_case(name)
+    V, E = c["V"], c["E"]
+    for h in SEQ:
+        p = gcx.ref.ordering(h, V, E)
+        assert p.view(np.uint64).tolist() == c[f"p_{h}"].view(np.uint64).tolist(), h
+        col = gcx.ref.jones_plassmann_coloring(V, E, p)
+        np.testing.assert_array_equal(col, c[f"color_{h}"])
+        assert gcx.ref.validate_coloring_par(V, E, col) == int(c[f"ncolors_{h}"])
+
+
+CLI = os.path.join(ROOT, "gnn-coloring_amd", "gcx", "coloring")
+QUICK = ["FF", "LF", "LLF", "SL", "SLL", "ID", "SDA", "SD", "GNN2", "GNN3", "GNN4"]
+
+
+def run_cli(tmp_path, name, *flags):
+    c = load_case(name)
+    path = tmp_path / f"{name}.mtx"
+    with open(path, "w") as f:
+        f.write("%%MatrixMarket matrix coordinate pattern general\n")
+        f.write(f"{int(c['N'])} {int(c['N'])} {len(c['I'])}\n")
+        for a, b in zip(c["I"].tolist(), c["J"].tolist()):
+            f.write(f"{a + 1} {b + 1}\n")
+    env = dict(os.environ, GCX_MODEL_DIR=os.path.join(ROOT, "models"))
+    out = subprocess.run([CLI, *flags, "-g", str(path)], capture_output=True, text=True,
+                         timeout=120, env=env)
+    assert out.returncode == 0, out.stderr
+    return c, [int(x) for x in out.stdout.split()]
+
+
+@pytest.mark.parametrize("name", ["rmat10", "er300", "mixed10"])
+def test_cli_quick_line(tmp_path, name):
+    """`coloring -q` prints the reference's colour counts for all eleven
+    orderings (main.c:137-187); -f zeroes LLF, SLL, ID, SDA and SD."""
+    c, got = run_cli(tmp_path, name, "-q")
+    assert got == [int(c[f"ncolors_{h}"]) for h in QUICK]
+    c, got = run_cli(tmp_path, name, "-q", "-f")
+    skipped = {"LLF", "SLL", "ID", "SDA", "SD"}
+    assert got == [0 if h in skipped else int(c[f"ncolors_{h}"]) for h in QUICK]
+    # -c 5: each column is the count after a 5-pass Culberson chain
+    c, got = run_cli(tmp_path, name, "-q", "-c", "5")
+    assert got[QUICK.index("GNN3")] == int(c["culb_counts"][-1])
 
 
 def test_reference_csr_utilities():

@@ -7,7 +7,9 @@ Checks, per oracle function:
   * brute force: sequential greedy in (p desc, id asc) order, written
     independently below, on the hand-made graphs and on random tiny graphs;
   * hand-derived known answers for SL / SLL (path, star, clique, a
-    threshold-jump graph);
+    threshold-jump graph) and ID / SD / SDA (star, clique, path, components,
+    the isolated last vertex); a graph on which a textbook priority queue
+    gives another ID order than the reference's heap;
   * the GNN in float64 as sparse matrix algebra (numpy/scipy), independent of
     the oracle's loop structure, against the FP64 and FP32 oracles;
   * planted mistakes: a corrupted colouring / CSR must be rejected.
@@ -20,7 +22,8 @@ import pytest
 import scipy.sparse as sp
 
 import oracle
-from conftest import HEUR, TINY, load_case
+from conftest import HEUR, SEQ, TINY, load_case
+from oracle import seqorder
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -319,3 +322,76 @@ def test_class_orders_keep_colour_count():
     key = oracle.class_keys(c0, 1, 0, 0)
     sizes = np.bincount(c0)
     assert key[c0 == np.argmax(sizes[1:]) + 1].max() == c0.max()
+
+
+# ------------------------------------------- ID / SD / SDA (incidence_degree.c)
+
+SEQ_FN = {"ID": seqorder.incidence_degree, "SD": seqorder.saturation_degree,
+          "SDA": seqorder.saturation_degree_alt}
+
+
+@pytest.mark.parametrize("name", TINY + ["rmat10"])
+@pytest.mark.parametrize("h", SEQ)
+def test_seq_orderings_match_reference(name, h):
+    c = load_case(name)
+    p = SEQ_FN[h](c["V"], c["E"])
+    assert p.view(np.uint64).tolist() == c[f"p_{h}"].view(np.uint64).tolist()
+    col, _ = oracle.greedy(c["V"], c["E"], p)
+    np.testing.assert_array_equal(col, c[f"color_{h}"])
+
+
+def test_seq_textbook_queue_is_told_apart():
+    """The reference's heap removal only sifts down; a correct priority queue
+    orders heapquirk10 differently, so the goldens pin the heap rule."""
+    c = load_case("heapquirk10")
+    assert not np.array_equal(seqorder.textbook_incidence_degree(c["V"], c["E"]), c["p_ID"])
+    np.testing.assert_array_equal(seqorder.incidence_degree(c["V"], c["E"]), c["p_ID"])
+
+
+def _pairs_graph(n, pairs):
+    I = np.array([a for a, _ in pairs], np.uint32)
+    J = np.array([b for _, b in pairs], np.uint32)
+    return oracle.csr_from_pairs(n, I, J)
+
+
+def test_seq_known_answers():
+    # star K1,5 with the centre at 2: centre first (largest degree), then the
+    # leaves, all tied on (1 ordered neighbour, degree 1): largest id first
+    V, E = _pairs_graph(6, [(2, x) for x in (0, 1, 3, 4, 5)])
+    for f in (seqorder.incidence_degree, seqorder.saturation_degree):
+        np.testing.assert_array_equal(f(V, E), [1, 2, 6, 3, 4, 5])
+    # SDA: the centre alone has the maximum degree; then all leaves share
+    # (saturation 1, degree 1) and one priority
+    np.testing.assert_array_equal(seqorder.saturation_degree_alt(V, E), [5, 5, 6, 5, 5, 5])
+    # clique K5: start at 0 (smallest id of maximum degree), then every
+    # vertex has the same count, so the largest id goes next
+    V, E = _pairs_graph(5, [(a, b) for a in range(5) for b in range(a + 1, 5)])
+    for f in (seqorder.incidence_degree, seqorder.saturation_degree):
+        np.testing.assert_array_equal(f(V, E), [5, 1, 2, 3, 4])
+    # path 0-1-2-3-4: start at 1 (first vertex of degree 2); 0 and 2 both
+    # have one ordered neighbour and 2 has the larger degree, so the walk runs
+    # down the path (3, then 4 over 0 by id) and ends at 0
+    V, E = _pairs_graph(5, [(0, 1), (1, 2), (2, 3), (3, 4)])
+    np.testing.assert_array_equal(seqorder.incidence_degree(V, E), [1, 5, 4, 3, 2])
+    # components in the order of their smallest vertex; an isolated last
+    # vertex is never walked (incidence_degree.c:160-164) and keeps -1
+    V, E = _pairs_graph(5, [(0, 1), (1, 2)])
+    np.testing.assert_array_equal(seqorder.incidence_degree(V, E), [3, 5, 4, 2, -1])
+    V, E = _pairs_graph(4, [(2, 3)])
+    np.testing.assert_array_equal(seqorder.incidence_degree(V, E), [4, 3, 2, 1])
+
+
+def test_seq_oracle_against_compiled_reference():
+    """Random small graphs, when oracle/_ref is built (development container)."""
+    try:
+        ref = oracle.Ref()
+    except ImportError:
+        pytest.skip("oracle/_ref not built")
+    rng = np.random.default_rng(21)
+    for _ in range(150):
+        n = int(rng.integers(1, 50))
+        m = int(rng.integers(0, 4 * n + 1))
+        V, E = oracle.csr_from_pairs(n, rng.integers(0, n, m).astype(np.uint32),
+                                     rng.integers(0, n, m).astype(np.uint32))
+        for h, f in SEQ_FN.items():
+            np.testing.assert_array_equal(f(V, E), ref.ordering(h, V, E))
