@@ -73,7 +73,8 @@ def serving(args, gcx):
     GNN2 GNN3) run three ways: BF16 GNN on `workers` streams with all
     requests queued at t = 0 (capacity), BF16 with Poisson arrivals at 70% of
     that capacity (latency), and the plain FP32 path one graph at a time
-    (1 worker, all queued)."""
+    (1 worker, all queued, cooperative ordered launches). The BF16 passes
+    share the device through plain ordered launches in `slots` slots."""
     import numpy as np
     k = args.serving_requests
     base = gcx.serving_trace(k, seed=11, nmax=args.serving_nmax)
@@ -92,13 +93,17 @@ def serving(args, gcx):
         rep = srv.run(reqs)
         return rep, time.perf_counter() - t0
 
-    srv = gcx.Server(workers=args.serving_workers, budget=int(args.serving_budget_gb * (1 << 30)))
+    workers = args.serving_mixed_workers
+    prev = gcx.get_ordered_launch()
+    gcx.set_ordered_launch(args.serving_mixed_slots)
+    srv = gcx.Server(workers=workers, budget=int(args.serving_budget_gb * (1 << 30)))
     srv.run(gcx.with_arrivals(base[:min(k, 16)], None))  # warm the pool and the kernels
     if args.serving_rate:
         # one Poisson pass at a given rate (long traces)
         rep2, wall2 = run(srv, gcx.with_arrivals(base, args.serving_rate, seed=12))
         srv.close()
-        return {"requests": k, "workers": args.serving_workers,
+        gcx.set_ordered_launch(prev)
+        return {"requests": k, "workers": workers, "ordered_launch_slots": args.serving_mixed_slots,
                 "budget_gb": args.serving_budget_gb,
                 "trace": f"seed 11, n log-uniform [1e3, {args.serving_nmax:.0e}], 40% R-MAT / 30% ER "
                          "(degree 8-64) / 30% RGG (degree ~12), FF/LF/SL/GNN2/GNN3, GNN in BF16",
@@ -110,14 +115,16 @@ def serving(args, gcx):
     lat = summary(rep2, wall2)
     peak = srv.peak_reserved()
     srv.close()
+    gcx.set_ordered_launch(0)
     plain = gcx.Server(workers=1, budget=int(args.serving_budget_gb * (1 << 30)))
     plain.run(gcx.with_arrivals(base[:min(k, 16)], None, precision=gcx.FP32))
     rep3, wall3 = run(plain, gcx.with_arrivals(base, None, precision=gcx.FP32))
     fp32 = summary(rep3, wall3)
     plain.close()
+    gcx.set_ordered_launch(prev)
     same = sum(1 for a, b in zip(rep, rep2) if (a.colors, a.digest) == (b.colors, b.digest))
     worse = [b.colors - a.colors for a, b in zip(rep3, rep) if a.status == 0 and b.status == 0]
-    return {"requests": k, "workers": args.serving_workers,
+    return {"requests": k, "workers": workers, "ordered_launch_slots": args.serving_mixed_slots,
             "budget_gb": args.serving_budget_gb, "peak_reserved_gb": peak / (1 << 30),
             "trace": f"seed 11, n log-uniform [1e3, {args.serving_nmax:.0e}], 40% R-MAT / 30% ER "
                      "(degree 8-64) / 30% RGG (degree ~12), FF/LF/SL/GNN2/GNN3, GNN in BF16",
@@ -345,7 +352,12 @@ def main():
     ap.add_argument("--serving-batch", type=int, nargs="*", default=[4, 8],
                     help="continuous-batching sizes for the resident R-MAT serving leg")
     ap.add_argument("--serving-nmax", type=float, default=4e6)
-    ap.add_argument("--serving-workers", type=int, default=8)
+    ap.add_argument("--serving-workers", type=int, default=8,
+                    help="worker streams of the resident R-MAT serving leg")
+    ap.add_argument("--serving-mixed-workers", type=int, default=16,
+                    help="worker streams of the mixed-trace serving leg")
+    ap.add_argument("--serving-mixed-slots", type=int, default=4,
+                    help="ordered-launch slots of the mixed-trace BF16 passes")
     ap.add_argument("--serving-budget-gb", type=float, default=48.0)
     ap.add_argument("--serving-rate", type=float, default=0.0,
                     help="run only a Poisson pass at this many requests/s (0 = capacity, "

@@ -894,6 +894,12 @@ int gcx_workspace_free_on(gcx_workspace *ws, void *stream)
 {
     if (!ws)
         return GCX_OK;
+    /* Hub kernels may still be queued on the side stream when a call failed
+     * between its fork and its join: order the frees after them, so the pool
+     * cannot hand these buffers to another allocation while they run. */
+    if (ws->aux && ws->ev_join &&
+        hipEventRecord((hipEvent_t)ws->ev_join, (hipStream_t)ws->aux) == hipSuccess)
+        (void)hipStreamWaitEvent((hipStream_t)gcx__pick(stream), (hipEvent_t)ws->ev_join, 0);
     void *bufs[] = {ws->key,    ws->count, ws->color,    ws->q[0],     ws->q[1],
                     ws->h[0],   ws->h[1],  ws->x,        ws->y,        ws->ctl,
                     ws->npred,  ws->state, ws->ovf,      ws->esplit,   ws->items[0],
