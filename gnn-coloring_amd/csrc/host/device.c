@@ -794,12 +794,12 @@ static size_t order_extra(uint32_t N)
 
 uint64_t gcx_workspace_footprint(uint32_t N, uint64_t M)
 {
-    /* per vertex: key 8, count 4, color 4, npred 4, word offsets 4, 4 lists
-     * x 4, hpos 8, rows x,y 2 x 64; per entry: E_split 4; plus the state and
-     * overflow colour words, two item queues and any sort scratch beyond the
-     * x rows */
+    /* per vertex: key 8, count 4, color 4, npred 4, word offsets 4, word
+     * descriptors 8, 4 lists x 4, hpos 8, rows x,y 2 x 64; per entry: E_split
+     * 4; plus the state and overflow colour words, two item queues and any
+     * sort scratch beyond the x rows */
     const uint64_t n = (uint64_t)N + 1;
-    return 176ull * n + 8ull * gcxk_jp_state_words(N, M) + 4ull * M +
+    return 184ull * n + 8ull * gcxk_jp_state_words(N, M) + 4ull * M +
            4ull * gcxk_jp_ovf_words(N, M) +
            2ull * sizeof(gcxk_item) * gcxk_jp_item_capacity(N, M) + 2 * sizeof(gcxk_ctl) + 256 +
            order_extra(N);
@@ -903,7 +903,7 @@ int gcx_workspace_free_on(gcx_workspace *ws, void *stream)
     void *bufs[] = {ws->key,    ws->count, ws->color,    ws->q[0],     ws->q[1],
                     ws->h[0],   ws->h[1],  ws->x,        ws->y,        ws->ctl,
                     ws->npred,  ws->state, ws->ovf,      ws->esplit,   ws->items[0],
-                    ws->items[1], ws->hpos, ws->order_buf, ws->wof, ws->pass};
+                    ws->items[1], ws->hpos, ws->order_buf, ws->wof, ws->wdesc, ws->pass};
     for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); i++)
         gcx__dfree(bufs[i], stream);
     pinned_ctl_put(ws->hctl);
@@ -952,6 +952,7 @@ int gcx_workspace_create_on(uint32_t N, uint64_t M, void *stream, gcx_workspace 
              gcx__dalloc((void **)&ws->npred, 4 * n, st) == GCX_OK &&
              gcx__dalloc((void **)&ws->state, 8 * gcxk_jp_state_words(N, M), st) == GCX_OK &&
              gcx__dalloc((void **)&ws->wof, 4 * n, st) == GCX_OK &&
+             gcx__dalloc((void **)&ws->wdesc, 8 * n, st) == GCX_OK &&
              gcx__dalloc((void **)&ws->hpos, 8 * n, st) == GCX_OK &&
              gcx__dalloc((void **)&ws->ovf, 4 * gcxk_jp_ovf_words(N, M), st) == GCX_OK &&
              gcx__dalloc((void **)&ws->esplit, 4 * (M ? M : 1), st) == GCX_OK &&
@@ -1301,7 +1302,7 @@ static int launch_ordered(gcx_workspace *ws, const gcx_graph *g, int mb, hipStre
     }
     /* 2 s per wait: far beyond any legitimate wait, and it still ends the kernel */
     if (rc == GCX_OK)
-        rc = gcxk_jp_ordered(g->V, ws->npred, ws->esplit, ws->state, ws->wof, ws->color, ws->ovf,
+        rc = gcxk_jp_ordered(g->V, ws->npred, ws->esplit, ws->state, ws->wdesc, ws->color, ws->ovf,
                              mb, ws->items[0], ws->ctl, grid, 200000000ull, k == 0, s);
     if (rc == GCX_OK && hipEventRecord(g_slot_done[slot], s) != hipSuccess)
         rc = GCX_ERR_HIP;
@@ -1317,7 +1318,7 @@ static int color_ordered(gcx_workspace *ws, const gcx_graph *g, int mb, hipStrea
     void *scratch = ws->order_buf ? ws->order_buf : (void *)ws->x;
     uint32_t *cnt = (uint32_t *)ws->y, *incl = cnt + ws->capN;
     TRY(gcxk_jp_order_items(g->N, g->V, ws->npred, ws->key, ws->key_bits, mb, scratch,
-                            ws->order_bytes, cnt, incl, ws->wof, ws->state,
+                            ws->order_bytes, cnt, incl, ws->wof, ws->wdesc, ws->state,
                             gcxk_jp_state_words(ws->capN, ws->capM), ws->items[0], ws->ctl, s));
     TRY(gcx__prof_end(tok, s));
     TRY(gcx__prof_begin(GCX_ST_JP_ROUNDS, s, &tok));

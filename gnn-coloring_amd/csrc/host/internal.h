@@ -50,6 +50,7 @@ struct gcx_workspace {
     uint32_t *npred;  /* JP: predecessors per vertex */
     uint64_t *state;  /* JP state words (gcxk_jp_state_words; layouts in jp.hip) */
     uint32_t *wof;    /* JP (ordered): first state word of each vertex, N + 1 */
+    uint64_t *wdesc;  /* JP (ordered): state word descriptor of each vertex, N */
     uint32_t *ovf;    /* JP: colour-set overflow words (colours > mb) */
     uint32_t *esplit; /* JP: rows split into predecessors | successors (capM) */
     gcxk_item *items[2]; /* JP: round queues; items[0] = greedy-order items */

@@ -176,7 +176,10 @@ static inline uint64_t gcxk_jp_ovf_words(uint32_t N, uint64_t M)
  * the predecessor counts sum to M/2. */
 static inline uint64_t gcxk_jp_state_words(uint32_t N, uint64_t M)
 {
-    return (uint64_t)N + 1u + M / 64u + 8u;
+    /* counting fields: ceil((np + 2) / 8) <= np / 8 + 9 / 8 words per vertex,
+     * bit sets fewer; the predecessor counts sum to at most M (M / 2 for a
+     * symmetric CSR) */
+    return (uint64_t)N + N / 8u + M / 8u + 64u;
 }
 /* Item queue capacity: every vertex contributes max(1, ceil(nsucc / 64)). */
 static inline uint64_t gcxk_jp_item_capacity(uint32_t N, uint64_t M)
@@ -215,15 +218,16 @@ int gcxk_jp_round(const uint32_t *dV, const uint32_t *npred, const uint32_t *esp
                   gcxk_ctl *ctl, uint32_t r, uint32_t grid, void *stream);
 /* Ordered schedule, step 1 (after gcxk_jp_init): the state word offsets
  * wof (N + 1 entries; vertex v owns words[wof[v] .. wof[v+1]), zeroed,
- * words_cap = gcxk_jp_state_words) and the items of every vertex in greedy
+ * words_cap = gcxk_jp_state_words), the per-vertex word descriptors wdesc
+ * (N entries) and the items of every vertex in greedy
  * order (p descending, id ascending) into `items`, their number into
  * ctl->total_items. scratch: gcxk_jp_order_scratch(N, key_bits) bytes; cnt:
  * N words, incl: N + 1 words. */
 size_t gcxk_jp_order_scratch(uint32_t N, int key_bits);
 int gcxk_jp_order_items(uint32_t N, const uint32_t *dV, const uint32_t *npred, const void *key,
                         int key_bits, int mb, void *scratch, size_t scratch_bytes, uint32_t *cnt,
-                        uint32_t *incl, uint32_t *wof, uint64_t *words, uint64_t words_cap,
-                        gcxk_item *items, gcxk_ctl *ctl, void *stream);
+                        uint32_t *incl, uint32_t *wof, uint64_t *wdesc, uint64_t *words,
+                        uint64_t words_cap, gcxk_item *items, gcxk_ctl *ctl, void *stream);
 /* Default workgroups of the ordered kernel: 6 per CU, at most what is
  * resident at once (0 on error). */
 uint32_t gcxk_jp_ordered_grid(void);
@@ -236,7 +240,7 @@ uint32_t gcxk_jp_ordered_grid(void);
  * shares the device with other streams (the caller keeps the grids of
  * concurrent ordered launches within the resident maximum). */
 int gcxk_jp_ordered(const uint32_t *dV, const uint32_t *npred, const uint32_t *esplit,
-                    uint64_t *words, const uint32_t *wof, int *color, uint32_t *ovf, int mb,
+                    uint64_t *words, const uint64_t *wdesc, int *color, uint32_t *ovf, int mb,
                     const gcxk_item *items, gcxk_ctl *ctl, uint32_t grid, uint64_t spin_ticks,
                     int cooperative, void *stream);
 /* Workgroups of the ordered kernel resident on the device at once (0 on

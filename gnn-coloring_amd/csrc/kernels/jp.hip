@@ -455,20 +455,53 @@ __global__ __launch_bounds__(kBlock) void jp_expand(uint32_t N, const uint32_t *
         ctl->total_items = end;
 }
 
-// Word counts of the ordered schedule's state: W(v) = min(8, ceil((npred+1)/mb))
-// words hold every colour v can take (mex <= npred + 1), capped at 8 words;
-// wn[N] = 0 closes the scan.
+// State words of the ordered schedule. A vertex with np predecessors uses one
+// of two layouts (W words each, wn[N] = 0 closes the scan):
+//  counting fields (np <= 255): fields of B bits (B = 4 for np <= 15, else
+//    8), F = 64 / B per word; field f counts the predecessors with colour
+//    f + 1, the last field of the last word those with any colour past the
+//    fields. W = ceil((np + 2) / F), so every colour v can take (mex <=
+//    np + 1) has its own field, and no field can exceed np < 2^B;
+//  bit sets (np > 255): word i = (reports << mb) | bits of colours
+//    i*mb+1 .. (i+1)*mb, W = min(8, ceil((np + 1) / mb)).
 constexpr uint32_t kMaxWords = 8;
+constexpr uint32_t kFieldMaxPred = 255;
+
+__device__ __forceinline__ uint32_t field_bits(uint32_t np)
+{
+    return np <= 15u ? 4u : (np <= kFieldMaxPred ? 8u : 0u);
+}
+
+__device__ __forceinline__ uint32_t state_words(uint32_t np, int mb)
+{
+    const uint32_t B = field_bits(np);
+    if (B)
+        return (np + 2u + 64u / B - 1u) / (64u / B);
+    const uint32_t w = (np + 1u + (uint32_t)mb - 1u) / (uint32_t)mb;
+    return w < kMaxWords ? w : kMaxWords;
+}
 
 __global__ __launch_bounds__(kBlock) void jp_word_counts(uint32_t N, const uint32_t *__restrict__ npred,
                                                          int mb, uint32_t *__restrict__ wn)
 {
     const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
-    if (v < N) {
-        const uint32_t w = (npred[v] + 1u + (uint32_t)mb - 1u) / (uint32_t)mb;
-        wn[v] = w < kMaxWords ? w : kMaxWords;
-    } else if (v == N) {
+    if (v < N)
+        wn[v] = state_words(npred[v], mb);
+    else if (v == N)
         wn[v] = 0u;
+}
+
+// Per-vertex descriptor read by the ordered kernel (one 8-byte load instead
+// of two offsets): first word (bits 0-39), word count (40-47), field width
+// B (48-55; 0 = bit-set words).
+__global__ __launch_bounds__(kBlock) void jp_word_desc(uint32_t N, const uint32_t *__restrict__ npred,
+                                                       const uint32_t *__restrict__ wof,
+                                                       uint64_t *__restrict__ wdesc)
+{
+    const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+    if (v < N) {
+        const uint64_t n = wof[v + 1] - wof[v];
+        wdesc[v] = (uint64_t)wof[v] | (n << 40) | ((uint64_t)field_bits(npred[v]) << 48);
     }
 }
 
@@ -483,16 +516,20 @@ __global__ __launch_bounds__(kBlock) void jp_zero_words(const uint32_t *__restri
         st[i] = 0ull;
 }
 
-// Ordered schedule. Vertex v owns state words st[wof[v] .. wof[v+1]); word i
-// is (reports << mb) | colour bits for colours i*mb+1 .. (i+1)*mb. A
-// predecessor with colour c reports into the word that holds c (OR, then
-// add 1 << mb, on one location: no wait); a colour past v's words can never
-// be v's mex, so it is only counted (add to word 0), except for vertices
-// with the full 8 words, whose higher colours go to overflow words (waited
-// for, then counted in word 0). v is ready when its words' reports sum to
-// npred[v]: reports only grow and each one sits in the word of its bit, so
-// loads whose reports sum to npred have seen every bit. Lanes 0..W-1 poll
-// the words in one load instruction.
+// Ordered schedule. Vertex v owns state words st[off .. off + W) in the
+// layout its descriptor names (jp_word_counts). Every report from a
+// predecessor with colour c is one atomic on one word:
+//  counting fields: add 1 to the field of c (or to the last field when c is
+//    past the fields); v's reports are the sum of its fields;
+//  bit sets: OR c's bit, then add 1 << mb, both on the word that holds c (one
+//    location: no wait between them); a colour past v's words can never be
+//    v's mex, so it is only counted (add to word 0), except for vertices with
+//    the full 8 words, whose higher colours go to overflow words (waited for,
+//    then counted in word 0).
+// v is ready when its reports sum to npred[v]: reports only grow and each one
+// sits in the word it marks, so loads whose reports sum to npred have seen
+// every mark. Lanes 0..W-1 poll the words in one load instruction. The
+// mex is the first zero field, or the first clear bit.
 //
 // Waits sleep between polls by how many reports are still missing: a wave
 // one or two reports from ready polls every 64 clocks, a wave that still
@@ -509,11 +546,40 @@ __device__ __forceinline__ void poll_sleep(uint32_t missing)
         __builtin_amdgcn_s_sleep(16);
 }
 
+__device__ __forceinline__ uint32_t field_sum(uint64_t w, uint32_t B)
+{
+    // per-word sums never exceed npred <= 255, so the byte sum has no carry
+    if (B == 4u)
+        w = (w & 0x0F0F0F0F0F0F0F0Full) + ((w >> 4) & 0x0F0F0F0F0F0F0F0Full);
+    return (uint32_t)((w * 0x0101010101010101ull) >> 56);
+}
+
+// The lowest bit of every zero field of w; the last field of the last word
+// (colours past the fields) never counts as free.
+__device__ __forceinline__ uint64_t zero_fields(uint64_t w, uint32_t B, bool last)
+{
+    uint64_t t, low;
+    if (B == 4u) {
+        t = w | (w >> 2);
+        t |= t >> 1;
+        low = 0x1111111111111111ull;
+    } else {
+        t = w | (w >> 4);
+        t |= t >> 2;
+        t |= t >> 1;
+        low = 0x0101010101010101ull;
+    }
+    uint64_t z = ~t & low;
+    if (last)
+        z &= ~(1ull << (64u - B));
+    return z;
+}
+
 __global__ __launch_bounds__(kBlock) void jp_ordered_kernel(const uint32_t *__restrict__ V,
                                                             const uint32_t *__restrict__ npred,
                                                             const uint32_t *__restrict__ esplit,
                                                             uint64_t *__restrict__ st,
-                                                            const uint32_t *__restrict__ wof,
+                                                            const uint64_t *__restrict__ wdesc,
                                                             int *__restrict__ color,
                                                             uint32_t *__restrict__ ovf,
                                                             const uint2 *__restrict__ items,
@@ -531,21 +597,30 @@ __global__ __launch_bounds__(kBlock) void jp_ordered_kernel(const uint32_t *__re
         const uint32_t u = it.x, k = it.y;
         const uint32_t Vu = V[u], Vu1 = V[u + 1];
         const uint32_t np = npred[u];
-        const uint32_t wo = wof[u], wn = wof[u + 1] - wo;
+        const uint64_t du = wdesc[u];
+        const uint32_t wo = (uint32_t)du, wn = (uint32_t)(du >> 40) & 0xFFu;
+        const uint32_t Bu = (uint32_t)(du >> 48) & 0xFFu;
         const uint32_t i = Vu + np + k * kChunk + (uint32_t)lane;
         const bool has = i < Vu1;
         const uint32_t v = has ? esplit[i] : 0u;
-        const uint32_t vo = has ? wof[v] : 0u, vn = has ? wof[v + 1] - vo : 0u;
+        const uint64_t dv = has ? wdesc[v] : 0ull;
+        const uint32_t vo = (uint32_t)dv, vn = (uint32_t)(dv >> 40) & 0xFFu;
+        const uint32_t Bv = (uint32_t)(dv >> 48) & 0xFFu;
         const bool mine = (uint32_t)lane < wn;
         uint64_t w;
         uint32_t polls = 0;
         uint64_t t0 = 0;
         for (;;) {
-            w = mine ? load_state(&st[wo + lane]) : ~0ull;
-            uint32_t r = mine ? (uint32_t)(w >> mb) : 0u;
+            w = mine ? load_state(&st[wo + lane]) : 0ull;
+            uint32_t r = mine ? (Bu ? field_sum(w, Bu) : (uint32_t)(w >> mb)) : 0u;
             r += __shfl_xor(r, 1);
             r += __shfl_xor(r, 2);
             r += __shfl_xor(r, 4);
+            r += __shfl_xor(r, 8);
+            if (wn > 16u) { // wave-uniform
+                r += __shfl_xor(r, 16);
+                r += __shfl_xor(r, 32);
+            }
             r = __shfl(r, 0);
             if (r == np)
                 break;
@@ -559,32 +634,53 @@ __global__ __launch_bounds__(kBlock) void jp_ordered_kernel(const uint32_t *__re
                 return;
             }
         }
-        // mex: the first word with a free colour bit (all 8 full: overflow)
-        const uint64_t fr = mine ? (~w & low) : 0ull;
-        const uint64_t bw = __ballot(fr != 0ull);
+        // mex: the first zero field, or the first word with a clear bit (all
+        // 8 words full: overflow)
         uint32_t c;
-        if (bw) {
-            const int fl = __ffsll((long long)bw) - 1;
-            const uint32_t lo = __shfl((uint32_t)fr, fl), hi = __shfl((uint32_t)(fr >> 32), fl);
-            c = (uint32_t)fl * (uint32_t)mb +
-                (uint32_t)__builtin_ctzll(((uint64_t)hi << 32) | lo) + 1u;
+        if (Bu) {
+            const uint64_t z = mine ? zero_fields(w, Bu, (uint32_t)lane == wn - 1u) : 0ull;
+            const uint64_t bz = __ballot(z != 0ull);
+            if (!bz) { // cannot happen: np + 1 colour fields, at most np counted
+                if (lane == 0)
+                    __hip_atomic_store(&ctl->err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                return;
+            }
+            const int fl = __ffsll((long long)bz) - 1;
+            const uint32_t lo = __shfl((uint32_t)z, fl), hi = __shfl((uint32_t)(z >> 32), fl);
+            c = (uint32_t)fl * (64u / Bu) +
+                (uint32_t)__builtin_ctzll(((uint64_t)hi << 32) | lo) / Bu + 1u;
         } else {
-            c = mex_overflow(cap, u, Vu, Vu1 - Vu, ovf);
+            const uint64_t fr = mine ? (~w & low) : 0ull;
+            const uint64_t bw = __ballot(fr != 0ull);
+            if (bw) {
+                const int fl = __ffsll((long long)bw) - 1;
+                const uint32_t lo = __shfl((uint32_t)fr, fl), hi = __shfl((uint32_t)(fr >> 32), fl);
+                c = (uint32_t)fl * (uint32_t)mb +
+                    (uint32_t)__builtin_ctzll(((uint64_t)hi << 32) | lo) + 1u;
+            } else {
+                c = mex_overflow(cap, u, Vu, Vu1 - Vu, ovf);
+            }
         }
         if (k == 0 && lane == 0)
             color[u] = (int)c;
-        const uint32_t wi = (c - 1u) / (uint32_t)mb;
         bool late = false;
         if (has) {
-            if (wi < vn) {
-                unsigned long long *p = (unsigned long long *)&st[vo + wi];
-                atomicOr(p, 1ull << (c - 1u - wi * (uint32_t)mb));
-                atomicAdd(p, (unsigned long long)one);
-            } else if (vn == kMaxWords) {
-                or_overflow(V, ovf, v, c, cap);
-                late = true;
+            if (Bv) {
+                const uint32_t F = 64u / Bv, top = vn * F - 1u;
+                const uint32_t fi = c - 1u < top ? c - 1u : top;
+                atomicAdd((unsigned long long *)&st[vo + fi / F], 1ull << (Bv * (fi % F)));
             } else {
-                atomicAdd((unsigned long long *)&st[vo], (unsigned long long)one);
+                const uint32_t wi = (c - 1u) / (uint32_t)mb;
+                if (wi < vn) {
+                    unsigned long long *p = (unsigned long long *)&st[vo + wi];
+                    atomicOr(p, 1ull << (c - 1u - wi * (uint32_t)mb));
+                    atomicAdd(p, (unsigned long long)one);
+                } else if (vn == kMaxWords) {
+                    or_overflow(V, ovf, v, c, cap);
+                    late = true;
+                } else {
+                    atomicAdd((unsigned long long *)&st[vo], (unsigned long long)one);
+                }
             }
         }
         if (__ballot(late)) {
@@ -931,8 +1027,9 @@ extern "C" size_t gcxk_jp_order_scratch(uint32_t N, int key_bits)
 extern "C" int gcxk_jp_order_items(uint32_t N, const uint32_t *dV, const uint32_t *npred,
                                    const void *key, int key_bits, int mb, void *scratch,
                                    size_t scratch_bytes, uint32_t *cnt, uint32_t *incl,
-                                   uint32_t *wof, uint64_t *words, uint64_t words_cap,
-                                   gcxk_item *items, gcxk_ctl *ctl, void *stream)
+                                   uint32_t *wof, uint64_t *wdesc, uint64_t *words,
+                                   uint64_t words_cap, gcxk_item *items, gcxk_ctl *ctl,
+                                   void *stream)
 {
     if (N == 0)
         return GCX_OK;
@@ -950,6 +1047,8 @@ extern "C" int gcxk_jp_order_items(uint32_t N, const uint32_t *dV, const uint32_
             return GCX_ERR_CAPACITY;
         GCXK_CHECK(rocprim::exclusive_scan(tmp, need, incl, wof, 0u, (size_t)N + 1,
                                            rocprim::plus<uint32_t>(), s));
+        jp_word_desc<<<div_up(N, kBlock), kBlock, 0, s>>>(N, npred, wof, wdesc);
+        GCXK_LAUNCH_CHECK();
         uint32_t zg = div_up((uint64_t)N + 1, kBlock);
         zg = zg < 2048u ? zg : 2048u;
         jp_zero_words<<<zg, kBlock, 0, s>>>(wof, N, words_cap, words);
@@ -1000,7 +1099,7 @@ extern "C" uint32_t gcxk_jp_ordered_grid(void)
 extern "C" uint32_t gcxk_jp_ordered_cap(void) { return gcxk_jp_ordered_grid() ? g_ordered_cap : 0; }
 
 extern "C" int gcxk_jp_ordered(const uint32_t *dV, const uint32_t *npred, const uint32_t *esplit,
-                               uint64_t *words, const uint32_t *wof, int *color, uint32_t *ovf,
+                               uint64_t *words, const uint64_t *wdesc, int *color, uint32_t *ovf,
                                int mb, const gcxk_item *items, gcxk_ctl *ctl, uint32_t grid,
                                uint64_t spin_ticks, int cooperative, void *stream)
 {
@@ -1013,7 +1112,7 @@ extern "C" int gcxk_jp_ordered(const uint32_t *dV, const uint32_t *npred, const 
         grid = g_ordered_cap;
     const uint2 *it = reinterpret_cast<const uint2 *>(items);
     void *args[] = {(void *)&dV,  (void *)&npred, (void *)&esplit, (void *)&words,
-                    (void *)&wof, (void *)&color, (void *)&ovf,    (void *)&it,
+                    (void *)&wdesc, (void *)&color, (void *)&ovf,  (void *)&it,
                     (void *)&ctl, (void *)&mb,    (void *)&spin_ticks};
     if (cooperative) {
         GCXK_CHECK(hipLaunchCooperativeKernel((const void *)jp_ordered_kernel, dim3(grid),

@@ -162,3 +162,66 @@ def test_shared_ordered_launch(slots, digests, gmodels):
             assert r.status == 0 and r.colors == rec[h]["ncolors"], h
     finally:
         gcx.set_ordered_launch(0)
+
+
+def _probe_graph():
+    """A 300-clique whose members take colours 1..300 in greedy order, plus
+    independent probe vertices of lower priority, each joined to clique
+    members chosen by colour. Probe predecessor counts sit on both sides of
+    every state layout boundary (4-bit fields up to 15, 8-bit fields up to
+    255, bit-set words above); the chosen colours fill the fields (mex just
+    past them), skip past the fields (counted in the last field) or leave
+    gaps."""
+    K = 300
+    I, J = np.triu_indices(K, 1)
+    pairs_i, pairs_j = [I.astype(np.int64)], [J.astype(np.int64)]
+    probes = []
+    for np_ in (1, 2, 14, 15, 16, 17, 126, 127, 254, 255, 256, 257, 299):
+        F = 16 if np_ <= 15 else 8
+        top = -(-(np_ + 2) // F) * F - 1  # colours past this go to the last field
+        sets = [list(range(1, np_ + 1)),                        # mex = np + 1
+                list(range(K - np_ + 1, K + 1)),                # high colours, mex 1
+                list(range(1, np_)) + [K]]                      # mex = np
+        if np_ >= 3:
+            sets.append(list(range(2, np_ + 2)))                # mex 1
+            sets.append([c for c in range(1, np_ + 2) if c != np_ // 2][:np_])
+        if top < K and np_ < top:
+            sets.append(list(range(1, np_)) + [top])            # the last colour field
+            sets.append(list(range(1, np_)) + [top + 1])        # one colour just past
+        for cols in sets:
+            assert len(cols) == np_
+            probes.append(cols)
+    N = K + len(probes)
+    for pi, cols in enumerate(probes):
+        v = K + pi
+        for c in cols:
+            pairs_i.append(np.array([c - 1]))
+            pairs_j.append(np.array([v]))
+    I = np.concatenate(pairs_i).astype(np.uint32)
+    J = np.concatenate(pairs_j).astype(np.uint32)
+    p = np.zeros(N, np.float64)
+    p[:K] = 1e6 - np.arange(K)  # member i precedes i + 1: colour i + 1
+    return N, I, J, p, probes
+
+
+@pytest.mark.parametrize("hub_degree", [0, 64])
+@pytest.mark.parametrize("sched", SCHEDULES + [(gcx.SCHED_ROUNDS, 0)])
+def test_state_layout_boundaries(sched, hub_degree):
+    N, I, J, p, probes = _probe_graph()
+    g = gcx.Graph.from_pairs(N, I, J)
+    if hub_degree:
+        g.set_hub_degree(hub_degree)  # clique members become hub rows
+    V, E = g.download()
+    ws = gcx.Workspace(g.N, g.M)
+    ws.set_schedule(*sched)
+    ws.set_priorities(g, p)
+    ws.color(g)
+    col = ws.colors(N)
+    ref, _ = oracle.greedy(V, E, p)
+    np.testing.assert_array_equal(col, ref)
+    np.testing.assert_array_equal(col[:300], np.arange(1, 301))
+    for pi, cols in enumerate(probes):
+        used = set(cols)
+        mex = next(c for c in range(1, len(cols) + 2) if c not in used)
+        assert col[300 + pi] == mex, (len(cols), pi)
+    assert ws.mex_check(g) == 0
