@@ -1056,14 +1056,16 @@ static int sl_round(void *p, uint32_t r, hipStream_t s)
     gcx_workspace *ws = c->ws;
     TRY(gcxk_sl_scan(ws->count, (uint32_t *)ws->key, ws->q[r & 1], ws->q[(r + 1) & 1], ws->h[0],
                      ws->ctl, c->log_variant, r, c->grid, s));
-    return gcxk_sl_dec(c->g->V, c->g->E, ws->count, (uint32_t *)ws->key, ws->h[0], ws->ctl, r,
-                       c->grid, s);
+    return gcxk_sl_dec(c->g->V, c->g->E, ws->count, (uint32_t *)ws->key, ws->h[0], ws->h[1],
+                       c->g->rank, c->g->perm, gcxk_sl_hub_slots(c->g->N), ws->ctl, r, c->grid,
+                       s);
 }
 
 static int order_sl(gcx_workspace *ws, const gcx_graph *g, int log_variant, hipStream_t s)
 {
     HIPC(hipMemsetAsync(ws->ctl, 0, sizeof(gcxk_ctl), s));
     TRY(gcxk_sl_init(g->N, g->V, ws->count, (uint32_t *)ws->key, ws->q[0], ws->ctl, s));
+    TRY(gcxk_sl_hub_bits(g->N, g->perm, ws->h[1], s));
     sl_ctx c = {ws, g, log_variant, g_round_grid};
     TRY(drive_rounds(ws, s, sl_round, &c, offsetof(gcxk_ctl, sl_done), g->N + 2 * ROUND_BATCH + 2));
     return GCX_OK;

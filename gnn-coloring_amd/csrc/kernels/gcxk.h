@@ -128,9 +128,18 @@ int gcxk_sl_init(uint32_t N, const uint32_t *dV, uint32_t *deg, uint32_t *key, u
 int gcxk_sl_scan(uint32_t *deg, uint32_t *key, const uint32_t *alive_in, uint32_t *alive_out,
                  uint32_t *gone, gcxk_ctl *ctl, int log_variant, uint32_t r, uint32_t grid,
                  void *stream);
-/* Round r, phase 2: decrement unremoved neighbours of removed vertices. */
+/* Round r, phase 2: decrement unremoved neighbours of removed vertices.
+ * Decrements to the vertices perm[0 .. nslot) (the highest degrees; hub_bit
+ * marks them, rank maps them to their slot) are summed per workgroup first;
+ * nslot is capped at gcxk_sl_hub_slots(N); hub_bit = NULL disables this. */
 int gcxk_sl_dec(const uint32_t *dV, const uint32_t *dE, uint32_t *deg, const uint32_t *key,
-                const uint32_t *gone, gcxk_ctl *ctl, uint32_t r, uint32_t grid, void *stream);
+                const uint32_t *gone, const uint32_t *hub_bit, const uint32_t *rank,
+                const uint32_t *perm, uint32_t nslot, gcxk_ctl *ctl, uint32_t r, uint32_t grid,
+                void *stream);
+/* Slots of the per-workgroup decrement sums (min(N, 1024)). */
+uint32_t gcxk_sl_hub_slots(uint32_t N);
+/* hub_bit (N / 32 + 1 words): bit v set for v in perm[0 .. gcxk_sl_hub_slots(N)). */
+int gcxk_sl_hub_bits(uint32_t N, const uint32_t *perm, uint32_t *hub_bit, void *stream);
 
 /* ---- GraphSAGE-16 (gnn.hip) ---- */
 /* FP32, bit-exact with the reference op order. kind 0 = input layer (x

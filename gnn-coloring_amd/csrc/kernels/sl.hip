@@ -18,20 +18,8 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr uint32_t kAlive = UINT32_MAX;
-
-__device__ __forceinline__ void wave_append(bool want, uint32_t v, uint32_t *q, uint32_t *n)
-{
-    const uint64_t b = __ballot(want);
-    if (b == 0)
-        return;
-    const int leader = __ffsll((long long)b) - 1;
-    uint32_t base = 0;
-    if (lane_id() == leader)
-        base = atomicAdd(n, (uint32_t)__popcll(b));
-    base = __shfl(base, leader);
-    if (want)
-        q[base + __popcll(b & lanemask_lt())] = v;
-}
+constexpr int kDecSteps = 4;
+constexpr int kScanPer = 8;
 
 __device__ __forceinline__ void wave_min_inv(uint32_t m, uint32_t *slot)
 {
@@ -106,54 +94,211 @@ __global__ __launch_bounds__(kBlock) void sl_scan_kernel(uint32_t *__restrict__ 
     if (n == 0)
         return;
     const uint64_t thr = log_variant ? (delta >= 32 ? (1ull << 32) : (1ull << delta)) : delta;
-    const uint32_t stride = gridDim.x * kBlock;
-    // Uniform trip count per wave so that the ballots see every lane.
-    const uint32_t n_pad = (n + kWave - 1) / kWave * kWave;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i - lane_id() < n_pad; i += stride) {
-        bool rm = false, keep = false;
-        uint32_t u = 0, inv = 0;
-        if (i < n) {
-            u = alive_in[i];
-            const uint32_t d = deg[u];
-            if ((uint64_t)d <= thr) {
-                rm = true;
-                key[u] = delta;
-            } else {
-                keep = true;
-                inv = UINT32_MAX - d;
+    // Tiles of kBlock * kScanPer entries: one reservation per list per tile
+    // (all blocks append to the same two counters, whose atomics serialise).
+    __shared__ uint32_t s_cnt[2][kWavesPerBlock];
+    __shared__ uint32_t s_base[2];
+    const uint32_t tile = kBlock * kScanPer;
+    const uint32_t ntiles = (n + tile - 1) / tile;
+    const int lane = lane_id(), w = threadIdx.x / kWave;
+    uint32_t inv = 0;
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) { // block-uniform trip count
+        uint32_t u[kScanPer];
+        bool rm[kScanPer], keep[kScanPer];
+        uint32_t cr = 0, ck = 0;
+#pragma unroll
+        for (int q = 0; q < kScanPer; q++) {
+            const uint32_t i = t * tile + (uint32_t)q * kBlock + threadIdx.x;
+            rm[q] = keep[q] = false;
+            u[q] = i < n ? alive_in[i] : 0u;
+            if (i < n) {
+                const uint32_t d = deg[u[q]];
+                if ((uint64_t)d <= thr) {
+                    rm[q] = true;
+                    key[u[q]] = delta;
+                } else {
+                    keep[q] = true;
+                    const uint32_t m = UINT32_MAX - d;
+                    inv = m > inv ? m : inv;
+                }
+            }
+            cr += rm[q] ? 1u : 0u;
+            ck += keep[q] ? 1u : 0u;
+        }
+        uint32_t ir = cr, ik = ck; // inclusive scans over the wave
+#pragma unroll
+        for (int off = 1; off < kWave; off <<= 1) {
+            const uint32_t a = __shfl_up(ir, off), b = __shfl_up(ik, off);
+            if (lane >= off) {
+                ir += a;
+                ik += b;
             }
         }
-        wave_append(rm, u, gone, &ctl->sl_gone[s]);
-        wave_append(keep, u, alive_out, &ctl->sl_alive[s1]);
-        wave_min_inv(inv, &ctl->sl_mininv[s]);
+        if (lane == kWave - 1) {
+            s_cnt[0][w] = ir;
+            s_cnt[1][w] = ik;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t tr = 0, tk = 0;
+            for (int q = 0; q < kWavesPerBlock; q++) {
+                const uint32_t a = s_cnt[0][q], b = s_cnt[1][q];
+                s_cnt[0][q] = tr;
+                s_cnt[1][q] = tk;
+                tr += a;
+                tk += b;
+            }
+            s_base[0] = tr ? atomicAdd(&ctl->sl_gone[s], tr) : 0u;
+            s_base[1] = tk ? atomicAdd(&ctl->sl_alive[s1], tk) : 0u;
+        }
+        __syncthreads();
+        uint32_t pr = s_base[0] + s_cnt[0][w] + ir - cr;
+        uint32_t pk = s_base[1] + s_cnt[1][w] + ik - ck;
+#pragma unroll
+        for (int q = 0; q < kScanPer; q++) {
+            if (rm[q])
+                gone[pr++] = u[q];
+            else if (keep[q])
+                alive_out[pk++] = u[q];
+        }
+        __syncthreads(); // s_cnt and s_base are rewritten by the next tile
     }
+    wave_min_inv(inv, &ctl->sl_mininv[s]);
 }
+
+// In rounds that remove at least kAggMinGone vertices, decrements to the
+// kHubSlots highest-degree vertices (hub_bit set, slot = degree-order rank)
+// are summed in LDS and applied once per block: in R-MAT graphs most removed
+// vertices hang off the same few hubs, whose counters would otherwise take
+// thousands of serialised atomics per round.
+constexpr uint32_t kHubSlots = 1024;
+constexpr uint32_t kAggMinGone = 1u << 16;
+
+__device__ __forceinline__ bool is_hub(const uint32_t *__restrict__ hub_bit, uint32_t v)
+{
+    return (hub_bit[v >> 5] >> (v & 31u)) & 1u;
+}
+
+// Decrement the alive neighbours among entries beg + t0, beg + t0 + stride,
+// ... (< end) of a removed vertex's row, STEPS entries per thread in flight
+// (loads, then atomics, then uses); folds the new degrees into inv.
+template <int STEPS>
+__device__ __forceinline__ uint32_t dec_row(uint32_t beg, uint32_t end, uint32_t t0, uint32_t stride,
+                                            const uint32_t *__restrict__ E,
+                                            uint32_t *__restrict__ deg,
+                                            const uint32_t *__restrict__ key,
+                                            const uint32_t *__restrict__ hub_bit,
+                                            const uint32_t *__restrict__ rank, uint32_t *s_hub,
+                                            uint32_t inv)
+{
+    for (uint32_t i0 = beg + t0; i0 < end; i0 += stride * STEPS) {
+        uint32_t v[STEPS], d[STEPS];
+        bool live[STEPS];
+#pragma unroll
+        for (int t = 0; t < STEPS; t++) {
+            const uint32_t i = i0 + (uint32_t)t * stride;
+            live[t] = i < end;
+            v[t] = live[t] ? E[i] : 0u;
+        }
+#pragma unroll
+        for (int t = 0; t < STEPS; t++) {
+            live[t] = live[t] && key[v[t]] == kAlive;
+            if (live[t] && hub_bit && is_hub(hub_bit, v[t])) {
+                atomicAdd(&s_hub[rank[v[t]]], 1u);
+                live[t] = false;
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < STEPS; t++)
+            d[t] = live[t] ? atomicSub(deg + v[t], 1u) : 0u;
+#pragma unroll
+        for (int t = 0; t < STEPS; t++) {
+            const uint32_t m = live[t] ? UINT32_MAX - (d[t] - 1u) : 0u;
+            inv = m > inv ? m : inv;
+        }
+    }
+    return inv;
+}
+
+// Each block takes the removed vertices in groups of one per wave. A wave
+// decrements its own vertex's row; a row longer than kBigRow is done by the
+// whole block after the group's short rows.
+constexpr uint32_t kBigRow = 2048;
 
 __global__ __launch_bounds__(kBlock) void sl_dec_kernel(const uint32_t *__restrict__ V,
                                                         const uint32_t *__restrict__ E,
                                                         uint32_t *__restrict__ deg,
                                                         const uint32_t *__restrict__ key,
                                                         const uint32_t *__restrict__ gone,
-                                                        gcxk_ctl *ctl, uint32_t r)
+                                                        const uint32_t *__restrict__ hub_bit,
+                                                        const uint32_t *__restrict__ rank,
+                                                        const uint32_t *__restrict__ perm,
+                                                        uint32_t nslot, gcxk_ctl *ctl, uint32_t r)
 {
+    __shared__ uint32_t s_beg[kWavesPerBlock], s_end[kWavesPerBlock];
+    __shared__ uint32_t s_hub[kHubSlots];
     const uint32_t s = r % 3;
     const uint32_t n = ctl->sl_gone[s];
-    const uint32_t w0 = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
-    const uint32_t nw = gridDim.x * kWavesPerBlock;
+    if (n == 0)
+        return;
+    // only rounds that remove many vertices pile up on the hubs' counters;
+    // in the others the LDS clear and flush cost more than they save
+    if (n < kAggMinGone)
+        nslot = 0;
+    if (nslot == 0)
+        hub_bit = nullptr;
+    for (uint32_t i = threadIdx.x; i < nslot; i += kBlock)
+        s_hub[i] = 0u;
+    __syncthreads();
+    const int w = threadIdx.x / kWave;
     uint32_t inv = 0;
-    for (uint32_t k = w0; k < n; k += nw) {
-        const uint32_t u = gone[k];
-        const uint32_t beg = V[u], end = V[u + 1];
-        for (uint32_t i = beg + lane_id(); i < end; i += kWave) {
-            const uint32_t v = E[i];
-            if (key[v] == kAlive) {
-                const uint32_t d = atomicSub(deg + v, 1u) - 1u;
-                const uint32_t m = UINT32_MAX - d;
-                inv = m > inv ? m : inv;
-            }
+    for (uint32_t g = blockIdx.x * kWavesPerBlock; g < n; g += gridDim.x * kWavesPerBlock) {
+        const uint32_t k = g + (uint32_t)w;
+        uint32_t beg = 0, end = 0;
+        if (k < n) {
+            const uint32_t u = gone[k];
+            beg = V[u];
+            end = V[u + 1];
+        }
+        const bool big = end - beg > kBigRow;
+        if (!big)
+            inv = dec_row<kDecSteps>(beg, end, (uint32_t)lane_id(), kWave, E, deg, key, hub_bit,
+                                     rank, s_hub, inv);
+        if (lane_id() == 0) {
+            s_beg[w] = big ? beg : 0u;
+            s_end[w] = big ? end : 0u;
+        }
+        __syncthreads();
+        for (int q = 0; q < kWavesPerBlock; q++)
+            if (s_end[q]) // block-uniform
+                inv = dec_row<kDecSteps>(s_beg[q], s_end[q], threadIdx.x, kBlock, E, deg, key,
+                                         hub_bit, rank, s_hub, inv);
+        __syncthreads(); // s_beg / s_end are rewritten by the next group
+    }
+    for (uint32_t i = threadIdx.x; i < nslot; i += kBlock) {
+        const uint32_t c = s_hub[i];
+        if (c) {
+            const uint32_t d = atomicSub(deg + perm[i], c) - c;
+            const uint32_t m = UINT32_MAX - d;
+            inv = m > inv ? m : inv;
         }
     }
     wave_min_inv(inv, &ctl->sl_mininv[s]);
+}
+
+// hub_bit: one bit per vertex, set for perm[0 .. nslot).
+__global__ __launch_bounds__(kBlock) void sl_hub_bits(uint32_t words, const uint32_t *__restrict__ perm,
+                                                      uint32_t nslot, uint32_t *__restrict__ hub_bit,
+                                                      int phase)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (phase == 0) {
+        if (i < words)
+            hub_bit[i] = 0u;
+    } else if (i < nslot) {
+        const uint32_t v = perm[i];
+        atomicOr(&hub_bit[v >> 5], 1u << (v & 31u));
+    }
 }
 
 } // namespace
@@ -178,10 +323,32 @@ extern "C" int gcxk_sl_scan(uint32_t *deg, uint32_t *key, const uint32_t *alive_
 }
 
 extern "C" int gcxk_sl_dec(const uint32_t *dV, const uint32_t *dE, uint32_t *deg,
-                           const uint32_t *key, const uint32_t *gone, gcxk_ctl *ctl, uint32_t r,
-                           uint32_t grid, void *stream)
+                           const uint32_t *key, const uint32_t *gone, const uint32_t *hub_bit,
+                           const uint32_t *rank, const uint32_t *perm, uint32_t nslot,
+                           gcxk_ctl *ctl, uint32_t r, uint32_t grid, void *stream)
 {
-    sl_dec_kernel<<<grid, kBlock, 0, as_stream(stream)>>>(dV, dE, deg, key, gone, ctl, r);
+    if (nslot > kHubSlots)
+        nslot = kHubSlots;
+    if (!hub_bit)
+        nslot = 0;
+    sl_dec_kernel<<<grid, kBlock, 0, as_stream(stream)>>>(dV, dE, deg, key, gone, hub_bit, rank,
+                                                          perm, nslot, ctl, r);
     GCXK_LAUNCH_CHECK();
+    return GCX_OK;
+}
+
+extern "C" uint32_t gcxk_sl_hub_slots(uint32_t N) { return N < kHubSlots ? N : kHubSlots; }
+
+extern "C" int gcxk_sl_hub_bits(uint32_t N, const uint32_t *perm, uint32_t *hub_bit, void *stream)
+{
+    const uint32_t words = N / 32u + 1u, nslot = gcxk_sl_hub_slots(N);
+    sl_hub_bits<<<div_up(words, kBlock), kBlock, 0, as_stream(stream)>>>(words, perm, nslot,
+                                                                         hub_bit, 0);
+    GCXK_LAUNCH_CHECK();
+    if (nslot) {
+        sl_hub_bits<<<div_up(nslot, kBlock), kBlock, 0, as_stream(stream)>>>(words, perm, nslot,
+                                                                             hub_bit, 1);
+        GCXK_LAUNCH_CHECK();
+    }
     return GCX_OK;
 }
