@@ -208,8 +208,9 @@ int gcxk_jp_clear(uint32_t N, uint64_t M, uint32_t *ovf, uint32_t nhub, uint32_t
                   const uint32_t *hubs, uint32_t *npred, uint64_t *state, int *color,
                   gcxk_ctl *ctl, void *stream);
 /* Orientation: key_bits = 32 or 64, mb = gcxk_jp_mask_bits(maxdeg), perm =
- * the graph's degree order (hubs first). E_split rows = predecessors then
- * successors; npred = predecessor counts; state = npred << mb; color = 1 for
+ * the graph's degree order (hubs first). E_split[V[u] + npred[u] ..
+ * V[u+1]) = u's successors (the predecessor part is left unwritten: nothing
+ * reads it); npred = predecessor counts; state = npred << mb; color = 1 for
  * degree-0 vertices, else 0. hchunk: per-hub prefix of 4096-edge chunks
  * (nhub + 1 entries), nhchunk = total chunks. The hub chunks run on
  * hub_stream (may equal stream), after gcxk_jp_clear; the caller joins. */

@@ -7,9 +7,10 @@
 // in (p descending, id ascending) order, so it does not depend on how the
 // work below is scheduled.
 //
-// Orientation (init): every row is split into predecessors first and
-// successors last (E_split, as coloring.c:241-264 does); npred[u] = number of
-// predecessors.
+// Orientation (init): npred[u] = number of predecessors, and u's successors
+// fill E_split[V[u] + npred[u] .. V[u+1]) (the successor part of the split
+// that coloring.c:241-264 builds). The predecessor part is never read --
+// colours arrive through the state words -- so it is not written.
 //
 // Per-vertex state word: state[v] = (pending predecessors << mb) | colour
 // bits, bit c-1 set when a predecessor took colour c (1 <= c <= mb), where
@@ -173,10 +174,11 @@ __global__ __launch_bounds__(kBlock) void jp_clear(uint64_t nwords, uint32_t *__
 }
 
 // Orientation in degree order (perm), 4 lanes per vertex so a wave's 16
-// vertices have near-equal degrees. Also writes every non-hub vertex's colour
-// slot (1 for degree 0, else 0), npred and state[u] = npred << mb. The hubs
-// (perm[0 .. nhub)) are left alone: jp_clear zeroed them and
-// jp_init_hub_chunks splits their rows and adds their counts.
+// vertices have near-equal degrees: successors go to the back of the row in
+// E_split. Also writes every non-hub vertex's colour slot (1 for degree 0,
+// else 0), npred and state[u] = npred << mb. The hubs (perm[0 .. nhub)) are
+// left alone: jp_clear zeroed them and jp_init_hub_chunks places their
+// successors and adds their counts.
 constexpr int kQuad = 4;
 constexpr int kQuadSteps = 4; // 16 edges in flight per vertex
 
@@ -221,12 +223,8 @@ __global__ __launch_bounds__(kBlock) void jp_init_quads(uint32_t N, const uint32
             for (int s = 0; s < kQuadSteps; s++) {
                 const uint32_t gp = (uint32_t)(__ballot(pr[s]) >> gsh) & 0xFu;
                 const uint32_t gs = (uint32_t)(__ballot(in[s] && !pr[s]) >> gsh) & 0xFu;
-                if (in[s]) {
-                    if (pr[s])
-                        esplit[beg + np + __popc(gp & below)] = v[s];
-                    else
-                        esplit[end - 1u - ns - __popc(gs & below)] = v[s];
-                }
+                if (in[s] && !pr[s])
+                    esplit[end - 1u - ns - __popc(gs & below)] = v[s];
                 np += __popc(gp);
                 ns += __popc(gs);
             }
@@ -239,10 +237,12 @@ __global__ __launch_bounds__(kBlock) void jp_init_quads(uint32_t N, const uint32
     }
 }
 
-// Hubs: chunk b of hub h covers edges [V[u] + b*kHubEdges, ...); positions in
-// E_split come from per-hub counters (predecessors from the front,
-// successors from the back); the order inside each part is irrelevant.
+// Hubs: chunk b of hub h covers edges [V[u] + b*kHubEdges, ...); successor
+// positions in E_split come from a per-hub counter (filled from the back of
+// the row; their order is irrelevant) and predecessors are only counted.
 constexpr uint32_t kHubEdges = GCXK_HUB_CHUNK_EDGES;
+
+constexpr int kHubPer = (int)(kHubEdges / kBlock); // entries per thread of a chunk
 
 template <typename K>
 __global__ __launch_bounds__(kBlock) void jp_init_hub_chunks(const uint32_t *__restrict__ V,
@@ -256,6 +256,8 @@ __global__ __launch_bounds__(kBlock) void jp_init_hub_chunks(const uint32_t *__r
                                                              uint32_t nhub,
                                                              uint32_t *__restrict__ hpos, int mb)
 {
+    __shared__ uint32_t s_cnt[2][kWavesPerBlock];
+    __shared__ uint32_t s_base;
     // hub of this block: last h with hchunk[h] <= blockIdx.x
     uint32_t lo = 0, hi = nhub;
     while (hi - lo > 1) {
@@ -270,39 +272,57 @@ __global__ __launch_bounds__(kBlock) void jp_init_hub_chunks(const uint32_t *__r
     const uint32_t cb = beg + (blockIdx.x - hchunk[h]) * kHubEdges;
     const uint32_t ce = cb + kHubEdges < end ? cb + kHubEdges : end;
     const K ku = key[u];
-    const int lane = lane_id();
-    uint32_t npb = 0;
-    for (uint32_t i0 = cb + (threadIdx.x - lane); i0 < ce; i0 += kBlock) {
-        const uint32_t i = i0 + lane;
-        const bool in = i < ce;
-        bool pr = false;
-        uint32_t v = 0;
-        if (in) {
-            v = E[i];
-            pr = precedes(key[v], v, ku, u);
-        }
-        const uint64_t bp = __ballot(in && pr), bs = __ballot(in && !pr);
-        uint32_t pbase = 0, sbase = 0;
-        if (lane == 0) {
-            if (bp)
-                pbase = atomicAdd(&hpos[2 * h], (uint32_t)__popcll(bp));
-            if (bs)
-                sbase = atomicAdd(&hpos[2 * h + 1], (uint32_t)__popcll(bs));
-        }
-        pbase = __shfl(pbase, 0);
-        sbase = __shfl(sbase, 0);
-        if (in) {
-            if (pr)
-                esplit[beg + pbase + __popcll(bp & lanemask_lt())] = v;
-            else
-                esplit[end - 1u - sbase - __popcll(bs & lanemask_lt())] = v;
-        }
-        npb += (uint32_t)__popcll(bp);
+    const int lane = lane_id(), w = threadIdx.x / kWave;
+    // all of the chunk's loads in flight, then one reservation per block
+    uint32_t v[kHubPer];
+    bool in[kHubPer], sc[kHubPer];
+#pragma unroll
+    for (int q = 0; q < kHubPer; q++) {
+        const uint32_t i = cb + (uint32_t)q * kBlock + threadIdx.x;
+        in[q] = i < ce;
+        v[q] = in[q] ? E[i] : 0u;
     }
-    if (lane == 0 && npb) {
-        atomicAdd(&npred[u], npb);
-        atomicAdd((unsigned long long *)&state[u], (unsigned long long)npb << mb);
+    uint32_t np = 0, ns = 0;
+#pragma unroll
+    for (int q = 0; q < kHubPer; q++) {
+        sc[q] = in[q] && !precedes(key[v[q]], v[q], ku, u);
+        np += (in[q] && !sc[q]) ? 1u : 0u;
+        ns += sc[q] ? 1u : 0u;
     }
+    uint32_t is = ns; // inclusive scan of the successor counts over the wave
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const uint32_t a = __shfl_up(is, off);
+        if (lane >= off)
+            is += a;
+    }
+    const uint32_t wp = group_sum32<kWave>(np);
+    if (lane == kWave - 1)
+        s_cnt[1][w] = is;
+    if (lane == 0)
+        s_cnt[0][w] = wp;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tp = 0, ts = 0;
+        for (int q = 0; q < kWavesPerBlock; q++) {
+            tp += s_cnt[0][q];
+            const uint32_t b = s_cnt[1][q];
+            s_cnt[1][q] = ts;
+            ts += b;
+        }
+        s_base = ts ? atomicAdd(&hpos[2 * h + 1], ts) : 0u;
+        if (tp) {
+            atomicAdd(&npred[u], tp);
+            atomicAdd((unsigned long long *)&state[u], (unsigned long long)tp << mb);
+        }
+    }
+    __syncthreads();
+    // successors fill the row from its back; their order is irrelevant
+    uint32_t pos = s_base + s_cnt[1][w] + is - ns;
+#pragma unroll
+    for (int q = 0; q < kHubPer; q++)
+        if (sc[q])
+            esplit[end - 1u - pos++] = v[q];
 }
 
 // ------------------------------------------------------------ round schedule
