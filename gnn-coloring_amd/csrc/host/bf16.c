@@ -78,7 +78,12 @@ int gcx__order_gnn_bf16(gcx_workspace *ws, const gcx_graph *g, const gcx_model *
     hipStream_t s = (hipStream_t)stream;
     const int L = m->layers;
     void *bufs[2] = {ws->x, ws->y};
-    const void *in = NULL;
+    /* layer 0 reads its neighbours' (degree, id) records from y, which
+     * layer 1 overwrites only after layer 0 has joined */
+    int rc0 = gcxk_gnn_input_rec(g->N, g->V, g->rank, m->md, g->fidv, ws->y, s);
+    if (rc0 != GCX_OK)
+        return rc0;
+    const void *in = ws->y;
     for (int l = 0; l < L; l++) {
         const int kind = l == 0 ? 0 : (l == L - 1 ? 2 : 1);
         const float *b = m->dparam + (size_t)l * LAYER_PARAMS + 512;

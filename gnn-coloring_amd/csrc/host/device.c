@@ -1074,8 +1074,11 @@ static int order_sl(gcx_workspace *ws, const gcx_graph *g, int log_variant, hipS
 static int order_gnn_fp32(gcx_workspace *ws, const gcx_graph *g, const gcx_model *m, hipStream_t s)
 {
     const int L = m->layers;
-    const float *in = NULL;
     float *bufs[2] = {ws->x, ws->y};
+    /* layer 0 reads its neighbours' (degree, id) records from y, which
+     * layer 1 overwrites only after layer 0 has joined */
+    TRY(gcxk_gnn_input_rec(g->N, g->V, g->rank, m->md, g->fidv, ws->y, s));
+    const float *in = ws->y;
     for (int l = 0; l < L; l++) {
         const int kind = l == 0 ? 0 : (l == L - 1 ? 2 : 1);
         const float *W = m->dparam + (size_t)l * LAYER_PARAMS;

@@ -142,8 +142,14 @@ uint32_t gcxk_sl_hub_slots(uint32_t N);
 int gcxk_sl_hub_bits(uint32_t N, const uint32_t *perm, uint32_t *hub_bit, void *stream);
 
 /* ---- GraphSAGE-16 (gnn.hip) ---- */
-/* FP32, bit-exact with the reference op order. kind 0 = input layer (x
- * unused), 1 = hidden (x -> out), 2 = output (x -> key, float order keys).
+/* Input-layer neighbour features by degree-order position: rec[rank[v]] =
+ * (float)deg(v) / md and the id feature (fidv[v], or (float)(v / N) when
+ * fidv is NULL), as 2 floats; rec holds 2 N floats. */
+int gcxk_gnn_input_rec(uint32_t N, const uint32_t *dV, const uint32_t *rank, int md,
+                       const float *fidv, void *rec, void *stream);
+/* FP32, bit-exact with the reference op order. kind 0 = input layer (x =
+ * gcxk_gnn_input_rec's records -> out), 1 = hidden (x -> out), 2 = output
+ * (x -> key, float order keys).
  * Rows are 16 floats, stored by degree-order position (row of vertex v =
  * rank[v]; neighbours read through Er, gcxk_rank_columns). Hub rows run on
  * hub_stream, the rest on stream in the degree order perm (non-hubs are
@@ -159,7 +165,8 @@ int gcxk_gnn_fp32_layer(int kind, uint32_t N, const uint32_t *dV, const uint32_t
  * split_mode 0: bf16 rows (16 x bf16 = 32 B), operands rounded to bf16.
  * split_mode 1: FP32 rows, operands and weights split into bf16 hi + lo,
  * three MFMAs per product (near-FP32 results).
- * Rows by degree-order position as in the FP32 layer. Wpack: the layer's two
+ * Rows by degree-order position as in the FP32 layer (input layer: x =
+ * gcxk_gnn_input_rec's records). Wpack: the layer's two
  * MFMA B operands in lane order, hi parts (512 bf16) then lo parts (512);
  * b: FP32 biases (16). */
 int gcxk_gnn_bf16_layer(int kind, int split_mode, uint32_t N, const uint32_t *dV,

@@ -134,6 +134,7 @@ __global__ __launch_bounds__(kBlock) void sage_quads(uint32_t N, const uint32_t 
     __syncthreads();
     const int q = threadIdx.x & 3;
     const float4 *X = reinterpret_cast<const float4 *>(x);
+    const float2 *rec = reinterpret_cast<const float2 *>(x); // input layer
     // whole quads leave the loop together, so the width-4 shuffles stay valid
     for (uint32_t g = p0 + blockIdx.x * kQuads + threadIdx.x / 4; g < p1;
          g += gridDim.x * kQuads) {
@@ -154,9 +155,9 @@ __global__ __launch_bounds__(kBlock) void sage_quads(uint32_t N, const uint32_t 
                     fd[t] = 0.0f;
                     fi[t] = 0.0f;
                     if (i < end) {
-                        const uint32_t v = E[i];
-                        fd[t] = (float)(V[v + 1] - V[v]) / fmd;
-                        fi[t] = fid_of(v, N, fidv);
+                        const float2 r = rec[Er[i]];
+                        fd[t] = r.x;
+                        fi[t] = r.y;
                     }
                 }
                 const uint32_t nb = end - i0 < 16u ? end - i0 : 16u;
@@ -289,9 +290,9 @@ __global__ __launch_bounds__(kHubBlock) void sage_hubs(uint32_t N, const uint32_
         const uint32_t i = beg + c * kHubChunk + tid;
         if (i < end) {
             if (kind == kInput) {
-                const uint32_t v = E[i];
-                buf[s][0][tid] = (float)(V[v + 1] - V[v]) / fmd;
-                buf[s][1][tid] = fid_of(v, N, fidv);
+                const float2 r = reinterpret_cast<const float2 *>(x)[Er[i]];
+                buf[s][0][tid] = r.x;
+                buf[s][1][tid] = r.y;
             } else {
                 const float4 *row = reinterpret_cast<const float4 *>(x + (size_t)Er[i] * F);
 #pragma unroll
@@ -400,7 +401,32 @@ int launch_layer(uint32_t N, const uint32_t *V, const uint32_t *E, const uint32_
     return GCX_OK;
 }
 
+// Input-layer neighbour features in degree order: rec[rank[v]] = (fdeg(v),
+// fid(v)), computed exactly as the layer would (one float division, one
+// double division), so the layer gathers one 8-byte record per neighbour
+// through Er instead of V[v], V[v+1] and the id.
+__global__ __launch_bounds__(kBlock) void input_rec(uint32_t N, const uint32_t *__restrict__ V,
+                                                    const uint32_t *__restrict__ rank, float fmd,
+                                                    const float *__restrict__ fidv,
+                                                    float2 *__restrict__ rec)
+{
+    const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+    if (v < N)
+        rec[rank[v]] = make_float2((float)(V[v + 1] - V[v]) / fmd, fid_of(v, N, fidv));
+}
+
 } // namespace
+
+extern "C" int gcxk_gnn_input_rec(uint32_t N, const uint32_t *dV, const uint32_t *rank, int md,
+                                  const float *fidv, void *rec, void *stream)
+{
+    if (N == 0)
+        return GCX_OK;
+    input_rec<<<div_up(N, kBlock), kBlock, 0, as_stream(stream)>>>(
+        N, dV, rank, (float)md, fidv, static_cast<float2 *>(rec));
+    GCXK_LAUNCH_CHECK();
+    return GCX_OK;
+}
 
 extern "C" int gcxk_gnn_fp32_layer(int kind, uint32_t N, const uint32_t *dV, const uint32_t *dE,
                                    const uint32_t *Er, const uint32_t *rank, const float *W,
@@ -411,7 +437,7 @@ extern "C" int gcxk_gnn_fp32_layer(int kind, uint32_t N, const uint32_t *dV, con
 {
     switch (kind) {
     case 0:
-        return launch_layer<kInput>(N, dV, dE, Er, rank, W, b, (float)md, nullptr, out, nullptr,
+        return launch_layer<kInput>(N, dV, dE, Er, rank, W, b, (float)md, x, out, nullptr,
                                     hubs, nhub, perm, fidv, stream, hub_stream);
     case 1:
         return launch_layer<kHidden>(N, dV, dE, Er, rank, W, b, 0.0f, x, out, nullptr, hubs, nhub,

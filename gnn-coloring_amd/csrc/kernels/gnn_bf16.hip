@@ -135,9 +135,9 @@ __global__ __launch_bounds__(kBlock) void sage_mfma_quads(uint32_t N, const uint
                     fd[t] = 0.0f;
                     fi[t] = 0.0f;
                     if (i < end) {
-                        const uint32_t v = E[i];
-                        fd[t] = (float)(V[v + 1] - V[v]) / fmd;
-                        fi[t] = fid_of(v, N, fidv);
+                        const float2 r = reinterpret_cast<const float2 *>(x)[Er[i]];
+                        fd[t] = r.x;
+                        fi[t] = r.y;
                     }
                 }
                 const uint32_t nb = end - i0 < 16u ? end - i0 : 16u;
@@ -294,9 +294,9 @@ __global__ __launch_bounds__(kHubChunk) void sage_mfma_hubs(uint32_t N, const ui
         const uint32_t i = beg + c * kHubChunk + tid;
         if (i < end) {
             if (kind == kInput) {
-                const uint32_t v = E[i];
-                buf[s][0][tid] = (float)(V[v + 1] - V[v]) / fmd;
-                buf[s][1][tid] = fid_of(v, N, fidv);
+                const float2 r = reinterpret_cast<const float2 *>(x)[Er[i]];
+                buf[s][0][tid] = r.x;
+                buf[s][1][tid] = r.y;
             } else {
                 const uint32_t v = Er[i];
 #pragma unroll
@@ -445,7 +445,7 @@ int layer(int kind, uint32_t N, const uint32_t *dV, const uint32_t *dE, const ui
 {
     switch (kind) {
     case 0:
-        return launch<kInput, mode>(N, dV, dE, Er, rank, Wpack, b, fmd, nullptr, out, nullptr,
+        return launch<kInput, mode>(N, dV, dE, Er, rank, Wpack, b, fmd, x, out, nullptr,
                                     hubs, nhub, perm, fidv, stream, hub_stream);
     case 1:
         return launch<kHidden, mode>(N, dV, dE, Er, rank, Wpack, b, fmd, x, out, nullptr, hubs,
