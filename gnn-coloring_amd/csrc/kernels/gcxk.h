@@ -245,7 +245,7 @@ int gcxk_jp_order_items(uint32_t N, const uint32_t *dV, const uint32_t *npred, c
                         int key_bits, int mb, void *scratch, size_t scratch_bytes, uint32_t *cnt,
                         uint32_t *incl, uint32_t *wof, uint64_t *wdesc, uint64_t *words,
                         uint64_t words_cap, gcxk_item *items, gcxk_ctl *ctl, void *stream);
-/* Default workgroups of the ordered kernel: 8 per CU, at most what is
+/* Default workgroups of the ordered kernel: 6 per CU, at most what is
  * resident at once (0 on error). */
 uint32_t gcxk_jp_ordered_grid(void);
 /* Ordered schedule, step 2: one launch of `grid` workgroups (0 =

@@ -1105,12 +1105,14 @@ extern "C" uint32_t gcxk_jp_ordered_grid(void)
                 hipSuccess ||
             per_cu <= 0)
             return 0;
-        // 8 workgroups per CU (all that fit): with counting-field reports,
-        // orientation + JP on the 160^3 stencil take 6.52 / 5.89 / 5.33 /
-        // 4.98 ms at 3 / 4 / 6 / 8 per CU, and R-MAT 22 is flat from 5 per CU
-        // on (7.72 / 7.65 / 7.64 ms at 5 / 6 / 8)
+        // 6 workgroups per CU (8 fit). Alone on the device, orientation + JP
+        // on the 160^3 stencil take 5.33 / 4.98 ms at 6 / 8 per CU and R-MAT
+        // 22 is flat (7.65 / 7.64 ms); but a launch that needs every resident
+        // slot cannot become resident while other streams' kernels hold some:
+        // with 8 serving workers, cooperative launches of 8 per CU fell from
+        // 69 to 0.6 jobs/s. 6 per CU leaves room for them.
         const uint32_t cap = (uint32_t)per_cu * (uint32_t)prop.multiProcessorCount;
-        const uint32_t want = 8u * (uint32_t)prop.multiProcessorCount;
+        const uint32_t want = 6u * (uint32_t)prop.multiProcessorCount;
         g_ordered_grid = want < cap ? want : cap;
         g_ordered_cap = cap;
     }
