@@ -80,9 +80,6 @@ int gcx__order_gnn_bf16(gcx_workspace *ws, const gcx_graph *g, const gcx_model *
     void *bufs[2] = {ws->x, ws->y};
     /* layer 0 reads its neighbours' (degree, id) records from y, which
      * layer 1 overwrites only after layer 0 has joined */
-    int rc0 = gcxk_gnn_input_rec(g->N, g->V, g->rank, m->md, g->fidv, ws->y, s);
-    if (rc0 != GCX_OK)
-        return rc0;
     const void *in = ws->y;
     for (int l = 0; l < L; l++) {
         const int kind = l == 0 ? 0 : (l == L - 1 ? 2 : 1);
@@ -92,6 +89,9 @@ int gcx__order_gnn_bf16(gcx_workspace *ws, const gcx_graph *g, const gcx_model *
         int rc = gcx__prof_begin(kind == 0 ? GCX_ST_GNN_IN : kind == 1 ? GCX_ST_GNN_HID : GCX_ST_GNN_OUT,
                                  s, &tok);
         if (rc != GCX_OK)
+            return rc;
+        if (kind == 0 &&
+            (rc = gcxk_gnn_input_rec(g->N, g->V, g->rank, m->md, g->fidv, ws->y, s)) != GCX_OK)
             return rc;
         hipStream_t hs = s;
         if (g->nhub) {

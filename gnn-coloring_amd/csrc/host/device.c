@@ -1077,7 +1077,6 @@ static int order_gnn_fp32(gcx_workspace *ws, const gcx_graph *g, const gcx_model
     float *bufs[2] = {ws->x, ws->y};
     /* layer 0 reads its neighbours' (degree, id) records from y, which
      * layer 1 overwrites only after layer 0 has joined */
-    TRY(gcxk_gnn_input_rec(g->N, g->V, g->rank, m->md, g->fidv, ws->y, s));
     const float *in = ws->y;
     for (int l = 0; l < L; l++) {
         const int kind = l == 0 ? 0 : (l == L - 1 ? 2 : 1);
@@ -1087,6 +1086,8 @@ static int order_gnn_fp32(gcx_workspace *ws, const gcx_graph *g, const gcx_model
         void *tok;
         TRY(gcx__prof_begin(kind == 0 ? GCX_ST_GNN_IN : kind == 1 ? GCX_ST_GNN_HID : GCX_ST_GNN_OUT,
                             s, &tok));
+        if (kind == 0)
+            TRY(gcxk_gnn_input_rec(g->N, g->V, g->rank, m->md, g->fidv, ws->y, s));
         hipStream_t hs = s;
         if (g->nhub) {
             HIPC(hipEventRecord((hipEvent_t)ws->ev_fork, s));
