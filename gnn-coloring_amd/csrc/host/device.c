@@ -296,11 +296,14 @@ int gcx_profile_read(double ms[GCX_ST_COUNT], uint64_t launches[GCX_ST_COUNT])
 
 /* -------------------------------------------------------------------- graph */
 
+static int block_get(void **p, size_t bytes, size_t *got, void *stream);
+static void block_put(void *p, size_t bytes, void *stream);
+
 uint64_t gcx_graph_footprint(uint32_t N, uint64_t M)
 {
     /* V and the hub chunk table (N + 1 each), E and Er, hubs, the degree
-     * order and its inverse */
-    return 8ull * ((uint64_t)N + 1) + 8ull * M + 12ull * N;
+     * order and its inverse, each part 256-byte aligned in one block */
+    return 8ull * ((uint64_t)N + 1) + 8ull * M + 12ull * N + 7ull * 256;
 }
 
 /* Degree statistics, hub list (ascending ids) with its 4096-edge chunk table
@@ -334,9 +337,37 @@ static int graph_stats(gcx_graph *g, hipStream_t s)
     return GCX_OK;
 }
 
+/* One allocation carved into the per-graph arrays (and V and E when with_ve
+ * is set): stream-ordered allocations and frees each cost host time, which
+ * adds up over the many small graphs of a serving trace. */
+static int graph_block(gcx_graph *g, int with_ve, void *stream)
+{
+    const uint64_t N = g->N, M = g->M;
+    struct {
+        uint32_t **p;
+        size_t bytes;
+    } part[] = {
+        {&g->hubs, 4 * (N ? N : 1)}, {&g->hchunk, 4 * (N + 1)}, {&g->perm, 4 * (N ? N : 1)},
+        {&g->rank, 4 * (N ? N : 1)}, {&g->Er, 4 * (M ? M : 1)},  {&g->V, 4 * (N + 1)},
+        {&g->E, 4 * (M ? M : 1)},
+    };
+    const size_t nparts = with_ve ? 7 : 5;
+    size_t total = 0;
+    for (size_t i = 0; i < nparts; i++)
+        total += (part[i].bytes + 255) & ~(size_t)255;
+    TRY(block_get(&g->blk, total, &g->blk_bytes, stream));
+    g->blk_ve = with_ve;
+    char *at = (char *)g->blk;
+    for (size_t i = 0; i < nparts; i++) {
+        *part[i].p = (uint32_t *)at;
+        at += (part[i].bytes + 255) & ~(size_t)255;
+    }
+    return GCX_OK;
+}
+
 static int graph_finish(gcx_graph *g, hipStream_t s)
 {
-    TRY(gcx__dalloc((void **)&g->hubs, 4ull * (g->N ? g->N : 1), s));
+    TRY(graph_block(g, 0, s));
     return graph_stats(g, s);
 }
 
@@ -345,8 +376,12 @@ int gcx_graph_free_on(gcx_graph *g, void *stream)
     if (!g)
         return GCX_OK;
     void *bufs[] = {g->V, g->E, g->hubs, g->hchunk, g->perm, g->rank, g->Er, g->fidv, g->goff};
+    /* the first 7 may live in the graph's block: V and E when blk_ve is
+     * set, the next 5 whenever there is a block */
     for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); i++)
-        gcx__dfree(bufs[i], stream);
+        if (!g->blk || (i >= 2 ? i >= 7 : !g->blk_ve))
+            gcx__dfree(bufs[i], stream);
+    block_put(g->blk, g->blk_bytes, stream);
     free(g->hgoff);
     free(g);
     return GCX_OK;
@@ -371,13 +406,7 @@ int gcx__graph_alloc(uint32_t N, uint64_t M, void *stream, gcx_graph **out)
         return GCX_ERR_NOMEM;
     g->N = N;
     g->M = M;
-    if (gcx__dalloc((void **)&g->V, 4ull * (N + 1ull), stream) != GCX_OK ||
-        gcx__dalloc((void **)&g->E, 4ull * (M ? M : 1), stream) != GCX_OK ||
-        gcx__dalloc((void **)&g->hubs, 4ull * (N ? N : 1), stream) != GCX_OK ||
-        gcx__dalloc((void **)&g->perm, 4ull * (N ? N : 1), stream) != GCX_OK ||
-        gcx__dalloc((void **)&g->rank, 4ull * (N ? N : 1), stream) != GCX_OK ||
-        gcx__dalloc((void **)&g->Er, 4ull * (M ? M : 1), stream) != GCX_OK ||
-        gcx__dalloc((void **)&g->hchunk, 4ull * (N + 1ull), stream) != GCX_OK) {
+    if (graph_block(g, 1, stream) != GCX_OK) {
         gcx_graph_free_on(g, stream);
         return GCX_ERR_NOMEM;
     }
@@ -796,13 +825,14 @@ uint64_t gcx_workspace_footprint(uint32_t N, uint64_t M)
 {
     /* per vertex: key 8, count 4, color 4, npred 4, word offsets 4, word
      * descriptors 8, 4 lists x 4, hpos 8, rows x,y 2 x 64; per entry: E_split
-     * 4; plus the state and overflow colour words, two item queues and any
-     * sort scratch beyond the x rows */
+     * 4; plus the state and overflow colour words, two item queues, any
+     * sort scratch beyond the x rows, and the 256-byte alignment of each of
+     * the 21 parts carved from one allocation */
     const uint64_t n = (uint64_t)N + 1;
     return 184ull * n + 8ull * gcxk_jp_state_words(N, M) + 4ull * M +
            4ull * gcxk_jp_ovf_words(N, M) +
            2ull * sizeof(gcxk_item) * gcxk_jp_item_capacity(N, M) + 2 * sizeof(gcxk_ctl) + 256 +
-           order_extra(N);
+           order_extra(N) + 21ull * 256;
 }
 
 /* Pinned control-block mirrors are recycled: freeing pinned host memory
@@ -890,6 +920,86 @@ static void ev_put(hipEvent_t e)
         (void)hipEventDestroy(e);
 }
 
+/* Freed workspace and graph blocks are kept for reuse: hipFreeAsync and
+ * hipMallocAsync cost 50-350 us of host time each, more than a small graph's
+ * whole colouring. A block is reused for a request of at least half its size
+ * after the new stream waits for the event recorded when it was put back;
+ * the cache holds at most BLOCK_CACHE blocks and BLOCK_CACHE_BYTES bytes, and
+ * evicts the largest block first. */
+#define BLOCK_CACHE 32
+#define BLOCK_CACHE_BYTES (16ull << 30)
+static struct {
+    void *p;
+    size_t bytes;
+    hipEvent_t ev;
+} g_bc[BLOCK_CACHE];
+static int g_bc_n = 0;
+static size_t g_bc_bytes = 0;
+static pthread_mutex_t g_bc_lock = PTHREAD_MUTEX_INITIALIZER;
+
+static int block_get(void **p, size_t bytes, size_t *got, void *stream)
+{
+    hipStream_t s = (hipStream_t)gcx__pick(stream);
+    pthread_mutex_lock(&g_bc_lock);
+    int best = -1;
+    for (int i = 0; i < g_bc_n; i++)
+        if (g_bc[i].bytes >= bytes && g_bc[i].bytes / 2 <= bytes &&
+            (best < 0 || g_bc[i].bytes < g_bc[best].bytes))
+            best = i;
+    if (best >= 0) {
+        void *q = g_bc[best].p;
+        const size_t have = g_bc[best].bytes;
+        hipEvent_t ev = g_bc[best].ev;
+        g_bc[best] = g_bc[--g_bc_n];
+        g_bc_bytes -= have;
+        pthread_mutex_unlock(&g_bc_lock);
+        const hipError_t e = hipStreamWaitEvent(s, ev, 0);
+        ev_put(ev);
+        if (e != hipSuccess) {
+            (void)hipFreeAsync(q, s);
+            return GCX_ERR_HIP;
+        }
+        *p = q;
+        *got = have;
+        return GCX_OK;
+    }
+    pthread_mutex_unlock(&g_bc_lock);
+    *got = bytes;
+    return gcx__dalloc(p, bytes, stream);
+}
+
+static void block_put(void *p, size_t bytes, void *stream)
+{
+    if (!p)
+        return;
+    hipStream_t s = (hipStream_t)gcx__pick(stream);
+    hipEvent_t ev = NULL;
+    if (bytes > BLOCK_CACHE_BYTES / 2 || !ev_get(&ev) || hipEventRecord(ev, s) != hipSuccess) {
+        ev_put(ev);
+        (void)hipFreeAsync(p, s);
+        return;
+    }
+    pthread_mutex_lock(&g_bc_lock);
+    while (g_bc_n > 0 && (g_bc_n == BLOCK_CACHE || g_bc_bytes + bytes > BLOCK_CACHE_BYTES)) {
+        int big = 0;
+        for (int i = 1; i < g_bc_n; i++)
+            if (g_bc[i].bytes > g_bc[big].bytes)
+                big = i;
+        /* the evicted block's last use is ordered before its event */
+        (void)hipStreamWaitEvent(s, g_bc[big].ev, 0);
+        (void)hipFreeAsync(g_bc[big].p, s);
+        ev_put(g_bc[big].ev);
+        g_bc_bytes -= g_bc[big].bytes;
+        g_bc[big] = g_bc[--g_bc_n];
+    }
+    g_bc[g_bc_n].p = p;
+    g_bc[g_bc_n].bytes = bytes;
+    g_bc[g_bc_n].ev = ev;
+    g_bc_n++;
+    g_bc_bytes += bytes;
+    pthread_mutex_unlock(&g_bc_lock);
+}
+
 int gcx_workspace_free_on(gcx_workspace *ws, void *stream)
 {
     if (!ws)
@@ -900,12 +1010,7 @@ int gcx_workspace_free_on(gcx_workspace *ws, void *stream)
     if (ws->aux && ws->ev_join &&
         hipEventRecord((hipEvent_t)ws->ev_join, (hipStream_t)ws->aux) == hipSuccess)
         (void)hipStreamWaitEvent((hipStream_t)gcx__pick(stream), (hipEvent_t)ws->ev_join, 0);
-    void *bufs[] = {ws->key,    ws->count, ws->color,    ws->q[0],     ws->q[1],
-                    ws->h[0],   ws->h[1],  ws->x,        ws->y,        ws->ctl,
-                    ws->npred,  ws->state, ws->ovf,      ws->esplit,   ws->items[0],
-                    ws->items[1], ws->hpos, ws->order_buf, ws->wof, ws->wdesc, ws->pass};
-    for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); i++)
-        gcx__dfree(bufs[i], stream);
+    block_put(ws->arena, ws->arena_bytes, stream);
     pinned_ctl_put(ws->hctl);
     aux_put((hipStream_t)ws->aux);
     void *evs[] = {ws->ev_fork, ws->ev_join, ws->ev_poll[0], ws->ev_poll[1]};
@@ -937,35 +1042,50 @@ int gcx_workspace_create_on(uint32_t N, uint64_t M, void *stream, gcx_workspace 
     ws->capN = N;
     ws->capM = M;
     const size_t n = (size_t)N + 1;
-    void *st = stream;
-    int ok = gcx__dalloc(&ws->key, 8 * n, st) == GCX_OK &&
-             gcx__dalloc((void **)&ws->count, 4 * n, st) == GCX_OK &&
-             gcx__dalloc((void **)&ws->color, 4 * n, st) == GCX_OK &&
-             gcx__dalloc((void **)&ws->q[0], 4 * n, st) == GCX_OK &&
-             gcx__dalloc((void **)&ws->q[1], 4 * n, st) == GCX_OK &&
-             gcx__dalloc((void **)&ws->h[0], 4 * n, st) == GCX_OK &&
-             gcx__dalloc((void **)&ws->h[1], 4 * n, st) == GCX_OK &&
-             gcx__dalloc((void **)&ws->x, 64 * n, st) == GCX_OK &&
-             gcx__dalloc((void **)&ws->y, 64 * n, st) == GCX_OK &&
-             gcx__dalloc((void **)&ws->pass, 256, st) == GCX_OK &&
-             gcx__dalloc((void **)&ws->ctl, 2 * sizeof(gcxk_ctl), st) == GCX_OK &&
-             gcx__dalloc((void **)&ws->npred, 4 * n, st) == GCX_OK &&
-             gcx__dalloc((void **)&ws->state, 8 * gcxk_jp_state_words(N, M), st) == GCX_OK &&
-             gcx__dalloc((void **)&ws->wof, 4 * n, st) == GCX_OK &&
-             gcx__dalloc((void **)&ws->wdesc, 8 * n, st) == GCX_OK &&
-             gcx__dalloc((void **)&ws->hpos, 8 * n, st) == GCX_OK &&
-             gcx__dalloc((void **)&ws->ovf, 4 * gcxk_jp_ovf_words(N, M), st) == GCX_OK &&
-             gcx__dalloc((void **)&ws->esplit, 4 * (M ? M : 1), st) == GCX_OK &&
-             gcx__dalloc((void **)&ws->items[0],
-                         sizeof(gcxk_item) * gcxk_jp_item_capacity(N, M), st) == GCX_OK &&
-             gcx__dalloc((void **)&ws->items[1],
-                         sizeof(gcxk_item) * gcxk_jp_item_capacity(N, M), st) == GCX_OK &&
-             (ws->hctl = pinned_ctl_get()) != NULL;
+    const size_t items = sizeof(gcxk_item) * gcxk_jp_item_capacity(N, M);
+    const size_t extra = order_extra(N);
+    ws->order_bytes = extra ? extra : 64 * n;
+    /* One pool allocation carved into every buffer (256-byte aligned): a
+     * serving job creates and frees a workspace, and each stream-ordered
+     * allocation or free costs tens to hundreds of microseconds of host time. */
+    struct {
+        void **p;
+        size_t bytes;
+    } part[] = {
+        {&ws->key, 8 * n},
+        {(void **)&ws->count, 4 * n},
+        {(void **)&ws->color, 4 * n},
+        {(void **)&ws->q[0], 4 * n},
+        {(void **)&ws->q[1], 4 * n},
+        {(void **)&ws->h[0], 4 * n},
+        {(void **)&ws->h[1], 4 * n},
+        {(void **)&ws->x, 64 * n},
+        {(void **)&ws->y, 64 * n},
+        {(void **)&ws->pass, 256},
+        {(void **)&ws->ctl, 2 * sizeof(gcxk_ctl)},
+        {(void **)&ws->npred, 4 * n},
+        {(void **)&ws->state, 8 * gcxk_jp_state_words(N, M)},
+        {(void **)&ws->wof, 4 * n},
+        {(void **)&ws->wdesc, 8 * n},
+        {(void **)&ws->hpos, 8 * n},
+        {(void **)&ws->ovf, 4 * gcxk_jp_ovf_words(N, M)},
+        {(void **)&ws->esplit, 4 * (M ? M : 1)},
+        {(void **)&ws->items[0], items},
+        {(void **)&ws->items[1], items},
+        {&ws->order_buf, extra},
+    };
+    const size_t nparts = sizeof(part) / sizeof(part[0]);
+    size_t total = 0;
+    for (size_t i = 0; i < nparts; i++)
+        total += (part[i].bytes + 255) & ~(size_t)255;
+    int ok = block_get(&ws->arena, total, &ws->arena_bytes, stream) == GCX_OK;
     if (ok) {
-        const size_t extra = order_extra(N);
-        ws->order_bytes = extra ? extra : 64 * n;
-        if (extra)
-            ok = gcx__dalloc(&ws->order_buf, extra, st) == GCX_OK;
+        char *at = (char *)ws->arena;
+        for (size_t i = 0; i < nparts; i++) {
+            *part[i].p = part[i].bytes ? (void *)at : NULL;
+            at += (part[i].bytes + 255) & ~(size_t)255;
+        }
+        ok = (ws->hctl = pinned_ctl_get()) != NULL;
     }
     if (ok) {
         hipStream_t aux;

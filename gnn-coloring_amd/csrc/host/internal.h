@@ -27,6 +27,9 @@ struct gcx_graph {
     uint32_t *goff;   /* device, count + 1, or NULL: first vertex of each packed graph */
     uint32_t *hgoff;  /* host copy of goff (malloc'ed), or NULL */
     uint32_t count;   /* packed graphs (0 = a single graph) */
+    void *blk;        /* one device allocation holding hubs, hchunk, perm, rank, Er */
+    int blk_ve;       /* ... and V and E */
+    size_t blk_bytes; /* size of blk */
 };
 
 struct gcx_model {
@@ -51,6 +54,8 @@ struct gcx_workspace {
     uint64_t *state;  /* JP state words (gcxk_jp_state_words; layouts in jp.hip) */
     uint32_t *wof;    /* JP (ordered): first state word of each vertex, N + 1 */
     uint64_t *wdesc;  /* JP (ordered): state word descriptor of each vertex, N */
+    void *arena;      /* the one device allocation all of the above are carved from */
+    size_t arena_bytes;
     uint32_t *ovf;    /* JP: colour-set overflow words (colours > mb) */
     uint32_t *esplit; /* JP: rows split into predecessors | successors (capM) */
     gcxk_item *items[2]; /* JP: round queues; items[0] = greedy-order items */

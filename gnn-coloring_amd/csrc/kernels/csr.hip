@@ -374,6 +374,22 @@ struct DevBuf {
     }
 };
 
+// Several temporaries carved from one DevBuf (256-byte aligned parts): each
+// stream-ordered allocation and free costs host time, which adds up over the
+// many small graphs of a serving trace.
+struct DevParts {
+    DevBuf buf;
+    size_t total = 0;
+    size_t add(size_t bytes)
+    {
+        const size_t at = total;
+        total += (bytes + 255) & ~(size_t)255;
+        return at;
+    }
+    hipError_t alloc(hipStream_t st) { return buf.alloc(total, st); }
+    template <typename T> T *at(size_t off) { return (T *)((char *)buf.p + off); }
+};
+
 // Sort keys for the degree order: maxdeg - deg (so ascending = degree
 // descending) and the vertex ids.
 __global__ __launch_bounds__(kBlock) void degree_keys(uint32_t N, const uint32_t *__restrict__ V,
@@ -399,33 +415,35 @@ extern "C" int gcxk_csr_from_pairs(uint32_t N, uint64_t L, const uint32_t *dI, c
         return GCX_ERR_ARG;
     const int b = bit_length(N - 1) ? bit_length(N - 1) : 1;
     const uint64_t n = 2 * L;
-    DevBuf keys, keys2, flag, pos, temp, temp2;
+    DevParts tmp;
     uint64_t *k_out = nullptr;
+    uint32_t *flag = nullptr, *pos = nullptr;
     uint32_t total = 0;
     if (L) {
-        GCXK_CHECK(keys.alloc(n * sizeof(uint64_t), s));
-        GCXK_CHECK(keys2.alloc(n * sizeof(uint64_t), s));
-        uint64_t *k_in = (uint64_t *)keys.p;
-        k_out = (uint64_t *)keys2.p;
+        size_t tb = 0, tb2 = 0;
+        GCXK_CHECK(rocprim::radix_sort_keys(nullptr, tb, (uint64_t *)nullptr, (uint64_t *)nullptr,
+                                            (size_t)n, 0, 2 * b, s));
+        GCXK_CHECK(rocprim::exclusive_scan(nullptr, tb2, (uint32_t *)nullptr, (uint32_t *)nullptr,
+                                           0u, (size_t)n, rocprim::plus<uint32_t>(), s));
+        const size_t o_in = tmp.add(n * sizeof(uint64_t)), o_out = tmp.add(n * sizeof(uint64_t));
+        const size_t o_flag = tmp.add(n * sizeof(uint32_t)), o_pos = tmp.add(n * sizeof(uint32_t));
+        const size_t o_t = tmp.add(tb ? tb : 1), o_t2 = tmp.add(tb2 ? tb2 : 1);
+        GCXK_CHECK(tmp.alloc(s));
+        uint64_t *k_in = tmp.at<uint64_t>(o_in);
+        k_out = tmp.at<uint64_t>(o_out);
+        flag = tmp.at<uint32_t>(o_flag);
+        pos = tmp.at<uint32_t>(o_pos);
         expand_pairs<<<div_up(L, kBlock), kBlock, 0, s>>>(L, dI, dJ, b, k_in);
         GCXK_LAUNCH_CHECK();
-        size_t tb = 0;
-        GCXK_CHECK(rocprim::radix_sort_keys(nullptr, tb, k_in, k_out, (size_t)n, 0, 2 * b, s));
-        GCXK_CHECK(temp.alloc(tb, s));
-        GCXK_CHECK(rocprim::radix_sort_keys(temp.p, tb, k_in, k_out, (size_t)n, 0, 2 * b, s));
-        GCXK_CHECK(flag.alloc(n * sizeof(uint32_t), s));
-        GCXK_CHECK(pos.alloc(n * sizeof(uint32_t), s));
-        mark_unique<<<div_up(n, kBlock), kBlock, 0, s>>>(n, k_out, b, (uint32_t *)flag.p);
+        GCXK_CHECK(rocprim::radix_sort_keys(tmp.at<void>(o_t), tb, k_in, k_out, (size_t)n, 0, 2 * b,
+                                            s));
+        mark_unique<<<div_up(n, kBlock), kBlock, 0, s>>>(n, k_out, b, flag);
         GCXK_LAUNCH_CHECK();
-        size_t tb2 = 0;
-        GCXK_CHECK(rocprim::exclusive_scan(nullptr, tb2, (uint32_t *)flag.p, (uint32_t *)pos.p, 0u,
-                                           (size_t)n, rocprim::plus<uint32_t>(), s));
-        GCXK_CHECK(temp2.alloc(tb2, s));
-        GCXK_CHECK(rocprim::exclusive_scan(temp2.p, tb2, (uint32_t *)flag.p, (uint32_t *)pos.p, 0u,
-                                           (size_t)n, rocprim::plus<uint32_t>(), s));
+        GCXK_CHECK(rocprim::exclusive_scan(tmp.at<void>(o_t2), tb2, flag, pos, 0u, (size_t)n,
+                                           rocprim::plus<uint32_t>(), s));
         uint32_t last[2] = {0, 0};
-        GCXK_CHECK(hipMemcpyAsync(&last[0], (uint32_t *)pos.p + (n - 1), 4, hipMemcpyDeviceToHost, s));
-        GCXK_CHECK(hipMemcpyAsync(&last[1], (uint32_t *)flag.p + (n - 1), 4, hipMemcpyDeviceToHost, s));
+        GCXK_CHECK(hipMemcpyAsync(&last[0], pos + (n - 1), 4, hipMemcpyDeviceToHost, s));
+        GCXK_CHECK(hipMemcpyAsync(&last[1], flag + (n - 1), 4, hipMemcpyDeviceToHost, s));
         GCXK_CHECK(hipStreamSynchronize(s));
         total = last[0] + last[1];
     }
@@ -433,8 +451,7 @@ extern "C" int gcxk_csr_from_pairs(uint32_t N, uint64_t L, const uint32_t *dI, c
     GCXK_CHECK(hipMallocAsync((void **)&E, (total ? total : 1) * sizeof(uint32_t), s));
     hipError_t e = hipSuccess;
     if (total) {
-        scatter_rows<<<div_up(n, kBlock), kBlock, 0, s>>>(n, k_out, (uint32_t *)flag.p,
-                                                           (uint32_t *)pos.p, b, N, total, dV, E);
+        scatter_rows<<<div_up(n, kBlock), kBlock, 0, s>>>(n, k_out, flag, pos, b, N, total, dV, E);
         e = hipGetLastError();
     } else {
         e = hipMemsetAsync(dV, 0, ((size_t)N + 1) * sizeof(uint32_t), s);
@@ -488,31 +505,18 @@ extern "C" int gcxk_degree_order(uint32_t N, const uint32_t *dV, uint32_t maxdeg
         return GCX_OK;
     hipStream_t s = as_stream(stream);
     const unsigned bits = maxdeg ? 32u - (unsigned)__builtin_clz(maxdeg) : 1u;
-    uint32_t *k0 = nullptr, *k1 = nullptr, *id = nullptr;
-    void *tmp = nullptr;
     size_t tb = 0;
-    int rc = GCX_ERR_HIP;
-    if (hipMallocAsync((void **)&k0, 4ull * N, s) != hipSuccess ||
-        hipMallocAsync((void **)&k1, 4ull * N, s) != hipSuccess ||
-        hipMallocAsync((void **)&id, 4ull * N, s) != hipSuccess)
-        goto done;
+    GCXK_CHECK(rocprim::radix_sort_pairs(nullptr, tb, (uint32_t *)nullptr, (uint32_t *)nullptr,
+                                         (uint32_t *)nullptr, (uint32_t *)nullptr, N, 0u, bits, s));
+    DevParts tmp;
+    const size_t o0 = tmp.add(4ull * N), o1 = tmp.add(4ull * N), oid = tmp.add(4ull * N);
+    const size_t ot = tmp.add(tb ? tb : 1);
+    GCXK_CHECK(tmp.alloc(s));
+    uint32_t *k0 = tmp.at<uint32_t>(o0), *k1 = tmp.at<uint32_t>(o1), *id = tmp.at<uint32_t>(oid);
     degree_keys<<<div_up(N, kBlock), kBlock, 0, s>>>(N, dV, maxdeg, k0, id);
-    if (hipGetLastError() != hipSuccess ||
-        rocprim::radix_sort_pairs(nullptr, tb, k0, k1, id, perm, N, 0u, bits, s) != hipSuccess ||
-        hipMallocAsync(&tmp, tb ? tb : 1, s) != hipSuccess ||
-        rocprim::radix_sort_pairs(tmp, tb, k0, k1, id, perm, N, 0u, bits, s) != hipSuccess)
-        goto done;
-    rc = GCX_OK;
-done:
-    if (k0)
-        (void)hipFreeAsync(k0, s);
-    if (k1)
-        (void)hipFreeAsync(k1, s);
-    if (id)
-        (void)hipFreeAsync(id, s);
-    if (tmp)
-        (void)hipFreeAsync(tmp, s);
-    return rc;
+    GCXK_LAUNCH_CHECK();
+    GCXK_CHECK(rocprim::radix_sort_pairs(tmp.at<void>(ot), tb, k0, k1, id, perm, N, 0u, bits, s));
+    return GCX_OK;
 }
 
 extern "C" int gcxk_graph_stats(uint32_t N, const uint32_t *dV, uint32_t hub_deg, uint32_t *dhub,
@@ -521,28 +525,28 @@ extern "C" int gcxk_graph_stats(uint32_t N, const uint32_t *dV, uint32_t hub_deg
 {
     hipStream_t s = as_stream(stream);
     *nhub = *maxdeg = *nchunk = 0;
-    DevBuf flag, chunks, pos, cpre, mx, temp;
     const size_t n1 = (size_t)N + 1;
-    GCXK_CHECK(flag.alloc(4 * n1, s));
-    GCXK_CHECK(chunks.alloc(4 * n1, s));
-    GCXK_CHECK(pos.alloc(4 * n1, s));
-    GCXK_CHECK(cpre.alloc(4 * n1, s));
-    GCXK_CHECK(mx.alloc(4, s));
-    GCXK_CHECK(hipMemsetAsync(mx.p, 0, 4, s));
-    uint32_t *f = (uint32_t *)flag.p, *c = (uint32_t *)chunks.p, *ps = (uint32_t *)pos.p,
-             *cp = (uint32_t *)cpre.p;
-    hub_flags<<<div_up(n1, kBlock), kBlock, 0, s>>>(N, dV, hub_deg, f, c, (uint32_t *)mx.p);
-    GCXK_LAUNCH_CHECK();
     size_t tb = 0;
-    GCXK_CHECK(rocprim::exclusive_scan(nullptr, tb, f, ps, 0u, n1, rocprim::plus<uint32_t>(), s));
-    GCXK_CHECK(temp.alloc(tb, s));
-    GCXK_CHECK(rocprim::exclusive_scan(temp.p, tb, f, ps, 0u, n1, rocprim::plus<uint32_t>(), s));
-    GCXK_CHECK(rocprim::exclusive_scan(temp.p, tb, c, cp, 0u, n1, rocprim::plus<uint32_t>(), s));
+    GCXK_CHECK(rocprim::exclusive_scan(nullptr, tb, (uint32_t *)nullptr, (uint32_t *)nullptr, 0u,
+                                       n1, rocprim::plus<uint32_t>(), s));
+    DevParts tmp;
+    const size_t of = tmp.add(4 * n1), oc = tmp.add(4 * n1), op = tmp.add(4 * n1),
+                 ocp = tmp.add(4 * n1), om = tmp.add(4), ot = tmp.add(tb ? tb : 1);
+    GCXK_CHECK(tmp.alloc(s));
+    uint32_t *f = tmp.at<uint32_t>(of), *c = tmp.at<uint32_t>(oc), *ps = tmp.at<uint32_t>(op),
+             *cp = tmp.at<uint32_t>(ocp), *mx = tmp.at<uint32_t>(om);
+    GCXK_CHECK(hipMemsetAsync(mx, 0, 4, s));
+    hub_flags<<<div_up(n1, kBlock), kBlock, 0, s>>>(N, dV, hub_deg, f, c, mx);
+    GCXK_LAUNCH_CHECK();
+    GCXK_CHECK(rocprim::exclusive_scan(tmp.at<void>(ot), tb, f, ps, 0u, n1,
+                                       rocprim::plus<uint32_t>(), s));
+    GCXK_CHECK(rocprim::exclusive_scan(tmp.at<void>(ot), tb, c, cp, 0u, n1,
+                                       rocprim::plus<uint32_t>(), s));
     hub_scatter<<<div_up(n1, kBlock), kBlock, 0, s>>>(N, f, ps, cp, dhub, dchunk);
     GCXK_LAUNCH_CHECK();
     uint32_t h[3] = {0, 0, 0};
     GCXK_CHECK(hipMemcpyAsync(&h[0], ps + N, 4, hipMemcpyDeviceToHost, s));
-    GCXK_CHECK(hipMemcpyAsync(&h[1], mx.p, 4, hipMemcpyDeviceToHost, s));
+    GCXK_CHECK(hipMemcpyAsync(&h[1], mx, 4, hipMemcpyDeviceToHost, s));
     GCXK_CHECK(hipMemcpyAsync(&h[2], cp + N, 4, hipMemcpyDeviceToHost, s));
     GCXK_CHECK(hipStreamSynchronize(s));
     *nhub = h[0];
