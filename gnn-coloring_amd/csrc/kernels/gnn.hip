@@ -16,24 +16,25 @@
 // Mapping: vertices below the hub degree run in degree order (the graph's
 // perm), 4 lanes per vertex and 16 vertices per wave (see sage_quads); the
 // sequential dependency is only on the ordered adds, the row loads are
-// issued ahead. Hubs (degree >= GCXK_HUB_DEG) get a 512-thread workgroup that stages 512
-// neighbour rows per step into LDS (double-buffered, feature-major) while one
-// 16-lane group performs the ordered sum; the hub kernel runs on a second
-// stream next to the group kernel.
+// issued ahead. Hubs (degree >= the graph's hub degree) get a 256-thread
+// workgroup that stages 256 neighbour rows per step into LDS (feature-major,
+// double-buffered, the next step's loads in flight during the sum) while one
+// 16-lane group performs the ordered sum (hub_sum.hip.h); the hub kernel runs
+// on a second stream next to the group kernel.
 //
 // This file is compiled with -ffp-contract=off: the only fused operations
 // are the explicit fmaf calls of the dense step.
 #include "common.hip.h"
+#include "hub_sum.hip.h"
 
 namespace {
 
 constexpr int kBlock = 256;
 constexpr int F = 16;
-constexpr int kHubBlock = 512;
-constexpr int kHubChunk = kHubBlock;
-// Row stride padding of the feature-major hub staging buffer: 4 floats put
-// the 16 summing lanes' 16-byte reads on distinct LDS banks.
-constexpr int kHubPad = 4;
+// Threads (= rows per staged chunk) of the hub kernel. Measured on R-MAT
+// scale 22 (GNN3 ordering): 256 -> 4.43 ms, 512 -> 4.87 ms, 1024 -> 6.08 ms;
+// more, smaller workgroups keep more of the hubs' add chains running.
+constexpr int kHubBlock = 256;
 
 enum LayerKind { kInput = 0, kHidden = 1, kOutput = 2 };
 
@@ -276,71 +277,27 @@ __global__ __launch_bounds__(kHubBlock) void sage_hubs(uint32_t N, const uint32_
                                                        const uint32_t *__restrict__ hubs,
                                                        const float *__restrict__ fidv)
 {
-    // Feature-major staging: buf[s][f][r] = feature f of row r of step s.
-    __shared__ __attribute__((aligned(16))) float buf[2][F][kHubChunk + kHubPad];
+    __shared__ __attribute__((aligned(16))) hubsum::Stage<kHubBlock> st;
     const uint32_t u = hubs[blockIdx.x];
     const uint32_t beg = V[u], end = V[u + 1];
     const uint32_t deg = end - beg;
     const int tid = threadIdx.x;
-    const bool summer = tid < F; // lanes 0..15 of wave 0 own the ordered sums
     const int j = tid & (F - 1);
-    float acc = 0.0f;
-
-    auto stage = [&](uint32_t c, int s) {
-        const uint32_t i = beg + c * kHubChunk + tid;
-        if (i < end) {
-            if (kind == kInput) {
-                const float2 r = reinterpret_cast<const float2 *>(x)[Er[i]];
-                buf[s][0][tid] = r.x;
-                buf[s][1][tid] = r.y;
-            } else {
-                const float4 *row = reinterpret_cast<const float4 *>(x + (size_t)Er[i] * F);
+    // Layer 0: lane 0 sums fdeg, lane 1 sums fid. Other layers: lane j sums
+    // feature j.
+    float acc;
+    if (kind == kInput) {
+        acc = hubsum::ordered_sum<kHubBlock, true>(st, beg, end, Er, [&](uint32_t v, float4 *row) {
+            const float2 r = reinterpret_cast<const float2 *>(x)[v];
+            row[0] = make_float4(r.x, r.y, 0.0f, 0.0f);
+        });
+    } else {
+        acc = hubsum::ordered_sum<kHubBlock, false>(st, beg, end, Er, [&](uint32_t v, float4 *row) {
+            const float4 *src = reinterpret_cast<const float4 *>(x + (size_t)v * F);
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const float4 r4 = row[q];
-                    buf[s][4 * q + 0][tid] = r4.x;
-                    buf[s][4 * q + 1][tid] = r4.y;
-                    buf[s][4 * q + 2][tid] = r4.z;
-                    buf[s][4 * q + 3][tid] = r4.w;
-                }
-            }
-        }
-    };
-
-    const uint32_t nchunk = (deg + kHubChunk - 1) / kHubChunk;
-    stage(0, 0);
-    __syncthreads();
-    for (uint32_t c = 0; c < nchunk; c++) {
-        const int s = c & 1;
-        if (c + 1 < nchunk)
-            stage(c + 1, s ^ 1);
-        if (summer) {
-            const uint32_t rows = deg - c * kHubChunk < (uint32_t)kHubChunk ? deg - c * kHubChunk
-                                                                          : (uint32_t)kHubChunk;
-            // Layer 0: lane 0 sums fdeg, lane 1 sums fid. Other layers: lane j
-            // sums feature j. 32 rows of LDS reads are issued ahead of the
-            // ordered adds so the add chain does not wait on each read.
-            if (kind != kInput || j < 2) {
-                const float *col = buf[s][j];
-                uint32_t r = 0;
-                for (; r + 32 <= rows; r += 32) {
-                    float4 q[8];
-#pragma unroll
-                    for (int t = 0; t < 8; t++)
-                        q[t] = *reinterpret_cast<const float4 *>(col + r + 4 * t);
-#pragma unroll
-                    for (int t = 0; t < 8; t++) {
-                        acc = acc + q[t].x;
-                        acc = acc + q[t].y;
-                        acc = acc + q[t].z;
-                        acc = acc + q[t].w;
-                    }
-                }
-                for (; r < rows; r++)
-                    acc = acc + col[r];
-            }
-        }
-        __syncthreads();
+            for (int q = 0; q < 4; q++)
+                row[q] = src[q];
+        });
     }
     if (tid >= kWave)
         return;

@@ -28,6 +28,7 @@
 // not depend on the hub degree; both kernels only read the previous layer and
 // write disjoint rows.
 #include "common.hip.h"
+#include "hub_sum.hip.h"
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -260,14 +261,17 @@ __global__ __launch_bounds__(kBlock) void sage_mfma_quads(uint32_t N, const uint
 }
 
 // One workgroup per hub. Neighbour rows are staged into LDS feature-major,
-// 512 per step (double-buffered), and lanes 0..15 of wave 0 sum them in CSR
-// order -- the order of the quad kernel's sums, so the mean is the same bits
-// whichever kernel handles the vertex. Wave 0 then runs the same MFMA
+// 512 per step with the next step's loads in flight (hub_sum.hip.h), and
+// lanes 0..15 of wave 0 sum them in CSR order -- the order of the quad
+// kernel's sums, so the mean is the same bits whichever kernel handles the
+// vertex. Wave 0 then runs the same MFMA
 // sequence with the hub as row 0 of the A tile (the other rows zero): each
 // output row of an MFMA depends only on its own A row, so the priorities and
 // rows do not depend on the hub degree.
+// Threads (= rows per staged chunk) of the hub kernel. Measured on R-MAT
+// scale 22 (GNN3 ordering, BF16): 256 -> 4.91 ms, 512 -> 4.63 ms,
+// 1024 -> 5.80 ms.
 constexpr int kHubChunk = 512;
-constexpr int kHubPad = 4;
 
 template <int kind, int mode>
 __global__ __launch_bounds__(kHubChunk) void sage_mfma_hubs(uint32_t N, const uint32_t *__restrict__ V,
@@ -282,65 +286,24 @@ __global__ __launch_bounds__(kHubChunk) void sage_mfma_hubs(uint32_t N, const ui
                                                             const uint32_t *__restrict__ hubs,
                                                             const float *__restrict__ fidv)
 {
-    __shared__ __attribute__((aligned(16))) float buf[2][F][kHubChunk + kHubPad];
+    __shared__ __attribute__((aligned(16))) hubsum::Stage<kHubChunk> st;
     const uint32_t u = hubs[blockIdx.x];
     const uint32_t beg = V[u], end = V[u + 1];
     const uint32_t deg = end - beg;
     const int tid = threadIdx.x;
-    const int j = tid & (F - 1);
-    float acc = 0.0f;
-
-    auto stage = [&](uint32_t c, int s) {
-        const uint32_t i = beg + c * kHubChunk + tid;
-        if (i < end) {
-            if (kind == kInput) {
-                const float2 r = reinterpret_cast<const float2 *>(x)[Er[i]];
-                buf[s][0][tid] = r.x;
-                buf[s][1][tid] = r.y;
-            } else {
-                const uint32_t v = Er[i];
+    // layer 0: lane 0 sums fdeg, lane 1 fid; else lane j feature j
+    float acc;
+    if (kind == kInput) {
+        acc = hubsum::ordered_sum<kHubChunk, true>(st, beg, end, Er, [&](uint32_t v, float4 *row) {
+            const float2 r = reinterpret_cast<const float2 *>(x)[v];
+            row[0] = make_float4(r.x, r.y, 0.0f, 0.0f);
+        });
+    } else {
+        acc = hubsum::ordered_sum<kHubChunk, false>(st, beg, end, Er, [&](uint32_t v, float4 *row) {
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const float4 r4 = load_slice<mode>(x, v, q);
-                    buf[s][4 * q + 0][tid] = r4.x;
-                    buf[s][4 * q + 1][tid] = r4.y;
-                    buf[s][4 * q + 2][tid] = r4.z;
-                    buf[s][4 * q + 3][tid] = r4.w;
-                }
-            }
-        }
-    };
-
-    const uint32_t nchunk = (deg + kHubChunk - 1) / kHubChunk;
-    stage(0, 0);
-    __syncthreads();
-    for (uint32_t c = 0; c < nchunk; c++) {
-        const int s = c & 1;
-        if (c + 1 < nchunk)
-            stage(c + 1, s ^ 1);
-        if (tid < F && (kind != kInput || j < 2)) {
-            // layer 0: lane 0 sums fdeg, lane 1 fid; else lane j feature j
-            const uint32_t rows = deg - c * kHubChunk < (uint32_t)kHubChunk ? deg - c * kHubChunk
-                                                                          : (uint32_t)kHubChunk;
-            const float *col = buf[s][j];
-            uint32_t r = 0;
-            for (; r + 32 <= rows; r += 32) {
-                float4 q[8];
-#pragma unroll
-                for (int t = 0; t < 8; t++)
-                    q[t] = *reinterpret_cast<const float4 *>(col + r + 4 * t);
-#pragma unroll
-                for (int t = 0; t < 8; t++) {
-                    acc = acc + q[t].x;
-                    acc = acc + q[t].y;
-                    acc = acc + q[t].z;
-                    acc = acc + q[t].w;
-                }
-            }
-            for (; r < rows; r++)
-                acc = acc + col[r];
-        }
-        __syncthreads();
+            for (int q = 0; q < 4; q++)
+                row[q] = load_slice<mode>(x, v, q);
+        });
     }
     if (tid >= kWave)
         return;
