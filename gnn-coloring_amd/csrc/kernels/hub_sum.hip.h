@@ -24,21 +24,50 @@ struct Stage {
 };
 
 // Sums the first `rows` entries of one staged feature column, in order,
-// into acc: 32 rows of LDS reads are issued ahead of their 32 adds.
+// into acc. Blocks of 32 rows alternate between two register sets so that
+// the LDS reads of the next block are in flight during the adds of the
+// current one; scheduling fences keep the compiler from moving the reads
+// back behind the adds (which would put every block's LDS latency on the
+// add chain).
 __device__ __forceinline__ float ordered_column_sum(const float *col, uint32_t rows, float acc)
 {
-    uint32_t r = 0;
-    for (; r + 32 <= rows; r += 32) {
-        float4 q[8];
+    auto rd = [&](float4(&q)[8], uint32_t r) {
 #pragma unroll
         for (int t = 0; t < 8; t++)
             q[t] = *reinterpret_cast<const float4 *>(col + r + 4 * t);
+    };
+    auto add = [&](const float4(&q)[8]) {
 #pragma unroll
         for (int t = 0; t < 8; t++) {
             acc = acc + q[t].x;
             acc = acc + q[t].y;
             acc = acc + q[t].z;
             acc = acc + q[t].w;
+        }
+    };
+    uint32_t r = 0;
+    if (rows >= 32) {
+        float4 A[8], B[8];
+        rd(A, 0);
+        // A holds rows [r-32, r), read but not yet added
+        for (r = 32; r + 64 <= rows; r += 64) {
+            rd(B, r);
+            __builtin_amdgcn_sched_barrier(0);
+            add(A);
+            __builtin_amdgcn_sched_barrier(0);
+            rd(A, r + 32);
+            __builtin_amdgcn_sched_barrier(0);
+            add(B);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        if (r + 32 <= rows) {
+            rd(B, r);
+            __builtin_amdgcn_sched_barrier(0);
+            add(A);
+            add(B);
+            r += 32;
+        } else {
+            add(A);
         }
     }
     for (; r < rows; r++)
@@ -53,8 +82,8 @@ __device__ __forceinline__ float ordered_column_sum(const float *col, uint32_t r
 // 0..15 (0..1) of wave 0; other threads return 0. Must be called by every
 // thread of the workgroup (it synchronises).
 //
-// Step c: issue the row loads of chunk c+1 (its index arrived during step
-// c-1) and the index load of chunk c+2, sum chunk c from LDS, then write
+// Step c: issue the row loads of chunk c+1 (its index was loaded two steps
+// earlier) and the index load of chunk c+3, sum chunk c from LDS, then write
 // chunk c+1's rows into the other LDS buffer and synchronise. The summing
 // wave waits for at most one memory round trip per chunk, overlapped with
 // its own sum.
@@ -71,7 +100,7 @@ __device__ __forceinline__ float ordered_sum(Stage<kRows> &st, uint32_t beg, uin
     if (nchunk == 0)
         return acc;
 
-    // 64-bit positions: the index load looks two chunks past the end. Index
+    // 64-bit positions: the index loads look three chunks past the end. Index
     // loads are unconditional, clamped to the last row (deg >= 1 here), so
     // no branch hides them from the compiler's in-order wait counting.
     auto pos = [&](uint32_t c) { return (uint64_t)beg + (uint64_t)c * kRows + (uint64_t)tid; };
@@ -96,25 +125,33 @@ __device__ __forceinline__ float ordered_sum(Stage<kRows> &st, uint32_t beg, uin
         }
     };
 
+    // Index loads run two chunks ahead of the row loads (ia: the next odd
+    // or even chunk, ib: the one after): under a saturated HBM a load takes
+    // longer than one chunk's sum. Steps go in pairs so the two index
+    // registers keep fixed roles.
     float4 row[Q];
     load(fetch_idx(0), row);
-    uint32_t idx = fetch_idx(1);
+    uint32_t ia = fetch_idx(1), ib = fetch_idx(2);
     put(0, 0, row);
     __syncthreads();
-    for (uint32_t c = 0; c < nchunk; c++) {
+    // The loads of the step after the last chunk are made anyway (clamped,
+    // in bounds) and not staged: a branch around them would cost the exact
+    // wait counts.
+    auto step = [&](uint32_t c, uint32_t &inext) {
         const int s = c & 1;
-        const bool more = c + 1 < nchunk;
-        if (more) {
-            load(idx, row);
-            idx = fetch_idx(c + 2);
-        }
+        load(inext, row);
+        inext = fetch_idx(c + 3);
         if (summer) {
             const uint32_t left = deg - c * kRows;
             acc = ordered_column_sum(st.v[s][tid], left < (uint32_t)kRows ? left : kRows, acc);
         }
-        if (more)
-            put(c + 1, s ^ 1, row);
+        put(c + 1, s ^ 1, row); // no-op past the last row
         __syncthreads();
+    };
+    for (uint32_t c = 0; c < nchunk; c += 2) {
+        step(c, ia); // chunk c+1's index is in ia, chunk c+3's goes there
+        if (c + 1 < nchunk)
+            step(c + 1, ib);
     }
     return acc;
 }
