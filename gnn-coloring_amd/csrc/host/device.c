@@ -306,7 +306,7 @@ uint64_t gcx_graph_footprint(uint32_t N, uint64_t M)
     return 8ull * ((uint64_t)N + 1) + 8ull * M + 12ull * N + 7ull * 256;
 }
 
-/* Degree statistics, hub list (ascending ids) with its 4096-edge chunk table
+/* Degree statistics, hub list (ascending ids) with its 1024-edge chunk table
  * for the JP orientation pass, and the degree order. The hub degree is the
  * caller's, else the tuner's for the graph's degree signature, else the
  * default. */

@@ -18,7 +18,7 @@ struct gcx_graph {
     int hub_fixed;    /* hub_deg set by the caller: the tuner leaves it */
     uint32_t nhub;
     uint32_t maxdeg;
-    uint32_t *hchunk; /* device, nhub+1: prefix of 4096-edge chunks per hub */
+    uint32_t *hchunk; /* device, nhub+1: prefix of 1024-edge chunks per hub */
     uint32_t nhchunk;
     uint32_t *perm;   /* device, N: ids by degree descending (hubs first) */
     uint32_t *rank;   /* device, N: rank[perm[i]] = i */
