@@ -570,6 +570,12 @@ def main():
         out["cpu_baseline"] = cpu_baseline(g, args.scale)
     else:
         out["cpu_baseline"] = None
+    # explicit teardown before exit: objects first, then the library's pools
+    ws.free()
+    g.free()
+    model.free()
+    stream.close()
+    gcx.shutdown()
     print(json.dumps(out))
 
 

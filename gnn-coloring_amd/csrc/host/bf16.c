@@ -93,23 +93,14 @@ int gcx__order_gnn_bf16(gcx_workspace *ws, const gcx_graph *g, const gcx_model *
         if (kind == 0 &&
             (rc = gcxk_gnn_input_rec(g->N, g->V, g->rank, m->md, g->fidv, ws->y, s)) != GCX_OK)
             return rc;
-        hipStream_t hs = s;
-        if (g->nhub) {
-            if (hipEventRecord((hipEvent_t)ws->ev_fork, s) != hipSuccess ||
-                hipStreamWaitEvent((hipStream_t)ws->aux, (hipEvent_t)ws->ev_fork, 0) != hipSuccess)
-                return GCX_ERR_HIP;
-            hs = (hipStream_t)ws->aux;
-        }
+        void *hs = gcx__fork_hubs(ws, g, 1, s);
+        if (!hs)
+            return GCX_ERR_HIP;
         rc = gcxk_gnn_bf16_layer(kind, split, g->N, g->V, g->E, g->Er, g->rank,
                                  m->dWpack + 1024 * (size_t)l, b, m->md, in, out, (uint32_t *)ws->key,
                                  g->hubs, g->nhub, g->perm, g->fidv, s, hs);
-        if (rc != GCX_OK)
+        if ((rc = gcx__join_hubs(ws, hs, s, rc)) != GCX_OK)
             return rc;
-        if (g->nhub) {
-            if (hipEventRecord((hipEvent_t)ws->ev_join, hs) != hipSuccess ||
-                hipStreamWaitEvent(s, (hipEvent_t)ws->ev_join, 0) != hipSuccess)
-                return GCX_ERR_HIP;
-        }
         if ((rc = gcx__prof_end(tok, s)) != GCX_OK)
             return rc;
         in = out;

@@ -836,35 +836,70 @@ uint64_t gcx_workspace_footprint(uint32_t N, uint64_t M)
 }
 
 /* Pinned control-block mirrors are recycled: freeing pinned host memory
- * synchronises the whole device, which would stall other streams' jobs. */
+ * synchronises the whole device, which would stall other streams' jobs. A
+ * block goes back with an event recorded on the freeing stream, and is
+ * handed out again only after that event: the round loop keeps device ->
+ * host snapshot copies into it in flight (drive_rounds), and one of them
+ * could otherwise land in the next owner's block and pass it a stale flag. */
 static pthread_mutex_t g_pin_lock = PTHREAD_MUTEX_INITIALIZER;
-static void *g_pin_free[64];
+static struct {
+    void *p;
+    hipEvent_t ev;
+} g_pin_free[64];
 static int g_pin_n = 0;
+
+static int ev_get(hipEvent_t *out);
+static void ev_put(hipEvent_t e);
 
 static void *pinned_ctl_get(void)
 {
     void *p = NULL;
+    hipEvent_t ev = NULL;
     pthread_mutex_lock(&g_pin_lock);
-    if (g_pin_n > 0)
-        p = g_pin_free[--g_pin_n];
+    if (g_pin_n > 0) {
+        g_pin_n--;
+        p = g_pin_free[g_pin_n].p;
+        ev = g_pin_free[g_pin_n].ev;
+    }
     pthread_mutex_unlock(&g_pin_lock);
+    if (p && ev && hipEventSynchronize(ev) != hipSuccess) {
+        /* the last copies into it may not have ended: never reuse it */
+        ev_put(ev);
+        return NULL;
+    }
+    ev_put(ev);
     if (!p && hipHostMalloc(&p, 4 * sizeof(gcxk_ctl), hipHostMallocDefault) != hipSuccess)
         p = NULL;
     return p;
 }
 
-static void pinned_ctl_put(void *p)
+static void pinned_ctl_put(void *p, void *stream)
 {
     if (!p)
         return;
+    hipEvent_t ev = NULL;
+    if (!ev_get(&ev) || hipEventRecord(ev, (hipStream_t)gcx__pick(stream)) != hipSuccess) {
+        /* cannot order the reuse after the copies: finish them, then keep it */
+        ev_put(ev);
+        ev = NULL;
+        if (hipStreamSynchronize((hipStream_t)gcx__pick(stream)) != hipSuccess)
+            return; /* leaked rather than reused while a copy may still land */
+    }
     pthread_mutex_lock(&g_pin_lock);
     if (g_pin_n < 64) {
-        g_pin_free[g_pin_n++] = p;
+        g_pin_free[g_pin_n].p = p;
+        g_pin_free[g_pin_n].ev = ev;
+        g_pin_n++;
         p = NULL;
+        ev = NULL;
     }
     pthread_mutex_unlock(&g_pin_lock);
-    if (p)
+    if (p) {
+        if (ev)
+            (void)hipEventSynchronize(ev);
         (void)hipHostFree(p);
+    }
+    ev_put(ev);
 }
 
 /* Side streams and events are recycled too: creating and destroying them per
@@ -1011,7 +1046,7 @@ int gcx_workspace_free_on(gcx_workspace *ws, void *stream)
         hipEventRecord((hipEvent_t)ws->ev_join, (hipStream_t)ws->aux) == hipSuccess)
         (void)hipStreamWaitEvent((hipStream_t)gcx__pick(stream), (hipEvent_t)ws->ev_join, 0);
     block_put(ws->arena, ws->arena_bytes, stream);
-    pinned_ctl_put(ws->hctl);
+    pinned_ctl_put(ws->hctl, stream);
     aux_put((hipStream_t)ws->aux);
     void *evs[] = {ws->ev_fork, ws->ev_join, ws->ev_poll[0], ws->ev_poll[1]};
     for (size_t i = 0; i < 4; i++)
@@ -1208,19 +1243,13 @@ static int order_gnn_fp32(gcx_workspace *ws, const gcx_graph *g, const gcx_model
                             s, &tok));
         if (kind == 0)
             TRY(gcxk_gnn_input_rec(g->N, g->V, g->rank, m->md, g->fidv, ws->y, s));
-        hipStream_t hs = s;
-        if (g->nhub) {
-            HIPC(hipEventRecord((hipEvent_t)ws->ev_fork, s));
-            HIPC(hipStreamWaitEvent((hipStream_t)ws->aux, (hipEvent_t)ws->ev_fork, 0));
-            hs = (hipStream_t)ws->aux;
-        }
-        TRY(gcxk_gnn_fp32_layer(kind, g->N, g->V, g->E, g->Er, g->rank, W, b, m->md, in, out,
-                                (uint32_t *)ws->key, g->hubs, g->nhub, g->perm, g->fidv, s,
-                                hs));
-        if (g->nhub) {
-            HIPC(hipEventRecord((hipEvent_t)ws->ev_join, hs));
-            HIPC(hipStreamWaitEvent(s, (hipEvent_t)ws->ev_join, 0));
-        }
+        void *hs = gcx__fork_hubs(ws, g, 1, s);
+        if (!hs)
+            return GCX_ERR_HIP;
+        TRY(gcx__join_hubs(ws, hs, s,
+                           gcxk_gnn_fp32_layer(kind, g->N, g->V, g->E, g->Er, g->rank, W, b, m->md,
+                                               in, out, (uint32_t *)ws->key, g->hubs, g->nhub,
+                                               g->perm, g->fidv, s, hs)));
         TRY(gcx__prof_end(tok, s));
         in = out;
     }
@@ -1319,27 +1348,33 @@ static int jp_round(void *p, uint32_t r, hipStream_t s)
                          ws->items, ws->ctl, r, c->grid, s);
 }
 
-/* Orientation pass shared by both schedules. */
 /* The stream for hub work: the workspace's side stream, forked after what s
  * has queued, when the graph has hubs and `fork` is set; else s. NULL on a
- * HIP error. join_hubs makes s wait for it. */
-static hipStream_t fork_hubs(gcx_workspace *ws, const gcx_graph *g, int fork, hipStream_t s)
+ * HIP error. */
+void *gcx__fork_hubs(gcx_workspace *ws, const gcx_graph *g, int fork, void *stream)
 {
+    hipStream_t s = (hipStream_t)stream;
     if (!fork || !g->nhub)
         return s;
     if (hipEventRecord((hipEvent_t)ws->ev_fork, s) != hipSuccess ||
         hipStreamWaitEvent((hipStream_t)ws->aux, (hipEvent_t)ws->ev_fork, 0) != hipSuccess)
         return NULL;
-    return (hipStream_t)ws->aux;
+    return ws->aux;
 }
 
-static int join_hubs(gcx_workspace *ws, hipStream_t hs, hipStream_t s)
+/* Makes s wait for the hub stream hs and returns rc (or GCX_ERR_HIP when rc
+ * is GCX_OK and the join fails). Every path after a successful fork calls
+ * it, failed launches included: a stream-ordered free on s must come after
+ * the hub kernels, or the pool could hand their buffers to another
+ * allocation while they still run. */
+int gcx__join_hubs(gcx_workspace *ws, void *hs, void *s, int rc)
 {
-    if (hs == s)
-        return GCX_OK;
-    HIPC(hipEventRecord((hipEvent_t)ws->ev_join, hs));
-    HIPC(hipStreamWaitEvent(s, (hipEvent_t)ws->ev_join, 0));
-    return GCX_OK;
+    if (hs == s || !hs)
+        return rc;
+    if (hipEventRecord((hipEvent_t)ws->ev_join, (hipStream_t)hs) != hipSuccess ||
+        hipStreamWaitEvent((hipStream_t)s, (hipEvent_t)ws->ev_join, 0) != hipSuccess)
+        return rc == GCX_OK ? GCX_ERR_HIP : rc;
+    return rc;
 }
 
 /* fork = 0 keeps everything on s (inside a captured graph). */
@@ -1347,13 +1382,13 @@ static int jp_prepare_ex(gcx_workspace *ws, const gcx_graph *g, int mb, int fork
 {
     TRY(gcxk_jp_clear(g->N, g->M, ws->ovf, g->nhub, ws->hpos, g->hubs, ws->npred, ws->state,
                       ws->color, ws->ctl, s));
-    hipStream_t hs = fork_hubs(ws, g, fork, s);
+    void *hs = gcx__fork_hubs(ws, g, fork, s);
     if (!hs)
         return GCX_ERR_HIP;
-    TRY(gcxk_jp_init(g->N, g->V, g->E, ws->key, ws->key_bits, mb, g->perm, ws->npred, ws->esplit,
-                     ws->state, ws->color, g->hubs, g->nhub, g->hchunk, g->nhchunk, ws->hpos, s,
-                     hs));
-    return join_hubs(ws, hs, s);
+    const int rc = gcxk_jp_init(g->N, g->V, g->E, ws->key, ws->key_bits, mb, g->perm, ws->npred,
+                                ws->esplit, ws->state, ws->color, g->hubs, g->nhub, g->hchunk,
+                                g->nhchunk, ws->hpos, s, hs);
+    return gcx__join_hubs(ws, hs, s, rc);
 }
 
 static int jp_prepare(gcx_workspace *ws, const gcx_graph *g, int mb, hipStream_t s)
@@ -1382,6 +1417,10 @@ static int color_rounds(gcx_workspace *ws, const gcx_graph *g, int mb, hipStream
  * in that slot; after a change of k the next launch waits for all slots. The
  * lock keeps pick + wait + launch + record atomic across host threads. */
 #define ORDERED_MAX_SLOTS 8
+/* Largest slot count accepted. A mixed serving trace with 8 slots, 16
+ * workers and GPU_MAX_HW_QUEUES=16 once ended in a GPU fault whose first
+ * error was never seen; 1, 2 and 4 slots have only run cleanly. */
+#define ORDERED_SLOTS_ACCEPTED 4
 static hipEvent_t g_slot_done[ORDERED_MAX_SLOTS];
 static int g_ordered_slots = 0;
 static int g_ordered_drain = 0;
@@ -1391,7 +1430,7 @@ static pthread_mutex_t g_ordered_lock = PTHREAD_MUTEX_INITIALIZER;
 int gcx_set_ordered_launch(int slots)
 {
     TRY(gcx__ensure_init());
-    if (slots < 0 || slots > ORDERED_MAX_SLOTS)
+    if (slots < 0 || slots > ORDERED_SLOTS_ACCEPTED)
         return GCX_ERR_ARG;
     pthread_mutex_lock(&g_ordered_lock);
     if (slots != g_ordered_slots)
@@ -1636,11 +1675,12 @@ int gcx_validate(gcx_workspace *ws, const gcx_graph *g, void *stream, int *max_c
     hipStream_t s = (hipStream_t)gcx__pick(stream);
     gcxk_ctl *vc = ws->ctl + 1;
     HIPC(hipMemsetAsync(vc, 0, sizeof(gcxk_ctl), s));
-    hipStream_t hs = fork_hubs(ws, g, 1, s);
+    void *hs = gcx__fork_hubs(ws, g, 1, s);
     if (!hs)
         return GCX_ERR_HIP;
-    TRY(gcxk_validate(g->N, g->V, g->E, ws->color, g->hubs, g->nhub, g->perm, vc, s, hs));
-    TRY(join_hubs(ws, hs, s));
+    TRY(gcx__join_hubs(ws, hs, s,
+                       gcxk_validate(g->N, g->V, g->E, ws->color, g->hubs, g->nhub, g->perm, vc,
+                                     s, hs)));
     HIPC(hipMemcpyAsync(ws->hctl + 1, vc, sizeof(gcxk_ctl), hipMemcpyDeviceToHost, s));
     HIPC(hipStreamSynchronize(s));
     *max_color = ws->hctl[1].bad ? 0 : (ws->hctl[1].maxcolor > 0 ? ws->hctl[1].maxcolor : 1);
@@ -1728,3 +1768,67 @@ int gcx__order_ff_only(gcx_workspace *ws, uint32_t N, void *stream)
 }
 
 int *gcx_workspace_colors_dev(gcx_workspace *ws) { return ws ? ws->color : NULL; }
+
+/* ----------------------------------------------------------------- teardown */
+
+int gcx_shutdown(void)
+{
+    if (!g_init)
+        return GCX_OK;
+    int rc = GCX_OK;
+    if (g_init_rc == GCX_OK && hipDeviceSynchronize() != hipSuccess)
+        rc = GCX_ERR_HIP;
+    /* cached blocks: every use is finished after the device sync */
+    pthread_mutex_lock(&g_bc_lock);
+    for (int i = 0; i < g_bc_n; i++) {
+        if (hipFreeAsync(g_bc[i].p, g_stream) != hipSuccess)
+            rc = GCX_ERR_HIP;
+        (void)hipEventDestroy(g_bc[i].ev);
+    }
+    g_bc_n = 0;
+    g_bc_bytes = 0;
+    pthread_mutex_unlock(&g_bc_lock);
+    if (g_stream && hipStreamSynchronize(g_stream) != hipSuccess)
+        rc = GCX_ERR_HIP;
+    pthread_mutex_lock(&g_pin_lock);
+    for (int i = 0; i < g_pin_n; i++) {
+        if (g_pin_free[i].ev)
+            (void)hipEventDestroy(g_pin_free[i].ev);
+        (void)hipHostFree(g_pin_free[i].p);
+    }
+    g_pin_n = 0;
+    for (int i = 0; i < g_aux_n; i++)
+        (void)hipStreamDestroy(g_aux_free[i]);
+    g_aux_n = 0;
+    for (int i = 0; i < g_ev_n; i++)
+        (void)hipEventDestroy(g_ev_free[i]);
+    g_ev_n = 0;
+    pthread_mutex_unlock(&g_pin_lock);
+    (void)gcx_profile_reset();
+    for (size_t i = 0; i < g_pool_n; i++)
+        (void)hipEventDestroy(g_pool[i]);
+    free(g_pool);
+    g_pool = NULL;
+    g_pool_n = g_pool_cap = 0;
+    free(g_prof);
+    g_prof = NULL;
+    g_prof_n = g_prof_cap = 0;
+    pthread_mutex_lock(&g_ordered_lock);
+    for (int i = 0; i < ORDERED_MAX_SLOTS; i++)
+        if (g_slot_done[i]) {
+            (void)hipEventDestroy(g_slot_done[i]);
+            g_slot_done[i] = NULL;
+        }
+    g_next_slot = 0;
+    g_ordered_drain = 0;
+    pthread_mutex_unlock(&g_ordered_lock);
+    if (g_stream)
+        (void)hipStreamDestroy(g_stream);
+    g_stream = NULL;
+    hipMemPool_t pool;
+    if (g_init_rc == GCX_OK && hipDeviceGetDefaultMemPool(&pool, 0) == hipSuccess)
+        (void)hipMemPoolTrimTo(pool, 0);
+    g_init = 0;
+    g_init_rc = GCX_OK;
+    return rc;
+}

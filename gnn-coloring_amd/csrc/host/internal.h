@@ -118,4 +118,10 @@ int gcx__ensure_init(void);
 void *gcx__default_stream(void);
 void *gcx__pick(void *stream);
 int gcx__prof_begin(int stage, void *stream, void **token);
+/* Hub work on the workspace's side stream: fork returns the stream to put
+ * it on (s itself when the graph has no hubs or fork is 0; NULL on a HIP
+ * error); join makes s wait for it and passes rc through. Every path after
+ * a successful fork joins, failed launches included. */
+void *gcx__fork_hubs(gcx_workspace *ws, const gcx_graph *g, int fork, void *s);
+int gcx__join_hubs(gcx_workspace *ws, void *hs, void *s, int rc);
 int gcx__prof_end(void *token, void *stream);

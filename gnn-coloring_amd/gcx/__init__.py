@@ -10,6 +10,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import sys
 
 import numpy as np
 
@@ -51,6 +52,7 @@ def _load():
         "gcx_status_string": (C.c_char_p, [C.c_int]),
         "gcx_version": (C.c_char_p, []),
         "gcx_init": (C.c_int, []),
+        "gcx_shutdown": (C.c_int, []),
         "gcx_stream_create": (C.c_int, [C.POINTER(vp)]),
         "gcx_stream_destroy": (C.c_int, [vp]),
         "gcx_stream_sync": (C.c_int, [vp]),
@@ -197,6 +199,13 @@ def init():
     _chk(_lib.gcx_init(), "gcx_init")
 
 
+def shutdown():
+    """Wait for the device, then release the library's cached blocks, pinned
+    buffers, streams and events (gcx_shutdown). Free graphs, workspaces,
+    models and servers first."""
+    _chk(_lib.gcx_shutdown(), "gcx_shutdown")
+
+
 def gpu_available() -> bool:
     return _lib.gcx_init() == OK
 
@@ -222,6 +231,8 @@ class Stream:
             self.handle = None
 
     def __del__(self):
+        if sys.is_finalizing():  # HIP may already be torn down
+            return
         try:
             self.close()
         except Exception:
@@ -243,6 +254,8 @@ class Event:
         return float(ms.value)
 
     def __del__(self):
+        if sys.is_finalizing():  # HIP may already be torn down
+            return
         try:
             _lib.gcx_event_destroy(self.handle)
         except Exception:
@@ -403,6 +416,8 @@ class Graph:
             self.handle = None
 
     def __del__(self):
+        if sys.is_finalizing():  # HIP may already be torn down
+            return
         try:
             self.free()
         except Exception:
@@ -451,6 +466,8 @@ class Model:
             self.handle = None
 
     def __del__(self):
+        if sys.is_finalizing():  # HIP may already be torn down
+            return
         try:
             self.free()
         except Exception:
@@ -561,6 +578,8 @@ class Workspace:
             self.handle = None
 
     def __del__(self):
+        if sys.is_finalizing():  # HIP may already be torn down
+            return
         try:
             self.free()
         except Exception:
@@ -849,6 +868,8 @@ class Server:
             self.handle = None
 
     def __del__(self):
+        if sys.is_finalizing():  # HIP may already be torn down
+            return
         try:
             self.close()
         except Exception:

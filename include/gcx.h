@@ -60,6 +60,12 @@ const char *gcx_status_string(int status);
 int gcx_init(void);
 /* Library version string and the gfx target it was built for. */
 const char *gcx_version(void);
+/* Waits for all device work, then releases what the library keeps for
+ * reuse: cached freed blocks, pinned control blocks, side streams, events,
+ * the ordered-launch slot events and the library's stream. Call it after
+ * freeing the graphs, workspaces, models and servers, before exit; a later
+ * call of any entry point initialises the library again. */
+int gcx_shutdown(void);
 
 /* ---- streams and timing (thin wrappers so callers need no HIP headers) */
 int gcx_stream_create(void **stream);
@@ -178,7 +184,7 @@ int gcx_color(gcx_workspace *ws, const gcx_graph *g, void *stream);
 enum gcx_schedule { GCX_SCHED_ROUNDS = 0, GCX_SCHED_ORDERED = 1, GCX_SCHED_AUTO = 2 };
 int gcx_set_schedule(gcx_workspace *ws, int schedule, uint32_t param);
 /* How ORDERED launches share the device (process-wide). 0: cooperative
- * launches, one at a time, each running alone on the device. k = 1..8:
+ * launches, one at a time, each running alone on the device. k = 1..4:
  * plain launches that overlap other streams' kernels, at most k at once,
  * each with at most 1/k of the resident workgroups. Colours are the same
  * under every setting. */
