@@ -68,6 +68,8 @@ int gcx__ensure_init(void)
         hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep) != hipSuccess)
         return g_init_rc = GCX_ERR_HIP;
     g_round_grid = gcxk_round_grid();
+    if (gcxk_noalloc_init() != GCX_OK)
+        return g_init_rc = GCX_ERR_HIP;
     return g_init_rc = GCX_OK;
 }
 

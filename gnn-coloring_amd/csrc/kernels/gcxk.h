@@ -282,6 +282,10 @@ int gcxk_color_digest(uint32_t N, const int *color, uint64_t *out, void *stream)
 /* Number of workgroups the round kernel should use (fills the GPU). */
 uint32_t gcxk_round_grid(void);
 
+/* The no-allocation entry point's device scratch (noalloc.hip): looks up its
+ * address on the current device; called by the library's initialisation. */
+int gcxk_noalloc_init(void);
+
 #ifdef __cplusplus
 }
 #endif

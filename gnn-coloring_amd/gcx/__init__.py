@@ -136,6 +136,7 @@ def _load():
         "jones_plassmann_setup": (vp, [C.c_uint32, C.c_uint32]),
         "jones_plassmann_cleanup": (None, [vp]),
         "jones_plassmann_internal": (None, [C.c_uint32, u32p, u32p, vp, f64p, i32p]),
+        "jones_plassmann_internal_noalloc": (C.c_int, [C.c_uint32, u32p, u32p, f64p, i32p]),
         "validate_coloring_par": (C.c_int, [C.c_uint32, u32p, u32p, i32p]),
         "validate_coloring": (C.c_int, [C.c_uint32, u32p, u32p, i32p]),
         "cgnn_graph_sage_16_par": (None, [C.c_int, f32p, C.c_uint32, u32p, u32p, C.c_int, f32p, f32p,
@@ -665,6 +666,31 @@ class ref:
         p = np.ascontiguousarray(p, np.float64)
         N = len(V) - 1
         return _take_malloced_ints(_lib.jones_plassmann_coloring(N, Vp, Ep, p.ctypes.data_as(f64p)), N)
+
+    @staticmethod
+    def jones_plassmann_internal(V, E, p):
+        """jones_plassmann_setup + jones_plassmann_internal + cleanup."""
+        V, Vp = _u32(V)
+        E, Ep = _u32(E)
+        p = np.ascontiguousarray(p, np.float64)
+        N = len(V) - 1
+        color = np.zeros(max(N, 1), np.int32)
+        h = _lib.jones_plassmann_setup(N, int(V[N]) if N else 0)
+        _lib.jones_plassmann_internal(N, Vp, Ep, h, p.ctypes.data_as(f64p), color.ctypes.data_as(i32p))
+        _lib.jones_plassmann_cleanup(h)
+        return color[:N]
+
+    @staticmethod
+    def jones_plassmann_internal_noalloc(V, E, p):
+        """The no-allocation variant: returns (status, colours)."""
+        V, Vp = _u32(V)
+        E, Ep = _u32(E)
+        p = np.ascontiguousarray(p, np.float64)
+        N = len(V) - 1
+        color = np.zeros(max(N, 1), np.int32)
+        rc = _lib.jones_plassmann_internal_noalloc(N, Vp, Ep, p.ctypes.data_as(f64p),
+                                                   color.ctypes.data_as(i32p))
+        return int(rc), color[:N]
 
     @staticmethod
     def greedy_coloring(V, E, p):

@@ -37,6 +37,23 @@ void jones_plassmann_internal(uint32_t N, const uint32_t *V, const uint32_t *E,
 /* Replaces coloring.c:332-351. */
 int validate_coloring_par(uint32_t N, const uint32_t *V, const uint32_t *E, const int *color);
 
+/* No counterpart in the reference. For callers that cannot allocate (code
+ * that must not call hipMalloc, such as tight serving loops or harnesses that
+ * forbid allocation): the colours of jones_plassmann_internal, with no
+ * setup handle and no allocation of any kind. Every device buffer of the
+ * call lives in one 128 MiB device array that the library holds for the
+ * process's lifetime. Limits: N <= GCX_NOALLOC_MAX_N vertices and
+ * V[N] <= GCX_NOALLOC_MAX_M entries; larger inputs return GCX_ERR_CAPACITY
+ * (-4) with a message on stderr. Returns 0 (GCX_OK) or a negative gcx.h
+ * status: GCX_ERR_ARG for out-of-range indices or a NaN priority.
+ * Synchronous; calls are serialised inside the process (they share the
+ * array). */
+#define GCX_NOALLOC_SCRATCH_MB 128
+#define GCX_NOALLOC_MAX_N (1u << 18)
+#define GCX_NOALLOC_MAX_M (1u << 23)
+int jones_plassmann_internal_noalloc(uint32_t N, const uint32_t *V, const uint32_t *E,
+                                     const double *p, int *color);
+
 #ifdef __cplusplus
 }
 #endif
