@@ -634,6 +634,14 @@ int gcx_graph_pack(const gcx_graph *const *graphs, uint32_t count, void *stream,
 
 uint32_t gcx_graph_packed_count(const gcx_graph *g) { return g ? g->count : 0; }
 
+int gcx_graph_prepare(gcx_graph *g, void *stream)
+{
+    TRY(gcx__ensure_init());
+    if (!g)
+        return GCX_ERR_ARG;
+    return graph_stats(g, (hipStream_t)gcx__pick(stream));
+}
+
 int gcx_graph_set_hub_degree(gcx_graph *g, uint32_t hub_degree, void *stream)
 {
     TRY(gcx__ensure_init());

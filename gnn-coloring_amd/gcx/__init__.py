@@ -77,6 +77,7 @@ def _load():
         "gcx_graph_footprint": (C.c_uint64, [C.c_uint32, C.c_uint64]),
         "gcx_graph_pack": (C.c_int, [C.POINTER(vp), C.c_uint32, vp, C.POINTER(vp)]),
         "gcx_graph_packed_count": (C.c_uint32, [vp]),
+        "gcx_graph_prepare": (C.c_int, [vp, vp]),
         "gcx_packed_colors": (C.c_int, [vp, vp, i32p, C.POINTER(C.c_uint64), vp]),
         "gcx_model_load": (C.c_int, [C.c_char_p, C.POINTER(vp)]),
         "gcx_model_from_params": (C.c_int, [C.c_int, f32p, C.c_int, C.POINTER(vp)]),
@@ -370,6 +371,11 @@ class Graph:
         h = vp()
         _chk(_lib.gcx_graph_pack(arr, len(graphs), _sh(stream), C.byref(h)), "graph pack")
         return cls(h)
+
+    def prepare(self, stream=None):
+        """Redo the one-time per-graph preparation (degree order, hub tables,
+        remapped columns); for timing it."""
+        _chk(_lib.gcx_graph_prepare(self.handle, _sh(stream)), "graph prepare")
 
     @property
     def hub_degree(self) -> int:

@@ -142,6 +142,11 @@ uint32_t gcx_graph_hub_degree(const gcx_graph *g);
  * features), in one pass over the device: many small graphs become one
  * launch sequence and one persistent JP kernel. */
 int gcx_graph_pack(const gcx_graph *const *graphs, uint32_t count, void *stream, gcx_graph **out);
+/* Redo the per-graph preparation every constructor does once: degree
+ * statistics, the hub list and its chunk table, the degree order, its
+ * inverse and the columns remapped to it (Er). Results do not change; the
+ * call exists to time that one-time cost. Synchronises `stream`. */
+int gcx_graph_prepare(gcx_graph *g, void *stream);
 /* Number of graphs packed into g (0 for a plain graph). */
 uint32_t gcx_graph_packed_count(const gcx_graph *g);
 /* Bytes of HBM a graph with N vertices and M entries occupies. */

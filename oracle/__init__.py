@@ -224,11 +224,22 @@ def culberson_classes(V, E, color, K, strategy, seed=0):
     return color, counts
 
 
-def bf16_round(x):
-    """Round float32 values to bfloat16 (nearest even), returned as float32."""
+def bf16_bits(x):
+    """float32 -> bfloat16 bit patterns (uint16), the library's rule
+    (gnn_bf16.hip / bf16.c to_bf16): finite values round to nearest, ties to
+    even (values past the largest bf16 round to Inf); Inf stays Inf; a NaN
+    keeps its sign and top payload bits and gets the quiet bit 0x40 when any
+    dropped payload bit was set, so it never turns into Inf."""
     b = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
-    b = (b + 0x7FFF + ((b >> 16) & 1)) & 0xFFFF0000
-    return b.astype(np.uint32).view(np.float32)
+    special = (b & 0x7F800000) == 0x7F800000
+    rounded = (b + 0x7FFF + ((b >> 16) & 1)) >> 16
+    kept = (b >> 16) | np.where((b & 0xFFFF) != 0, 0x40, 0).astype(np.uint64)
+    return np.where(special, kept, rounded).astype(np.uint16)
+
+
+def bf16_round(x):
+    """Round float32 values to bfloat16 (bf16_bits), returned as float32."""
+    return (bf16_bits(x).astype(np.uint32) << np.uint32(16)).view(np.float32)
 
 
 def gnn_bf16_emulated(param, V, E, md=100, rounding=True, split=False):

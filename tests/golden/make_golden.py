@@ -20,6 +20,17 @@ Outputs
                     reference's own driver reads it), scale 20, and with
                     --big scale 22 and the 160^3 27-point stencil
   models.npz        the weights as parsed by the reference's gnn_setup
+
+With --configs (the batching and precision configurations):
+  digests.json      rmat20.culb_GNN3_64: a 64-pass Culberson chain (reverse
+                    class order, the reference's -c) from the reference's
+                    GNN3 priorities on R-MAT scale 20; rmat24: CSR and GNN4
+                    (m4) FP32 priority / colour digests on R-MAT scale 24
+  rmat24_fp64_sample.npz
+                    the FP64 network (the oracle) at 65,536 seeded sample
+                    vertices of R-MAT 24, for m4 and Glorot-uniform weight
+                    sets 1 and 2 (tests/precision_study.glorot_params), with
+                    the FP32 reference priorities at the same vertices
 """
 from __future__ import annotations
 
@@ -113,11 +124,74 @@ def tiny_graphs():
     return g
 
 
+def config_goldens(ref, models):
+    """Goldens for the batching (64-pass chain) and precision (R-MAT 24)
+    configurations."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import precision_study as ps
+
+    path = os.path.join(OUT, "digests.json")
+    digests = json.load(open(path))
+    t = time.time()
+    N, I, J = gcx.rmat_pairs(20, 16, seed=1)
+    V, E = oracle.csr_from_pairs(N, I, J)
+    assert sha(V) == digests["rmat20"]["sha_V"] and sha(E) == digests["rmat20"]["sha_E"]
+    param, md = models[3]
+    c, counts = culberson(ref, V, E, ref.gnn(param, V, E, md), 64)
+    digests["rmat20"]["culb_GNN3_64"] = {"sha_color": sha(c), "counts": counts.tolist()}
+    print("rmat20 chain of 64:", counts[0], "->", counts[-1], "%.1fs" % (time.time() - t), flush=True)
+    json.dump(digests, open(path, "w"), indent=1, sort_keys=True)
+    del V, E, I, J
+    t = time.time()
+    N, I, J = gcx.rmat_pairs(24, 16, seed=1)
+    V, E = oracle.csr_from_pairs(N, I, J)
+    del I, J
+    rec = {"graph": {"kind": "rmat", "scale": 24, "edge_factor": 16, "abc": [0.57, 0.19, 0.19],
+                     "seed": 1},
+           "N": int(N), "M": int(V[-1]), "sha_V": sha(V), "sha_E": sha(E), "csr_via": "oracle"}
+    param, md = models[4]
+    p = ref.gnn(param, V, E, md)
+    col = ref.jp(V, E, p)
+    rec["GNN4"] = {"sha_p": sha(p), "sha_color": sha(col), "ncolors": int(ref.validate(V, E, col))}
+    print("rmat24", rec["N"], rec["M"], rec["GNN4"]["ncolors"], "%.1fs" % (time.time() - t),
+          flush=True)
+    digests["rmat24"] = rec
+    json.dump(digests, open(path, "w"), indent=1, sort_keys=True)
+    rng = np.random.default_rng(2024)
+    ids = np.sort(rng.choice(N, 65536, replace=False)).astype(np.uint32)
+    arrs = {"ids": ids, "p32_m4": p[ids]}
+    sets = {"m4": (param, md), "glorot1": (ps.glorot_params(4, 1), 100),
+            "glorot2": (ps.glorot_params(4, 2), 100)}
+    for name, (w, mdl) in sets.items():
+        t = time.time()
+        p64 = oracle.gnn_fp64(w, V, E, mdl)
+        arrs[f"p64_{name}"] = p64[ids]
+        arrs[f"scale_{name}"] = np.float64(np.abs(p64).max())
+        c64 = ref.jp(V, E, p64)
+        arrs[f"ncolors64_{name}"] = np.int32(ref.validate(V, E, c64))
+        if name != "m4":
+            p32 = ref.gnn(w, V, E, mdl)
+            arrs[f"p32_{name}"] = p32[ids]
+            c32 = ref.jp(V, E, p32)
+            arrs[f"ncolors32_{name}"] = np.int32(ref.validate(V, E, c32))
+        else:
+            arrs["ncolors32_m4"] = np.int32(rec["GNN4"]["ncolors"])
+        print("rmat24 fp64", name, int(arrs[f"ncolors64_{name}"]), int(arrs[f"ncolors32_{name}"]),
+              "%.1fs" % (time.time() - t), flush=True)
+    np.savez_compressed(os.path.join(OUT, "rmat24_fp64_sample.npz"), **arrs)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--big", action="store_true", help="also scale 22 and the 160^3 stencil")
+    ap.add_argument("--configs", action="store_true",
+                    help="only the batching / precision configuration goldens")
     args = ap.parse_args()
     ref = oracle.Ref()
+    if args.configs:
+        models = {l: ref.model_params(REF_TREE, l) for l in (2, 3, 4)}
+        config_goldens(ref, models)
+        return
     models = {l: ref.model_params(REF_TREE, l) for l in (2, 3, 4)}
     np.savez_compressed(os.path.join(OUT, "models.npz"),
                         **{f"m{l}": models[l][0] for l in models},

@@ -268,6 +268,47 @@ def test_checkers_reject_planted_mistakes(models):
     assert oracle.graph_validate(V, E2) == 0
 
 
+def test_bf16_rounding_pinned():
+    """bf16_bits against torch.bfloat16 (round to nearest even) on random bit
+    patterns of every class, and hand-derived cases a truncating or
+    round-half-up rule, or one that lets NaNs become Inf, would fail."""
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(11)
+    bits = np.concatenate([
+        rng.integers(0, 1 << 32, 1 << 20, dtype=np.uint64).astype(np.uint32),
+        # exact ties and their neighbours: low 16 bits 0x7FFF, 0x8000, 0x8001
+        (rng.integers(0, 1 << 16, 1 << 14, dtype=np.uint64).astype(np.uint32) << 16)
+        | rng.choice(np.array([0x7FFF, 0x8000, 0x8001], np.uint32), 1 << 14),
+    ])
+    x = bits.view(np.float32)
+    mine = oracle.bf16_bits(x)
+    ref = torch.from_numpy(x.copy()).to(torch.bfloat16).view(torch.int16).numpy().view(np.uint16)
+    nan = np.isnan(x)
+    np.testing.assert_array_equal(mine[~nan], ref[~nan])
+    back = (mine.astype(np.uint32) << 16).view(np.float32)
+    assert np.isnan(back[nan]).all()  # NaNs stay NaN (torch: its canonical NaN)
+    cases = {
+        0x3F800000: 0x3F80,  # 1.0 exact
+        0x3F808000: 0x3F80,  # tie, even neighbour below: down
+        0x3F818000: 0x3F82,  # tie, odd below: up (truncation gives 0x3F81)
+        0x3F808001: 0x3F81,  # just past the tie: up
+        0x3F807FFF: 0x3F80,  # just below the tie: down (half-up would agree; 0x3F818000 does not)
+        0xBF818000: 0xBF82,  # the same on a negative value
+        0x7F7FFFFF: 0x7F80,  # largest float: rounds to Inf
+        0x7F7F7FFF: 0x7F7F,  # largest bf16 below it stays finite
+        0x80000000: 0x8000,  # -0
+        0x00008000: 0x0000,  # subnormal tie to even (0)
+        0x00018000: 0x0002,  # subnormal tie to even (up)
+        0x7F800000: 0x7F80,  # +Inf
+        0xFF800000: 0xFF80,  # -Inf
+        0x7F800001: 0x7FC0,  # NaN with only low payload bits: quiet NaN, not Inf
+        0xFFFFFFFF: 0xFFFF,  # all-ones NaN: stays NaN (an unchecked add wraps it to +0)
+        0x7FC00000: 0x7FC0,  # quiet NaN
+    }
+    got = oracle.bf16_bits(np.array(list(cases), np.uint32).view(np.float32))
+    assert [hex(v) for v in got] == [hex(v) for v in cases.values()]
+
+
 @pytest.mark.parametrize("name", ["er300", "rmat10", "mixed10"])
 @pytest.mark.parametrize("layers", [2, 3, 4])
 def test_bf16_emulation_structure(name, layers, models):
