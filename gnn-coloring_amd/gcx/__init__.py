@@ -15,7 +15,9 @@ import sys
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libgcx.so")
+# GCX_LIB_PATH selects another build of the same library (the host
+# sanitizer build used by the CPU tests)
+LIB_PATH = os.environ.get("GCX_LIB_PATH") or os.path.join(_HERE, "libgcx.so")
 CLI_PATH = os.path.join(_HERE, "coloring")
 MODEL_DIR = os.path.normpath(os.path.join(_HERE, "..", "..", "models"))
 

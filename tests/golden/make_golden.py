@@ -31,6 +31,10 @@ With --configs (the batching and precision configurations):
                     vertices of R-MAT 24, for m4 and Glorot-uniform weight
                     sets 1 and 2 (tests/precision_study.glorot_params), with
                     the FP32 reference priorities at the same vertices
+
+With --cli: cli_ref.json, the stdout of the reference's own driver
+(src/main.c linked against its own library: oracle/_ref/coloring_ref) for
+-q, -q -f, -q -c 5 and the full runner mode on five golden graphs.
 """
 from __future__ import annotations
 
@@ -181,12 +185,43 @@ def config_goldens(ref, models):
     np.savez_compressed(os.path.join(OUT, "rmat24_fp64_sample.npz"), **arrs)
 
 
+CLI_GRAPHS = ["rmat10", "er300", "mixed10", "jump8", "stencil6"]
+CLI_ARGS = [["-q"], ["-q", "-f"], ["-q", "-c", "5"], ["-c", "3", "2"]]
+
+
+def cli_goldens():
+    """stdout of the reference's own driver (src/main.c linked against its
+    own library, oracle/_ref/coloring_ref) on the golden graphs, for the
+    drop-in check of the same driver linked against libgcx.so."""
+    import subprocess
+    exe = os.path.join(ROOT, "oracle", "_ref", "coloring_ref")
+    out = {}
+    for name in CLI_GRAPHS:
+        fn = f"{name}.npz" if name.startswith("rmat") else f"tiny_{name}.npz"
+        with np.load(os.path.join(OUT, fn), allow_pickle=False) as z:
+            N, I, J = int(z["N"]), z["I"], z["J"]
+        with tempfile.TemporaryDirectory() as d:
+            path = os.path.join(d, f"{name}.mtx")
+            write_mtx(path, N, I, J)
+            for args in CLI_ARGS:
+                r = subprocess.run([exe, *args, "-g", path], capture_output=True, text=True,
+                                   cwd=ROOT, timeout=600, check=True)
+                out[f"{name} {' '.join(args)}"] = r.stdout
+                print(name, args, r.stdout.strip().splitlines()[:2], flush=True)
+    json.dump(out, open(os.path.join(OUT, "cli_ref.json"), "w"), indent=1, sort_keys=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--big", action="store_true", help="also scale 22 and the 160^3 stencil")
     ap.add_argument("--configs", action="store_true",
                     help="only the batching / precision configuration goldens")
+    ap.add_argument("--cli", action="store_true",
+                    help="only the reference driver's output (cli_ref.json)")
     args = ap.parse_args()
+    if args.cli:
+        cli_goldens()
+        return
     ref = oracle.Ref()
     if args.configs:
         models = {l: ref.model_params(REF_TREE, l) for l in (2, 3, 4)}
