@@ -419,6 +419,10 @@ def main():
                     help="run only a Poisson pass at this many requests/s (0 = capacity, "
                          "70%% Poisson and plain FP32 passes)")
     ap.add_argument("--schedule", choices=["auto", "rounds", "ordered"], default="auto")
+    ap.add_argument("--head", type=int, default=None,
+                    help="longest head prefix of the ordered schedule (0 = off; default: the "
+                         "library's)")
+    ap.add_argument("--head-density", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is "
@@ -435,6 +439,8 @@ def main():
           f"({time.time() - t:.1f}s to generate + build CSR on the GPU)", file=sys.stderr, flush=True)
     model = gcx.Model.layers_file(3)
     ws = gcx.Workspace(N, M)
+    if args.head is not None:
+        ws.set_ordered_head(args.head, args.head_density)
     stream = gcx.Stream()
     # one-time per-graph preparation (degree order, hub tables, Er), outside
     # the timed step: best of 3

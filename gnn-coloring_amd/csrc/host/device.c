@@ -1155,6 +1155,8 @@ int gcx_workspace_create_on(uint32_t N, uint64_t M, void *stream, gcx_workspace 
         return GCX_ERR_NOMEM;
     }
     memset(ws->hctl, 0, 4 * sizeof(gcxk_ctl));
+    ws->head_max = 0; /* the head is opt-in (gcx_set_ordered_head) */
+    ws->head_density = GCX_HEAD_DENSITY;
     *out = ws;
     return GCX_OK;
 }
@@ -1397,7 +1399,7 @@ static int jp_prepare_ex(gcx_workspace *ws, const gcx_graph *g, int mb, int fork
         return GCX_ERR_HIP;
     const int rc = gcxk_jp_init(g->N, g->V, g->E, ws->key, ws->key_bits, mb, g->perm, ws->npred,
                                 ws->esplit, ws->state, ws->color, g->hubs, g->nhub, g->hchunk,
-                                g->nhchunk, ws->hpos, s, hs);
+                                g->nhchunk, ws->hpos, 0u, s, hs);
     return gcx__join_hubs(ws, hs, s, rc);
 }
 
@@ -1485,26 +1487,73 @@ static int launch_ordered(gcx_workspace *ws, const gcx_graph *g, int mb, hipStre
     return rc;
 }
 
-static int color_ordered(gcx_workspace *ws, const gcx_graph *g, int mb, hipStream_t s)
+/* After an ordered launch: the control block back, and whether a wait hit
+ * its bound (GCX_ERR_INVALID: the caller redoes the colouring). */
+static int ordered_done(gcx_workspace *ws, hipStream_t s)
 {
-    void *tok = NULL;
-    TRY(gcx__prof_begin(GCX_ST_JP_INIT, s, &tok));
-    TRY(jp_prepare(ws, g, mb, s));
-    void *scratch = ws->order_buf ? ws->order_buf : (void *)ws->x;
-    uint32_t *cnt = (uint32_t *)ws->y, *incl = cnt + ws->capN;
-    TRY(gcxk_jp_order_items(g->N, g->V, ws->npred, ws->key, ws->key_bits, mb, scratch,
-                            ws->order_bytes, cnt, incl, ws->wof, ws->wdesc, ws->state,
-                            gcxk_jp_state_words(ws->capN, ws->capM), ws->items[0], ws->ctl, s));
-    TRY(gcx__prof_end(tok, s));
-    TRY(gcx__prof_begin(GCX_ST_JP_ROUNDS, s, &tok));
-    const int rc = launch_ordered(ws, g, mb, s);
-    TRY(rc);
-    TRY(gcx__prof_end(tok, s));
     HIPC(hipMemcpyAsync(ws->hctl + 2, ws->ctl, sizeof(gcxk_ctl), hipMemcpyDeviceToHost, s));
     HIPC(hipStreamSynchronize(s));
     ws->last_launches = 1;
     ws->last_ordered = 1;
     return ws->hctl[2].err ? GCX_ERR_INVALID : GCX_OK;
+}
+
+/* Ordered schedule without a head (the default): orientation against the
+ * workspace keys, the greedy-order items, one persistent launch. With a
+ * head (gcx_set_ordered_head): the greedy order first (sort), then 32-bit rank keys
+ * of that order, then the orientation against the rank keys -- which also
+ * writes the predecessor positions of the first head_max vertices for the
+ * head kernel -- then state words and items; the head kernel colours the
+ * dense top of the order in one workgroup, and the persistent launch the
+ * rest. The rank keys live in ws->count (SL scratch, free during JP). */
+static int color_ordered(gcx_workspace *ws, const gcx_graph *g, int mb, hipStream_t s)
+{
+    void *tok = NULL;
+    TRY(gcx__prof_begin(GCX_ST_JP_INIT, s, &tok));
+    void *scratch = ws->order_buf ? ws->order_buf : (void *)ws->x;
+    uint32_t *cnt = (uint32_t *)ws->y, *incl = cnt + ws->capN;
+    uint32_t *ids = NULL, *rk = ws->count;
+    const uint32_t head_max = ws->head_max;
+    if (!head_max) {
+        /* no head: orientation on the workspace keys, then the greedy order */
+        TRY(jp_prepare(ws, g, mb, s));
+        TRY(gcxk_jp_order_items(g->N, g->V, ws->npred, ws->key, ws->key_bits, mb, scratch,
+                                ws->order_bytes, cnt, incl, ws->wof, ws->wdesc, ws->state,
+                                gcxk_jp_state_words(ws->capN, ws->capM), ws->items[0], ws->ctl,
+                                s));
+        TRY(gcx__prof_end(tok, s));
+        TRY(gcx__prof_begin(GCX_ST_JP_ROUNDS, s, &tok));
+        TRY(launch_ordered(ws, g, mb, s));
+        TRY(gcx__prof_end(tok, s));
+        return ordered_done(ws, s);
+    }
+    TRY(gcxk_jp_clear(g->N, g->M, ws->ovf, g->nhub, ws->hpos, g->hubs, ws->npred, ws->state,
+                      ws->color, ws->ctl, s));
+    TRY(gcxk_jp_sort_ids(g->N, ws->key, ws->key_bits, scratch, ws->order_bytes, &ids, s));
+    TRY(gcxk_jp_rank_keys(g->N, ids, rk, s));
+    void *hs = gcx__fork_hubs(ws, g, 1, s);
+    if (!hs)
+        return GCX_ERR_HIP;
+    TRY(gcx__join_hubs(ws, hs, s,
+                       gcxk_jp_init(g->N, g->V, g->E, rk, 32, mb, g->perm, ws->npred, ws->esplit,
+                                    ws->state, ws->color, g->hubs, g->nhub, g->hchunk, g->nhchunk,
+                                    ws->hpos, head_max, s, hs)));
+    TRY(gcxk_jp_order_items_ranked(g->N, g->V, ws->npred, ids, rk, head_max, ws->head_density,
+                                   ws->key_bits, mb, scratch, ws->order_bytes, cnt, incl, ws->wof,
+                                   ws->wdesc, ws->state, gcxk_jp_state_words(ws->capN, ws->capM),
+                                   ws->items[0], ws->ctl, s));
+    TRY(gcx__prof_end(tok, s));
+    TRY(gcx__prof_begin(GCX_ST_JP_ROUNDS, s, &tok));
+    /* 2 s per wait, as in the persistent launch */
+    /* the head table lives in the second item queue (rounds schedule only):
+     * 8 bytes per position, and the queue holds 8 (N + M / 64 + 2) */
+    if (head_max)
+        TRY(gcxk_jp_head(g->V, ws->npred, ws->esplit, ids, ws->items[1], ws->color, ws->ctl,
+                         200000000ull, s));
+    const int rc = launch_ordered(ws, g, mb, s);
+    TRY(rc);
+    TRY(gcx__prof_end(tok, s));
+    return ordered_done(ws, s);
 }
 
 int gcx_color(gcx_workspace *ws, const gcx_graph *g, void *stream)
@@ -1525,6 +1574,23 @@ int gcx_color(gcx_workspace *ws, const gcx_graph *g, void *stream)
         ws->last_ordered = 0;
     }
     return color_rounds(ws, g, mb, s);
+}
+
+int gcx_set_ordered_head(gcx_workspace *ws, uint32_t max_positions, uint32_t min_density)
+{
+    if (!ws || max_positions > gcxk_jp_head_max())
+        return GCX_ERR_ARG;
+    ws->head_max = max_positions;
+    ws->head_density = min_density;
+    return GCX_OK;
+}
+
+int gcx_last_head(const gcx_workspace *ws, uint32_t *positions)
+{
+    if (!ws || !positions)
+        return GCX_ERR_ARG;
+    *positions = ws->last_ordered && !ws->last_fallback ? ws->hctl[2].head_n : 0u;
+    return GCX_OK;
 }
 
 int gcx_set_schedule(gcx_workspace *ws, int schedule, uint32_t param)

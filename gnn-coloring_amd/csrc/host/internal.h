@@ -77,7 +77,15 @@ struct gcx_workspace {
     void *order_buf;        /* sort scratch when it does not fit in x */
     size_t order_bytes;     /* bytes of sort scratch available */
     uint32_t curN;    /* vertex count of the graph the contents belong to */
+    uint32_t head_max;     /* ordered schedule: longest head prefix (0 = no head) */
+    uint32_t head_density; /* ... and the mean predecessor count that makes a block dense */
 };
+
+/* Default head density: a 1024-position block of the greedy order belongs to
+ * the head when its vertices have at least this many predecessors on
+ * average (R-MAT scale 22, GNN3: ~460-1000 in the first 32,768 positions;
+ * 27-point stencils and Erdos-Renyi graphs stay far below it). */
+#define GCX_HEAD_DENSITY 64u
 
 /* Return GCX_ERR_HIP when a HIP call fails / the status of a failing call. */
 #define HIPC(expr)                                                              \

@@ -35,6 +35,7 @@ typedef struct {
     uint32_t sl_done;
     uint32_t total_items;   /* JP (ordered): work items in greedy order */
     uint32_t err;           /* JP (ordered): a wave hit its wait limit */
+    uint32_t head_n;        /* JP (ordered): greedy-order positions the head kernel colours */
 } gcxk_ctl;
 
 /* ---- graph helpers (csr.hip) ---- */
@@ -224,7 +225,40 @@ int gcxk_jp_clear(uint32_t N, uint64_t M, uint32_t *ovf, uint32_t nhub, uint32_t
 int gcxk_jp_init(uint32_t N, const uint32_t *dV, const uint32_t *dE, const void *key,
                  int key_bits, int mb, const uint32_t *perm, uint32_t *npred, uint32_t *esplit,
                  uint64_t *state, int *color, const uint32_t *hubs, uint32_t nhub,
-                 const uint32_t *hchunk, uint32_t nhchunk, uint32_t *hpos, void *stream, void *hub_stream);
+                 const uint32_t *hchunk, uint32_t nhchunk, uint32_t *hpos, uint32_t head_max,
+                 void *stream, void *hub_stream);
+/* head_max > 0 (with 32-bit rank keys from gcxk_jp_rank_keys only): every
+ * vertex whose greedy-order position N - 1 - key is below head_max also
+ * gets its predecessors' positions in E_split[V[u] .. V[u] + npred[u]), for
+ * the head kernel. */
+/* Greedy order alone: ids (inside scratch, sized by gcxk_jp_order_scratch)
+ * = the vertices sorted by key descending, id ascending. */
+int gcxk_jp_sort_ids(uint32_t N, const void *key, int key_bits, void *scratch,
+                     size_t scratch_bytes, uint32_t **ids, void *stream);
+/* rk[ids[i]] = N - 1 - i: 32-bit keys of the same order with no ties. */
+int gcxk_jp_rank_keys(uint32_t N, const uint32_t *ids, uint32_t *rk, void *stream);
+/* Longest prefix the head kernel may colour (positions). */
+uint32_t gcxk_jp_head_max(void);
+/* Ordered schedule after gcxk_jp_sort_ids + rank keys + orientation with
+ * head_max: ctl->head_n = the dense head of the order (leading 1024-position
+ * blocks whose mean predecessor count is at least head_density, at most
+ * head_max positions; 0 = no head), then state words, descriptors (head
+ * vertices flagged) and items as gcxk_jp_order_items. key_bits: the keys
+ * the ids were sorted by (sizes the scratch layout). */
+int gcxk_jp_order_items_ranked(uint32_t N, const uint32_t *dV, const uint32_t *npred,
+                               const uint32_t *ids, const uint32_t *rk, uint32_t head_max,
+                               uint32_t head_density, int key_bits, int mb, void *scratch,
+                               size_t scratch_bytes, uint32_t *cnt, uint32_t *incl, uint32_t *wof,
+                               uint64_t *wdesc, uint64_t *words, uint64_t words_cap,
+                               gcxk_item *items, gcxk_ctl *ctl, void *stream);
+/* Head kernel: one workgroup colours greedy-order positions 0 .. head_n
+ * (LDS colours); run before gcxk_jp_ordered, which takes those colours as
+ * given. hdr: 8 bytes of scratch per head position (gcxk_jp_head_max() at
+ * most). A wait past spin_ticks sets ctl->err (the ordered launch then does
+ * nothing and the caller redoes the colouring). */
+int gcxk_jp_head(const uint32_t *dV, const uint32_t *npred, const uint32_t *esplit,
+                 const uint32_t *ids, void *hdr, int *color, gcxk_ctl *ctl, uint64_t spin_ticks,
+                 void *stream);
 /* Round schedule: queue the roots (vertices with successors and no
  * predecessors) as round-0 items in q0 / ctl->qn[0]. */
 int gcxk_jp_roots(uint32_t N, const uint32_t *dV, const uint32_t *npred, gcxk_item *q0,

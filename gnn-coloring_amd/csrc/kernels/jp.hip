@@ -191,7 +191,8 @@ __global__ __launch_bounds__(kBlock) void jp_init_quads(uint32_t N, const uint32
                                                         uint32_t *__restrict__ npred,
                                                         uint32_t *__restrict__ esplit,
                                                         uint64_t *__restrict__ state,
-                                                        int *__restrict__ color, int mb)
+                                                        int *__restrict__ color, int mb,
+                                                        uint32_t head_max)
 {
     const uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     const uint32_t pi = (uint32_t)(t / kQuad);
@@ -206,9 +207,13 @@ __global__ __launch_bounds__(kBlock) void jp_init_quads(uint32_t N, const uint32
     uint32_t np = 0, ns = 0;
     if (end > beg) {
         const K ku = key[u];
+        // rank keys: the predecessors of a head-prefix vertex go to the front
+        // of its row as greedy-order positions (N - 1 - key)
+        const bool emit = head_max != 0u && N - 1u - (uint32_t)ku < head_max;
         const uint32_t below = (1u << gl) - 1u;
         for (uint32_t i0 = beg; i0 < end; i0 += kQuad * kQuadSteps) {
             uint32_t v[kQuadSteps];
+            K kv[kQuadSteps];
             bool in[kQuadSteps], pr[kQuadSteps];
 #pragma unroll
             for (int s = 0; s < kQuadSteps; s++) {
@@ -217,14 +222,18 @@ __global__ __launch_bounds__(kBlock) void jp_init_quads(uint32_t N, const uint32
                 v[s] = in[s] ? E[i] : 0u;
             }
 #pragma unroll
-            for (int s = 0; s < kQuadSteps; s++)
-                pr[s] = in[s] && precedes(key[v[s]], v[s], ku, u);
+            for (int s = 0; s < kQuadSteps; s++) {
+                kv[s] = in[s] ? key[v[s]] : K(0);
+                pr[s] = in[s] && precedes(kv[s], v[s], ku, u);
+            }
 #pragma unroll
             for (int s = 0; s < kQuadSteps; s++) {
                 const uint32_t gp = (uint32_t)(__ballot(pr[s]) >> gsh) & 0xFu;
                 const uint32_t gs = (uint32_t)(__ballot(in[s] && !pr[s]) >> gsh) & 0xFu;
                 if (in[s] && !pr[s])
                     esplit[end - 1u - ns - __popc(gs & below)] = v[s];
+                if (emit && pr[s])
+                    esplit[beg + np + __popc(gp & below)] = N - 1u - (uint32_t)kv[s];
                 np += __popc(gp);
                 ns += __popc(gs);
             }
@@ -254,10 +263,11 @@ __global__ __launch_bounds__(kBlock) void jp_init_hub_chunks(const uint32_t *__r
                                                              const uint32_t *__restrict__ hubs,
                                                              const uint32_t *__restrict__ hchunk,
                                                              uint32_t nhub,
-                                                             uint32_t *__restrict__ hpos, int mb)
+                                                             uint32_t *__restrict__ hpos, int mb,
+                                                             uint32_t N, uint32_t head_max)
 {
-    __shared__ uint32_t s_cnt[2][kWavesPerBlock];
-    __shared__ uint32_t s_base;
+    __shared__ uint32_t s_cnt[3][kWavesPerBlock]; // [1] successors, [2] predecessors
+    __shared__ uint32_t s_base, s_pbase;
     // hub of this block: last h with hchunk[h] <= blockIdx.x
     uint32_t lo = 0, hi = nhub;
     while (hi - lo > 1) {
@@ -275,6 +285,7 @@ __global__ __launch_bounds__(kBlock) void jp_init_hub_chunks(const uint32_t *__r
     const int lane = lane_id(), w = threadIdx.x / kWave;
     // all of the chunk's loads in flight, then one reservation per block
     uint32_t v[kHubPer];
+    K kv[kHubPer];
     bool in[kHubPer], sc[kHubPer];
 #pragma unroll
     for (int q = 0; q < kHubPer; q++) {
@@ -285,44 +296,59 @@ __global__ __launch_bounds__(kBlock) void jp_init_hub_chunks(const uint32_t *__r
     uint32_t np = 0, ns = 0;
 #pragma unroll
     for (int q = 0; q < kHubPer; q++) {
-        sc[q] = in[q] && !precedes(key[v[q]], v[q], ku, u);
+        kv[q] = in[q] ? key[v[q]] : K(0);
+        sc[q] = in[q] && !precedes(kv[q], v[q], ku, u);
         np += (in[q] && !sc[q]) ? 1u : 0u;
         ns += sc[q] ? 1u : 0u;
     }
-    uint32_t is = ns; // inclusive scan of the successor counts over the wave
+    // rank keys: a head-prefix hub also writes its predecessors' greedy-order
+    // positions (N - 1 - key) to the front of its row
+    const bool emit = head_max != 0u && N - 1u - (uint32_t)ku < head_max;
+    uint32_t is = ns, ip = np; // inclusive scans of the counts over the wave
 #pragma unroll
     for (int off = 1; off < kWave; off <<= 1) {
-        const uint32_t a = __shfl_up(is, off);
-        if (lane >= off)
+        const uint32_t a = __shfl_up(is, off), b = __shfl_up(ip, off);
+        if (lane >= off) {
             is += a;
+            ip += b;
+        }
     }
-    const uint32_t wp = group_sum32<kWave>(np);
-    if (lane == kWave - 1)
+    if (lane == kWave - 1) {
         s_cnt[1][w] = is;
-    if (lane == 0)
-        s_cnt[0][w] = wp;
+        s_cnt[2][w] = ip;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t tp = 0, ts = 0;
         for (int q = 0; q < kWavesPerBlock; q++) {
-            tp += s_cnt[0][q];
-            const uint32_t b = s_cnt[1][q];
+            const uint32_t b = s_cnt[1][q], c = s_cnt[2][q];
             s_cnt[1][q] = ts;
+            s_cnt[2][q] = tp;
             ts += b;
+            tp += c;
         }
         s_base = ts ? atomicAdd(&hpos[2 * h + 1], ts) : 0u;
+        s_pbase = (tp && emit) ? atomicAdd(&hpos[2 * h], tp) : 0u;
         if (tp) {
             atomicAdd(&npred[u], tp);
             atomicAdd((unsigned long long *)&state[u], (unsigned long long)tp << mb);
         }
     }
     __syncthreads();
-    // successors fill the row from its back; their order is irrelevant
+    // successors fill the row from its back, predecessor positions (head
+    // prefix only) from its front; the order within each part is irrelevant
     uint32_t pos = s_base + s_cnt[1][w] + is - ns;
 #pragma unroll
     for (int q = 0; q < kHubPer; q++)
         if (sc[q])
             esplit[end - 1u - pos++] = v[q];
+    if (emit) {
+        uint32_t pp = s_pbase + s_cnt[2][w] + ip - np;
+#pragma unroll
+        for (int q = 0; q < kHubPer; q++)
+            if (in[q] && !sc[q])
+                esplit[beg + pp++] = N - 1u - (uint32_t)kv[q];
+    }
 }
 
 // ------------------------------------------------------------ round schedule
@@ -513,15 +539,408 @@ __global__ __launch_bounds__(kBlock) void jp_word_counts(uint32_t N, const uint3
 
 // Per-vertex descriptor read by the ordered kernel (one 8-byte load instead
 // of two offsets): first word (bits 0-39), word count (40-47), field width
-// B (48-55; 0 = bit-set words).
+// B (48-55; 0 = bit-set words), and bit 56 set when the head kernel colours
+// the vertex (rank keys rk given and its greedy-order position N - 1 - rk[v]
+// below ctl->head_n).
+constexpr int kHeadFlagBit = 56;
+
 __global__ __launch_bounds__(kBlock) void jp_word_desc(uint32_t N, const uint32_t *__restrict__ npred,
                                                        const uint32_t *__restrict__ wof,
+                                                       const uint32_t *__restrict__ rk,
+                                                       const gcxk_ctl *__restrict__ ctl,
                                                        uint64_t *__restrict__ wdesc)
 {
     const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
     if (v < N) {
         const uint64_t n = wof[v + 1] - wof[v];
-        wdesc[v] = (uint64_t)wof[v] | (n << 40) | ((uint64_t)field_bits(npred[v]) << 48);
+        const bool head = rk && N - 1u - rk[v] < ctl->head_n;
+        wdesc[v] = (uint64_t)wof[v] | (n << 40) | ((uint64_t)field_bits(npred[v]) << 48) |
+                   ((uint64_t)head << kHeadFlagBit);
+    }
+}
+
+// Rank keys: rk[ids[i]] = N - 1 - i for the vertices in greedy order (ids), so
+// larger = earlier and no two are equal.
+__global__ __launch_bounds__(kBlock) void jp_rank_keys(uint32_t N, const uint32_t *__restrict__ ids,
+                                                       uint32_t *__restrict__ rk)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < N)
+        rk[ids[i]] = N - 1u - i;
+}
+
+// ------------------------------------------------------------- head kernel
+//
+// The top of the greedy order of a skewed graph is a dense core whose
+// vertices form most of the DAG's depth: on R-MAT scale 22 (GNN3 order) the
+// first 32,768 positions hold 2,516 of the 2,741 levels of the longest
+// chain, each of their vertices has at most ~1000 predecessors, and every
+// predecessor of a vertex in such a prefix lies in the prefix too. One
+// workgroup colours that prefix with the colours in LDS: a hop of the chain
+// is then an LDS store seen by an LDS poll instead of a device-scope atomic
+// seen by a device-scope poll. The rest of the graph follows in the ordered
+// kernel, which takes the head's colours as given.
+// Waits read the real-time clock (for their 2 s bound) only from their 64th
+// poll on: s_memrealtime is a scalar memory read, and the next LDS or
+// scalar access waits for it (one shared counter), which put the clock's
+// latency on every hop of the chain when the first poll read it.
+constexpr uint32_t kClockPoll = 64;
+
+constexpr int kHeadBlock = 1024;
+constexpr int kHeadWaves = kHeadBlock / kWave;
+constexpr uint32_t kHeadMax = 32768;          // prefix positions (32-bit colours in LDS)
+constexpr uint32_t kHeadWin = 4096;           // colours per mex window
+constexpr uint32_t kHeadWinWords = kHeadWin / 32;
+constexpr int kHeadSlots = 16;                // predecessor positions per lane and batch
+constexpr uint32_t kHeadBatch = kHeadSlots * kWave;
+constexpr uint32_t kHeadBlockPos = 1024;      // positions per density block
+constexpr uint32_t kHeadWaveWords = kHeadWinWords + kWave; // bitmap + a word per lane
+constexpr size_t kHeadLds = (size_t)kHeadWaves * kHeadWaveWords * 4 + 4 * kHeadMax + 4;
+
+// Head extent: the leading 1024-position blocks of the greedy order whose
+// mean predecessor count is at least min_density (a dense core), at most
+// head_max positions; ctl->head_n = their total (0: no head).
+__global__ __launch_bounds__(kHeadBlock) void jp_head_extent(uint32_t N, const uint32_t *__restrict__ ids,
+                                                             const uint32_t *__restrict__ npred,
+                                                             uint32_t head_max,
+                                                             uint32_t min_density, gcxk_ctl *ctl)
+{
+    __shared__ uint32_t part[kHeadWaves];
+    __shared__ uint32_t s_go;
+    const uint32_t lim = N < head_max ? N : head_max;
+    const int w = threadIdx.x / kWave;
+    uint32_t P = 0;
+    for (uint32_t b = 0; b < lim; b += kHeadBlockPos) {
+        const uint32_t i = b + threadIdx.x;
+        uint32_t x = i < lim ? npred[ids[i]] : 0u;
+        x = group_sum32<kWave>(x);
+        if (lane_id() == 0)
+            part[w] = x;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint64_t sum = 0;
+            for (int q = 0; q < kHeadWaves; q++)
+                sum += part[q];
+            const uint32_t n = lim - b < kHeadBlockPos ? lim - b : kHeadBlockPos;
+            s_go = sum >= (uint64_t)min_density * n;
+        }
+        __syncthreads();
+        if (!s_go)
+            break;
+        P = b + kHeadBlockPos < lim ? b + kHeadBlockPos : lim;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        ctl->head_n = P;
+}
+
+// Head table: hdr[i] = (first entry of the row, predecessor count) of the
+// vertex at greedy-order position i < ctl->head_n, so the head kernel reads
+// one record per position instead of ids -> V and npred.
+__global__ __launch_bounds__(kBlock) void jp_head_table(const uint32_t *__restrict__ V,
+                                                        const uint32_t *__restrict__ npred,
+                                                        const uint32_t *__restrict__ ids,
+                                                        const gcxk_ctl *__restrict__ ctl,
+                                                        uint2 *__restrict__ hdr)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < ctl->head_n) {
+        const uint32_t u = ids[i];
+        hdr[i] = make_uint2(V[u], npred[u]);
+    }
+}
+
+// A head position's work: its vertex, row start, predecessor count and the
+// first batch of its predecessor positions (registers).
+struct head_vtx {
+    uint32_t u, base, np;
+    uint32_t pos[kHeadSlots];
+};
+
+struct head_hdr {
+    uint32_t u, base, np;
+};
+
+__device__ __forceinline__ head_hdr head_fetch_hdr(uint32_t i, uint32_t P,
+                                                   const uint32_t *__restrict__ ids,
+                                                   const uint2 *__restrict__ hdr)
+{
+    head_hdr h = {0u, 0u, 0u};
+    if (i < P) {
+        const uint2 r = hdr[i];
+        h.u = ids[i];
+        h.base = r.x;
+        h.np = r.y;
+    }
+    return h;
+}
+
+__device__ __forceinline__ void head_take(head_vtx &x, const head_hdr &h)
+{
+    x.u = h.u;
+    x.base = h.base;
+    x.np = h.np;
+}
+
+// Loads are unconditional (slots past the list re-read its last entry, and a
+// vertex without predecessors reads entry 0), so the number of loads in
+// flight is the same on every path and the compiler can wait for exactly
+// the older ones; slots past the list are masked out by the caller.
+__device__ __forceinline__ void head_fetch_list(head_vtx &x, uint32_t b0,
+                                                const uint32_t *__restrict__ esplit, int lane)
+{
+    const uint32_t last = x.np ? x.base + x.np - 1u : 0u;
+#pragma unroll
+    for (int t = 0; t < kHeadSlots; t++) {
+        const uint32_t j = x.base + b0 + (uint32_t)(kWave * t + lane);
+        x.pos[t] = esplit[j < last ? j : last];
+    }
+}
+
+// Colour one head position from its predecessors' LDS colours; x.pos holds
+// the first batch. Returns false when a wait hit its bound (ctl->err set).
+// LDS colours: relaxed workgroup-scope atomics keep them LDS accesses
+// (ds_read / ds_write) that other waves' stores become visible to.
+__device__ __forceinline__ uint32_t lds_get(const uint32_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Generic colouring of a head position: any number of predecessors (in
+// batches of 1024) and colours past the first window (4096 colours per pass
+// over the, by then complete, list). x.pos holds the first batch. Returns
+// the colour, or 0 when a wait hit its bound (ctl->err set).
+__device__ __noinline__ uint32_t head_colour_any(head_vtx &x, uint32_t *bm, uint32_t *col,
+                                                 const uint32_t *__restrict__ esplit,
+                                                 gcxk_ctl *ctl, uint64_t spin_ticks, int lane)
+{
+    uint32_t c = x.np ? 0u : 1u;
+    for (uint32_t win = 0; c == 0; win++) {
+        const uint32_t lo = win * kHeadWin; // window: colours lo + 1 .. lo + kHeadWin
+        for (uint32_t b0 = 0; b0 < x.np; b0 += kHeadBatch) {
+            if (b0 || win)
+                head_fetch_list(x, b0, esplit, lane);
+            uint32_t need = 0;
+#pragma unroll
+            for (int t = 0; t < kHeadSlots; t++)
+                need |= b0 + (uint32_t)(kWave * t + lane) < x.np ? 1u << t : 0u;
+            uint32_t polls = 0;
+            uint64_t t0 = 0;
+            for (;;) {
+                uint32_t cv[kHeadSlots];
+#pragma unroll
+                for (int t = 0; t < kHeadSlots; t++)
+                    cv[t] = lds_get(&col[x.pos[t]]);
+#pragma unroll
+                for (int t = 0; t < kHeadSlots; t++) {
+                    if ((need & (1u << t)) && cv[t]) {
+                        need &= ~(1u << t);
+                        if (cv[t] > lo && cv[t] <= lo + kHeadWin)
+                            atomicOr(&bm[(cv[t] - lo - 1u) / 32u], 1u << ((cv[t] - lo - 1u) % 32u));
+                    }
+                }
+                if (__ballot(need != 0u) == 0ull)
+                    break;
+                if (polls == kClockPoll)
+                    t0 = __builtin_amdgcn_s_memrealtime();
+                __builtin_amdgcn_s_sleep(1);
+                if ((++polls & 255u) == 0u &&
+                    (__builtin_amdgcn_s_memrealtime() - t0 > spin_ticks ||
+                     __hip_atomic_load(&ctl->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+                    if (lane == 0)
+                        __hip_atomic_store(&ctl->err, 1u, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                    return 0u;
+                }
+            }
+        }
+        const uint32_t w0 = bm[lane], w1 = bm[lane + kWave];
+        const uint64_t f0 = __ballot(w0 != 0xFFFFFFFFu), f1 = __ballot(w1 != 0xFFFFFFFFu);
+        if (f0) {
+            const int fl = __ffsll((long long)f0) - 1;
+            c = lo + 32u * (uint32_t)fl +
+                (uint32_t)__builtin_ctz(~__builtin_amdgcn_readlane(w0, fl)) + 1u;
+        } else if (f1) {
+            const int fl = __ffsll((long long)f1) - 1;
+            c = lo + 32u * (uint32_t)(kWave + fl) +
+                (uint32_t)__builtin_ctz(~__builtin_amdgcn_readlane(w1, fl)) + 1u;
+        }
+        bm[lane] = 0u;
+        bm[lane + kWave] = 0u;
+    }
+    return c;
+}
+
+// Smallest colour of the window (1 .. 4096) whose bit is clear in the wave's
+// bitmap words (w0: words 0-63, w1: words 64-127, one of each per lane); 0
+// when all are set. Marks it set in w0 / w1 of its lane.
+__device__ __forceinline__ uint32_t head_take_free(uint32_t &w0, uint32_t &w1, int lane)
+{
+    const uint64_t f0 = __ballot(w0 != 0xFFFFFFFFu), f1 = __ballot(w1 != 0xFFFFFFFFu);
+    const uint64_t f = f0 ? f0 : f1;
+    if (!f)
+        return 0u;
+    const int fl = __ffsll((long long)f) - 1;
+    const uint32_t word = f0 ? __builtin_amdgcn_readlane(w0, fl) : __builtin_amdgcn_readlane(w1, fl);
+    const uint32_t b = (uint32_t)__builtin_ctz(~word);
+    if (lane == fl) {
+        if (f0)
+            w0 |= 1u << b;
+        else
+            w1 |= 1u << b;
+    }
+    return 32u * (uint32_t)(f0 ? fl : kWave + fl) + b + 1u;
+}
+
+// Wait until `*p >= target` (LDS word); false when the wait hit its bound.
+__device__ __forceinline__ bool head_wait_ge(const uint32_t *p, uint32_t target, uint32_t &polls,
+                                             uint64_t &t0, gcxk_ctl *ctl, uint64_t spin_ticks,
+                                             int lane, uint32_t &got)
+{
+    while ((got = lds_get(p)) < target) {
+        __builtin_amdgcn_s_sleep(1);
+        if (++polls == kClockPoll)
+            t0 = __builtin_amdgcn_s_memrealtime();
+        if ((polls & 255u) == 0u &&
+            (__builtin_amdgcn_s_memrealtime() - t0 > spin_ticks ||
+             __hip_atomic_load(&ctl->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+            if (lane == 0)
+                __hip_atomic_store(&ctl->err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+    }
+    return true;
+}
+
+// Colour of one head position, or 0 when a wait hit its bound. The
+// watermark *wm is the smallest position not yet coloured. Common case (at
+// most 1024 predecessors, colours below 4096): with L the predecessor
+// latest in the greedy order, the wave waits until wm >= L -- then every
+// other predecessor is coloured -- folds those colours into its LDS bitmap
+// in one gather, finds the two smallest colours absent from it, m0 < m1,
+// and waits for L's colour cl: the colour is cl == m0 ? m1 : m0. Only that
+// last wait is on the chain. Everything else goes to head_colour_any.
+__device__ __forceinline__ uint32_t head_colour(head_vtx &x, uint32_t *bm, uint32_t *col,
+                                                const uint32_t *wm,
+                                                const uint32_t *__restrict__ esplit,
+                                                gcxk_ctl *ctl, uint64_t spin_ticks, int lane)
+{
+    if (x.np == 0u)
+        return 1u;
+    if (x.np > kHeadBatch)
+        return head_colour_any(x, bm, col, esplit, ctl, spin_ticks, lane);
+    uint32_t latest = 0;
+#pragma unroll
+    for (int t = 0; t < kHeadSlots; t++) {
+        const bool in = (uint32_t)(kWave * t + lane) < x.np;
+        latest = in && x.pos[t] > latest ? x.pos[t] : latest;
+    }
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        const uint32_t y = __shfl_xor(latest, off);
+        latest = y > latest ? y : latest;
+    }
+    uint32_t polls = 0, got = 0;
+    uint64_t t0 = 0;
+    if (!head_wait_ge(wm, latest, polls, t0, ctl, spin_ticks, lane, got))
+        return 0u;
+    // every predecessor but `latest` is coloured: fold them (branch-free;
+    // lanes with nothing to add OR 0 into a word of their own)
+    uint32_t cv[kHeadSlots];
+#pragma unroll
+    for (int t = 0; t < kHeadSlots; t++)
+        cv[t] = lds_get(&col[x.pos[t]]);
+#pragma unroll
+    for (int t = 0; t < kHeadSlots; t++) {
+        const bool ok = (uint32_t)(kWave * t + lane) < x.np && x.pos[t] != latest;
+        const bool in_win = ok && cv[t] <= kHeadWin;
+        const uint32_t widx = in_win ? (cv[t] - 1u) / 32u : kHeadWinWords + (uint32_t)lane;
+        atomicOr(&bm[widx], in_win ? 1u << ((cv[t] - 1u) % 32u) : 0u);
+    }
+    uint32_t w0 = bm[lane], w1 = bm[lane + kWave];
+    const uint32_t m0 = head_take_free(w0, w1, lane);
+    const uint32_t m1 = head_take_free(w0, w1, lane);
+    if (m1 == 0u) // colours past the window: the generic path (the bitmap
+                  // keeps what was folded; folding it again changes nothing)
+        return head_colour_any(x, bm, col, esplit, ctl, spin_ticks, lane);
+    bm[lane] = 0u;
+    bm[lane + kWave] = 0u;
+    uint32_t cl = 0;
+    if (!head_wait_ge(&col[latest], 1u, polls, t0, ctl, spin_ticks, lane, cl))
+        return 0u;
+    return cl == m0 ? m1 : m0;
+}
+
+// Record colour c of position i and advance the watermark past it and any
+// positions already coloured after it. A wave that finds the watermark at
+// its own position moves it with a compare-and-swap, then re-checks the next
+// position: a wave colouring that position meanwhile either sees the moved
+// watermark (and advances it itself) or is seen by the re-check, so the
+// watermark never stalls behind a coloured position. Lane 0 only.
+__device__ __forceinline__ void head_finish(uint32_t i, uint32_t c, uint32_t u, uint32_t P,
+                                            uint32_t *col, uint32_t *wm, int *__restrict__ color)
+{
+    __hip_atomic_store(&col[i], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    color[u] = (int)c;
+    uint32_t k = i;
+    while (lds_get(wm) == k) {
+        uint32_t n = k + 1u;
+        while (n < P && lds_get(&col[n]) != 0u)
+            n++;
+        uint32_t expect = k;
+        if (!__hip_atomic_compare_exchange_strong(wm, &expect, n, __ATOMIC_RELAXED,
+                                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
+            break;
+        if (n >= P || lds_get(&col[n]) == 0u)
+            break; // a wave colouring n after this sees wm == n
+        k = n;     // n was coloured meanwhile: keep going from it
+    }
+}
+
+// One workgroup of 16 waves; wave w colours positions w, w + 16, ... of the
+// prefix in order. A position's predecessors are the greedy-order positions
+// the orientation pass wrote to the front of its row (esplit), at most 1024
+// per batch in registers. The wave loads the batch and waits -- on one LDS
+// word, the colour of the predecessor latest in the greedy order -- so that
+// memory latency falls into the wait; once that colour is set it gathers
+// all the predecessors' colours into its LDS bitmap (polling again for any
+// still missing) and takes the first clear bit: the mex. Colours past a
+// window of 4096 take further passes over the (by then complete) list. The
+// next position's header is loaded one position ahead.
+//
+// The first unfinished position has all its predecessors finished and its
+// wave is at it, so the kernel completes; every wait is bounded all the
+// same.
+__global__ __launch_bounds__(kHeadBlock) void jp_head_kernel(const uint32_t *__restrict__ esplit,
+                                                             const uint32_t *__restrict__ ids,
+                                                             const uint2 *__restrict__ hdr,
+                                                             int *__restrict__ color, gcxk_ctl *ctl,
+                                                             uint64_t spin_ticks)
+{
+    extern __shared__ uint32_t head_lds[];
+    const int w = threadIdx.x / kWave, lane = lane_id();
+    uint32_t *bm = head_lds + (size_t)w * kHeadWaveWords;
+    uint32_t *col = head_lds + (size_t)kHeadWaves * kHeadWaveWords;
+    uint32_t *wm = col + kHeadMax; // watermark: smallest position not yet coloured
+    const uint32_t P = ctl->head_n;
+    for (uint32_t i = threadIdx.x; i < P; i += kHeadBlock)
+        col[i] = 0u;
+    for (uint32_t i = threadIdx.x; i < (uint32_t)kHeadWaves * kHeadWaveWords; i += kHeadBlock)
+        head_lds[i] = 0u;
+    if (threadIdx.x == 0)
+        *wm = 0u;
+    __syncthreads();
+    head_vtx x;
+    head_hdr next = head_fetch_hdr((uint32_t)w, P, ids, hdr);
+    for (uint32_t i = (uint32_t)w; i < P; i += kHeadWaves) {
+        head_take(x, next);
+        next = head_fetch_hdr(i + kHeadWaves, P, ids, hdr);
+        head_fetch_list(x, 0, esplit, lane);
+        const uint32_t c = head_colour(x, bm, col, wm, esplit, ctl, spin_ticks, lane);
+        if (c == 0u)
+            return;
+        if (lane == 0)
+            head_finish(i, c, x.u, P, col, wm, color);
     }
 }
 
@@ -606,7 +1025,7 @@ __global__ __launch_bounds__(kBlock) void jp_ordered_kernel(const uint32_t *__re
                                                             gcxk_ctl *ctl, int mb,
                                                             uint64_t spin_ticks)
 {
-    const uint32_t n = ctl->total_items;
+    const uint32_t n = ctl->err ? 0u : ctl->total_items; // a failed head ends the launch
     const int lane = lane_id();
     const uint64_t one = 1ull << mb, low = one - 1ull;
     const uint32_t cap = kMaxWords * (uint32_t)mb;
@@ -626,11 +1045,16 @@ __global__ __launch_bounds__(kBlock) void jp_ordered_kernel(const uint32_t *__re
         const uint64_t dv = has ? wdesc[v] : 0ull;
         const uint32_t vo = (uint32_t)dv, vn = (uint32_t)(dv >> 40) & 0xFFu;
         const uint32_t Bv = (uint32_t)(dv >> 48) & 0xFFu;
+        // head-coloured successors get no reports (nothing waits on them)
+        const bool report = has && !((dv >> kHeadFlagBit) & 1ull);
         const bool mine = (uint32_t)lane < wn;
-        uint64_t w;
+        uint64_t w = 0;
         uint32_t polls = 0;
         uint64_t t0 = 0;
-        for (;;) {
+        // a head-coloured vertex (coloured by the head kernel, which ran
+        // before this launch) only reports its colour to its successors
+        const bool head_u = (du >> kHeadFlagBit) & 1ull;
+        for (; !head_u;) {
             w = mine ? load_state(&st[wo + lane]) : 0ull;
             uint32_t r = mine ? (Bu ? field_sum(w, Bu) : (uint32_t)(w >> mb)) : 0u;
             r += __shfl_xor(r, 1);
@@ -644,7 +1068,7 @@ __global__ __launch_bounds__(kBlock) void jp_ordered_kernel(const uint32_t *__re
             r = __shfl(r, 0);
             if (r == np)
                 break;
-            if (polls == 0)
+            if (polls == kClockPoll)
                 t0 = __builtin_amdgcn_s_memrealtime();
             poll_sleep(np - r);
             if ((++polls & 255u) == 0u &&
@@ -657,7 +1081,9 @@ __global__ __launch_bounds__(kBlock) void jp_ordered_kernel(const uint32_t *__re
         // mex: the first zero field, or the first word with a clear bit (all
         // 8 words full: overflow)
         uint32_t c;
-        if (Bu) {
+        if (head_u) {
+            c = (uint32_t)color[u];
+        } else if (Bu) {
             const uint64_t z = mine ? zero_fields(w, Bu, (uint32_t)lane == wn - 1u) : 0ull;
             const uint64_t bz = __ballot(z != 0ull);
             if (!bz) { // cannot happen: np + 1 colour fields, at most np counted
@@ -681,10 +1107,10 @@ __global__ __launch_bounds__(kBlock) void jp_ordered_kernel(const uint32_t *__re
                 c = mex_overflow(cap, u, Vu, Vu1 - Vu, ovf);
             }
         }
-        if (k == 0 && lane == 0)
+        if (k == 0 && lane == 0 && !head_u)
             color[u] = (int)c;
         bool late = false;
-        if (has) {
+        if (report) {
             if (Bv) {
                 const uint32_t F = 64u / Bv, top = vn * F - 1u;
                 const uint32_t fi = c - 1u < top ? c - 1u : top;
@@ -969,24 +1395,29 @@ extern "C" int gcxk_jp_init(uint32_t N, const uint32_t *dV, const uint32_t *dE, 
                             int key_bits, int mb, const uint32_t *perm, uint32_t *npred,
                             uint32_t *esplit, uint64_t *state, int *color, const uint32_t *hubs,
                             uint32_t nhub, const uint32_t *hchunk, uint32_t nhchunk,
-                            uint32_t *hpos, void *stream, void *hub_stream)
+                            uint32_t *hpos, uint32_t head_max, void *stream, void *hub_stream)
 {
     if (N == 0)
         return GCX_OK;
+    if (head_max && key_bits != 32)
+        return GCX_ERR_ARG; // predecessor positions come from rank keys
     hipStream_t s = as_stream(stream), hs = as_stream(hub_stream);
     const uint32_t grid = div_up((uint64_t)N * kQuad, kBlock);
     if (key_bits == 32) {
         jp_init_quads<uint32_t><<<grid, kBlock, 0, s>>>(N, dV, dE, (const uint32_t *)key, perm,
-                                                        nhub, npred, esplit, state, color, mb);
+                                                        nhub, npred, esplit, state, color, mb,
+                                                        head_max);
         if (nhub)
             jp_init_hub_chunks<uint32_t><<<nhchunk, kBlock, 0, hs>>>(
-                dV, dE, (const uint32_t *)key, npred, state, esplit, hubs, hchunk, nhub, hpos, mb);
+                dV, dE, (const uint32_t *)key, npred, state, esplit, hubs, hchunk, nhub, hpos, mb,
+                N, head_max);
     } else {
         jp_init_quads<uint64_t><<<grid, kBlock, 0, s>>>(N, dV, dE, (const uint64_t *)key, perm,
-                                                        nhub, npred, esplit, state, color, mb);
+                                                        nhub, npred, esplit, state, color, mb, 0u);
         if (nhub)
             jp_init_hub_chunks<uint64_t><<<nhchunk, kBlock, 0, hs>>>(
-                dV, dE, (const uint64_t *)key, npred, state, esplit, hubs, hchunk, nhub, hpos, mb);
+                dV, dE, (const uint64_t *)key, npred, state, esplit, hubs, hchunk, nhub, hpos, mb,
+                N, 0u);
     }
     GCXK_LAUNCH_CHECK();
     return GCX_OK;
@@ -1044,6 +1475,49 @@ extern "C" size_t gcxk_jp_order_scratch(uint32_t N, int key_bits)
     return order_tmp_offset(N, ks) + need;
 }
 
+// State word offsets, descriptors (head flags when rk is given) and zeroed
+// words; incl (N + 1) is used for the per-vertex word counts.
+static int order_words(uint32_t N, const uint32_t *npred, const uint32_t *rk, int mb, char *tmp,
+                       size_t tmp_bytes, uint32_t *incl, uint32_t *wof, uint64_t *wdesc,
+                       uint64_t *words, uint64_t words_cap, gcxk_ctl *ctl, hipStream_t s)
+{
+    size_t need = 0;
+    jp_word_counts<<<div_up((uint64_t)N + 1, kBlock), kBlock, 0, s>>>(N, npred, mb, incl);
+    GCXK_LAUNCH_CHECK();
+    GCXK_CHECK(rocprim::exclusive_scan(nullptr, need, incl, wof, 0u, (size_t)N + 1,
+                                       rocprim::plus<uint32_t>(), s));
+    if (need > tmp_bytes)
+        return GCX_ERR_CAPACITY;
+    GCXK_CHECK(rocprim::exclusive_scan(tmp, need, incl, wof, 0u, (size_t)N + 1,
+                                       rocprim::plus<uint32_t>(), s));
+    jp_word_desc<<<div_up(N, kBlock), kBlock, 0, s>>>(N, npred, wof, rk, ctl, wdesc);
+    GCXK_LAUNCH_CHECK();
+    uint32_t zg = div_up((uint64_t)N + 1, kBlock);
+    zg = zg < 2048u ? zg : 2048u;
+    jp_zero_words<<<zg, kBlock, 0, s>>>(wof, N, words_cap, words);
+    GCXK_LAUNCH_CHECK();
+    return GCX_OK;
+}
+
+// Work items of the vertices in greedy order (ids): counts into cnt, their
+// inclusive scan into incl, then the items and ctl->total_items.
+static int order_items_from(uint32_t N, const uint32_t *dV, const uint32_t *npred,
+                            const uint32_t *ids, char *tmp, size_t tmp_bytes, uint32_t *cnt,
+                            uint32_t *incl, gcxk_item *items, gcxk_ctl *ctl, hipStream_t s)
+{
+    size_t need = 0;
+    jp_item_counts<<<div_up(N, kBlock), kBlock, 0, s>>>(N, dV, npred, ids, cnt);
+    GCXK_LAUNCH_CHECK();
+    GCXK_CHECK(rocprim::inclusive_scan(nullptr, need, cnt, incl, N, rocprim::plus<uint32_t>(), s));
+    if (need > tmp_bytes)
+        return GCX_ERR_CAPACITY;
+    GCXK_CHECK(rocprim::inclusive_scan(tmp, need, cnt, incl, N, rocprim::plus<uint32_t>(), s));
+    jp_expand<<<div_up(N, kBlock), kBlock, 0, s>>>(N, ids, incl, reinterpret_cast<uint2 *>(items),
+                                                   ctl);
+    GCXK_LAUNCH_CHECK();
+    return GCX_OK;
+}
+
 extern "C" int gcxk_jp_order_items(uint32_t N, const uint32_t *dV, const uint32_t *npred,
                                    const void *key, int key_bits, int mb, void *scratch,
                                    size_t scratch_bytes, uint32_t *cnt, uint32_t *incl,
@@ -1054,43 +1528,90 @@ extern "C" int gcxk_jp_order_items(uint32_t N, const uint32_t *dV, const uint32_
     if (N == 0)
         return GCX_OK;
     hipStream_t s = as_stream(stream);
-    {
-        // state words: counts into incl (N + 1), exclusive scan -> wof, zero
-        const size_t ks = key_bits == 32 ? 4 : 8;
-        char *tmp = static_cast<char *>(scratch) + order_tmp_offset(N, ks);
-        size_t tmp_bytes = scratch_bytes - order_tmp_offset(N, ks), need = 0;
-        jp_word_counts<<<div_up((uint64_t)N + 1, kBlock), kBlock, 0, s>>>(N, npred, mb, incl);
-        GCXK_LAUNCH_CHECK();
-        GCXK_CHECK(rocprim::exclusive_scan(nullptr, need, incl, wof, 0u, (size_t)N + 1,
-                                           rocprim::plus<uint32_t>(), s));
-        if (need > tmp_bytes)
-            return GCX_ERR_CAPACITY;
-        GCXK_CHECK(rocprim::exclusive_scan(tmp, need, incl, wof, 0u, (size_t)N + 1,
-                                           rocprim::plus<uint32_t>(), s));
-        jp_word_desc<<<div_up(N, kBlock), kBlock, 0, s>>>(N, npred, wof, wdesc);
-        GCXK_LAUNCH_CHECK();
-        uint32_t zg = div_up((uint64_t)N + 1, kBlock);
-        zg = zg < 2048u ? zg : 2048u;
-        jp_zero_words<<<zg, kBlock, 0, s>>>(wof, N, words_cap, words);
-        GCXK_LAUNCH_CHECK();
-    }
-    uint32_t *ids = nullptr;
-    const int rc = key_bits == 32 ? sort_order<uint32_t>(N, key, scratch, scratch_bytes, &ids, s)
-                                  : sort_order<uint64_t>(N, key, scratch, scratch_bytes, &ids, s);
+    const size_t ks = key_bits == 32 ? 4 : 8;
+    if (order_tmp_offset(N, ks) > scratch_bytes)
+        return GCX_ERR_CAPACITY;
+    char *tmp = static_cast<char *>(scratch) + order_tmp_offset(N, ks);
+    const size_t tmp_bytes = scratch_bytes - order_tmp_offset(N, ks);
+    int rc = order_words(N, npred, nullptr, mb, tmp, tmp_bytes, incl, wof, wdesc, words, words_cap,
+                         ctl, s);
     if (rc != GCX_OK)
         return rc;
-    jp_item_counts<<<div_up(N, kBlock), kBlock, 0, s>>>(N, dV, npred, ids, cnt);
-    GCXK_LAUNCH_CHECK();
+    uint32_t *ids = nullptr;
+    rc = key_bits == 32 ? sort_order<uint32_t>(N, key, scratch, scratch_bytes, &ids, s)
+                        : sort_order<uint64_t>(N, key, scratch, scratch_bytes, &ids, s);
+    if (rc != GCX_OK)
+        return rc;
     // the sort's temporary storage is free again: reuse it for the scan
+    return order_items_from(N, dV, npred, ids, tmp, tmp_bytes, cnt, incl, items, ctl, s);
+}
+
+extern "C" int gcxk_jp_sort_ids(uint32_t N, const void *key, int key_bits, void *scratch,
+                                size_t scratch_bytes, uint32_t **ids, void *stream)
+{
+    *ids = nullptr;
+    if (N == 0)
+        return GCX_OK;
+    hipStream_t s = as_stream(stream);
+    return key_bits == 32 ? sort_order<uint32_t>(N, key, scratch, scratch_bytes, ids, s)
+                          : sort_order<uint64_t>(N, key, scratch, scratch_bytes, ids, s);
+}
+
+extern "C" int gcxk_jp_rank_keys(uint32_t N, const uint32_t *ids, uint32_t *rk, void *stream)
+{
+    if (N == 0)
+        return GCX_OK;
+    jp_rank_keys<<<div_up(N, kBlock), kBlock, 0, as_stream(stream)>>>(N, ids, rk);
+    GCXK_LAUNCH_CHECK();
+    return GCX_OK;
+}
+
+extern "C" int gcxk_jp_order_items_ranked(uint32_t N, const uint32_t *dV, const uint32_t *npred,
+                                          const uint32_t *ids, const uint32_t *rk,
+                                          uint32_t head_max, uint32_t head_density, int key_bits,
+                                          int mb, void *scratch, size_t scratch_bytes,
+                                          uint32_t *cnt, uint32_t *incl, uint32_t *wof,
+                                          uint64_t *wdesc, uint64_t *words, uint64_t words_cap,
+                                          gcxk_item *items, gcxk_ctl *ctl, void *stream)
+{
+    if (N == 0)
+        return GCX_OK;
+    hipStream_t s = as_stream(stream);
     const size_t ks = key_bits == 32 ? 4 : 8;
-    char *tmp = static_cast<char *>(scratch) + order_tmp_offset(N, ks);
-    size_t tmp_bytes = scratch_bytes - order_tmp_offset(N, ks), need = 0;
-    GCXK_CHECK(rocprim::inclusive_scan(nullptr, need, cnt, incl, N, rocprim::plus<uint32_t>(), s));
-    if (need > tmp_bytes)
+    if (order_tmp_offset(N, ks) > scratch_bytes)
         return GCX_ERR_CAPACITY;
-    GCXK_CHECK(rocprim::inclusive_scan(tmp, need, cnt, incl, N, rocprim::plus<uint32_t>(), s));
-    jp_expand<<<div_up(N, kBlock), kBlock, 0, s>>>(N, ids, incl, reinterpret_cast<uint2 *>(items),
-                                                   ctl);
+    char *tmp = static_cast<char *>(scratch) + order_tmp_offset(N, ks);
+    const size_t tmp_bytes = scratch_bytes - order_tmp_offset(N, ks);
+    if (head_max > kHeadMax)
+        head_max = kHeadMax;
+    if (head_max) {
+        jp_head_extent<<<1, kHeadBlock, 0, s>>>(N, ids, npred, head_max, head_density, ctl);
+        GCXK_LAUNCH_CHECK();
+    }
+    const int rc = order_words(N, npred, head_max ? rk : nullptr, mb, tmp, tmp_bytes, incl, wof,
+                               wdesc, words, words_cap, ctl, s);
+    if (rc != GCX_OK)
+        return rc;
+    return order_items_from(N, dV, npred, ids, tmp, tmp_bytes, cnt, incl, items, ctl, s);
+}
+
+extern "C" uint32_t gcxk_jp_head_max(void) { return kHeadMax; }
+
+extern "C" int gcxk_jp_head(const uint32_t *dV, const uint32_t *npred, const uint32_t *esplit,
+                            const uint32_t *ids, void *hdr, int *color, gcxk_ctl *ctl,
+                            uint64_t spin_ticks, void *stream)
+{
+    static int lds_set = 0; // the LDS limit attribute is per function, set once
+    if (!lds_set) {
+        GCXK_CHECK(hipFuncSetAttribute((const void *)jp_head_kernel,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kHeadLds));
+        lds_set = 1;
+    }
+    hipStream_t s = as_stream(stream);
+    uint2 *h = static_cast<uint2 *>(hdr);
+    jp_head_table<<<div_up(kHeadMax, kBlock), kBlock, 0, s>>>(dV, npred, ids, ctl, h);
+    GCXK_LAUNCH_CHECK();
+    jp_head_kernel<<<1, kHeadBlock, kHeadLds, s>>>(esplit, ids, h, color, ctl, spin_ticks);
     GCXK_LAUNCH_CHECK();
     return GCX_OK;
 }

@@ -117,7 +117,7 @@ int color_by_rounds(uint32_t N, uint64_t M, void *b, const layout &L, int key_bi
         rc = gcxk_jp_init(N, V, at<uint32_t>(b, L.E), at<void>(b, L.key), key_bits, mb,
                           at<uint32_t>(b, L.perm), at<uint32_t>(b, L.npred),
                           at<uint32_t>(b, L.esplit), at<uint64_t>(b, L.state),
-                          at<int>(b, L.color), nullptr, 0, nullptr, 0, nullptr, s, s);
+                          at<int>(b, L.color), nullptr, 0, nullptr, 0, nullptr, 0u, s, s);
     gcxk_item *q[2] = {at<gcxk_item>(b, L.items0), at<gcxk_item>(b, L.items1)};
     if (rc == GCX_OK)
         rc = gcxk_jp_roots(N, V, at<uint32_t>(b, L.npred), q[0], ctl, s);
@@ -214,7 +214,7 @@ int color_noalloc(uint32_t N, const uint32_t *V, const uint32_t *E, const double
         rc = gcxk_jp_init(N, at<uint32_t>(b, L.V), at<uint32_t>(b, L.E), at<void>(b, L.key),
                           key_bits, mb, at<uint32_t>(b, L.perm), at<uint32_t>(b, L.npred),
                           at<uint32_t>(b, L.esplit), at<uint64_t>(b, L.state),
-                          at<int>(b, L.color), nullptr, 0, nullptr, 0, nullptr, s, s);
+                          at<int>(b, L.color), nullptr, 0, nullptr, 0, nullptr, 0u, s, s);
     if (rc == GCX_OK)
         rc = gcxk_jp_order_items(N, at<uint32_t>(b, L.V), at<uint32_t>(b, L.npred),
                                  at<void>(b, L.key), key_bits, mb, at<void>(b, L.sort),

@@ -94,6 +94,8 @@ def _load():
         "gcx_recolor": (C.c_int, [vp, vp, C.c_int, vp]),
         "gcx_recolor_ex": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_uint64, vp]),
         "gcx_set_schedule": (C.c_int, [vp, C.c_int, C.c_uint32]),
+        "gcx_set_ordered_head": (C.c_int, [vp, C.c_uint32, C.c_uint32]),
+        "gcx_last_head": (C.c_int, [vp, u32p]),
         "gcx_validate": (C.c_int, [vp, vp, vp, i32p]),
         "gcx_mex_check": (C.c_int, [vp, vp, vp, C.POINTER(C.c_uint64)]),
         "gcx_color_digest": (C.c_int, [vp, vp, vp, C.POINTER(C.c_uint64)]),
@@ -573,6 +575,17 @@ class Workspace:
     def launches(self) -> int:
         r = C.c_uint32()
         _chk(_lib.gcx_last_launches(self.handle, C.byref(r)), "launches")
+        return int(r.value)
+
+    def set_ordered_head(self, max_positions=32768, min_density=64):
+        """Head of the ORDERED schedule (0 positions = off, the default);
+        colours do not change."""
+        _chk(_lib.gcx_set_ordered_head(self.handle, max_positions, min_density), "ordered head")
+
+    def head(self) -> int:
+        """Greedy-order positions the head coloured in the last color()."""
+        r = C.c_uint32()
+        _chk(_lib.gcx_last_head(self.handle, C.byref(r)), "last head")
         return int(r.value)
 
     def fell_back(self) -> bool:

@@ -195,6 +195,20 @@ int gcx_set_schedule(gcx_workspace *ws, int schedule, uint32_t param);
  * under every setting. */
 int gcx_set_ordered_launch(int slots);
 int gcx_get_ordered_launch(void);
+/* The ORDERED schedule's head (no reference counterpart): before the
+ * persistent launch, one workgroup colours the dense top of the greedy order
+ * with the colours in LDS -- on skewed graphs most of the DAG's depth sits
+ * there. It covers the leading 1024-position blocks whose vertices have at
+ * least min_density predecessors on average, at most max_positions (<=
+ * 32768; 0 turns the head off). Colours never change. Off by default: on
+ * R-MAT scale 22 (GNN3) it is still slower than the persistent launch alone
+ * (an LDS hop in the head costs 0.2-0.8 us against ~2 us for a device-scope
+ * hop, but folding each vertex's ~500 predecessor colours has not been
+ * taken off the chain). Suggested setting: 32768 positions, density 64. */
+int gcx_set_ordered_head(gcx_workspace *ws, uint32_t max_positions, uint32_t min_density);
+/* Greedy-order positions the head coloured in the last gcx_color (0: no
+ * head, or not the ORDERED schedule). */
+int gcx_last_head(const gcx_workspace *ws, uint32_t *positions);
 /* 1 when the last gcx_color fell back from ORDERED to ROUNDS. */
 int gcx_last_fallback(const gcx_workspace *ws, uint32_t *fell_back);
 /* Culberson iterated greedy: `iterations` more passes, each with p = colour
