@@ -78,9 +78,7 @@ int gcx__order_gnn_bf16(gcx_workspace *ws, const gcx_graph *g, const gcx_model *
     hipStream_t s = (hipStream_t)stream;
     const int L = m->layers;
     void *bufs[2] = {ws->x, ws->y};
-    /* layer 0 reads its neighbours' (degree, id) records from y, which
-     * layer 1 overwrites only after layer 0 has joined */
-    const void *in = ws->y;
+    const void *in = NULL; /* layer 0 reads only the graph */
     for (int l = 0; l < L; l++) {
         const int kind = l == 0 ? 0 : (l == L - 1 ? 2 : 1);
         const float *b = m->dparam + (size_t)l * LAYER_PARAMS + 512;
@@ -90,15 +88,12 @@ int gcx__order_gnn_bf16(gcx_workspace *ws, const gcx_graph *g, const gcx_model *
                                  s, &tok);
         if (rc != GCX_OK)
             return rc;
-        if (kind == 0 &&
-            (rc = gcxk_gnn_input_rec(g->N, g->V, g->rank, m->md, g->fidv, ws->y, s)) != GCX_OK)
-            return rc;
         void *hs = gcx__fork_hubs(ws, g, 1, s);
         if (!hs)
             return GCX_ERR_HIP;
-        rc = gcxk_gnn_bf16_layer(kind, split, g->N, g->V, g->E, g->Er, g->rank,
-                                 m->dWpack + 1024 * (size_t)l, b, m->md, in, out, (uint32_t *)ws->key,
-                                 g->hubs, g->nhub, g->perm, g->fidv, s, hs);
+        rc = gcxk_gnn_bf16_layer(kind, split, g->N, g->V, g->E, g->Ed, g->Er, g->rank,
+                                 m->dWpack + 1024 * (size_t)l, b, m->md, in, out,
+                                 (uint32_t *)ws->key, g->hubs, g->nhub, g->perm, g->gspan, s, hs);
         if ((rc = gcx__join_hubs(ws, hs, s, rc)) != GCX_OK)
             return rc;
         if ((rc = gcx__prof_end(tok, s)) != GCX_OK)

@@ -303,9 +303,10 @@ static void block_put(void *p, size_t bytes, void *stream);
 
 uint64_t gcx_graph_footprint(uint32_t N, uint64_t M)
 {
-    /* V and the hub chunk table (N + 1 each), E and Er, hubs, the degree
-     * order and its inverse, each part 256-byte aligned in one block */
-    return 8ull * ((uint64_t)N + 1) + 8ull * M + 12ull * N + 7ull * 256;
+    /* V and the hub chunk table (N + 1 each), E, Er and Ed (each with
+     * GCXK_ER_PAD entries of slack), hubs, the degree order and its inverse,
+     * each part 256-byte aligned in one block */
+    return 8ull * ((uint64_t)N + 1) + 12ull * (M + GCXK_ER_PAD) + 12ull * N + 8ull * 256;
 }
 
 /* Degree statistics, hub list (ascending ids) with its 1024-edge chunk table
@@ -332,8 +333,10 @@ static int graph_stats(gcx_graph *g, hipStream_t s)
     if (!g->rank)
         TRY(gcx__dalloc((void **)&g->rank, 4ull * (g->N ? g->N : 1), s));
     if (!g->Er)
-        TRY(gcx__dalloc((void **)&g->Er, 4ull * (g->M ? g->M : 1), s));
-    TRY(gcxk_rank_columns(g->N, g->M, g->perm, g->E, g->rank, g->Er, s));
+        TRY(gcx__dalloc((void **)&g->Er, 4ull * (g->M + GCXK_ER_PAD), s));
+    if (!g->Ed)
+        TRY(gcx__dalloc((void **)&g->Ed, 4ull * (g->M + GCXK_ER_PAD), s));
+    TRY(gcxk_rank_columns(g->N, g->M, g->perm, g->V, g->E, g->rank, g->Er, g->Ed, s));
     /* creation returns a graph usable on any stream */
     HIPC(hipStreamSynchronize(s));
     return GCX_OK;
@@ -349,11 +352,12 @@ static int graph_block(gcx_graph *g, int with_ve, void *stream)
         uint32_t **p;
         size_t bytes;
     } part[] = {
-        {&g->hubs, 4 * (N ? N : 1)}, {&g->hchunk, 4 * (N + 1)}, {&g->perm, 4 * (N ? N : 1)},
-        {&g->rank, 4 * (N ? N : 1)}, {&g->Er, 4 * (M ? M : 1)},  {&g->V, 4 * (N + 1)},
-        {&g->E, 4 * (M ? M : 1)},
+        {&g->hubs, 4 * (N ? N : 1)},     {&g->hchunk, 4 * (N + 1)},
+        {&g->perm, 4 * (N ? N : 1)},     {&g->rank, 4 * (N ? N : 1)},
+        {&g->Er, 4 * (M + GCXK_ER_PAD)}, {&g->Ed, 4 * (M + GCXK_ER_PAD)},
+        {&g->V, 4 * (N + 1)},            {&g->E, 4 * (M + GCXK_ER_PAD)},
     };
-    const size_t nparts = with_ve ? 7 : 5;
+    const size_t nparts = with_ve ? 8 : 6;
     size_t total = 0;
     for (size_t i = 0; i < nparts; i++)
         total += (part[i].bytes + 255) & ~(size_t)255;
@@ -377,11 +381,12 @@ int gcx_graph_free_on(gcx_graph *g, void *stream)
 {
     if (!g)
         return GCX_OK;
-    void *bufs[] = {g->V, g->E, g->hubs, g->hchunk, g->perm, g->rank, g->Er, g->fidv, g->goff};
-    /* the first 7 may live in the graph's block: V and E when blk_ve is
-     * set, the next 5 whenever there is a block */
+    void *bufs[] = {g->V,    g->E,  g->hubs, g->hchunk, g->perm,
+                    g->rank, g->Er, g->Ed,   g->gspan,  g->goff};
+    /* the first 8 may live in the graph's block: V and E when blk_ve is
+     * set, the next 6 whenever there is a block */
     for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); i++)
-        if (!g->blk || (i >= 2 ? i >= 7 : !g->blk_ve))
+        if (!g->blk || (i >= 2 ? i >= 8 : !g->blk_ve))
             gcx__dfree(bufs[i], stream);
     block_put(g->blk, g->blk_bytes, stream);
     free(g->hgoff);
@@ -599,7 +604,7 @@ int gcx_graph_pack(const gcx_graph *const *graphs, uint32_t count, void *stream,
     gcx_graph *g = NULL;
     int rc = gcx__graph_alloc((uint32_t)N, M, s, &g);
     if (rc == GCX_OK)
-        rc = gcx__dalloc((void **)&g->fidv, 4ull * N, s);
+        rc = gcx__dalloc((void **)&g->gspan, 8ull * N, s);
     if (rc == GCX_OK)
         rc = gcx__dalloc((void **)&g->goff, 4ull * ((size_t)count + 1), s);
     if (rc == GCX_OK && hipMemcpyAsync(g->goff, off, 4ull * ((size_t)count + 1),
@@ -610,7 +615,7 @@ int gcx_graph_pack(const gcx_graph *const *graphs, uint32_t count, void *stream,
     uint64_t eoff = 0;
     for (uint32_t i = 0; rc == GCX_OK && i < count; i++) {
         const gcx_graph *h = graphs[i];
-        rc = gcxk_pack_graph(h->N, h->M, h->V, h->E, off[i], (uint32_t)eoff, g->V, g->E, g->fidv,
+        rc = gcxk_pack_graph(h->N, h->M, h->V, h->E, off[i], (uint32_t)eoff, g->V, g->E, g->gspan,
                              s);
         eoff += h->M;
     }
@@ -1242,9 +1247,7 @@ static int order_gnn_fp32(gcx_workspace *ws, const gcx_graph *g, const gcx_model
 {
     const int L = m->layers;
     float *bufs[2] = {ws->x, ws->y};
-    /* layer 0 reads its neighbours' (degree, id) records from y, which
-     * layer 1 overwrites only after layer 0 has joined */
-    const float *in = ws->y;
+    const float *in = NULL; /* layer 0 reads only the graph */
     for (int l = 0; l < L; l++) {
         const int kind = l == 0 ? 0 : (l == L - 1 ? 2 : 1);
         const float *W = m->dparam + (size_t)l * LAYER_PARAMS;
@@ -1253,15 +1256,13 @@ static int order_gnn_fp32(gcx_workspace *ws, const gcx_graph *g, const gcx_model
         void *tok;
         TRY(gcx__prof_begin(kind == 0 ? GCX_ST_GNN_IN : kind == 1 ? GCX_ST_GNN_HID : GCX_ST_GNN_OUT,
                             s, &tok));
-        if (kind == 0)
-            TRY(gcxk_gnn_input_rec(g->N, g->V, g->rank, m->md, g->fidv, ws->y, s));
         void *hs = gcx__fork_hubs(ws, g, 1, s);
         if (!hs)
             return GCX_ERR_HIP;
         TRY(gcx__join_hubs(ws, hs, s,
-                           gcxk_gnn_fp32_layer(kind, g->N, g->V, g->E, g->Er, g->rank, W, b, m->md,
-                                               in, out, (uint32_t *)ws->key, g->hubs, g->nhub,
-                                               g->perm, g->fidv, s, hs)));
+                           gcxk_gnn_fp32_layer(kind, g->N, g->V, g->E, g->Ed, g->Er, g->rank, W,
+                                               b, m->md, in, out, (uint32_t *)ws->key, g->hubs,
+                                               g->nhub, g->perm, g->gspan, s, hs)));
         TRY(gcx__prof_end(tok, s));
         in = out;
     }

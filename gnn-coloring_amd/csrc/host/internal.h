@@ -22,12 +22,13 @@ struct gcx_graph {
     uint32_t nhchunk;
     uint32_t *perm;   /* device, N: ids by degree descending (hubs first) */
     uint32_t *rank;   /* device, N: rank[perm[i]] = i */
-    uint32_t *Er;     /* device, M: rank[E[j]] (GNN rows live at degree-order positions) */
-    float *fidv;      /* device, N, or NULL: GNN id feature per vertex (batches) */
+    uint32_t *Er;     /* device, M + pad: rank[E[j]] (GNN rows live at degree-order positions) */
+    uint32_t *Ed;     /* device, M + pad: deg(E[j]) (the GNN input layer's degree feature) */
+    uint32_t *gspan;  /* device, 2N, or NULL: (first vertex, count) of each vertex's packed graph */
     uint32_t *goff;   /* device, count + 1, or NULL: first vertex of each packed graph */
     uint32_t *hgoff;  /* host copy of goff (malloc'ed), or NULL */
     uint32_t count;   /* packed graphs (0 = a single graph) */
-    void *blk;        /* one device allocation holding hubs, hchunk, perm, rank, Er */
+    void *blk;        /* one device allocation holding hubs, hchunk, perm, rank, Er, Ed */
     int blk_ve;       /* ... and V and E */
     size_t blk_bytes; /* size of blk */
 };

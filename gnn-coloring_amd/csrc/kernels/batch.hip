@@ -5,8 +5,8 @@
 // its CSR is canonical when each graph's is, ids keep their order inside a
 // graph (JP's id tie-break is unchanged), and no edge joins two graphs: JP on
 // the pack gives every graph exactly the colours it gets alone. The GNN's
-// id feature must stay graph-local, so the pack carries fid[v] =
-// (float)((double)(v - off_g) / n_g).
+// id feature must stay graph-local, (float)((double)(v - off_g) / n_g), so
+// the pack carries each vertex's graph span gspan[v] = (off_g, n_g).
 #include "common.hip.h"
 
 namespace {
@@ -16,12 +16,12 @@ constexpr int kBlock = 256;
 __global__ __launch_bounds__(kBlock) void pack_rows(uint32_t n, const uint32_t *__restrict__ V,
                                                     uint32_t off, uint32_t eoff,
                                                     uint32_t *__restrict__ Vout,
-                                                    float *__restrict__ fid)
+                                                    uint2 *__restrict__ gspan)
 {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i < n) {
         Vout[off + i] = eoff + V[i];
-        fid[off + i] = (float)((double)i / (double)n);
+        gspan[off + i] = make_uint2(off, n);
     }
 }
 
@@ -91,11 +91,12 @@ __global__ __launch_bounds__(kBlock) void packed_stats(const uint32_t *__restric
 
 extern "C" int gcxk_pack_graph(uint32_t n, uint64_t m, const uint32_t *dV, const uint32_t *dE,
                                uint32_t off, uint32_t eoff, uint32_t *Vout, uint32_t *Eout,
-                               float *fid, void *stream)
+                               uint32_t *gspan, void *stream)
 {
     hipStream_t s = as_stream(stream);
     if (n)
-        pack_rows<<<div_up(n, kBlock), kBlock, 0, s>>>(n, dV, off, eoff, Vout, fid);
+        pack_rows<<<div_up(n, kBlock), kBlock, 0, s>>>(n, dV, off, eoff, Vout,
+                                                       reinterpret_cast<uint2 *>(gspan));
     if (m)
         pack_cols<<<div_up(m, kBlock), kBlock, 0, s>>>(m, dE, off, Eout + eoff);
     GCXK_LAUNCH_CHECK();

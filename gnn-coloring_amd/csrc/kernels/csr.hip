@@ -448,7 +448,8 @@ extern "C" int gcxk_csr_from_pairs(uint32_t N, uint64_t L, const uint32_t *dI, c
         total = last[0] + last[1];
     }
     uint32_t *E = nullptr;
-    GCXK_CHECK(hipMallocAsync((void **)&E, (total ? total : 1) * sizeof(uint32_t), s));
+    // GCXK_ER_PAD entries of slack: the GNN input layer reads whole 16-byte words
+    GCXK_CHECK(hipMallocAsync((void **)&E, (total + GCXK_ER_PAD) * sizeof(uint32_t), s));
     hipError_t e = hipSuccess;
     if (total) {
         scatter_rows<<<div_up(n, kBlock), kBlock, 0, s>>>(n, k_out, flag, pos, b, N, total, dV, E);
@@ -467,33 +468,52 @@ extern "C" int gcxk_csr_from_pairs(uint32_t N, uint64_t L, const uint32_t *dI, c
     return GCX_OK;
 }
 
-// rank[perm[i]] = i: position of each vertex in the degree order.
-__global__ __launch_bounds__(kBlock) void rank_of(uint32_t N, const uint32_t *__restrict__ perm,
-                                                  uint32_t *__restrict__ rank)
+// rank[v] = i for v = perm[i] (position of each vertex in the degree order),
+// and rd[v] = (rank[v], deg(v)): the two per-vertex values the column pass
+// gathers, in one 8-byte word.
+__global__ __launch_bounds__(kBlock) void rank_deg(uint32_t N, const uint32_t *__restrict__ perm,
+                                                   const uint32_t *__restrict__ V,
+                                                   uint32_t *__restrict__ rank,
+                                                   uint2 *__restrict__ rd)
 {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < N)
-        rank[perm[i]] = i;
+    if (i < N) {
+        const uint32_t v = perm[i];
+        rank[v] = i;
+        rd[v] = make_uint2(i, V[v + 1] - V[v]);
+    }
 }
 
-// Er[j] = rank[E[j]]: the CSR's columns as degree-order positions.
+// Er[j] = rank[E[j]] (the CSR's columns as degree-order positions) and
+// Ed[j] = deg(E[j]), then GCXK_ER_PAD zeros in both.
 __global__ __launch_bounds__(kBlock) void rank_cols(uint64_t M, const uint32_t *__restrict__ E,
-                                                    const uint32_t *__restrict__ rank,
-                                                    uint32_t *__restrict__ Er)
+                                                    const uint2 *__restrict__ rd,
+                                                    uint32_t *__restrict__ Er,
+                                                    uint32_t *__restrict__ Ed)
 {
     const uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (j < M)
-        Er[j] = rank[E[j]];
+    if (j < M) {
+        const uint2 x = rd[E[j]];
+        Er[j] = x.x;
+        Ed[j] = x.y;
+    } else if (j < M + GCXK_ER_PAD) {
+        Er[j] = 0u;
+        Ed[j] = 0u;
+    }
 }
 
-extern "C" int gcxk_rank_columns(uint32_t N, uint64_t M, const uint32_t *perm, const uint32_t *dE,
-                                 uint32_t *rank, uint32_t *Er, void *stream)
+extern "C" int gcxk_rank_columns(uint32_t N, uint64_t M, const uint32_t *perm,
+                                 const uint32_t *dV, const uint32_t *dE, uint32_t *rank,
+                                 uint32_t *Er, uint32_t *Ed, void *stream)
 {
     hipStream_t s = as_stream(stream);
+    DevBuf rd;
+    GCXK_CHECK(rd.alloc(8ull * N, s));
     if (N)
-        rank_of<<<div_up(N, kBlock), kBlock, 0, s>>>(N, perm, rank);
-    if (M)
-        rank_cols<<<(unsigned)div_up(M, (uint64_t)kBlock), kBlock, 0, s>>>(M, dE, rank, Er);
+        rank_deg<<<div_up(N, kBlock), kBlock, 0, s>>>(N, perm, dV, rank,
+                                                      static_cast<uint2 *>(rd.p));
+    rank_cols<<<(unsigned)div_up(M + GCXK_ER_PAD, (uint64_t)kBlock), kBlock, 0, s>>>(
+        M, dE, static_cast<const uint2 *>(rd.p), Er, Ed);
     GCXK_LAUNCH_CHECK();
     return GCX_OK;
 }

@@ -52,11 +52,17 @@ int gcxk_graph_stats(uint32_t N, const uint32_t *dV, uint32_t hub_deg, uint32_t 
                      void *stream);
 /* Degree order: perm = vertex ids by degree descending, id ascending (so the
  * nhub hubs come first). Stream-ordered temporaries; no synchronisation. */
-/* rank[perm[i]] = i and Er[j] = rank[E[j]]: the columns as positions in the
+/* rank[perm[i]] = i, Er[j] = rank[E[j]] (the columns as positions in the
  * degree order, so that rows stored in degree order put the hubs' rows
- * together (they take most neighbour reads on skewed graphs). */
-int gcxk_rank_columns(uint32_t N, uint64_t M, const uint32_t *perm, const uint32_t *dE,
-                      uint32_t *rank, uint32_t *Er, void *stream);
+ * together: they take most neighbour reads on skewed graphs) and
+ * Ed[j] = deg(E[j]) (the GNN input layer's degree feature, read in CSR order
+ * instead of gathered). Er and Ed hold M + GCXK_ER_PAD entries, the padding
+ * written as 0; E must also be allocated with GCXK_ER_PAD entries of slack
+ * (the input layer reads whole 16-byte words). */
+#define GCXK_ER_PAD 8u
+int gcxk_rank_columns(uint32_t N, uint64_t M, const uint32_t *perm, const uint32_t *dV,
+                      const uint32_t *dE, uint32_t *rank, uint32_t *Er, uint32_t *Ed,
+                      void *stream);
 int gcxk_degree_order(uint32_t N, const uint32_t *dV, uint32_t maxdeg, uint32_t *perm,
                       void *stream);
 /* CSR validity (sorted, simple, symmetric): *valid = 1 or 0. */
@@ -83,9 +89,11 @@ int gcxk_gen_rgg(uint32_t n, uint32_t cells, uint64_t r2, uint64_t seed, uint32_
 /* ---- packed graphs (batch.hip) ---- */
 /* Copy one graph (n vertices, m entries) into a pack at vertex offset off and
  * entry offset eoff (Vout rows, Eout columns shifted by off) and write its
- * graph-local id features fid[off + i] = (float)((double)i / n). */
+ * vertices' graph span gspan[2(off + i)] = off, gspan[2(off + i) + 1] = n
+ * (the GNN's id feature is graph-local: (float)((double)(v - off) / n)). */
 int gcxk_pack_graph(uint32_t n, uint64_t m, const uint32_t *dV, const uint32_t *dE, uint32_t off,
-                    uint32_t eoff, uint32_t *Vout, uint32_t *Eout, float *fid, void *stream);
+                    uint32_t eoff, uint32_t *Vout, uint32_t *Eout, uint32_t *gspan,
+                    void *stream);
 /* Vout[N] = M. */
 int gcxk_pack_close(uint32_t N, uint32_t M, uint32_t *Vout, void *stream);
 /* Per packed graph (goff: count + 1 first vertices): largest colour into
@@ -143,38 +151,35 @@ uint32_t gcxk_sl_hub_slots(uint32_t N);
 int gcxk_sl_hub_bits(uint32_t N, const uint32_t *perm, uint32_t *hub_bit, void *stream);
 
 /* ---- GraphSAGE-16 (gnn.hip) ---- */
-/* Input-layer neighbour features by degree-order position: rec[rank[v]] =
- * (float)deg(v) / md and the id feature (fidv[v], or (float)(v / N) when
- * fidv is NULL), as 2 floats; rec holds 2 N floats. */
-int gcxk_gnn_input_rec(uint32_t N, const uint32_t *dV, const uint32_t *rank, int md,
-                       const float *fidv, void *rec, void *stream);
-/* FP32, bit-exact with the reference op order. kind 0 = input layer (x =
- * gcxk_gnn_input_rec's records -> out), 1 = hidden (x -> out), 2 = output
- * (x -> key, float order keys).
+/* FP32, bit-exact with the reference op order. kind 0 = input layer (reads
+ * the graph only: neighbour ids E and their degrees Ed -> out), 1 = hidden
+ * (x -> out), 2 = output (x -> key, float order keys).
  * Rows are 16 floats, stored by degree-order position (row of vertex v =
  * rank[v]; neighbours read through Er, gcxk_rank_columns). Hub rows run on
  * hub_stream, the rest on stream in the degree order perm (non-hubs are
  * perm[nhub .. N)); the caller joins the two streams before the next layer.
- * Priorities (key) stay indexed by vertex id. fidv (input layer only):
- * per-vertex id feature, NULL = (float)(v / N). */
+ * Priorities (key) stay indexed by vertex id. gspan (input layer only): 2N
+ * values, (first vertex, vertex count) of each vertex's packed graph, or
+ * NULL for a single graph (id feature (float)((double)v / N)). E and Ed must
+ * carry GCXK_ER_PAD readable entries past M. */
 int gcxk_gnn_fp32_layer(int kind, uint32_t N, const uint32_t *dV, const uint32_t *dE,
-                        const uint32_t *Er, const uint32_t *rank, const float *W, const float *b,
-                        int md, const float *x, float *out, uint32_t *key, const uint32_t *hubs,
-                        uint32_t nhub, const uint32_t *perm, const float *fidv, void *stream,
-                        void *hub_stream);
+                        const uint32_t *Ed, const uint32_t *Er, const uint32_t *rank,
+                        const float *W, const float *b, int md, const float *x, float *out,
+                        uint32_t *key, const uint32_t *hubs, uint32_t nhub, const uint32_t *perm,
+                        const uint32_t *gspan, void *stream, void *hub_stream);
 /* MFMA layer transform with BF16 operands and FP32 accumulation.
  * split_mode 0: bf16 rows (16 x bf16 = 32 B), operands rounded to bf16.
  * split_mode 1: FP32 rows, operands and weights split into bf16 hi + lo,
  * three MFMAs per product (near-FP32 results).
- * Rows by degree-order position as in the FP32 layer (input layer: x =
- * gcxk_gnn_input_rec's records). Wpack: the layer's two
+ * Graph arrays and rows as in the FP32 layer. Wpack: the layer's two
  * MFMA B operands in lane order, hi parts (512 bf16) then lo parts (512);
  * b: FP32 biases (16). */
 int gcxk_gnn_bf16_layer(int kind, int split_mode, uint32_t N, const uint32_t *dV,
-                        const uint32_t *dE, const uint32_t *Er, const uint32_t *rank,
-                        const uint16_t *Wpack, const float *b, int md, const void *x, void *out,
-                        uint32_t *key, const uint32_t *hubs, uint32_t nhub, const uint32_t *perm,
-                        const float *fidv, void *stream, void *hub_stream);
+                        const uint32_t *dE, const uint32_t *Ed, const uint32_t *Er,
+                        const uint32_t *rank, const uint16_t *Wpack, const float *b, int md,
+                        const void *x, void *out, uint32_t *key, const uint32_t *hubs,
+                        uint32_t nhub, const uint32_t *perm, const uint32_t *gspan, void *stream,
+                        void *hub_stream);
 
 /* ---- Jones-Plassmann (jp.hip) ---- */
 /* Work item: (vertex, chunk of 64 successor edges). Stored as two uint32. */

@@ -26,6 +26,7 @@
 // are the explicit fmaf calls of the dense step.
 #include "common.hip.h"
 #include "hub_sum.hip.h"
+#include "input_sum.hip.h"
 
 namespace {
 
@@ -39,18 +40,6 @@ constexpr int kHubBlock = 256;
 enum LayerKind { kInput = 0, kHidden = 1, kOutput = 2 };
 
 __device__ __forceinline__ float relu(float c) { return c > 0.0f ? c : 0.0f; }
-
-__device__ __forceinline__ float fid(uint32_t v, uint32_t N)
-{
-    return (float)((double)v / (double)N);
-}
-
-// Id feature of v: v / N, or the caller's per-vertex value (a batch of graphs
-// packed into one uses each vertex's id and size within its own graph).
-__device__ __forceinline__ float fid_of(uint32_t v, uint32_t N, const float *__restrict__ fidv)
-{
-    return fidv ? fidv[v] : fid(v, N);
-}
 
 // Dense step for lane j of a 16-lane group; A[i] for i < 16 is a_lo of lane
 // i, for i >= 16 a_hi of lane i - 16.
@@ -115,17 +104,17 @@ constexpr int kQuads = kBlock / 4; // vertices per workgroup step
 constexpr int kUnroll = 8;         // neighbour rows in flight per lane
 
 template <int kind>
-__global__ __launch_bounds__(kBlock) void sage_quads(uint32_t N, const uint32_t *__restrict__ V,
-                                                     const uint32_t *__restrict__ E,
+__global__ __launch_bounds__(kBlock) void sage_quads(const uint32_t *__restrict__ V,
                                                      const uint32_t *__restrict__ Er,
                                                      const float *__restrict__ W,
-                                                     const float *__restrict__ b, float fmd,
+                                                     const float *__restrict__ b,
                                                      const float *__restrict__ x,
                                                      float *__restrict__ out,
                                                      uint32_t *__restrict__ key,
                                                      const uint32_t *__restrict__ perm, uint32_t p0,
-                                                     uint32_t p1, const float *__restrict__ fidv)
+                                                     uint32_t p1)
 {
+    static_assert(kind != kInput, "layer 0 runs in sage_lanes_input");
     __shared__ float4 Ws[2 * F][4];
     __shared__ float4 bs[4];
     for (int t = threadIdx.x; t < 2 * F * F; t += kBlock)
@@ -135,7 +124,6 @@ __global__ __launch_bounds__(kBlock) void sage_quads(uint32_t N, const uint32_t 
     __syncthreads();
     const int q = threadIdx.x & 3;
     const float4 *X = reinterpret_cast<const float4 *>(x);
-    const float2 *rec = reinterpret_cast<const float2 *>(x); // input layer
     // whole quads leave the loop together, so the width-4 shuffles stay valid
     for (uint32_t g = p0 + blockIdx.x * kQuads + threadIdx.x / 4; g < p1;
          g += gridDim.x * kQuads) {
@@ -144,103 +132,58 @@ __global__ __launch_bounds__(kBlock) void sage_quads(uint32_t N, const uint32_t 
         const float fdeg_f = (float)(end - beg);
         const float4 bb = bs[q];
         float c[4] = {bb.x, bb.y, bb.z, bb.w};
-        if (kind == kInput) {
-            // lane q computes the features of neighbours i0 + 4t + q, then
-            // every lane adds all 16 in CSR order
-            float sd = 0.0f, si = 0.0f;
-            for (uint32_t i0 = beg; i0 < end; i0 += 16) {
-                float fd[4], fi[4];
+        float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
+        uint32_t i = beg;
+        for (; i + kUnroll <= end; i += kUnroll) {
+            uint32_t v[kUnroll];
 #pragma unroll
-                for (int t = 0; t < 4; t++) {
-                    const uint32_t i = i0 + 4 * t + q;
-                    fd[t] = 0.0f;
-                    fi[t] = 0.0f;
-                    if (i < end) {
-                        const float2 r = rec[Er[i]];
-                        fd[t] = r.x;
-                        fi[t] = r.y;
-                    }
-                }
-                const uint32_t nb = end - i0 < 16u ? end - i0 : 16u;
+            for (int t = 0; t < kUnroll; t++)
+                v[t] = Er[i + t];
+            float4 r[kUnroll];
 #pragma unroll
-                for (int t = 0; t < 4; t++)
+            for (int t = 0; t < kUnroll; t++)
+                r[t] = X[(size_t)v[t] * 4 + q];
 #pragma unroll
-                    for (int m = 0; m < 4; m++) {
-                        const float a = __shfl(fd[t], m, 4), e = __shfl(fi[t], m, 4);
-                        if ((uint32_t)(4 * t + m) < nb) {
-                            sd = sd + a;
-                            si = si + e;
-                        }
-                    }
+            for (int t = 0; t < kUnroll; t++) {
+                s0 = s0 + r[t].x;
+                s1 = s1 + r[t].y;
+                s2 = s2 + r[t].z;
+                s3 = s3 + r[t].w;
             }
-            if (fdeg_f > 0.0f) {
-                sd = sd / fdeg_f;
-                si = si / fdeg_f;
-            }
-            const float selfd = fdeg_f / fmd, selfi = fid_of(u, N, fidv);
+        }
+        for (; i < end; i++) {
+            const float4 r = X[(size_t)Er[i] * 4 + q];
+            s0 = s0 + r.x;
+            s1 = s1 + r.y;
+            s2 = s2 + r.z;
+            s3 = s3 + r.w;
+        }
+        if (fdeg_f > 0.0f) {
+            s0 = s0 / fdeg_f;
+            s1 = s1 / fdeg_f;
+            s2 = s2 / fdeg_f;
+            s3 = s3 / fdeg_f;
+        }
+        const float4 self = X[(size_t)g * 4 + q];
+        const float ag[4] = {s0, s1, s2, s3};
+        const float sf[4] = {self.x, self.y, self.z, self.w};
 #pragma unroll
-            for (int i = 0; i < 2 * F; i++) {
-                const float a = i < 8 ? sd : i < 16 ? si : i < 24 ? selfd : selfi;
-                const float4 w = Ws[i][q];
-                c[0] = fmaf(w.x, a, c[0]);
-                c[1] = fmaf(w.y, a, c[1]);
-                c[2] = fmaf(w.z, a, c[2]);
-                c[3] = fmaf(w.w, a, c[3]);
-            }
-        } else {
-            float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
-            uint32_t i = beg;
-            for (; i + kUnroll <= end; i += kUnroll) {
-                uint32_t v[kUnroll];
+        for (int i = 0; i < F; i++) {
+            const float a = __shfl(ag[i % 4], i / 4, 4);
+            const float4 w = Ws[i][q];
+            c[0] = fmaf(w.x, a, c[0]);
+            c[1] = fmaf(w.y, a, c[1]);
+            c[2] = fmaf(w.z, a, c[2]);
+            c[3] = fmaf(w.w, a, c[3]);
+        }
 #pragma unroll
-                for (int t = 0; t < kUnroll; t++)
-                    v[t] = Er[i + t];
-                float4 r[kUnroll];
-#pragma unroll
-                for (int t = 0; t < kUnroll; t++)
-                    r[t] = X[(size_t)v[t] * 4 + q];
-#pragma unroll
-                for (int t = 0; t < kUnroll; t++) {
-                    s0 = s0 + r[t].x;
-                    s1 = s1 + r[t].y;
-                    s2 = s2 + r[t].z;
-                    s3 = s3 + r[t].w;
-                }
-            }
-            for (; i < end; i++) {
-                const float4 r = X[(size_t)Er[i] * 4 + q];
-                s0 = s0 + r.x;
-                s1 = s1 + r.y;
-                s2 = s2 + r.z;
-                s3 = s3 + r.w;
-            }
-            if (fdeg_f > 0.0f) {
-                s0 = s0 / fdeg_f;
-                s1 = s1 / fdeg_f;
-                s2 = s2 / fdeg_f;
-                s3 = s3 / fdeg_f;
-            }
-            const float4 self = X[(size_t)g * 4 + q];
-            const float ag[4] = {s0, s1, s2, s3};
-            const float sf[4] = {self.x, self.y, self.z, self.w};
-#pragma unroll
-            for (int i = 0; i < F; i++) {
-                const float a = __shfl(ag[i % 4], i / 4, 4);
-                const float4 w = Ws[i][q];
-                c[0] = fmaf(w.x, a, c[0]);
-                c[1] = fmaf(w.y, a, c[1]);
-                c[2] = fmaf(w.z, a, c[2]);
-                c[3] = fmaf(w.w, a, c[3]);
-            }
-#pragma unroll
-            for (int i = 0; i < F; i++) {
-                const float a = __shfl(sf[i % 4], i / 4, 4);
-                const float4 w = Ws[F + i][q];
-                c[0] = fmaf(w.x, a, c[0]);
-                c[1] = fmaf(w.y, a, c[1]);
-                c[2] = fmaf(w.z, a, c[2]);
-                c[3] = fmaf(w.w, a, c[3]);
-            }
+        for (int i = 0; i < F; i++) {
+            const float a = __shfl(sf[i % 4], i / 4, 4);
+            const float4 w = Ws[F + i][q];
+            c[0] = fmaf(w.x, a, c[0]);
+            c[1] = fmaf(w.y, a, c[1]);
+            c[2] = fmaf(w.z, a, c[2]);
+            c[3] = fmaf(w.w, a, c[3]);
         }
         float h[4];
 #pragma unroll
@@ -262,37 +205,107 @@ __global__ __launch_bounds__(kBlock) void sage_quads(uint32_t N, const uint32_t 
     }
 }
 
+// ------------------------------------------------------- input lane kernel
+
+constexpr int kLaneSteps = 1; // blocks of 8 neighbours per load step (2 need more registers than pay)
+
+// Layer 0 for the vertices below the hub degree (perm[p0 .. p1), heaviest
+// first): one lane per vertex, so a wave holds 64 vertices of near-equal
+// degree and each lane keeps its own two ordered sums (insum::lane_sums,
+// streaming E and Ed). The dense step runs in the lane:
+// c_j = fmaf(W[i][j], A[i], c_j) for i = 0..31, with the weights read from
+// LDS as broadcast float4s. One workgroup per 256 positions: the dispatcher
+// hands out the light workgroups as the heavy ones finish.
+template <bool kRecip>
+__global__ __launch_bounds__(kBlock) void sage_lanes_input(
+    uint32_t N, const uint32_t *__restrict__ V, const uint32_t *__restrict__ E,
+    const uint32_t *__restrict__ Ed, const float *__restrict__ W, const float *__restrict__ b,
+    float fmd, float *__restrict__ out, const uint32_t *__restrict__ perm, uint32_t p0,
+    uint32_t p1, const uint2 *__restrict__ gspan)
+{
+    __shared__ float4 Ws[2 * F][4];
+    __shared__ float4 bs[4];
+    for (int t = threadIdx.x; t < 2 * F * F; t += kBlock)
+        reinterpret_cast<float *>(Ws)[t] = W[t];
+    if (threadIdx.x < F)
+        reinterpret_cast<float *>(bs)[threadIdx.x] = b[threadIdx.x];
+    __syncthreads();
+    const uint32_t g = p0 + blockIdx.x * kBlock + threadIdx.x;
+    if (g >= p1)
+        return;
+    const uint32_t u = perm[g];
+    const uint32_t beg = V[u], end = V[u + 1];
+    const uint2 sp = insum::span_of(u, N, gspan);
+    const float2 s = insum::lane_sums<kLaneSteps, kRecip>(beg, end, E, Ed, fmd, sp);
+    const float fdeg_f = (float)(end - beg);
+    float sd = s.x, si = s.y;
+    if (fdeg_f > 0.0f) {
+        sd = sd / fdeg_f;
+        si = si / fdeg_f;
+    }
+    const float selfd = insum::deg_feature(end - beg, fmd), selfi = insum::id_feature(u, sp);
+    float c[F];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const float4 bb = bs[q];
+        c[4 * q] = bb.x;
+        c[4 * q + 1] = bb.y;
+        c[4 * q + 2] = bb.z;
+        c[4 * q + 3] = bb.w;
+    }
+#pragma unroll
+    for (int i = 0; i < 2 * F; i++) {
+        const float a = i < 8 ? sd : i < 16 ? si : i < 24 ? selfd : selfi;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const float4 w = Ws[i][q];
+            c[4 * q] = fmaf(w.x, a, c[4 * q]);
+            c[4 * q + 1] = fmaf(w.y, a, c[4 * q + 1]);
+            c[4 * q + 2] = fmaf(w.z, a, c[4 * q + 2]);
+            c[4 * q + 3] = fmaf(w.w, a, c[4 * q + 3]);
+        }
+    }
+    float4 *row = reinterpret_cast<float4 *>(out + (size_t)g * F);
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+        row[q] = make_float4(relu(c[4 * q]), relu(c[4 * q + 1]), relu(c[4 * q + 2]),
+                             relu(c[4 * q + 3]));
+}
+
 // -------------------------------------------------------------- hub kernel
 
+// Neighbours per loader thread per chunk of the input layer's hub sums:
+// 192 loader threads x 8 = 1536 per chunk (measured on R-MAT scale 22 against
+// 4 and 16).
+constexpr int kHubInRows = 8;
+
 template <int kind>
-__global__ __launch_bounds__(kHubBlock) void sage_hubs(uint32_t N, const uint32_t *__restrict__ V,
-                                                       const uint32_t *__restrict__ E,
-                                                       const uint32_t *__restrict__ Er,
-                                                       const uint32_t *__restrict__ rank,
-                                                       const float *__restrict__ W,
+__global__ __launch_bounds__(kHubBlock) void sage_hubs(GnnCsr gr, const float *__restrict__ W,
                                                        const float *__restrict__ b, float fmd,
                                                        const float *__restrict__ x,
                                                        float *__restrict__ out,
-                                                       uint32_t *__restrict__ key,
-                                                       const uint32_t *__restrict__ hubs,
-                                                       const float *__restrict__ fidv)
+                                                       uint32_t *__restrict__ key)
 {
-    __shared__ __attribute__((aligned(16))) hubsum::Stage<kHubBlock> st;
-    const uint32_t u = hubs[blockIdx.x];
-    const uint32_t beg = V[u], end = V[u + 1];
+    // the input layer stages 2 features per neighbour, the others 16
+    __shared__ __attribute__((aligned(16))) union {
+        hubsum::Stage<kHubBlock> rows;
+        insum::PairStage<kHubBlock, kHubInRows> feats;
+    } st;
+    const uint32_t u = gr.perm[blockIdx.x]; // hubs by degree, largest first
+    const uint32_t beg = gr.V[u], end = gr.V[u + 1];
     const uint32_t deg = end - beg;
     const int tid = threadIdx.x;
     const int j = tid & (F - 1);
     // Layer 0: lane 0 sums fdeg, lane 1 sums fid. Other layers: lane j sums
     // feature j.
     float acc;
+    uint2 sp = make_uint2(0u, gr.N);
     if (kind == kInput) {
-        acc = hubsum::ordered_sum<kHubBlock, true>(st, beg, end, Er, [&](uint32_t v, float4 *row) {
-            const float2 r = reinterpret_cast<const float2 *>(x)[v];
-            row[0] = make_float4(r.x, r.y, 0.0f, 0.0f);
-        });
+        sp = insum::span_of(u, gr.N, gr.gspan);
+        acc = insum::pair_sums(st.feats, beg, end, gr.E, gr.Ed, fmd, sp);
     } else {
-        acc = hubsum::ordered_sum<kHubBlock, false>(st, beg, end, Er, [&](uint32_t v, float4 *row) {
+        acc = hubsum::ordered_sum<kHubBlock, false>(st.rows, beg, end, gr.Er,
+                                                    [&](uint32_t v, float4 *row) {
             const float4 *src = reinterpret_cast<const float4 *>(x + (size_t)v * F);
 #pragma unroll
             for (int q = 0; q < 4; q++)
@@ -306,6 +319,7 @@ __global__ __launch_bounds__(kHubBlock) void sage_hubs(uint32_t N, const uint32_
     load_weights(W, j, w);
     const float bias = b[j];
     const float fdeg_f = (float)deg;
+    const uint32_t row = gr.rank[u];
     float h;
     if (kind == kInput) {
         float sd = __shfl(acc, 0, F), si = __shfl(acc, 1, F);
@@ -313,12 +327,12 @@ __global__ __launch_bounds__(kHubBlock) void sage_hubs(uint32_t N, const uint32_
             sd = sd / fdeg_f;
             si = si / fdeg_f;
         }
-        h = dense_input(w, bias, sd, si, fdeg_f / fmd, fid_of(u, N, fidv));
+        h = dense_input(w, bias, sd, si, insum::deg_feature(deg, fmd), insum::id_feature(u, sp));
     } else {
         float a = acc;
         if (fdeg_f > 0.0f)
             a = a / fdeg_f;
-        h = dense_group(w, bias, a, x[(size_t)rank[u] * F + j]);
+        h = dense_group(w, bias, a, x[(size_t)row * F + j]);
     }
     if (tid < F) {
         if (kind == kOutput) {
@@ -326,7 +340,7 @@ __global__ __launch_bounds__(kHubBlock) void sage_hubs(uint32_t N, const uint32_
             if (tid == 0)
                 key[u] = float_order_key(p);
         } else {
-            out[(size_t)rank[u] * F + j] = h;
+            out[(size_t)row * F + j] = h;
         }
     }
 }
@@ -336,72 +350,53 @@ __global__ __launch_bounds__(kHubBlock) void sage_hubs(uint32_t N, const uint32_
 constexpr uint32_t kQuadGrid = 2048;
 
 template <int kind>
-int launch_layer(uint32_t N, const uint32_t *V, const uint32_t *E, const uint32_t *Er,
-                 const uint32_t *rank, const float *W, const float *b, float fmd, const float *x,
-                 float *out, uint32_t *key, const uint32_t *hubs, uint32_t nhub,
-                 const uint32_t *perm, const float *fidv, void *stream, void *hub_stream)
+int launch_layer(const GnnCsr &gr, const float *W, const float *b, float fmd, bool recip_ok,
+                 const float *x, float *out, uint32_t *key, void *stream, void *hub_stream)
 {
-    if (N == 0)
+    if (gr.N == 0)
         return GCX_OK;
     // Hubs go first on their own stream so that they overlap the quads.
-    if (nhub)
-        sage_hubs<kind><<<nhub, kHubBlock, 0, as_stream(hub_stream)>>>(N, V, E, Er, rank, W, b, fmd,
-                                                                       x, out, key, hubs, fidv);
-    const uint32_t rest = N - nhub;
-    if (rest) {
+    if (gr.nhub)
+        sage_hubs<kind><<<gr.nhub, kHubBlock, 0, as_stream(hub_stream)>>>(gr, W, b, fmd, x, out,
+                                                                          key);
+    const uint32_t rest = gr.N - gr.nhub;
+    if constexpr (kind == kInput) {
+        const uint32_t grid = div_up(rest, (uint32_t)kBlock);
+        if (rest && recip_ok)
+            sage_lanes_input<true><<<grid, kBlock, 0, as_stream(stream)>>>(
+                gr.N, gr.V, gr.E, gr.Ed, W, b, fmd, out, gr.perm, gr.nhub, gr.N, gr.gspan);
+        else if (rest)
+            sage_lanes_input<false><<<grid, kBlock, 0, as_stream(stream)>>>(
+                gr.N, gr.V, gr.E, gr.Ed, W, b, fmd, out, gr.perm, gr.nhub, gr.N, gr.gspan);
+    } else if (rest) {
         uint32_t grid = div_up(rest, kQuads);
         grid = grid < kQuadGrid ? grid : kQuadGrid;
-        sage_quads<kind><<<grid, kBlock, 0, as_stream(stream)>>>(N, V, E, Er, W, b, fmd, x, out,
-                                                                 key, perm, nhub, N, fidv);
+        sage_quads<kind><<<grid, kBlock, 0, as_stream(stream)>>>(gr.V, gr.Er, W, b, x, out, key,
+                                                                 gr.perm, gr.nhub, gr.N);
     }
     GCXK_LAUNCH_CHECK();
     return GCX_OK;
 }
 
-// Input-layer neighbour features in degree order: rec[rank[v]] = (fdeg(v),
-// fid(v)), computed exactly as the layer would (one float division, one
-// double division), so the layer gathers one 8-byte record per neighbour
-// through Er instead of V[v], V[v+1] and the id.
-__global__ __launch_bounds__(kBlock) void input_rec(uint32_t N, const uint32_t *__restrict__ V,
-                                                    const uint32_t *__restrict__ rank, float fmd,
-                                                    const float *__restrict__ fidv,
-                                                    float2 *__restrict__ rec)
-{
-    const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
-    if (v < N)
-        rec[rank[v]] = make_float2((float)(V[v + 1] - V[v]) / fmd, fid_of(v, N, fidv));
-}
-
 } // namespace
 
-extern "C" int gcxk_gnn_input_rec(uint32_t N, const uint32_t *dV, const uint32_t *rank, int md,
-                                  const float *fidv, void *rec, void *stream)
-{
-    if (N == 0)
-        return GCX_OK;
-    input_rec<<<div_up(N, kBlock), kBlock, 0, as_stream(stream)>>>(
-        N, dV, rank, (float)md, fidv, static_cast<float2 *>(rec));
-    GCXK_LAUNCH_CHECK();
-    return GCX_OK;
-}
-
 extern "C" int gcxk_gnn_fp32_layer(int kind, uint32_t N, const uint32_t *dV, const uint32_t *dE,
-                                   const uint32_t *Er, const uint32_t *rank, const float *W,
-                                   const float *b, int md, const float *x, float *out,
-                                   uint32_t *key, const uint32_t *hubs, uint32_t nhub,
-                                   const uint32_t *perm, const float *fidv, void *stream,
+                                   const uint32_t *Ed, const uint32_t *Er, const uint32_t *rank,
+                                   const float *W, const float *b, int md, const float *x,
+                                   float *out, uint32_t *key, const uint32_t *hubs, uint32_t nhub,
+                                   const uint32_t *perm, const uint32_t *gspan, void *stream,
                                    void *hub_stream)
 {
+    const GnnCsr gr = {dV, dE, Ed, Er, rank, perm, hubs, reinterpret_cast<const uint2 *>(gspan), N,
+                    nhub};
     switch (kind) {
     case 0:
-        return launch_layer<kInput>(N, dV, dE, Er, rank, W, b, (float)md, x, out, nullptr,
-                                    hubs, nhub, perm, fidv, stream, hub_stream);
+        return launch_layer<kInput>(gr, W, b, (float)md, insum::recip_md_ok(md), nullptr, out,
+                                    nullptr, stream, hub_stream);
     case 1:
-        return launch_layer<kHidden>(N, dV, dE, Er, rank, W, b, 0.0f, x, out, nullptr, hubs, nhub,
-                                     perm, nullptr, stream, hub_stream);
+        return launch_layer<kHidden>(gr, W, b, 0.0f, false, x, out, nullptr, stream, hub_stream);
     case 2:
-        return launch_layer<kOutput>(N, dV, dE, Er, rank, W, b, 0.0f, x, nullptr, key, hubs, nhub,
-                                     perm, nullptr, stream, hub_stream);
+        return launch_layer<kOutput>(gr, W, b, 0.0f, false, x, nullptr, key, stream, hub_stream);
     default:
         return GCX_ERR_ARG;
     }

@@ -29,6 +29,7 @@
 // write disjoint rows.
 #include "common.hip.h"
 #include "hub_sum.hip.h"
+#include "input_sum.hip.h"
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -39,6 +40,8 @@ constexpr int F = 16;
 constexpr int kBlock = 256;
 constexpr int kUnroll = 8;        // neighbour rows in flight per lane
 constexpr int kGrid = 2048;       // persistent workgroups of the quad kernel
+constexpr int kLaneSteps = 1;     // input layer: blocks of 8 neighbours per load step
+constexpr int kHubInRows = 4;     // input layer hubs: neighbours per loader thread per chunk
 
 enum { kInput = 0, kHidden = 1, kOutput = 2 };
 enum { kBf16 = 0, kSplit = 1 };
@@ -53,14 +56,6 @@ __device__ __forceinline__ uint16_t to_bf16(float f)
 }
 
 __device__ __forceinline__ float from_bf16(uint32_t h) { return __builtin_bit_cast(float, h << 16); }
-
-__device__ __forceinline__ float fid(uint32_t v, uint32_t N) { return (float)((double)v / (double)N); }
-
-// Id feature: v / N, or the caller's per-vertex value (graph-local for batches).
-__device__ __forceinline__ float fid_of(uint32_t v, uint32_t N, const float *__restrict__ fidv)
-{
-    return fidv ? fidv[v] : fid(v, N);
-}
 
 // Slice q (features 4q .. 4q+3) of row r: 8 bytes of bf16 or 16 bytes of FP32.
 template <int mode>
@@ -84,18 +79,17 @@ __device__ __forceinline__ void split(float a, short &hi, short &lo)
 }
 
 template <int kind, int mode>
-__global__ __launch_bounds__(kBlock) void sage_mfma_quads(uint32_t N, const uint32_t *__restrict__ V,
-                                                          const uint32_t *__restrict__ E,
+__global__ __launch_bounds__(kBlock) void sage_mfma_quads(const uint32_t *__restrict__ V,
                                                           const uint32_t *__restrict__ Er,
                                                           const uint16_t *__restrict__ Wpack,
-                                                          const float *__restrict__ bias, float fmd,
+                                                          const float *__restrict__ bias,
                                                           const void *__restrict__ x,
                                                           void *__restrict__ out,
                                                           uint32_t *__restrict__ key,
                                                           const uint32_t *__restrict__ perm,
-                                                          uint32_t p0, uint32_t p1,
-                                                          const float *__restrict__ fidv)
+                                                          uint32_t p0, uint32_t p1)
 {
+    static_assert(kind != kInput, "layer 0 runs in sage_mfma_lanes_input");
     const int lane = lane_id();
     const int q = lane & 3;       // quad lane: features 4q .. 4q+3 of vertex lane / 4
     const int mi = lane & 15;     // MFMA lane: A row mi / C column mi
@@ -124,90 +118,47 @@ __global__ __launch_bounds__(kBlock) void sage_mfma_quads(uint32_t N, const uint
         }
         const float fdeg = (float)(end - beg);
         float m[4], sf[4];
-        if (kind == kInput) {
-            // lane q computes the features of neighbours i0 + 4t + q, then
-            // every lane adds all 16 in CSR order (the hub kernel's order)
-            float sd = 0.0f, si = 0.0f;
-            for (uint32_t i0 = beg; i0 < end; i0 += 16) {
-                float fd[4], fi[4];
+        float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
+        uint32_t i = beg;
+        for (; i + kUnroll <= end; i += kUnroll) {
+            uint32_t v[kUnroll];
 #pragma unroll
-                for (int t = 0; t < 4; t++) {
-                    const uint32_t i = i0 + 4 * t + q;
-                    fd[t] = 0.0f;
-                    fi[t] = 0.0f;
-                    if (i < end) {
-                        const float2 r = reinterpret_cast<const float2 *>(x)[Er[i]];
-                        fd[t] = r.x;
-                        fi[t] = r.y;
-                    }
-                }
-                const uint32_t nb = end - i0 < 16u ? end - i0 : 16u;
+            for (int t = 0; t < kUnroll; t++)
+                v[t] = Er[i + t];
+            float4 r[kUnroll];
 #pragma unroll
-                for (int t = 0; t < 4; t++)
+            for (int t = 0; t < kUnroll; t++)
+                r[t] = load_slice<mode>(x, v[t], q);
 #pragma unroll
-                    for (int m2 = 0; m2 < 4; m2++) {
-                        const float a = __shfl(fd[t], m2, 4), e = __shfl(fi[t], m2, 4);
-                        if ((uint32_t)(4 * t + m2) < nb) {
-                            sd = sd + a;
-                            si = si + e;
-                        }
-                    }
+            for (int t = 0; t < kUnroll; t++) {
+                s0 = s0 + r[t].x;
+                s1 = s1 + r[t].y;
+                s2 = s2 + r[t].z;
+                s3 = s3 + r[t].w;
             }
-            if (fdeg > 0.0f) {
-                sd = sd / fdeg;
-                si = si / fdeg;
-            }
-            // k = 4q .. 4q+3 of [sd x8, si x8] and of [fdeg x8, fid x8]
-            const float a = q < 2 ? sd : si;
-            const float b = valid ? (q < 2 ? fdeg / fmd : fid_of(u, N, fidv)) : 0.0f;
-#pragma unroll
-            for (int e = 0; e < 4; e++) {
-                m[e] = a;
-                sf[e] = b;
-            }
-        } else {
-            float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
-            uint32_t i = beg;
-            for (; i + kUnroll <= end; i += kUnroll) {
-                uint32_t v[kUnroll];
-#pragma unroll
-                for (int t = 0; t < kUnroll; t++)
-                    v[t] = Er[i + t];
-                float4 r[kUnroll];
-#pragma unroll
-                for (int t = 0; t < kUnroll; t++)
-                    r[t] = load_slice<mode>(x, v[t], q);
-#pragma unroll
-                for (int t = 0; t < kUnroll; t++) {
-                    s0 = s0 + r[t].x;
-                    s1 = s1 + r[t].y;
-                    s2 = s2 + r[t].z;
-                    s3 = s3 + r[t].w;
-                }
-            }
-            for (; i < end; i++) {
-                const float4 r = load_slice<mode>(x, Er[i], q);
-                s0 = s0 + r.x;
-                s1 = s1 + r.y;
-                s2 = s2 + r.z;
-                s3 = s3 + r.w;
-            }
-            if (fdeg > 0.0f) {
-                s0 = s0 / fdeg;
-                s1 = s1 / fdeg;
-                s2 = s2 / fdeg;
-                s3 = s3 / fdeg;
-            }
-            m[0] = s0;
-            m[1] = s1;
-            m[2] = s2;
-            m[3] = s3;
-            const float4 self = valid ? load_slice<mode>(x, pos, q) : make_float4(0.f, 0.f, 0.f, 0.f);
-            sf[0] = self.x;
-            sf[1] = self.y;
-            sf[2] = self.z;
-            sf[3] = self.w;
         }
+        for (; i < end; i++) {
+            const float4 r = load_slice<mode>(x, Er[i], q);
+            s0 = s0 + r.x;
+            s1 = s1 + r.y;
+            s2 = s2 + r.z;
+            s3 = s3 + r.w;
+        }
+        if (fdeg > 0.0f) {
+            s0 = s0 / fdeg;
+            s1 = s1 / fdeg;
+            s2 = s2 / fdeg;
+            s3 = s3 / fdeg;
+        }
+        m[0] = s0;
+        m[1] = s1;
+        m[2] = s2;
+        m[3] = s3;
+        const float4 self = valid ? load_slice<mode>(x, pos, q) : make_float4(0.f, 0.f, 0.f, 0.f);
+        sf[0] = self.x;
+        sf[1] = self.y;
+        sf[2] = self.z;
+        sf[3] = self.w;
         // A operands: lane (mi, mq) takes slice mq of vertex row mi
         bf16x4 ah0, ah1, al0, al1;
 #pragma unroll
@@ -260,6 +211,94 @@ __global__ __launch_bounds__(kBlock) void sage_mfma_quads(uint32_t N, const uint
     }
 }
 
+// Layer 0 for the vertices below the hub degree: one lane per vertex, 64
+// vertices per wave (insum::lane_sums, the FP32 path's sums), then four
+// 16-vertex MFMA tiles. A row mi of tile t is the vertex of lane 16t + mi:
+// K half 0 is [sd x8, si x8], K half 1 [fdeg/md x8, fid x8], so lane
+// (mi, mq) takes sd or si (fdeg/md or fid) of that vertex by mq < 2.
+template <int mode, bool kRecip>
+__global__ __launch_bounds__(kBlock) void sage_mfma_lanes_input(GnnCsr gr,
+                                                                const uint16_t *__restrict__ Wpack,
+                                                                const float *__restrict__ bias,
+                                                                float fmd, void *__restrict__ out)
+{
+    const uint32_t p0 = gr.nhub, p1 = gr.N;
+    const int lane = lane_id();
+    const int mi = lane & 15, mq = lane >> 4;
+    const uint32_t wave = (blockIdx.x * kBlock + threadIdx.x) / kWave;
+    const uint32_t base = p0 + wave * (uint32_t)kWave;
+    if (base >= p1) // wave-uniform: the MFMAs below run with every lane
+        return;
+    const bf16x4 bh0 = *reinterpret_cast<const bf16x4 *>(Wpack + 4 * lane);
+    const bf16x4 bh1 = *reinterpret_cast<const bf16x4 *>(Wpack + 256 + 4 * lane);
+    bf16x4 bl0 = {0, 0, 0, 0}, bl1 = {0, 0, 0, 0};
+    if (mode == kSplit) {
+        bl0 = *reinterpret_cast<const bf16x4 *>(Wpack + 512 + 4 * lane);
+        bl1 = *reinterpret_cast<const bf16x4 *>(Wpack + 768 + 4 * lane);
+    }
+    const float bj = bias[mi];
+    const uint32_t pos = base + (uint32_t)lane;
+    const bool valid = pos < p1;
+    const uint32_t u = valid ? gr.perm[pos] : 0u;
+    uint32_t beg = 0, end = 0;
+    if (valid) {
+        beg = gr.V[u];
+        end = gr.V[u + 1];
+    }
+    const uint2 sp = insum::span_of(u, gr.N, gr.gspan);
+    const float2 s = insum::lane_sums<kLaneSteps, kRecip>(beg, end, gr.E, gr.Ed, fmd, sp);
+    const float fdeg = (float)(end - beg);
+    float sd = s.x, si = s.y;
+    if (fdeg > 0.0f) {
+        sd = sd / fdeg;
+        si = si / fdeg;
+    }
+    const float selfd = insum::deg_feature(end - beg, fmd);
+    const float selfi = valid ? insum::id_feature(u, sp) : 0.0f;
+#pragma unroll
+    for (int t = 0; t < kWave / 16; t++) {
+        const int src = 16 * t + mi;
+        const float sdt = __shfl(sd, src), sit = __shfl(si, src);
+        const float fdt = __shfl(selfd, src), fit = __shfl(selfi, src);
+        const float a = mq < 2 ? sdt : sit, b = mq < 2 ? fdt : fit;
+        bf16x4 ah0, ah1, al0, al1;
+        if (mode == kSplit) {
+            short h0, l0, h1, l1;
+            split(a, h0, l0);
+            split(b, h1, l1);
+            ah0 = bf16x4{h0, h0, h0, h0};
+            al0 = bf16x4{l0, l0, l0, l0};
+            ah1 = bf16x4{h1, h1, h1, h1};
+            al1 = bf16x4{l1, l1, l1, l1};
+        } else {
+            const short h0 = (short)to_bf16(a), h1 = (short)to_bf16(b);
+            ah0 = bf16x4{h0, h0, h0, h0};
+            ah1 = bf16x4{h1, h1, h1, h1};
+        }
+        f32x4 c = {bj, bj, bj, bj};
+        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah0, bh0, c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah1, bh1, c, 0, 0, 0);
+        if (mode == kSplit) {
+            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah0, bl0, c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah1, bl1, c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al0, bh0, c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al1, bh1, c, 0, 0, 0);
+        }
+        // c[e] = H[tile row 4mq + e][feature mi]
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const size_t row = base + (uint32_t)(16 * t + 4 * mq + e); // degree-order position
+            if (row < p1) {
+                const float h = c[e] > 0.0f ? c[e] : 0.0f;
+                if (mode == kBf16)
+                    reinterpret_cast<uint16_t *>(out)[row * F + mi] = to_bf16(h);
+                else
+                    reinterpret_cast<float *>(out)[row * F + mi] = h;
+            }
+        }
+    }
+}
+
 // One workgroup per hub. Neighbour rows are staged into LDS feature-major,
 // 512 per step with the next step's loads in flight (hub_sum.hip.h), and
 // lanes 0..15 of wave 0 sum them in CSR order -- the order of the quad
@@ -274,32 +313,31 @@ __global__ __launch_bounds__(kBlock) void sage_mfma_quads(uint32_t N, const uint
 constexpr int kHubChunk = 512;
 
 template <int kind, int mode>
-__global__ __launch_bounds__(kHubChunk) void sage_mfma_hubs(uint32_t N, const uint32_t *__restrict__ V,
-                                                            const uint32_t *__restrict__ E,
-                                                            const uint32_t *__restrict__ Er,
-                                                            const uint32_t *__restrict__ rank,
+__global__ __launch_bounds__(kHubChunk) void sage_mfma_hubs(GnnCsr gr,
                                                             const uint16_t *__restrict__ Wpack,
-                                                            const float *__restrict__ bias, float fmd,
-                                                            const void *__restrict__ x,
+                                                            const float *__restrict__ bias,
+                                                            float fmd, const void *__restrict__ x,
                                                             void *__restrict__ out,
-                                                            uint32_t *__restrict__ key,
-                                                            const uint32_t *__restrict__ hubs,
-                                                            const float *__restrict__ fidv)
+                                                            uint32_t *__restrict__ key)
 {
-    __shared__ __attribute__((aligned(16))) hubsum::Stage<kHubChunk> st;
-    const uint32_t u = hubs[blockIdx.x];
-    const uint32_t beg = V[u], end = V[u + 1];
+    // the input layer stages 2 features per neighbour, the others 16
+    __shared__ __attribute__((aligned(16))) union {
+        hubsum::Stage<kHubChunk> rows;
+        insum::PairStage<kHubChunk, kHubInRows> feats;
+    } st;
+    const uint32_t u = gr.perm[blockIdx.x]; // hubs by degree, largest first
+    const uint32_t beg = gr.V[u], end = gr.V[u + 1];
     const uint32_t deg = end - beg;
     const int tid = threadIdx.x;
     // layer 0: lane 0 sums fdeg, lane 1 fid; else lane j feature j
     float acc;
+    uint2 sp = make_uint2(0u, gr.N);
     if (kind == kInput) {
-        acc = hubsum::ordered_sum<kHubChunk, true>(st, beg, end, Er, [&](uint32_t v, float4 *row) {
-            const float2 r = reinterpret_cast<const float2 *>(x)[v];
-            row[0] = make_float4(r.x, r.y, 0.0f, 0.0f);
-        });
+        sp = insum::span_of(u, gr.N, gr.gspan);
+        acc = insum::pair_sums(st.feats, beg, end, gr.E, gr.Ed, fmd, sp);
     } else {
-        acc = hubsum::ordered_sum<kHubChunk, false>(st, beg, end, Er, [&](uint32_t v, float4 *row) {
+        acc = hubsum::ordered_sum<kHubChunk, false>(st.rows, beg, end, gr.Er,
+                                                    [&](uint32_t v, float4 *row) {
 #pragma unroll
             for (int q = 0; q < 4; q++)
                 row[q] = load_slice<mode>(x, v, q);
@@ -310,7 +348,7 @@ __global__ __launch_bounds__(kHubChunk) void sage_mfma_hubs(uint32_t N, const ui
     const int lane = tid;
     const int mi = lane & 15, mq = lane >> 4;
     const float fdeg = (float)deg;
-    const uint32_t self_row = rank[u];
+    const uint32_t self_row = gr.rank[u];
     // mean features (lane j holds feature j; layer 0: lanes 0, 1)
     float mean = acc;
     if (fdeg > 0.0f)
@@ -322,7 +360,7 @@ __global__ __launch_bounds__(kHubChunk) void sage_mfma_hubs(uint32_t N, const ui
         if (kind == kInput) {
             const float sd = __shfl(mean, 0), si = __shfl(mean, 1);
             a[e] = mq < 2 ? sd : si;
-            b[e] = mq < 2 ? fdeg / fmd : fid_of(u, N, fidv);
+            b[e] = mq < 2 ? insum::deg_feature(deg, fmd) : insum::id_feature(u, sp);
         } else {
             a[e] = __shfl(mean, k);
             const float4 r4 = load_slice<mode>(x, self_row, mq);
@@ -379,43 +417,50 @@ __global__ __launch_bounds__(kHubChunk) void sage_mfma_hubs(uint32_t N, const ui
 }
 
 template <int kind, int mode>
-int launch(uint32_t N, const uint32_t *V, const uint32_t *E, const uint32_t *Er, const uint32_t *rank,
-           const uint16_t *Wpack, const float *b, float fmd, const void *x, void *out,
-           uint32_t *key, const uint32_t *hubs, uint32_t nhub, const uint32_t *perm,
-           const float *fidv, void *stream, void *hub_stream)
+int launch(const GnnCsr &gr, const uint16_t *Wpack, const float *b, float fmd, bool recip_ok,
+           const void *x, void *out, uint32_t *key, void *stream, void *hub_stream)
 {
-    if (N == 0)
+    if (gr.N == 0)
         return GCX_OK;
-    if (nhub)
-        sage_mfma_hubs<kind, mode><<<nhub, kHubChunk, 0, as_stream(hub_stream)>>>(
-            N, V, E, Er, rank, Wpack, b, fmd, x, out, key, hubs, fidv);
-    if (N > nhub) {
-        const uint32_t waves = div_up(N - nhub, 16u);
+    if (gr.nhub)
+        sage_mfma_hubs<kind, mode><<<gr.nhub, kHubChunk, 0, as_stream(hub_stream)>>>(
+            gr, Wpack, b, fmd, x, out, key);
+    if constexpr (kind == kInput) {
+        if (gr.N > gr.nhub) {
+            const uint32_t waves = div_up(gr.N - gr.nhub, (uint32_t)kWave);
+            const uint32_t grid = div_up(waves, (uint32_t)(kBlock / kWave));
+            if (recip_ok)
+                sage_mfma_lanes_input<mode, true>
+                    <<<grid, kBlock, 0, as_stream(stream)>>>(gr, Wpack, b, fmd, out);
+            else
+                sage_mfma_lanes_input<mode, false>
+                    <<<grid, kBlock, 0, as_stream(stream)>>>(gr, Wpack, b, fmd, out);
+        }
+    } else if (gr.N > gr.nhub) {
+        const uint32_t waves = div_up(gr.N - gr.nhub, 16u);
         const uint32_t need = div_up(waves, (uint32_t)(kBlock / kWave));
         sage_mfma_quads<kind, mode><<<need < (uint32_t)kGrid ? need : (uint32_t)kGrid, kBlock, 0,
-                                      as_stream(stream)>>>(N, V, E, Er, Wpack, b, fmd, x, out,
-                                                           key, perm, nhub, N, fidv);
+                                      as_stream(stream)>>>(gr.V, gr.Er, Wpack, b, x, out, key,
+                                                           gr.perm, gr.nhub, gr.N);
     }
     GCXK_LAUNCH_CHECK();
     return GCX_OK;
 }
 
 template <int mode>
-int layer(int kind, uint32_t N, const uint32_t *dV, const uint32_t *dE, const uint32_t *Er,
-          const uint32_t *rank, const uint16_t *Wpack, const float *b, float fmd, const void *x,
-          void *out, uint32_t *key, const uint32_t *hubs, uint32_t nhub, const uint32_t *perm,
-          const float *fidv, void *stream, void *hub_stream)
+int layer(int kind, const GnnCsr &gr, const uint16_t *Wpack, const float *b, int md,
+          const void *x, void *out, uint32_t *key, void *stream, void *hub_stream)
 {
+    const float fmd = (float)md;
+    const bool ok = insum::recip_md_ok(md);
     switch (kind) {
     case 0:
-        return launch<kInput, mode>(N, dV, dE, Er, rank, Wpack, b, fmd, x, out, nullptr,
-                                    hubs, nhub, perm, fidv, stream, hub_stream);
+        return launch<kInput, mode>(gr, Wpack, b, fmd, ok, nullptr, out, nullptr, stream,
+                                    hub_stream);
     case 1:
-        return launch<kHidden, mode>(N, dV, dE, Er, rank, Wpack, b, fmd, x, out, nullptr, hubs,
-                                     nhub, perm, nullptr, stream, hub_stream);
+        return launch<kHidden, mode>(gr, Wpack, b, fmd, ok, x, out, nullptr, stream, hub_stream);
     case 2:
-        return launch<kOutput, mode>(N, dV, dE, Er, rank, Wpack, b, fmd, x, nullptr, key, hubs,
-                                     nhub, perm, nullptr, stream, hub_stream);
+        return launch<kOutput, mode>(gr, Wpack, b, fmd, ok, x, nullptr, key, stream, hub_stream);
     default:
         return GCX_ERR_ARG;
     }
@@ -424,15 +469,14 @@ int layer(int kind, uint32_t N, const uint32_t *dV, const uint32_t *dE, const ui
 } // namespace
 
 extern "C" int gcxk_gnn_bf16_layer(int kind, int split_mode, uint32_t N, const uint32_t *dV,
-                                   const uint32_t *dE, const uint32_t *Er, const uint32_t *rank,
-                                   const uint16_t *Wpack, const float *b, int md, const void *x,
-                                   void *out, uint32_t *key, const uint32_t *hubs, uint32_t nhub,
-                                   const uint32_t *perm, const float *fidv, void *stream,
-                                   void *hub_stream)
+                                   const uint32_t *dE, const uint32_t *Ed, const uint32_t *Er,
+                                   const uint32_t *rank, const uint16_t *Wpack, const float *b,
+                                   int md, const void *x, void *out, uint32_t *key,
+                                   const uint32_t *hubs, uint32_t nhub, const uint32_t *perm,
+                                   const uint32_t *gspan, void *stream, void *hub_stream)
 {
-    const float fmd = (float)md;
-    return split_mode ? layer<kSplit>(kind, N, dV, dE, Er, rank, Wpack, b, fmd, x, out, key, hubs,
-                                      nhub, perm, fidv, stream, hub_stream)
-                      : layer<kBf16>(kind, N, dV, dE, Er, rank, Wpack, b, fmd, x, out, key, hubs,
-                                     nhub, perm, fidv, stream, hub_stream);
+    const GnnCsr gr = {dV, dE, Ed, Er, rank, perm, hubs, reinterpret_cast<const uint2 *>(gspan), N,
+                       nhub};
+    return split_mode ? layer<kSplit>(kind, gr, Wpack, b, md, x, out, key, stream, hub_stream)
+                      : layer<kBf16>(kind, gr, Wpack, b, md, x, out, key, stream, hub_stream);
 }
