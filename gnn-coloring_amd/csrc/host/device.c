@@ -1455,8 +1455,11 @@ int gcx_set_ordered_launch(int slots)
 
 int gcx_get_ordered_launch(void) { return g_ordered_slots; }
 
-/* Enqueues the ordered kernel on s under the launch policy above. */
-static int launch_ordered(gcx_workspace *ws, const gcx_graph *g, int mb, hipStream_t s)
+/* Enqueues the ordered kernel on s under the launch policy above: the pull
+ * kernel (pull = 1: positions in greedy order, headers in items[0], colours by
+ * position in wof, order ids in *ids) or the push kernel (state words). */
+static int launch_ordered(gcx_workspace *ws, const gcx_graph *g, int mb, int pull,
+                          const uint32_t *ids, hipStream_t s)
 {
     pthread_mutex_lock(&g_ordered_lock);
     int rc = GCX_OK;
@@ -1470,16 +1473,20 @@ static int launch_ordered(gcx_workspace *ws, const gcx_graph *g, int mb, hipStre
         if ((g_ordered_drain || i == (int)slot) && hipStreamWaitEvent(s, g_slot_done[i], 0) != hipSuccess)
             rc = GCX_ERR_HIP;
     g_ordered_drain = 0;
-    uint32_t grid = ws->ordered_grid ? ws->ordered_grid : gcxk_jp_ordered_grid();
+    uint32_t grid = ws->ordered_grid ? ws->ordered_grid
+                                     : (pull ? gcxk_pull_grid() : gcxk_jp_ordered_grid());
     if (k) {
-        const uint32_t share = gcxk_jp_ordered_cap() / (uint32_t)k;
+        const uint32_t share = (pull ? gcxk_pull_cap() : gcxk_jp_ordered_cap()) / (uint32_t)k;
         if (grid > share)
             grid = share;
         if (grid == 0)
             rc = GCX_ERR_HIP;
     }
     /* 2 s per wait: far beyond any legitimate wait, and it still ends the kernel */
-    if (rc == GCX_OK)
+    if (rc == GCX_OK && pull)
+        rc = gcxk_pull_colour(g->N, ws->items[0], ids, ws->esplit, ws->wof, ws->color, ws->ctl,
+                              grid, 200000000ull, k == 0, s);
+    else if (rc == GCX_OK)
         rc = gcxk_jp_ordered(g->V, ws->npred, ws->esplit, ws->state, ws->wdesc, ws->color, ws->ovf,
                              mb, ws->items[0], ws->ctl, grid, 200000000ull, k == 0, s);
     if (rc == GCX_OK && hipEventRecord(g_slot_done[slot], s) != hipSuccess)
@@ -1507,6 +1514,32 @@ static int ordered_done(gcx_workspace *ws, hipStream_t s)
  * head kernel -- then state words and items; the head kernel colours the
  * dense top of the order in one workgroup, and the persistent launch the
  * rest. The rank keys live in ws->count (SL scratch, free during JP). */
+/* Ordered schedule with pulled colours (the default): the greedy order (sort),
+ * positions rank[ids[j]] = j (ws->count), the orientation writing every
+ * row's predecessor positions (hub rows on the side stream), then one
+ * persistent launch. Headers live in items[0], colours by position in wof. */
+static int color_pull(gcx_workspace *ws, const gcx_graph *g, int mb, hipStream_t s)
+{
+    void *tok = NULL;
+    TRY(gcx__prof_begin(GCX_ST_JP_INIT, s, &tok));
+    void *scratch = ws->order_buf ? ws->order_buf : (void *)ws->x;
+    uint32_t *ids = NULL;
+    TRY(gcxk_jp_sort_ids(g->N, ws->key, ws->key_bits, scratch, ws->order_bytes, &ids, s));
+    TRY(gcxk_pull_rank(g->N, ids, ws->count, g->nhub, ws->hpos, ws->ctl, s));
+    void *hs = gcx__fork_hubs(ws, g, 1, s);
+    if (!hs)
+        return GCX_ERR_HIP;
+    TRY(gcx__join_hubs(ws, hs, s,
+                       gcxk_pull_orient(g->N, g->V, g->E, ws->count, g->perm, g->hubs, g->nhub,
+                                        g->hchunk, g->nhchunk, ws->hpos, ws->esplit,
+                                        ws->items[0], ws->wof, s, hs)));
+    TRY(gcx__prof_end(tok, s));
+    TRY(gcx__prof_begin(GCX_ST_JP_ROUNDS, s, &tok));
+    TRY(launch_ordered(ws, g, mb, 1, ids, s));
+    TRY(gcx__prof_end(tok, s));
+    return ordered_done(ws, s);
+}
+
 static int color_ordered(gcx_workspace *ws, const gcx_graph *g, int mb, hipStream_t s)
 {
     void *tok = NULL;
@@ -1524,7 +1557,7 @@ static int color_ordered(gcx_workspace *ws, const gcx_graph *g, int mb, hipStrea
                                 s));
         TRY(gcx__prof_end(tok, s));
         TRY(gcx__prof_begin(GCX_ST_JP_ROUNDS, s, &tok));
-        TRY(launch_ordered(ws, g, mb, s));
+        TRY(launch_ordered(ws, g, mb, 0, NULL, s));
         TRY(gcx__prof_end(tok, s));
         return ordered_done(ws, s);
     }
@@ -1551,7 +1584,7 @@ static int color_ordered(gcx_workspace *ws, const gcx_graph *g, int mb, hipStrea
     if (head_max)
         TRY(gcxk_jp_head(g->V, ws->npred, ws->esplit, ids, ws->items[1], ws->color, ws->ctl,
                          200000000ull, s));
-    const int rc = launch_ordered(ws, g, mb, s);
+    const int rc = launch_ordered(ws, g, mb, 0, NULL, s);
     TRY(rc);
     TRY(gcx__prof_end(tok, s));
     return ordered_done(ws, s);
@@ -1568,7 +1601,7 @@ int gcx_color(gcx_workspace *ws, const gcx_graph *g, void *stream)
     const int sched = ws->schedule == GCX_SCHED_AUTO ? gcx__tuned_schedule(g) : ws->schedule;
     ws->last_ordered = 0;
     if (sched == GCX_SCHED_ORDERED && g->N) {
-        const int rc = color_ordered(ws, g, mb, s);
+        const int rc = ws->head_max ? color_ordered(ws, g, mb, s) : color_pull(ws, g, mb, s);
         if (rc != GCX_ERR_INVALID)
             return rc;
         ws->last_fallback = 1; /* a wait hit its bound: redo with rounds */
