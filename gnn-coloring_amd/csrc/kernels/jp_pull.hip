@@ -12,18 +12,23 @@
 //   hdr[j]   = (V[ids[j]], np(ids[j]));
 //   pcol[j]  = colour of position j, 0 while it has none.
 // Predecessors of a vertex are exactly the earlier positions among its
-// neighbours, so v precedes u iff rank[v] < rank[u].
+// neighbours, so v precedes u iff rank[v] < rank[u]. (Gathering the
+// positions through the degree-order columns Er instead of E, which puts the
+// hubs' words together, was 2% slower on R-MAT 22.)
 //
-// One persistent launch: wave w colours positions w, w + W, w + 2W, ... in
-// order. A position gathers its predecessors' colours from pcol and waits for
-// the missing ones; nothing is pushed to successors, so a hop of the DAG's
-// longest chain is one colour store seen by one load (no atomics). While
-// more than kFast predecessors are missing the wave polls only the latest
-// of them in greedy order (one word), re-gathering the others when it is
-// coloured or every kRegather polls. Once at most kFast are missing, the
-// nmiss + 1 smallest colours absent from the predecessors seen so far are
-// the only candidates: the colour is the smallest candidate that none of the
-// missing predecessors takes, so the last wait is on those few words alone.
+// One persistent launch: wave w colours positions in increasing order (see
+// jp_pull_kernel for which). A position gathers its predecessors' colours
+// from pcol and waits for the missing ones; nothing is pushed to successors,
+// so a hop of the DAG's longest chain is one colour store seen by one load
+// (no atomics). While more than kFast predecessors are missing the wave
+// polls only the latest of them in greedy order (one word), re-gathering the
+// others when it is coloured or every kRegather polls (re-gathering only
+// when it is coloured was 1.6x slower on R-MAT 22: the latest predecessor
+// in greedy order is often not the last one coloured). Once at most kFast
+// are missing, the nmiss + 1 smallest colours absent from the predecessors
+// seen so far are the only candidates: the colour is the smallest candidate
+// that none of the missing predecessors takes, so the last wait is on those
+// few words alone.
 //
 // The earliest uncoloured position has all its predecessors coloured and its
 // wave is at it (all of the wave's earlier positions are coloured), so the
@@ -44,9 +49,12 @@ constexpr uint32_t kBatch = kSlots * kWave; // largest list kept in registers
 constexpr uint32_t kWin = 4096;             // colours per bitmap window
 constexpr uint32_t kWinWords = kWin / 32;
 constexpr uint32_t kWaveWords = kWinWords + kWave; // bitmap + a dummy word per lane
-constexpr uint32_t kFast = 8;               // missing predecessors polled word by word
+constexpr uint32_t kFast = 16;              // missing predecessors polled word by word
 constexpr uint32_t kRegather = 8;           // latest-missing polls between re-gathers
 constexpr uint32_t kClockPoll = 64;         // polls before the wall clock is read
+constexpr uint32_t kSingleRounds = 16;      // positions per wave coloured one at a time first
+constexpr uint32_t kChunkPos = 4;           // consecutive positions per chunk after that
+constexpr uint32_t kQuarter = kWave / kChunkPos; // lanes per position in a small chunk
 
 __device__ __forceinline__ uint32_t load_col(const uint32_t *p)
 {
@@ -451,6 +459,77 @@ __device__ __forceinline__ uint32_t pull_any(uint32_t beg, uint32_t np, uint32_t
     }
 }
 
+// Colour of one position with a whole wave (any predecessor count); 0 when
+// a wait hit its bound.
+__device__ __forceinline__ uint32_t pull_one(uint32_t beg, uint32_t np, uint32_t *bm,
+                                             const uint32_t *__restrict__ esplit,
+                                             const uint32_t *pcol, gcxk_ctl *ctl,
+                                             uint64_t spin_ticks, int lane)
+{
+    if (np == 0u)
+        return 1u;
+    if (np <= kWave)
+        return pull_list<1>(beg, np, bm, esplit, pcol, ctl, spin_ticks, lane);
+    if (np <= 2u * kWave)
+        return pull_list<2>(beg, np, bm, esplit, pcol, ctl, spin_ticks, lane);
+    if (np <= 4u * kWave)
+        return pull_list<4>(beg, np, bm, esplit, pcol, ctl, spin_ticks, lane);
+    if (np <= 8u * kWave)
+        return pull_list<8>(beg, np, bm, esplit, pcol, ctl, spin_ticks, lane);
+    if (np <= kBatch)
+        return pull_list<kSlots>(beg, np, bm, esplit, pcol, ctl, spin_ticks, lane);
+    return pull_any(beg, np, bm, esplit, pcol, ctl, spin_ticks, lane);
+}
+
+// Four consecutive positions at once, 16 lanes each, every one with at most
+// 16 predecessors: lane g of a group loads predecessor g and waits for its
+// colour; the mex (<= 17) is the first clear bit of the group's OR of
+// 1 << (colour - 1). A group stores its colour as soon as it has it (a later
+// position of the chunk may wait for it). False when a wait hit its bound.
+__device__ __forceinline__ bool pull_quarter(uint32_t jq, uint32_t n, uint32_t beg, uint32_t np,
+                                             uint32_t u, const uint32_t *__restrict__ esplit,
+                                             uint32_t *pcol, int *__restrict__ color,
+                                             gcxk_ctl *ctl, uint64_t spin_ticks, int lane)
+{
+    const uint32_t g = (uint32_t)lane & (kQuarter - 1u);
+    const uint32_t sh = (uint32_t)lane & ~(kQuarter - 1u);
+    const bool in = g < np;
+    const uint32_t pp = in ? esplit[beg + g] : 0u;
+    uint32_t cv = in ? load_col(&pcol[pp]) : 0u;
+    bool stored = false;
+    uint32_t polls = 0;
+    uint64_t t0 = 0;
+    for (;;) {
+        const bool miss = in && cv == 0u;
+        const uint32_t gm = (uint32_t)(__ballot(miss) >> sh) & 0xFFFFu;
+        uint32_t m = in && cv != 0u && cv <= 32u ? 1u << (cv - 1u) : 0u;
+#pragma unroll
+        for (int off = (int)kQuarter / 2; off > 0; off >>= 1)
+            m |= __shfl_xor(m, off);
+        if (gm == 0u && !stored) {
+            const uint32_t c = (uint32_t)__builtin_ctz(~m) + 1u;
+            if (g == 0u && jq < n) {
+                __hip_atomic_store(&pcol[jq], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                color[u] = (int)c;
+            }
+            stored = true;
+        }
+        if (__ballot(!stored) == 0ull)
+            return true;
+        __builtin_amdgcn_s_sleep(1);
+        if (!wait_ok(polls, t0, ctl, spin_ticks, lane))
+            return false;
+        if (miss)
+            cv = load_col(&pcol[pp]);
+    }
+}
+
+// Positions [0, T), T = min(N, kSingleRounds * W): one position per wave at a
+// time, wave w taking w, w + W, ... Positions [T, N): chunks of kChunkPos
+// consecutive positions, wave w taking chunks w, w + W, ...; a chunk whose
+// positions all have at most kQuarter predecessors is coloured by four
+// 16-lane groups at once, any other chunk one position at a time. Each wave
+// meets its positions in increasing order either way.
 __global__ __launch_bounds__(kBlock, 4) void jp_pull_kernel(uint32_t N, const uint2 *__restrict__ hdr,
                                                          const uint32_t *__restrict__ ids,
                                                          const uint32_t *__restrict__ esplit,
@@ -463,34 +542,54 @@ __global__ __launch_bounds__(kBlock, 4) void jp_pull_kernel(uint32_t N, const ui
     const uint32_t n = ctl->err ? 0u : N;
     const uint32_t w0 = blockIdx.x * kWavesPerBlock + (uint32_t)w;
     const uint32_t nw = gridDim.x * kWavesPerBlock;
-    uint2 h = w0 < n ? hdr[w0] : make_uint2(0u, 0u);
-    uint32_t u = w0 < n ? ids[w0] : 0u;
-    for (uint32_t j = w0; j < n; j += nw) {
+    const uint32_t T = (uint64_t)kSingleRounds * nw < n ? kSingleRounds * nw : n;
+    uint2 h = w0 < T ? hdr[w0] : make_uint2(0u, 0u);
+    uint32_t u = w0 < T ? ids[w0] : 0u;
+    for (uint32_t j = w0; j < T; j += nw) {
         // the next position's header and id are loaded a position ahead
         const uint32_t jn = j + nw;
-        const uint2 hn = jn < n ? hdr[jn] : make_uint2(0u, 0u);
-        const uint32_t un = jn < n ? ids[jn] : 0u;
-        const uint32_t beg = h.x, np = h.y;
-        uint32_t c;
-        if (np == 0u)
-            c = 1u;
-        else if (np <= kWave)
-            c = pull_list<1>(beg, np, bm, esplit, pcol, ctl, spin_ticks, lane);
-        else if (np <= 2u * kWave)
-            c = pull_list<2>(beg, np, bm, esplit, pcol, ctl, spin_ticks, lane);
-        else if (np <= 4u * kWave)
-            c = pull_list<4>(beg, np, bm, esplit, pcol, ctl, spin_ticks, lane);
-        else if (np <= 8u * kWave)
-            c = pull_list<8>(beg, np, bm, esplit, pcol, ctl, spin_ticks, lane);
-        else if (np <= kBatch)
-            c = pull_list<kSlots>(beg, np, bm, esplit, pcol, ctl, spin_ticks, lane);
-        else
-            c = pull_any(beg, np, bm, esplit, pcol, ctl, spin_ticks, lane);
+        const uint2 hn = jn < T ? hdr[jn] : make_uint2(0u, 0u);
+        const uint32_t un = jn < T ? ids[jn] : 0u;
+        const uint32_t c = pull_one(h.x, h.y, bm, esplit, pcol, ctl, spin_ticks, lane);
         if (c == 0u)
             return; // a wait hit its bound: ctl->err is set
         if (lane == 0) {
             __hip_atomic_store(&pcol[j], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             color[u] = (int)c;
+        }
+        h = hn;
+        u = un;
+    }
+    // chunks: group q = lane / 16 holds position c0 + q
+    const uint32_t q = (uint32_t)lane / kQuarter;
+    const uint32_t stride = kChunkPos * nw;
+    uint32_t c0 = T + kChunkPos * w0;
+    uint32_t jq = c0 + q;
+    h = c0 < n && jq < n ? hdr[jq] : make_uint2(0u, 0u);
+    u = c0 < n && jq < n ? ids[jq] : 0u;
+    for (; c0 < n; c0 += stride) {
+        jq = c0 + q;
+        const uint32_t cn = c0 + stride, jn = cn + q;
+        const uint2 hn = cn < n && jn < n ? hdr[jn] : make_uint2(0u, 0u);
+        const uint32_t un = cn < n && jn < n ? ids[jn] : 0u;
+        const bool small = __ballot(h.y > kQuarter) == 0ull;
+        if (small) {
+            if (!pull_quarter(jq, n, h.x, h.y, u, esplit, pcol, color, ctl, spin_ticks, lane))
+                return;
+        } else {
+            for (uint32_t k = 0; k < kChunkPos && c0 + k < n; k++) {
+                const uint32_t src = k * kQuarter;
+                const uint32_t beg = __shfl(h.x, src), np = __shfl(h.y, src),
+                               uk = __shfl(u, src);
+                const uint32_t c = pull_one(beg, np, bm, esplit, pcol, ctl, spin_ticks, lane);
+                if (c == 0u)
+                    return;
+                if (lane == 0) {
+                    __hip_atomic_store(&pcol[c0 + k], c, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                    color[uk] = (int)c;
+                }
+            }
         }
         h = hn;
         u = un;
