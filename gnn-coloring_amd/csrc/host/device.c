@@ -1233,6 +1233,9 @@ static int sl_round(void *p, uint32_t r, hipStream_t s)
                        s);
 }
 
+/* Persistent launches (launch_ordered, below) */
+enum { LAUNCH_JP_PUSH = 0, LAUNCH_JP_PULL = 1 };
+
 static int order_sl(gcx_workspace *ws, const gcx_graph *g, int log_variant, hipStream_t s)
 {
     HIPC(hipMemsetAsync(ws->ctl, 0, sizeof(gcxk_ctl), s));
@@ -1456,9 +1459,9 @@ int gcx_set_ordered_launch(int slots)
 int gcx_get_ordered_launch(void) { return g_ordered_slots; }
 
 /* Enqueues the ordered kernel on s under the launch policy above: the pull
- * kernel (pull = 1: positions in greedy order, headers in items[0], colours by
- * position in wof, order ids in *ids) or the push kernel (state words). */
-static int launch_ordered(gcx_workspace *ws, const gcx_graph *g, int mb, int pull,
+ * kernel (positions in greedy order, headers in items[0], colours by
+ * position in wof, order ids in ids) or the push kernel (state words). */
+static int launch_ordered(gcx_workspace *ws, const gcx_graph *g, int mb, int kind,
                           const uint32_t *ids, hipStream_t s)
 {
     pthread_mutex_lock(&g_ordered_lock);
@@ -1473,17 +1476,19 @@ static int launch_ordered(gcx_workspace *ws, const gcx_graph *g, int mb, int pul
         if ((g_ordered_drain || i == (int)slot) && hipStreamWaitEvent(s, g_slot_done[i], 0) != hipSuccess)
             rc = GCX_ERR_HIP;
     g_ordered_drain = 0;
-    uint32_t grid = ws->ordered_grid ? ws->ordered_grid
-                                     : (pull ? gcxk_pull_grid() : gcxk_jp_ordered_grid());
+    uint32_t grid = ws->ordered_grid        ? ws->ordered_grid
+                    : kind == LAUNCH_JP_PULL ? gcxk_pull_grid()
+                                             : gcxk_jp_ordered_grid();
     if (k) {
-        const uint32_t share = (pull ? gcxk_pull_cap() : gcxk_jp_ordered_cap()) / (uint32_t)k;
+        const uint32_t cap = kind == LAUNCH_JP_PULL ? gcxk_pull_cap() : gcxk_jp_ordered_cap();
+        const uint32_t share = cap / (uint32_t)k;
         if (grid > share)
             grid = share;
         if (grid == 0)
             rc = GCX_ERR_HIP;
     }
     /* 2 s per wait: far beyond any legitimate wait, and it still ends the kernel */
-    if (rc == GCX_OK && pull)
+    if (rc == GCX_OK && kind == LAUNCH_JP_PULL)
         rc = gcxk_pull_colour(g->N, ws->items[0], ids, ws->esplit, ws->wof, ws->color, ws->ctl,
                               grid, 200000000ull, k == 0, s);
     else if (rc == GCX_OK)
@@ -1535,7 +1540,7 @@ static int color_pull(gcx_workspace *ws, const gcx_graph *g, int mb, hipStream_t
                                         ws->items[0], ws->wof, s, hs)));
     TRY(gcx__prof_end(tok, s));
     TRY(gcx__prof_begin(GCX_ST_JP_ROUNDS, s, &tok));
-    TRY(launch_ordered(ws, g, mb, 1, ids, s));
+    TRY(launch_ordered(ws, g, mb, LAUNCH_JP_PULL, ids, s));
     TRY(gcx__prof_end(tok, s));
     return ordered_done(ws, s);
 }
@@ -1557,7 +1562,7 @@ static int color_ordered(gcx_workspace *ws, const gcx_graph *g, int mb, hipStrea
                                 s));
         TRY(gcx__prof_end(tok, s));
         TRY(gcx__prof_begin(GCX_ST_JP_ROUNDS, s, &tok));
-        TRY(launch_ordered(ws, g, mb, 0, NULL, s));
+        TRY(launch_ordered(ws, g, mb, LAUNCH_JP_PUSH, NULL, s));
         TRY(gcx__prof_end(tok, s));
         return ordered_done(ws, s);
     }
@@ -1584,7 +1589,7 @@ static int color_ordered(gcx_workspace *ws, const gcx_graph *g, int mb, hipStrea
     if (head_max)
         TRY(gcxk_jp_head(g->V, ws->npred, ws->esplit, ids, ws->items[1], ws->color, ws->ctl,
                          200000000ull, s));
-    const int rc = launch_ordered(ws, g, mb, 0, NULL, s);
+    const int rc = launch_ordered(ws, g, mb, LAUNCH_JP_PUSH, NULL, s);
     TRY(rc);
     TRY(gcx__prof_end(tok, s));
     return ordered_done(ws, s);

@@ -2,12 +2,13 @@
 // FP32 accumulation), in two precisions:
 //
 //   BF16   rows stored as bf16 (32 B per vertex); neighbour means accumulated
-//          in FP32; the dense step multiplies the bf16-rounded operands
+//          in FP32 (blocked order, below); the dense step multiplies the
+//          bf16-rounded operands
 //          [mean | self] (16 vertices x 32) by the bf16-rounded weights
 //          (32 x 16) on v_mfma_f32_16x16x16_bf16 with an FP32 bias; ReLU in
 //          FP32; the next layer's rows are rounded to bf16; the output layer
 //          sums the 16 FP32 features into the priority.
-//   SPLIT  rows stored in FP32 and summed in CSR order as in the FP32 path;
+//   SPLIT  rows stored in FP32 and summed in FP32 (blocked order, below);
 //          every operand a and weight w is split into bf16 parts
 //          a = a_hi + a_lo, w = w_hi + w_lo, and the dense step runs three
 //          MFMAs per K half: a_hi w_hi + a_hi w_lo + a_lo w_hi (the a_lo w_lo
@@ -16,6 +17,14 @@
 //
 // Rows live at degree-order positions (row of v = rank[v]; neighbours read
 // through Er = rank[E]), as in the FP32 kernel.
+//
+// Neighbour sums of the hidden and output layers are blocked: blocks of 32
+// consecutive CSR entries, each summed in order, block sums added in order
+// (hubsum::kSumBlock). The quad and hub kernels use the same order, so
+// results do not depend on the hub degree, and a hub's dependent add chain
+// is deg / 32 long (the FP32 path keeps the reference's single chain, which
+// made the largest hub, 162,724 entries on R-MAT 22, the layer's critical
+// path). The input layer keeps the FP32 path's sums.
 //
 // Mapping (as in the FP32 kernel): vertices below the hub degree in the
 // graph's degree order, 4 lanes per vertex, 16 vertices per wave; lane q of a
@@ -118,31 +127,41 @@ __global__ __launch_bounds__(kBlock) void sage_mfma_quads(const uint32_t *__rest
         }
         const float fdeg = (float)(end - beg);
         float m[4], sf[4];
+        // blocked sums (hub_sum.hip.h): blocks of kSumBlock entries in
+        // order, the block sums added in order -- as the hub kernel does
         float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
-        uint32_t i = beg;
-        for (; i + kUnroll <= end; i += kUnroll) {
-            uint32_t v[kUnroll];
+        for (uint32_t b0 = beg; b0 < end; b0 += hubsum::kSumBlock) {
+            const uint32_t be = b0 + hubsum::kSumBlock < end ? b0 + hubsum::kSumBlock : end;
+            float t0 = 0.0f, t1 = 0.0f, t2 = 0.0f, t3 = 0.0f;
+            uint32_t i = b0;
+            for (; i + kUnroll <= be; i += kUnroll) {
+                uint32_t v[kUnroll];
 #pragma unroll
-            for (int t = 0; t < kUnroll; t++)
-                v[t] = Er[i + t];
-            float4 r[kUnroll];
+                for (int t = 0; t < kUnroll; t++)
+                    v[t] = Er[i + t];
+                float4 r[kUnroll];
 #pragma unroll
-            for (int t = 0; t < kUnroll; t++)
-                r[t] = load_slice<mode>(x, v[t], q);
+                for (int t = 0; t < kUnroll; t++)
+                    r[t] = load_slice<mode>(x, v[t], q);
 #pragma unroll
-            for (int t = 0; t < kUnroll; t++) {
-                s0 = s0 + r[t].x;
-                s1 = s1 + r[t].y;
-                s2 = s2 + r[t].z;
-                s3 = s3 + r[t].w;
+                for (int t = 0; t < kUnroll; t++) {
+                    t0 = t0 + r[t].x;
+                    t1 = t1 + r[t].y;
+                    t2 = t2 + r[t].z;
+                    t3 = t3 + r[t].w;
+                }
             }
-        }
-        for (; i < end; i++) {
-            const float4 r = load_slice<mode>(x, Er[i], q);
-            s0 = s0 + r.x;
-            s1 = s1 + r.y;
-            s2 = s2 + r.z;
-            s3 = s3 + r.w;
+            for (; i < be; i++) {
+                const float4 r = load_slice<mode>(x, Er[i], q);
+                t0 = t0 + r.x;
+                t1 = t1 + r.y;
+                t2 = t2 + r.z;
+                t3 = t3 + r.w;
+            }
+            s0 = s0 + t0;
+            s1 = s1 + t1;
+            s2 = s2 + t2;
+            s3 = s3 + t3;
         }
         if (fdeg > 0.0f) {
             s0 = s0 / fdeg;
@@ -300,10 +319,10 @@ __global__ __launch_bounds__(kBlock) void sage_mfma_lanes_input(GnnCsr gr,
 }
 
 // One workgroup per hub. Neighbour rows are staged into LDS feature-major,
-// 512 per step with the next step's loads in flight (hub_sum.hip.h), and
-// lanes 0..15 of wave 0 sum them in CSR order -- the order of the quad
-// kernel's sums, so the mean is the same bits whichever kernel handles the
-// vertex. Wave 0 then runs the same MFMA
+// 512 per step with the next step's loads in flight (hub_sum.hip.h); 256
+// threads sum the chunk's 16 blocks x 16 features and lanes 0..15 of wave 0
+// add the block sums in order -- the quad kernel's order, so the mean is the
+// same bits whichever kernel handles the vertex. Wave 0 then runs the same MFMA
 // sequence with the hub as row 0 of the A tile (the other rows zero): each
 // output row of an MFMA depends only on its own A row, so the priorities and
 // rows do not depend on the hub degree.
@@ -322,7 +341,7 @@ __global__ __launch_bounds__(kHubChunk) void sage_mfma_hubs(GnnCsr gr,
 {
     // the input layer stages 2 features per neighbour, the others 16
     __shared__ __attribute__((aligned(16))) union {
-        hubsum::Stage<kHubChunk> rows;
+        hubsum::BlockedStage<kHubChunk> rows;
         insum::PairStage<kHubChunk, kHubInRows> feats;
     } st;
     const uint32_t u = gr.perm[blockIdx.x]; // hubs by degree, largest first
@@ -336,7 +355,7 @@ __global__ __launch_bounds__(kHubChunk) void sage_mfma_hubs(GnnCsr gr,
         sp = insum::span_of(u, gr.N, gr.gspan);
         acc = insum::pair_sums(st.feats, beg, end, gr.E, gr.Ed, fmd, sp);
     } else {
-        acc = hubsum::ordered_sum<kHubChunk, false>(st.rows, beg, end, gr.Er,
+        acc = hubsum::ordered_sum<kHubChunk, false, true>(st.rows, beg, end, gr.Er,
                                                     [&](uint32_t v, float4 *row) {
 #pragma unroll
             for (int q = 0; q < 4; q++)

@@ -23,6 +23,20 @@ struct Stage {
     float v[2][kF][kRows + kPad];
 };
 
+// Blocked sums (the BF16 / BF16X3 layers): a row's neighbours are summed in
+// blocks of kSumBlock consecutive CSR entries, each block in order from 0,
+// and the block sums are added in order: acc = ((0 + B0) + B1) + ... The
+// quad kernel sums a vertex's row the same way, so the bits do not depend on
+// which kernel handles a vertex, and a hub's add chain is deg / kSumBlock
+// long instead of deg.
+constexpr int kSumBlock = 32;
+
+template <int kRows>
+struct BlockedStage {
+    float v[2][kF][kRows + kPad];
+    float bs[2][kRows / kSumBlock][kF]; // a chunk's block sums, by parity
+};
+
 // Sums the first `rows` entries of one staged feature column, in order,
 // into acc. Blocks of 32 rows alternate between two register sets so that
 // the LDS reads of the next block are in flight during the adds of the
@@ -87,10 +101,37 @@ __device__ __forceinline__ float ordered_column_sum(const float *col, uint32_t r
 // chunk c+1's rows into the other LDS buffer and synchronise. The summing
 // wave waits for at most one memory round trip per chunk, overlapped with
 // its own sum.
-template <int kRows, bool pair, typename Load>
-__device__ __forceinline__ float ordered_sum(Stage<kRows> &st, uint32_t beg, uint32_t end,
+// Sum of column f's rows [r0, r1) of one staged chunk, in order from 0.
+__device__ __forceinline__ float block_column_sum(const float *col, uint32_t r0, uint32_t r1)
+{
+    float acc = 0.0f;
+    uint32_t r = r0;
+    if (r1 - r0 == (uint32_t)kSumBlock) {
+        float4 q[kSumBlock / 4];
+#pragma unroll
+        for (int t = 0; t < kSumBlock / 4; t++)
+            q[t] = *reinterpret_cast<const float4 *>(col + r0 + 4 * t);
+#pragma unroll
+        for (int t = 0; t < kSumBlock / 4; t++) {
+            acc = acc + q[t].x;
+            acc = acc + q[t].y;
+            acc = acc + q[t].z;
+            acc = acc + q[t].w;
+        }
+        return acc;
+    }
+    for (; r < r1; r++)
+        acc = acc + col[r];
+    return acc;
+}
+
+template <int kRows, bool pair, bool blocked = false, typename StageT, typename Load>
+__device__ __forceinline__ float ordered_sum(StageT &st, uint32_t beg, uint32_t end,
                                              const uint32_t *__restrict__ Er, Load load)
 {
+    static_assert(!blocked || (!pair && kRows % kSumBlock == 0 && kRows >= kF * 1),
+                  "blocked sums: 16 features, whole blocks per chunk");
+    constexpr uint32_t kBlocks = blocked ? kRows / kSumBlock : 1;
     constexpr int Q = pair ? 1 : 4;
     const int tid = threadIdx.x;
     const uint32_t deg = end - beg;
@@ -141,9 +182,24 @@ __device__ __forceinline__ float ordered_sum(Stage<kRows> &st, uint32_t beg, uin
         const int s = c & 1;
         load(inext, row);
         inext = fetch_idx(c + 3);
-        if (summer) {
-            const uint32_t left = deg - c * kRows;
-            acc = ordered_column_sum(st.v[s][tid], left < (uint32_t)kRows ? left : kRows, acc);
+        const uint32_t left = deg - c * kRows;
+        const uint32_t rows = left < (uint32_t)kRows ? left : kRows;
+        if constexpr (blocked) {
+            // thread (block b, feature f) sums chunk c's block b of column f;
+            // the summers add chunk c-1's block sums (full chunk: kBlocks),
+            // written before the previous step's barrier
+            const uint32_t b = (uint32_t)tid / kF, fe = (uint32_t)tid % kF;
+            if (b < kBlocks && b * kSumBlock < rows) {
+                const uint32_t r1 = (b + 1) * kSumBlock < rows ? (b + 1) * kSumBlock : rows;
+                st.bs[s][b][fe] = block_column_sum(st.v[s][fe], b * kSumBlock, r1);
+            }
+            if (summer && c > 0) {
+#pragma unroll
+                for (uint32_t q = 0; q < kBlocks; q++)
+                    acc = acc + st.bs[s ^ 1][q][tid];
+            }
+        } else if (summer) {
+            acc = ordered_column_sum(st.v[s][tid], rows, acc);
         }
         put(c + 1, s ^ 1, row); // no-op past the last row
         __syncthreads();
@@ -152,6 +208,15 @@ __device__ __forceinline__ float ordered_sum(Stage<kRows> &st, uint32_t beg, uin
         step(c, ia); // chunk c+1's index is in ia, chunk c+3's goes there
         if (c + 1 < nchunk)
             step(c + 1, ib);
+    }
+    if constexpr (blocked) {
+        // the last chunk's block sums (visible after the last step's barrier)
+        if (summer) {
+            const uint32_t c = nchunk - 1, left = deg - c * kRows;
+            const uint32_t nb = (left + kSumBlock - 1) / kSumBlock;
+            for (uint32_t q = 0; q < nb; q++)
+                acc = acc + st.bs[c & 1][q][tid];
+        }
     }
     return acc;
 }

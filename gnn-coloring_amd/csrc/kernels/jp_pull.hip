@@ -54,7 +54,8 @@ constexpr uint32_t kRegather = 8;           // latest-missing polls between re-g
 constexpr uint32_t kClockPoll = 64;         // polls before the wall clock is read
 constexpr uint32_t kSingleRounds = 16;      // positions per wave coloured one at a time first
 constexpr uint32_t kChunkPos = 4;           // consecutive positions per chunk after that
-constexpr uint32_t kQuarter = kWave / kChunkPos; // lanes per position in a small chunk
+constexpr uint32_t kQuarter = kWave / kChunkPos; // lanes per position in a chunk's groups
+constexpr uint32_t kSmall = 64;             // most predecessors a group position may have
 
 __device__ __forceinline__ uint32_t load_col(const uint32_t *p)
 {
@@ -373,10 +374,12 @@ __device__ __forceinline__ uint32_t pull_list(uint32_t beg, uint32_t np, uint32_
         return take_free(w0, w1, lane);
     // candidates: the nmiss + 1 smallest colours absent so far (all within
     // the window: np + 1 <= 1025 < kWin)
-    uint32_t cand[kFast + 1];
-#pragma unroll
-    for (uint32_t k = 0; k <= kFast; k++)
-        cand[k] = k <= nmiss ? take_free(w0, w1, lane) : 0u;
+    // lane k holds the k-th candidate
+    uint32_t cand = 0;
+    for (uint32_t k = 0; k <= nmiss; k++) {
+        const uint32_t c = take_free(w0, w1, lane);
+        cand = (uint32_t)lane == k ? c : cand;
+    }
     // lane k < nmiss polls the k-th missing predecessor
     uint32_t mine = 0, k = 0;
 #pragma unroll
@@ -401,17 +404,18 @@ __device__ __forceinline__ uint32_t pull_list(uint32_t beg, uint32_t np, uint32_
         if (!wait_ok(polls, t0, ctl, spin_ticks, lane))
             return 0u;
     }
-#pragma unroll
-    for (uint32_t q = 0; q <= kFast; q++)
-        if (q <= nmiss && __ballot(poller && got == cand[q]) == 0ull)
-            return cand[q];
+    for (uint32_t q = 0; q <= nmiss; q++) {
+        const uint32_t c = __builtin_amdgcn_readlane(cand, q);
+        if (__ballot(poller && got == c) == 0ull)
+            return c;
+    }
     return 0u; // unreachable: nmiss colours exclude at most nmiss candidates
 }
 
 // Any number of predecessors: batches of kBatch, each waited for in full, per
 // colour window of kWin (a further window only when the first is full, which
 // needs np >= kWin). 0 when a wait hit its bound.
-__device__ __forceinline__ uint32_t pull_any(uint32_t beg, uint32_t np, uint32_t *bm,
+__device__ __noinline__ uint32_t pull_any(uint32_t beg, uint32_t np, uint32_t *bm,
                                           const uint32_t *__restrict__ esplit, const uint32_t *pcol,
                                           gcxk_ctl *ctl, uint64_t spin_ticks, int lane)
 {
@@ -433,6 +437,9 @@ __device__ __forceinline__ uint32_t pull_any(uint32_t beg, uint32_t np, uint32_t
 #pragma unroll
             for (int s = 0; s < kSlots; s++)
                 miss |= b0 + (uint32_t)(kWave * s + lane) < np ? 1u << s : 0u;
+            // gather, then while any is missing poll the latest missing one
+            // in greedy order and re-gather when it is coloured or every
+            // kRegather polls (as pull_list's slow phase)
             for (;;) {
 #pragma unroll
                 for (int s = 0; s < kSlots; s++)
@@ -444,11 +451,22 @@ __device__ __forceinline__ uint32_t pull_any(uint32_t beg, uint32_t np, uint32_t
                     miss &= now ? ~(1u << s) : ~0u;
                     mark(bm, cv[s], lo, now, lane);
                 }
-                if (__ballot(miss != 0u) == 0ull)
+                const uint32_t nmiss = wave_sum<kWave>((uint32_t)__popc(miss));
+                if (nmiss == 0u)
                     break;
-                __builtin_amdgcn_s_sleep(16);
-                if (!wait_ok(polls, t0, ctl, spin_ticks, lane))
-                    return 0u;
+                uint32_t L = 0;
+#pragma unroll
+                for (int s = 0; s < kSlots; s++)
+                    L = ((miss >> s) & 1u) && p[s] > L ? p[s] : L;
+                L = wave_max(L);
+                for (uint32_t q = 0; q < kRegather; q++) {
+                    const uint32_t cl = lane == 0 ? load_col(&pcol[L]) : 0u;
+                    if (__shfl(cl, 0))
+                        break;
+                    far_sleep(nmiss);
+                    if (!wait_ok(polls, t0, ctl, spin_ticks, lane))
+                        return 0u;
+                }
             }
         }
         wave_sync();
@@ -481,34 +499,53 @@ __device__ __forceinline__ uint32_t pull_one(uint32_t beg, uint32_t np, uint32_t
     return pull_any(beg, np, bm, esplit, pcol, ctl, spin_ticks, lane);
 }
 
-// Four consecutive positions at once, 16 lanes each, every one with at most
-// 16 predecessors: lane g of a group loads predecessor g and waits for its
-// colour; the mex (<= 17) is the first clear bit of the group's OR of
-// 1 << (colour - 1). A group stores its colour as soon as it has it (a later
-// position of the chunk may wait for it). False when a wait hit its bound.
-__device__ __forceinline__ bool pull_quarter(uint32_t jq, uint32_t n, uint32_t beg, uint32_t np,
+// Up to four consecutive positions at once, 16 lanes each (groups with
+// active = false take no part), every one with at most kSmall = 64
+// predecessors: lane g of a group loads predecessors g, g + 16, g + 32,
+// g + 48 and waits for their colours; the mex (<= 65) is the first clear bit
+// of the group's OR of 1 << (colour - 1), or 65 when colours 1..64 are all
+// taken. A group stores its colour as soon as it has it (a later position of
+// the chunk may wait for it). False when a wait hit its bound.
+__device__ __forceinline__ bool pull_quarter(bool active, uint32_t jq, uint32_t beg, uint32_t np,
                                              uint32_t u, const uint32_t *__restrict__ esplit,
                                              uint32_t *pcol, int *__restrict__ color,
                                              gcxk_ctl *ctl, uint64_t spin_ticks, int lane)
 {
+    constexpr int S = (int)(kSmall / kQuarter);
     const uint32_t g = (uint32_t)lane & (kQuarter - 1u);
     const uint32_t sh = (uint32_t)lane & ~(kQuarter - 1u);
-    const bool in = g < np;
-    const uint32_t pp = in ? esplit[beg + g] : 0u;
-    uint32_t cv = in ? load_col(&pcol[pp]) : 0u;
-    bool stored = false;
+    uint32_t pp[S], cv[S];
+    uint32_t in = 0;
+#pragma unroll
+    for (int t = 0; t < S; t++)
+        in |= active && g + kQuarter * (uint32_t)t < np ? 1u << t : 0u;
+#pragma unroll
+    for (int t = 0; t < S; t++)
+        pp[t] = (in >> t) & 1u ? esplit[beg + g + kQuarter * (uint32_t)t] : 0u;
+#pragma unroll
+    for (int t = 0; t < S; t++)
+        cv[t] = (in >> t) & 1u ? load_col(&pcol[pp[t]]) : 0u;
+    bool stored = !active;
     uint32_t polls = 0;
     uint64_t t0 = 0;
     for (;;) {
-        const bool miss = in && cv == 0u;
-        const uint32_t gm = (uint32_t)(__ballot(miss) >> sh) & 0xFFFFu;
-        uint32_t m = in && cv != 0u && cv <= 32u ? 1u << (cv - 1u) : 0u;
+        uint32_t miss = 0;
+        uint64_t m = 0;
 #pragma unroll
-        for (int off = (int)kQuarter / 2; off > 0; off >>= 1)
-            m |= __shfl_xor(m, off);
+        for (int t = 0; t < S; t++) {
+            const bool on = (in >> t) & 1u;
+            miss |= on && cv[t] == 0u ? 1u << t : 0u;
+            m |= on && cv[t] != 0u && cv[t] <= 64u ? 1ull << (cv[t] - 1u) : 0ull;
+        }
+        const uint32_t gm = (uint32_t)(__ballot(miss != 0u) >> sh) & 0xFFFFu;
+#pragma unroll
+        for (int off = (int)kQuarter / 2; off > 0; off >>= 1) {
+            const uint32_t lo = __shfl_xor((uint32_t)m, off), hi = __shfl_xor((uint32_t)(m >> 32), off);
+            m |= ((uint64_t)hi << 32) | lo;
+        }
         if (gm == 0u && !stored) {
-            const uint32_t c = (uint32_t)__builtin_ctz(~m) + 1u;
-            if (g == 0u && jq < n) {
+            const uint32_t c = ~m ? (uint32_t)__builtin_ctzll(~m) + 1u : 65u;
+            if (g == 0u) {
                 __hip_atomic_store(&pcol[jq], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 color[u] = (int)c;
             }
@@ -519,17 +556,19 @@ __device__ __forceinline__ bool pull_quarter(uint32_t jq, uint32_t n, uint32_t b
         __builtin_amdgcn_s_sleep(1);
         if (!wait_ok(polls, t0, ctl, spin_ticks, lane))
             return false;
-        if (miss)
-            cv = load_col(&pcol[pp]);
+#pragma unroll
+        for (int t = 0; t < S; t++)
+            if ((miss >> t) & 1u)
+                cv[t] = load_col(&pcol[pp[t]]);
     }
 }
 
 // Positions [0, T), T = min(N, kSingleRounds * W): one position per wave at a
 // time, wave w taking w, w + W, ... Positions [T, N): chunks of kChunkPos
-// consecutive positions, wave w taking chunks w, w + W, ...; a chunk whose
-// positions all have at most kQuarter predecessors is coloured by four
-// 16-lane groups at once, any other chunk one position at a time. Each wave
-// meets its positions in increasing order either way.
+// consecutive positions, wave w taking chunks w, w + W, ...; within a chunk
+// the positions with at most kSmall predecessors are coloured by 16-lane
+// groups at once and the others by the whole wave, in order. Each wave meets
+// its positions in increasing order either way.
 __global__ __launch_bounds__(kBlock, 4) void jp_pull_kernel(uint32_t N, const uint2 *__restrict__ hdr,
                                                          const uint32_t *__restrict__ ids,
                                                          const uint32_t *__restrict__ esplit,
@@ -572,24 +611,37 @@ __global__ __launch_bounds__(kBlock, 4) void jp_pull_kernel(uint32_t N, const ui
         const uint32_t cn = c0 + stride, jn = cn + q;
         const uint2 hn = cn < n && jn < n ? hdr[jn] : make_uint2(0u, 0u);
         const uint32_t un = cn < n && jn < n ? ids[jn] : 0u;
-        const bool small = __ballot(h.y > kQuarter) == 0ull;
-        if (small) {
-            if (!pull_quarter(jq, n, h.x, h.y, u, esplit, pcol, color, ctl, spin_ticks, lane))
-                return;
-        } else {
-            for (uint32_t k = 0; k < kChunkPos && c0 + k < n; k++) {
-                const uint32_t src = k * kQuarter;
+        // large positions (more than kSmall predecessors) take the whole
+        // wave one at a time; the small ones between them go by groups, so
+        // every group phase only holds positions before the next large one
+        const uint64_t lm = __ballot(jq < n && h.y > kSmall);
+        uint32_t large = 0; // bit q: position c0 + q is large
+#pragma unroll
+        for (uint32_t k = 0; k < kChunkPos; k++)
+            large |= (uint32_t)((lm >> (k * kQuarter)) & 1ull) << k;
+        for (uint32_t k = 0; k < kChunkPos && c0 + k < n;) {
+            const uint32_t rest = large >> k;
+            const uint32_t f = rest ? k + (uint32_t)__builtin_ctz(rest) : kChunkPos;
+            if (f > k) { // small positions k .. f - 1
+                const bool act = q >= k && q < f && jq < n;
+                if (!pull_quarter(act, jq, h.x, h.y, u, esplit, pcol, color, ctl, spin_ticks,
+                                  lane))
+                    return;
+            }
+            if (f < kChunkPos && c0 + f < n) {
+                const uint32_t src = f * kQuarter;
                 const uint32_t beg = __shfl(h.x, src), np = __shfl(h.y, src),
-                               uk = __shfl(u, src);
+                               uf = __shfl(u, src);
                 const uint32_t c = pull_one(beg, np, bm, esplit, pcol, ctl, spin_ticks, lane);
                 if (c == 0u)
                     return;
                 if (lane == 0) {
-                    __hip_atomic_store(&pcol[c0 + k], c, __ATOMIC_RELAXED,
+                    __hip_atomic_store(&pcol[c0 + f], c, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
-                    color[uk] = (int)c;
+                    color[uf] = (int)c;
                 }
             }
+            k = f + 1u;
         }
         h = hn;
         u = un;

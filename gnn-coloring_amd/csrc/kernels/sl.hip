@@ -220,9 +220,13 @@ __device__ __forceinline__ uint32_t dec_row(uint32_t beg, uint32_t end, uint32_t
     return inv;
 }
 
-// Each block takes the removed vertices in groups of one per wave. A wave
-// decrements its own vertex's row; a row longer than kBigRow is done by the
-// whole block after the group's short rows.
+// In rounds that remove many vertices, rows of at most kLaneRow entries are
+// decremented by one thread, rows of at most kBigRow by one wave, longer
+// rows by the whole block (one wave per removed vertex made R-MAT 22's first
+// rounds, 1.8 M isolated and 0.55 M degree-1 vertices, take 0.6-0.7 ms each);
+// in the others every removed vertex gets a wave (a thread per vertex left
+// most of the grid idle in rounds that remove a few thousand long rows).
+constexpr uint32_t kLaneRow = 16;
 constexpr uint32_t kBigRow = 2048;
 
 __global__ __launch_bounds__(kBlock) void sl_dec_kernel(const uint32_t *__restrict__ V,
@@ -235,7 +239,8 @@ __global__ __launch_bounds__(kBlock) void sl_dec_kernel(const uint32_t *__restri
                                                         const uint32_t *__restrict__ perm,
                                                         uint32_t nslot, gcxk_ctl *ctl, uint32_t r)
 {
-    __shared__ uint32_t s_beg[kWavesPerBlock], s_end[kWavesPerBlock];
+    __shared__ uint32_t s_beg[kBlock], s_end[kBlock];
+    __shared__ uint32_t s_nrow;
     __shared__ uint32_t s_hub[kHubSlots];
     const uint32_t s = r % 3;
     const uint32_t n = ctl->sl_gone[s];
@@ -249,31 +254,71 @@ __global__ __launch_bounds__(kBlock) void sl_dec_kernel(const uint32_t *__restri
         hub_bit = nullptr;
     for (uint32_t i = threadIdx.x; i < nslot; i += kBlock)
         s_hub[i] = 0u;
-    __syncthreads();
     const int w = threadIdx.x / kWave;
     uint32_t inv = 0;
-    for (uint32_t g = blockIdx.x * kWavesPerBlock; g < n; g += gridDim.x * kWavesPerBlock) {
-        const uint32_t k = g + (uint32_t)w;
-        uint32_t beg = 0, end = 0;
-        if (k < n) {
-            const uint32_t u = gone[k];
-            beg = V[u];
-            end = V[u + 1];
+    if (n >= gridDim.x * (kBlock / 4u)) {
+        // many removed vertices (early rounds: mostly degree 0 or 1): a
+        // thread per removed vertex; rows of at most kLaneRow entries are
+        // walked by their own thread, longer ones are listed in LDS and taken
+        // by a wave each, rows past kBigRow by the whole block
+        for (uint32_t g = blockIdx.x * kBlock; g < n; g += gridDim.x * kBlock) {
+            if (threadIdx.x == 0)
+                s_nrow = 0u;
+            __syncthreads();
+            const uint32_t k = g + threadIdx.x;
+            uint32_t beg = 0, end = 0;
+            if (k < n) {
+                const uint32_t u = gone[k];
+                beg = V[u];
+                end = V[u + 1];
+            }
+            if (end - beg <= kLaneRow) {
+                inv = dec_row<kDecSteps>(beg, end, 0u, 1u, E, deg, key, hub_bit, rank, s_hub, inv);
+            } else {
+                const uint32_t slot = atomicAdd(&s_nrow, 1u);
+                s_beg[slot] = beg;
+                s_end[slot] = end;
+            }
+            __syncthreads();
+            const uint32_t m = s_nrow;
+            for (uint32_t j = (uint32_t)w; j < m; j += kWavesPerBlock)
+                if (s_end[j] - s_beg[j] <= kBigRow)
+                    inv = dec_row<kDecSteps>(s_beg[j], s_end[j], (uint32_t)lane_id(), kWave, E, deg,
+                                             key, hub_bit, rank, s_hub, inv);
+            for (uint32_t j = 0; j < m; j++)
+                if (s_end[j] - s_beg[j] > kBigRow) // block-uniform
+                    inv = dec_row<kDecSteps>(s_beg[j], s_end[j], threadIdx.x, kBlock, E, deg, key,
+                                             hub_bit, rank, s_hub, inv);
+            __syncthreads(); // s_nrow, s_beg and s_end are rewritten by the next group
         }
-        const bool big = end - beg > kBigRow;
-        if (!big)
-            inv = dec_row<kDecSteps>(beg, end, (uint32_t)lane_id(), kWave, E, deg, key, hub_bit,
-                                     rank, s_hub, inv);
-        if (lane_id() == 0) {
-            s_beg[w] = big ? beg : 0u;
-            s_end[w] = big ? end : 0u;
+    } else {
+        // few removed vertices (late rounds: long rows): a wave per removed
+        // vertex, so that every wave of the grid has a row; rows past kBigRow
+        // by the whole block after the group's short rows
+        __syncthreads(); // s_hub cleared
+        for (uint32_t g = blockIdx.x * kWavesPerBlock; g < n; g += gridDim.x * kWavesPerBlock) {
+            const uint32_t k = g + (uint32_t)w;
+            uint32_t beg = 0, end = 0;
+            if (k < n) {
+                const uint32_t u = gone[k];
+                beg = V[u];
+                end = V[u + 1];
+            }
+            const bool big = end - beg > kBigRow;
+            if (!big)
+                inv = dec_row<kDecSteps>(beg, end, (uint32_t)lane_id(), kWave, E, deg, key, hub_bit,
+                                         rank, s_hub, inv);
+            if (lane_id() == 0) {
+                s_beg[w] = big ? beg : 0u;
+                s_end[w] = big ? end : 0u;
+            }
+            __syncthreads();
+            for (int q = 0; q < kWavesPerBlock; q++)
+                if (s_end[q]) // block-uniform
+                    inv = dec_row<kDecSteps>(s_beg[q], s_end[q], threadIdx.x, kBlock, E, deg, key,
+                                             hub_bit, rank, s_hub, inv);
+            __syncthreads(); // s_beg / s_end are rewritten by the next group
         }
-        __syncthreads();
-        for (int q = 0; q < kWavesPerBlock; q++)
-            if (s_end[q]) // block-uniform
-                inv = dec_row<kDecSteps>(s_beg[q], s_end[q], threadIdx.x, kBlock, E, deg, key,
-                                         hub_bit, rank, s_hub, inv);
-        __syncthreads(); // s_beg / s_end are rewritten by the next group
     }
     for (uint32_t i = threadIdx.x; i < nslot; i += kBlock) {
         const uint32_t c = s_hub[i];
