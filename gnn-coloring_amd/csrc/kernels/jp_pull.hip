@@ -537,7 +537,7 @@ __device__ __forceinline__ bool pull_quarter(bool active, uint32_t jq, uint32_t 
             miss |= on && cv[t] == 0u ? 1u << t : 0u;
             m |= on && cv[t] != 0u && cv[t] <= 64u ? 1ull << (cv[t] - 1u) : 0ull;
         }
-        const uint32_t gm = (uint32_t)(__ballot(miss != 0u) >> sh) & 0xFFFFu;
+        const uint32_t gm = (uint32_t)(__ballot(miss != 0u) >> sh) & ((1u << kQuarter) - 1u);
 #pragma unroll
         for (int off = (int)kQuarter / 2; off > 0; off >>= 1) {
             const uint32_t lo = __shfl_xor((uint32_t)m, off), hi = __shfl_xor((uint32_t)(m >> 32), off);
@@ -565,7 +565,9 @@ __device__ __forceinline__ bool pull_quarter(bool active, uint32_t jq, uint32_t 
 
 // Positions [0, T), T = min(N, kSingleRounds * W): one position per wave at a
 // time, wave w taking w, w + W, ... Positions [T, N): chunks of kChunkPos
-// consecutive positions, wave w taking chunks w, w + W, ...; within a chunk
+// consecutive positions, wave w taking chunks w, w + W, ... (runs of 16 to
+// 256 consecutive positions per wave were 1.3-20x slower on R-MAT 22: a
+// wave's run waits on the runs before it); within a chunk
 // the positions with at most kSmall predecessors are coloured by 16-lane
 // groups at once and the others by the whole wave, in order. Each wave meets
 // its positions in increasing order either way.
@@ -599,7 +601,7 @@ __global__ __launch_bounds__(kBlock, 4) void jp_pull_kernel(uint32_t N, const ui
         h = hn;
         u = un;
     }
-    // chunks: group q = lane / 16 holds position c0 + q
+    // chunks: group q = lane / kQuarter holds position c0 + q
     const uint32_t q = (uint32_t)lane / kQuarter;
     const uint32_t stride = kChunkPos * nw;
     uint32_t c0 = T + kChunkPos * w0;
