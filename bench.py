@@ -28,19 +28,29 @@ sys.path.insert(0, os.path.join(ROOT, "gnn-coloring_amd"))
 METRIC = "GNN3+JP colored edges/s BF16 vs FP32 at equal #colors; serving p50/p99 ms"
 HBM_PEAK_GBS = 8000.0
 # kernels behind the timed stages (the JP stage's kernel depends on the schedule)
-KERNEL_OF_STAGE = {"jp_init": "jp_init_quads + jp_init_hub_chunks",
+KERNEL_OF_STAGE = {"jp_init": "pull_orient_quads + pull_orient_hub_chunks",
                    "gnn_hid": "sage_quads<1> + sage_hubs<1>", "gnn_out": "sage_quads<2> + sage_hubs<2>",
-                   "gnn_in": "sage_quads<0> + sage_hubs<0>"}
+                   "gnn_in": "sage_lanes_input + sage_hubs<0>"}
+# the JP stage's kernel by schedule
+JP_KERNEL = {"ordered": "jp_pull_kernel", "rounds": "jp_round_kernel"}
+
 
 # Algorithmic bytes per launch (model: every neighbour-row gather counted once,
 # 64-byte FP32 rows, 8-byte priorities; N vertices, M CSR entries).
-def algorithmic_bytes(stage, N, M, launches=1, R=64):
+# JP, ordered schedule (pulled colours): the orientation reads the greedy
+# order and scatters positions (8N), reads V (4N), E (4M) and each
+# neighbour's position (4M), writes the predecessor positions (M/2 of them,
+# 2M) and a header and colour slot per position (12N); the colouring kernel
+# reads a header and id per position (12N), each predecessor's position and
+# colour (M/2 of each, 4M) and writes two colours (8N). The round schedule:
+# orientation 8N + 12M, rounds 8N + 8M.
+def algorithmic_bytes(stage, N, M, launches=1, R=64, schedule="ordered"):
     per_stage = {
         "gnn_in": 4 * N + 12 * M + R * N,
         "gnn_hid": 4 * N + (4 + R) * M + 2 * R * N,
         "gnn_out": 4 * N + (4 + R) * M + R * N + 8 * N,
-        "jp_init": 8 * N + 12 * M,
-        "jp_rounds": 8 * N + 8 * M,
+        "jp_init": 24 * N + 10 * M if schedule == "ordered" else 8 * N + 12 * M,
+        "jp_rounds": 20 * N + 4 * M if schedule == "ordered" else 8 * N + 8 * M,
     }
     return per_stage[stage] / max(launches, 1)
 
@@ -392,6 +402,107 @@ def batching(args, gcx):
     return out
 
 
+# ------------------------------------------------ rocprofv3 database summaries
+#
+# `bench.py --profile-summary kernels.db [fetch.db write.db]` prints the
+# per-kernel table of a --kernel-trace run and, for --pmc runs, each counter
+# per dispatch (FETCH_SIZE / WRITE_SIZE in KiB); given the FETCH_SIZE and
+# WRITE_SIZE runs of the same command, also the HBM bytes per step of every
+# timed stage (JSON for profiles/traffic.json).
+# Counter calibration: the streaming copy kernel (copy_u32x4, 1 GiB read and
+# 1 GiB written per dispatch) gives the factor from each counter's KiB to
+# bytes.
+
+# kernels of each timed stage (FP32 GNN3 + ordered JP); the JP sort's radix
+# passes are left out (their rocPRIM kernels also build the CSR)
+STAGE_KERNELS = {
+    "gnn_in": ("sage_lanes_input", "sage_hubs<0>"),
+    "gnn_hid": ("sage_quads<1>", "sage_hubs<1>"),
+    "gnn_out": ("sage_quads<2>", "sage_hubs<2>"),
+    "jp_init": ("pull_clear", "pull_rank", "pull_orient_quads", "pull_orient_hub_chunks",
+                "pull_hub_headers", "jp_order_keys"),
+    "jp_rounds": ("jp_pull_kernel",),
+}
+
+
+def _short_kernel(n):
+    n = n.replace("(anonymous namespace)::", "")
+    depth = 0
+    for i in range(len(n) - 1, -1, -1):  # drop the argument list
+        if n[i] == ")":
+            depth += 1
+        elif n[i] == "(":
+            depth -= 1
+            if depth == 0:
+                n = n[:i]
+                break
+    n = n.replace("void ", "")
+    if "rocprim::" in n:
+        n = "rocprim " + ("radix sort" if "sort" in n else "scan" if "scan" in n else "kernel")
+    return n
+
+
+def profile_summary(paths):
+    import sqlite3
+    out = {}
+    for p in paths:
+        c = sqlite3.connect(p)
+        views = {r[0] for r in c.execute("select name from sqlite_master")}
+        pmc = "pmc_events" in views and c.execute("select count(*) from pmc_events").fetchone()[0]
+        if pmc:
+            # a counter may come as several rows per dispatch (one per hardware
+            # instance): sum them per dispatch, then average over dispatches
+            rows = c.execute("select name, counter_name, count(*), avg(v) from (select name, "
+                             "counter_name, dispatch_id, sum(counter_value) as v from pmc_events "
+                             "group by name, counter_name, dispatch_id) group by name, "
+                             "counter_name").fetchall()
+            agg = {}
+            for n, cn, k, v in rows:
+                key = (_short_kernel(n), cn)
+                a = agg.setdefault(key, [0, 0.0])
+                a[1] = (a[1] * a[0] + v * k) / (a[0] + k)
+                a[0] += k
+            print(f"\n### {os.path.basename(p)}: counter averages per dispatch\n")
+            print("| kernel | counter | dispatches | value per dispatch |")
+            print("|---|---|---|---|")
+            for (n, cn), (k, v) in sorted(agg.items(), key=lambda x: -x[1][1] * x[1][0]):
+                print(f"| `{n}` | {cn} | {k} | {v:.1f} |")
+            for (n, cn), (k, v) in agg.items():
+                out.setdefault(cn, {})[n] = v
+        else:
+            rows = c.execute("select name, count(*), sum(duration), avg(duration), min(duration), "
+                             "max(duration) from kernels group by name").fetchall()
+            agg = {}
+            for n, k, tot, a, mi, ma in rows:
+                sn = _short_kernel(n)
+                g = agg.setdefault(sn, [0, 0.0, float("inf"), 0.0])
+                g[0] += k
+                g[1] += tot
+                g[2] = min(g[2], mi)
+                g[3] = max(g[3], ma)
+            total = sum(g[1] for g in agg.values())
+            print(f"\n### {os.path.basename(p)}: kernel times\n")
+            print("| kernel | calls | total ms | avg us | min us | max us | % of GPU time |")
+            print("|---|---|---|---|---|---|---|")
+            for n, (k, tot, mi, ma) in sorted(agg.items(), key=lambda x: -x[1][1])[:45]:
+                print(f"| `{n}` | {k} | {tot / 1e6:.3f} | {tot / k / 1e3:.1f} | {mi / 1e3:.1f} | "
+                      f"{ma / 1e3:.1f} | {100 * tot / total:.1f} |")
+    if "FETCH_SIZE" in out and "WRITE_SIZE" in out:
+        f, w = out["FETCH_SIZE"], out["WRITE_SIZE"]
+        ff = (1 << 30) / (1024.0 * f["copy_u32x4"]) if f.get("copy_u32x4") else 1.0
+        fw = (1 << 30) / (1024.0 * w["copy_u32x4"]) if w.get("copy_u32x4") else 1.0
+        print(f"\ncalibration (copy_u32x4, 1 GiB each way): FETCH_SIZE x{ff:.3f}, WRITE_SIZE x{fw:.3f}")
+        traffic = {}
+        for st, ks in STAGE_KERNELS.items():
+            b = 0.0
+            for kn in set(f) | set(w):
+                if any(kn.startswith(k) for k in ks):
+                    b += 1024.0 * (f.get(kn, 0.0) * ff + w.get(kn, 0.0) * fw)
+            traffic[st] = int(b)
+        print("stage HBM bytes per step (profiles/traffic.json, key rmat<scale>_<stage>):")
+        print(json.dumps(traffic, indent=1))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -423,7 +534,13 @@ def main():
                     help="longest head prefix of the ordered schedule (0 = off; default: the "
                          "library's)")
     ap.add_argument("--head-density", type=int, default=64)
+    ap.add_argument("--profile-summary", nargs="+", metavar="DB",
+                    help="summarise rocprofv3 databases (kernel trace; FETCH_SIZE and WRITE_SIZE "
+                         "runs) and exit")
     args = ap.parse_args()
+    if args.profile_summary:
+        profile_summary(args.profile_summary)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is "
               "described in DESIGN.md and not built", file=sys.stderr)
@@ -520,7 +637,8 @@ def main():
     else:
         n_launch = counts[dominant]
     avg_ms = stage_ms[dominant] / max(n_launch, 1)
-    per_launch = algorithmic_bytes(dominant, N, M, launches if dominant == "jp_rounds" else 1)
+    per_launch = algorithmic_bytes(dominant, N, M, launches if dominant == "jp_rounds" else 1,
+                                   schedule=best)
     achieved = per_launch / (avg_ms * 1e-3) / 1e9
     prof_path = os.path.join(ROOT, "profiles", "traffic.json")
     tmap = json.load(open(prof_path)) if os.path.exists(prof_path) else {}
@@ -531,7 +649,7 @@ def main():
         if stage_ms[k] <= 0 or k not in ("gnn_in", "gnn_hid", "gnn_out", "jp_init", "jp_rounds"):
             continue
         nl = launches * args.steps if k == "jp_rounds" else counts[k]
-        per = algorithmic_bytes(k, N, M, launches if k == "jp_rounds" else 1)
+        per = algorithmic_bytes(k, N, M, launches if k == "jp_rounds" else 1, schedule=best)
         ms_l = stage_ms[k] / max(nl, 1)
         t = tmap.get(f"rmat{args.scale}_{k}")
         stage_roof[k] = {"ms_per_launch": ms_l, "algorithmic_gbs": per / (ms_l * 1e-3) / 1e9,
@@ -596,8 +714,8 @@ def main():
                                  "and chunk table, rank and Er = rank[E]"},
         "schedule_warmup_ms": sched_ms,
         "roofline": {"bound": "hbm",
-                     "kernel": ("jp_ordered_kernel" if best == "ordered" else "jp_round_kernel")
-                     if dominant == "jp_rounds" else KERNEL_OF_STAGE.get(dominant, dominant),
+                     "kernel": JP_KERNEL[best] if dominant == "jp_rounds"
+                     else KERNEL_OF_STAGE.get(dominant, dominant),
                      "stage": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "avg_launch_ms": avg_ms, "launches": n_launch,
