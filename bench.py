@@ -135,14 +135,25 @@ def serving(args, gcx):
     k = args.serving_requests
     base = gcx.serving_trace(k, seed=11, nmax=args.serving_nmax)
 
+    express_bytes = int(args.serving_express_mb * (1 << 20))
+
     def summary(rep, wall):
         ok = [r for r in rep if r.status == 0]
         lat = np.array([1000.0 * (r.t_done - r.t_arrival) for r in ok])
         edges = float(sum(r.M for r in ok))
-        return {"requests": len(rep), "ok": len(ok), "seconds": wall,
-                "graphs_per_s": len(ok) / wall, "colored_edges_per_s": edges / wall,
-                "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-                "mean_ms": float(lat.mean())}
+        out = {"requests": len(rep), "ok": len(ok), "seconds": wall,
+               "graphs_per_s": len(ok) / wall, "colored_edges_per_s": edges / wall,
+               "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+               "mean_ms": float(lat.mean())}
+        # latency by size class: jobs of at most the express footprint, and the rest
+        small = np.array([r.footprint <= express_bytes for r in ok])
+        for name, m in (("small", small), ("large", ~small)):
+            if m.any():
+                out[f"{name}_jobs"] = int(m.sum())
+                out[f"{name}_p50_ms"] = float(np.percentile(lat[m], 50))
+                out[f"{name}_p99_ms"] = float(np.percentile(lat[m], 99))
+        out["mean_wait_ms"] = float(np.mean([1000.0 * (r.t_admit - r.t_arrival) for r in ok]))
+        return out
 
     def run(srv, reqs):
         t0 = time.perf_counter()
@@ -180,6 +191,11 @@ def serving(args, gcx):
     rep2, wall2 = run(srv, gcx.with_arrivals(base, rate, seed=12))
     lat = summary(rep2, wall2)
     peak = srv.peak_reserved()
+    # the same arrivals with express workers: small jobs may overtake large ones
+    srv.set_express(args.serving_express_workers, express_bytes)
+    rep4, wall4 = run(srv, gcx.with_arrivals(base, rate, seed=12))
+    lat_x = summary(rep4, wall4)
+    srv.set_express(0, 0)
     srv.close()
     gcx.set_ordered_launch(0)
     plain = gcx.Server(workers=1, budget=int(args.serving_budget_gb * (1 << 30)))
@@ -188,16 +204,20 @@ def serving(args, gcx):
     fp32 = summary(rep3, wall3)
     plain.close()
     gcx.set_ordered_launch(prev)
-    same = sum(1 for a, b in zip(rep, rep2) if (a.colors, a.digest) == (b.colors, b.digest))
+    same = sum(1 for a, b, c in zip(rep, rep2, rep4)
+               if (a.colors, a.digest) == (b.colors, b.digest) == (c.colors, c.digest))
     worse = [b.colors - a.colors for a, b in zip(rep3, rep) if a.status == 0 and b.status == 0]
     return {"requests": k, "workers": workers, "ordered_launch_slots": args.serving_mixed_slots,
             "budget_gb": args.serving_budget_gb, "peak_reserved_gb": peak / (1 << 30),
             "trace": f"seed 11, n log-uniform [1e3, {args.serving_nmax:.0e}], 40% R-MAT / 30% ER "
                      "(degree 8-64) / 30% RGG (degree ~12), FF/LF/SL/GNN2/GNN3, GNN in BF16",
             "bf16_capacity": cap, "bf16_poisson": dict(lat, rate_per_s=rate),
+            "bf16_poisson_express": dict(lat_x, rate_per_s=rate,
+                                         express_workers=args.serving_express_workers,
+                                         express_max_footprint_mb=args.serving_express_mb),
             "plain_fp32_one_at_a_time": fp32,
             "capacity_speedup_vs_plain_fp32": cap["colored_edges_per_s"] / fp32["colored_edges_per_s"],
-            "same_result_queued_vs_poisson": f"{same}/{k}",
+            "same_result_queued_vs_poisson_vs_express": f"{same}/{k}",
             "bf16_minus_fp32_colors_max": max(worse) if worse else None}
 
 
@@ -526,6 +546,11 @@ def main():
     ap.add_argument("--serving-mixed-slots", type=int, default=4,
                     help="ordered-launch slots of the mixed-trace BF16 passes")
     ap.add_argument("--serving-budget-gb", type=float, default=48.0)
+    ap.add_argument("--serving-express-workers", type=int, default=4,
+                    help="mixed-trace workers reserved for small jobs in the express Poisson pass")
+    ap.add_argument("--serving-express-mb", type=float, default=256.0,
+                    help="largest job footprint (MiB) the express workers admit; also the "
+                         "small/large split of the latency report")
     ap.add_argument("--serving-rate", type=float, default=0.0,
                     help="run only a Poisson pass at this many requests/s (0 = capacity, "
                          "70%% Poisson and plain FP32 passes)")

@@ -4,7 +4,10 @@
  * The submitting thread queues each request at its arrival time; worker
  * threads take the queue head when its footprint fits the remaining budget
  * (first come, first served: a large head job waits for memory rather than
- * being overtaken) and run generate -> order -> colour -> validate -> digest
+ * being overtaken) -- except express workers (gcx_server_set_express), which
+ * take the earliest queued job whose footprint is at most the express size,
+ * so small jobs are not held behind multi-million-vertex ones -- and run
+ * generate -> order -> colour -> validate -> digest
  * on their own stream, then free the job's memory in stream order so no call
  * synchronises the whole device. Jobs on different workers overlap on the
  * GPU; ordered JP launches are serialised by the library itself.
@@ -36,6 +39,8 @@ struct gcx_server {
     const gcx_graph **resident;
     uint32_t nresident, resident_cap;
     int max_batch; /* continuous batching of resident-graph jobs (<= 1: off) */
+    int express_workers;    /* workers 0 .. express_workers-1 take small jobs only */
+    uint64_t express_bytes; /* ... those with a footprint of at most this */
     pthread_mutex_t lock;
     pthread_cond_t cv;      /* queue / memory changed */
     pthread_cond_t done_cv; /* a job finished */
@@ -44,6 +49,7 @@ struct gcx_server {
     const gcx_request *req;
     gcx_reply *rep;
     size_t *queue, head, tail, done, n;
+    unsigned char *taken; /* taken[k]: queue entry k was admitted (express workers skip ahead) */
     uint64_t in_use, peak;
     struct timespec t0;
 };
@@ -236,11 +242,27 @@ static void run_batch(worker *w, const size_t *idx, int n)
     (void)gcx_graph_free_on(pack, s);
 }
 
-static int head_fits(const gcx_server *S)
+/* Queue position of the job worker w may admit now, or -1: the head when it
+ * fits the budget; for an express worker, the earliest job of at most the
+ * express size that fits together with the head. Express admissions thus
+ * never hold the head back for memory: while the head waits for memory,
+ * the queue is first come, first served for every worker. */
+static long pick_job(gcx_server *S, const worker *w)
 {
+    while (S->head < S->tail && S->taken[S->head])
+        S->head++;
     if (S->head >= S->tail)
-        return 0;
-    return S->in_use + S->rep[S->queue[S->head]].footprint <= S->budget;
+        return -1;
+    const uint64_t head_fp = S->rep[S->queue[S->head]].footprint;
+    if (w->id >= S->express_workers)
+        return S->in_use + head_fp <= S->budget ? (long)S->head : -1;
+    for (size_t k = S->head; k < S->tail; k++) {
+        const uint64_t fp = S->rep[S->queue[k]].footprint;
+        const uint64_t need = S->in_use + fp + (k == S->head ? 0 : head_fp);
+        if (!S->taken[k] && fp <= S->express_bytes && need <= S->budget)
+            return (long)k;
+    }
+    return -1;
 }
 
 static void *worker_main(void *arg)
@@ -249,23 +271,28 @@ static void *worker_main(void *arg)
     gcx_server *S = w->srv;
     pthread_mutex_lock(&S->lock);
     for (;;) {
-        while (!S->stop && !head_fits(S))
+        long k0 = -1;
+        while (!S->stop && (k0 = pick_job(S, w)) < 0)
             pthread_cond_wait(&S->cv, &S->lock);
         if (S->stop)
             break;
-        const size_t i = S->queue[S->head++];
+        S->taken[k0] = 1;
+        const size_t i = S->queue[k0];
         gcx_reply *rp = &S->rep[i];
         size_t idx[MAX_BATCH];
         int n = 1;
         idx[0] = i;
         uint64_t reserve = rp->footprint;
-        if (S->max_batch > 1 && resident_graph(S, &S->req[i])) {
+        if (S->max_batch > 1 && w->id >= S->express_workers && resident_graph(S, &S->req[i])) {
             /* take the compatible jobs queued right behind it while the
              * packed pass fits the budget */
             const gcx_graph *g0 = resident_graph(S, &S->req[i]);
             uint64_t nv = g0->N, ne = g0->M;
-            while (n < S->max_batch && n < MAX_BATCH && S->head < S->tail) {
-                const size_t j = S->queue[S->head];
+            for (size_t k = (size_t)k0 + 1; n < S->max_batch && n < MAX_BATCH && k < S->tail;
+                 k++) {
+                if (S->taken[k])
+                    continue;
+                const size_t j = S->queue[k];
                 const gcx_graph *gj = resident_graph(S, &S->req[j]);
                 if (!batchable(S, &S->req[i], &S->req[j]) || nv + gj->N >= UINT32_MAX ||
                     ne + gj->M >= UINT32_MAX ||
@@ -274,7 +301,7 @@ static void *worker_main(void *arg)
                 nv += gj->N;
                 ne += gj->M;
                 idx[n++] = j;
-                S->head++;
+                S->taken[k] = 1;
             }
             if (n > 1)
                 reserve = pack_footprint(nv, ne);
@@ -371,8 +398,12 @@ int gcx_server_run(gcx_server *S, const gcx_request *reqs, size_t n, gcx_reply *
         if (reqs[i].arrival < reqs[i - 1].arrival)
             return GCX_ERR_ARG;
     size_t *queue = malloc(sizeof(size_t) * (n ? n : 1));
-    if (!queue)
+    unsigned char *taken = calloc(n ? n : 1, 1);
+    if (!queue || !taken) {
+        free(queue);
+        free(taken);
         return GCX_ERR_NOMEM;
+    }
     memset(replies, 0, sizeof(gcx_reply) * n);
     for (size_t i = 0; i < n; i++)
         replies[i].footprint = job_footprint(S, &reqs[i]);
@@ -380,6 +411,7 @@ int gcx_server_run(gcx_server *S, const gcx_request *reqs, size_t n, gcx_reply *
     S->req = reqs;
     S->rep = replies;
     S->queue = queue;
+    S->taken = taken;
     S->head = S->tail = S->done = 0;
     S->n = n;
     S->in_use = S->peak = 0;
@@ -414,10 +446,12 @@ int gcx_server_run(gcx_server *S, const gcx_request *reqs, size_t n, gcx_reply *
     while (S->done < n)
         pthread_cond_wait(&S->done_cv, &S->lock);
     S->queue = NULL;
+    S->taken = NULL;
     S->req = NULL;
     S->rep = NULL;
     pthread_mutex_unlock(&S->lock);
     free(queue);
+    free(taken);
     return GCX_OK;
 }
 
@@ -461,6 +495,20 @@ int gcx_server_set_batching(gcx_server *S, int max_batch)
     const int active = S->req != NULL;
     if (!active)
         S->max_batch = max_batch;
+    pthread_mutex_unlock(&S->lock);
+    return active ? GCX_ERR_ARG : GCX_OK;
+}
+
+int gcx_server_set_express(gcx_server *S, int workers, uint64_t max_footprint)
+{
+    if (!S || workers < 0 || workers >= S->nworkers)
+        return GCX_ERR_ARG;
+    pthread_mutex_lock(&S->lock);
+    const int active = S->req != NULL;
+    if (!active) {
+        S->express_workers = workers;
+        S->express_bytes = max_footprint;
+    }
     pthread_mutex_unlock(&S->lock);
     return active ? GCX_ERR_ARG : GCX_OK;
 }

@@ -106,6 +106,7 @@ def _load():
         "gcx_server_peak_reserved": (C.c_uint64, [vp]),
         "gcx_server_add_graph": (C.c_int, [vp, vp, u32p]),
         "gcx_server_set_batching": (C.c_int, [vp, C.c_int]),
+        "gcx_server_set_express": (C.c_int, [vp, C.c_int, C.c_uint64]),
         "gcx_set_ordered_launch": (C.c_int, [C.c_int]),
         "gcx_degree_signature": (C.c_uint32, [vp]),
         "gcx_tune": (C.c_int, [vp, vp, C.c_int, vp, vp]),
@@ -897,6 +898,11 @@ class Server:
         """Pack up to max_batch compatible queued resident-graph jobs into one
         pass (0 or 1 = off)."""
         _chk(_lib.gcx_server_set_batching(self.handle, max_batch), "server batching")
+
+    def set_express(self, workers: int, max_footprint: int):
+        """Reserve workers 0 .. workers-1 for jobs of at most max_footprint
+        bytes, taken in arrival order ahead of larger queued jobs (0 = off)."""
+        _chk(_lib.gcx_server_set_express(self.handle, workers, max_footprint), "server express")
 
     def run(self, requests):
         """Run requests (sorted by arrival); returns the list of Reply."""

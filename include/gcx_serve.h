@@ -85,6 +85,14 @@ uint64_t gcx_server_peak_reserved(const gcx_server *s);
  * are those of the jobs alone (per-graph colour count and digest). Not while
  * a run is active. */
 int gcx_server_set_batching(gcx_server *s, int max_batch);
+/* Express workers (0 = off, the default): workers 0 .. workers - 1 admit
+ * only jobs whose footprint (gcx_request_footprint, or the workspace of a
+ * resident-graph job) is at most max_footprint bytes, taking the earliest
+ * such job in the queue even when larger jobs arrived before it; the other
+ * workers keep first-come-first-served order. Small jobs then do not wait
+ * behind multi-million-vertex builds for a worker. workers must be below the
+ * server's worker count. Not while a run is active. */
+int gcx_server_set_express(gcx_server *s, int workers, uint64_t max_footprint);
 
 #ifdef __cplusplus
 }

@@ -176,3 +176,31 @@ def test_serve_batching_matches_alone():
     out = srv.run(reqs[:8])
     assert [(r.colors, r.digest) for r in out] == [(r.colors, r.digest) for r in alone[:8]]
     srv.close()
+
+
+def test_serve_express_small_jobs_overtake():
+    """Express workers take small jobs ahead of larger queued ones: replies
+    equal the first-come-first-served run, large jobs never run on an
+    express worker, and the small jobs queued behind three large builds
+    finish before the last large one is admitted."""
+    large = [gcx.make_request(gcx.GEN_RMAT, 1 << 20, 16.0, seed=31 + i, heuristic="FF")
+             for i in range(3)]
+    small = [gcx.make_request(gcx.GEN_ER, 2000 + 500 * i, 8.0, seed=41 + i, heuristic="LF")
+             for i in range(4)]
+    reqs = large + small  # all queued at t = 0, large first
+    limit = max(gcx.request_footprint(q) for q in small)
+    assert limit < min(gcx.request_footprint(q) for q in large)
+    srv = gcx.Server(workers=2, budget=16 << 30)
+    with pytest.raises(gcx.GcxError):
+        srv.set_express(2, limit)  # needs at least one general worker
+    fifo = srv.run(reqs)
+    srv.set_express(1, limit)
+    expr = srv.run(reqs)
+    srv.set_express(0, 0)
+    srv.close()
+    assert all(r.status == 0 for r in fifo + expr)
+    assert [(r.colors, r.digest) for r in fifo] == [(r.colors, r.digest) for r in expr]
+    assert all(r.worker == 1 for r in expr[:3])
+    last_large_admit = max(r.t_admit for r in expr[:3])
+    assert all(r.t_done <= last_large_admit for r in expr[3:]), \
+        [(r.t_admit, r.t_done) for r in expr]
