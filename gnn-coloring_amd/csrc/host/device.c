@@ -1226,11 +1226,15 @@ static int sl_round(void *p, uint32_t r, hipStream_t s)
 {
     sl_ctx *c = (sl_ctx *)p;
     gcx_workspace *ws = c->ws;
-    TRY(gcxk_sl_scan(ws->count, (uint32_t *)ws->key, ws->q[r & 1], ws->q[(r + 1) & 1], ws->h[0],
-                     ws->ctl, c->log_variant, r, c->grid, s));
-    return gcxk_sl_dec(c->g->V, c->g->E, ws->count, (uint32_t *)ws->key, ws->h[0], ws->h[1],
-                       c->g->rank, c->g->perm, gcxk_sl_hub_slots(c->g->N), ws->ctl, r, c->grid,
-                       s);
+    const gcx_graph *g = c->g;
+    uint32_t *keyr = (uint32_t *)ws->key + (g->N + 1);
+    /* long rows' chunks (gcxk_sl_chunk_capacity(M) pairs) in E_split, which
+     * only the colouring uses */
+    TRY(gcxk_sl_scan(g->V, ws->count, keyr, (uint32_t *)ws->key, g->perm, ws->q[r & 1],
+                     ws->q[(r + 1) & 1], ws->h[0], ws->esplit, ws->ctl, c->log_variant, r, c->grid,
+                     s));
+    return gcxk_sl_dec(g->N, g->V, g->Er, ws->count, keyr, ws->h[0], ws->esplit, ws->ctl,
+                       c->log_variant, r, s);
 }
 
 /* Persistent launches (launch_ordered, below) */
@@ -1239,8 +1243,10 @@ enum { LAUNCH_JP_PUSH = 0, LAUNCH_JP_PULL = 1 };
 static int order_sl(gcx_workspace *ws, const gcx_graph *g, int log_variant, hipStream_t s)
 {
     HIPC(hipMemsetAsync(ws->ctl, 0, sizeof(gcxk_ctl), s));
-    TRY(gcxk_sl_init(g->N, g->V, ws->count, (uint32_t *)ws->key, ws->q[0], ws->ctl, s));
-    TRY(gcxk_sl_hub_bits(g->N, g->perm, ws->h[1], s));
+    /* degrees and alive flags by degree-order rank: count and the upper half
+     * of key (key holds 2 (N + 1) words) */
+    TRY(gcxk_sl_init(g->N, g->V, g->perm, ws->count, (uint32_t *)ws->key + (g->N + 1), ws->q[0],
+                     ws->ctl, s));
     sl_ctx c = {ws, g, log_variant, g_round_grid};
     TRY(drive_rounds(ws, s, sl_round, &c, offsetof(gcxk_ctl, sl_done), g->N + 2 * ROUND_BATCH + 2));
     return GCX_OK;

@@ -28,7 +28,7 @@ typedef struct {
     uint32_t nan_count;     /* priorities that are NaN (rejected) */
     uint32_t nonfloat;      /* priorities not exactly representable as float */
     uint32_t sl_alive[3];   /* SL: alive-list length, slot = round % 3 */
-    uint32_t sl_gone[3];    /* SL: removed-this-round list length */
+    uint32_t sl_gone[3];    /* SL: removed-this-round short-row list length */
     uint32_t sl_mininv[3];  /* SL: UINT32_MAX - min remaining degree */
     uint32_t sl_delta[3];   /* SL: threshold of the round */
     uint32_t sl_rounds;
@@ -36,6 +36,7 @@ typedef struct {
     uint32_t total_items;   /* JP (ordered): work items in greedy order */
     uint32_t err;           /* JP (ordered): a wave hit its wait limit */
     uint32_t head_n;        /* JP (ordered): greedy-order positions the head kernel colours */
+    uint32_t sl_chunks[3];  /* SL: removed-this-round long-row chunk list length */
 } gcxk_ctl;
 
 /* ---- graph helpers (csr.hip) ---- */
@@ -129,26 +130,28 @@ int gcxk_keys64_to_doubles(uint32_t N, const uint64_t *key, double *p, void *str
 int gcxk_copy_bytes(uint64_t bytes, const void *src, void *dst, void *stream);
 
 /* ---- SL / SLL peeling (sl.hip) ---- */
-/* key[u] = UINT32_MAX marks "not yet removed"; a removed vertex's key is the
- * round threshold (delta) at its removal. ctl must be zeroed. */
-int gcxk_sl_init(uint32_t N, const uint32_t *dV, uint32_t *deg, uint32_t *key, uint32_t *alive0,
-                 gcxk_ctl *ctl, void *stream);
-/* Round r, phase 1: remove alive vertices with degree <= threshold. */
-int gcxk_sl_scan(uint32_t *deg, uint32_t *key, const uint32_t *alive_in, uint32_t *alive_out,
-                 uint32_t *gone, gcxk_ctl *ctl, int log_variant, uint32_t r, uint32_t grid,
-                 void *stream);
-/* Round r, phase 2: decrement unremoved neighbours of removed vertices.
- * Decrements to the vertices perm[0 .. nslot) (the highest degrees; hub_bit
- * marks them, rank maps them to their slot) are summed per workgroup first;
- * nslot is capped at gcxk_sl_hub_slots(N); hub_bit = NULL disables this. */
-int gcxk_sl_dec(const uint32_t *dV, const uint32_t *dE, uint32_t *deg, const uint32_t *key,
-                const uint32_t *gone, const uint32_t *hub_bit, const uint32_t *rank,
-                const uint32_t *perm, uint32_t nslot, gcxk_ctl *ctl, uint32_t r, uint32_t grid,
-                void *stream);
-/* Slots of the per-workgroup decrement sums (min(N, 1024)). */
-uint32_t gcxk_sl_hub_slots(uint32_t N);
-/* hub_bit (N / 32 + 1 words): bit v set for v in perm[0 .. gcxk_sl_hub_slots(N)). */
-int gcxk_sl_hub_bits(uint32_t N, const uint32_t *perm, uint32_t *hub_bit, void *stream);
+/* The peel's state is kept at degree-order ranks (perm / rank): degr and
+ * keyr (N entries each) are the degrees and alive flags by rank (keyr[a] =
+ * UINT32_MAX: alive, else the threshold at removal), the alive lists hold
+ * ranks; key (by vertex id) receives each vertex's threshold (delta) at its
+ * removal. ctl must be zeroed. */
+int gcxk_sl_init(uint32_t N, const uint32_t *dV, const uint32_t *perm, uint32_t *degr,
+                 uint32_t *keyr, uint32_t *alive0, gcxk_ctl *ctl, void *stream);
+/* Round r, phase 1: remove alive vertices with degree <= threshold. gone
+ * receives the removed vertices whose rows have 1 .. 16 entries, chunks
+ * (capacity gcxk_sl_chunk_capacity(M) pairs) the CSR ranges [begin, end) of
+ * at most 64 entries that cover the longer rows. */
+int gcxk_sl_scan(const uint32_t *dV, const uint32_t *degr, uint32_t *keyr, uint32_t *key,
+                 const uint32_t *perm, const uint32_t *alive_in, uint32_t *alive_out,
+                 uint32_t *gone, void *chunks, gcxk_ctl *ctl, int log_variant, uint32_t r,
+                 uint32_t grid, void *stream);
+uint64_t gcxk_sl_chunk_capacity(uint64_t M);
+/* Round r, phase 2: decrement unremoved neighbours of removed vertices,
+ * reading neighbour ranks from Er = rank[E]. Decrements to the first 32768
+ * ranks are summed per workgroup in LDS and applied once per rank. */
+int gcxk_sl_dec(uint32_t N, const uint32_t *dV, const uint32_t *Er, uint32_t *degr,
+                const uint32_t *keyr, const uint32_t *gone, const void *chunks, gcxk_ctl *ctl,
+                int log_variant, uint32_t r, void *stream);
 
 /* ---- GraphSAGE-16 (gnn.hip) ---- */
 /* FP32, bit-exact with the reference op order. kind 0 = input layer (reads

@@ -11,15 +11,41 @@
 // The min is kept inverted (UINT32_MAX - m) so that a zeroed slot is the
 // identity of atomicMax. Slots rotate by round % 3 so that each kernel resets
 // only a slot no kernel of the current round touches.
+//
+// Rank space: the peel keeps its per-vertex state (degrees, alive flags,
+// alive lists) at degree-order ranks (perm / rank: degree descending, ties
+// by id) and reads neighbours from Er = rank[E], so the highest-degree
+// vertices are the first ranks. They take most decrements: on R-MAT 22,
+// 78% of SL's 61.5 M decrements go to the 32768 first ranks, and in the
+// heavy late rounds all of them (round 221: 4.7 M decrements onto 16.7 K
+// vertices). Every workgroup sums its decrements to the first kSumSlots
+// ranks in LDS and applies each sum with one atomic, adjacent ranks from
+// adjacent lanes; only decrements to later ranks are device atomics. On
+// MI355X, device-scope atomics scattered over a few thousand hot addresses
+// run at about 4.5 G/s; summed in LDS and flushed in rank order the same
+// decrements run at about 170 G/s.
 #include "common.hip.h"
 
 namespace {
 
-constexpr int kBlock = 256;
-constexpr int kWavesPerBlock = kBlock / kWave;
+constexpr int kBlock = 256;   // init and scan
+constexpr int kDBlock = 1024; // decrements: one workgroup per CU or per two
+constexpr int kDWaves = kDBlock / kWave;
 constexpr uint32_t kAlive = UINT32_MAX;
 constexpr int kDecSteps = 4;
 constexpr int kScanPer = 8;
+// LDS decrement sums: two ranks per 32-bit word (16-bit halves), 64 KB (a
+// 128 KB table of 65536 ranks was 0.8 ms slower on R-MAT 22 SL at one
+// workgroup per CU). A workgroup's sum for one rank is at most the number
+// of removed rows it handles (one entry per row and neighbour in a simple
+// graph); the sums are used only while that bound stays below 2^16, so the
+// low half never carries into the high one.
+constexpr uint32_t kSumWords = 16384;
+constexpr uint32_t kSumSlots = 2 * kSumWords;
+constexpr uint64_t kMaxRowsPerGroup = 65535;
+constexpr uint32_t kLaneRow = 16;   // rows up to this length: a thread each
+constexpr uint32_t kChunk = kWave;  // longer rows: chunks of a lane per entry
+constexpr int kChunkSteps = 4;
 
 __device__ __forceinline__ void wave_min_inv(uint32_t m, uint32_t *slot)
 {
@@ -29,22 +55,30 @@ __device__ __forceinline__ void wave_min_inv(uint32_t m, uint32_t *slot)
         atomicMax(slot, m);
 }
 
+__device__ __forceinline__ void fold_min(uint32_t d, uint32_t &inv)
+{
+    const uint32_t m = UINT32_MAX - d;
+    inv = m > inv ? m : inv;
+}
+
 __global__ __launch_bounds__(kBlock) void sl_init_kernel(uint32_t N, const uint32_t *__restrict__ V,
-                                                         uint32_t *__restrict__ deg,
-                                                         uint32_t *__restrict__ key,
+                                                         const uint32_t *__restrict__ perm,
+                                                         uint32_t *__restrict__ degr,
+                                                         uint32_t *__restrict__ keyr,
                                                          uint32_t *__restrict__ alive0,
                                                          gcxk_ctl *ctl)
 {
-    const uint32_t u = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t a = blockIdx.x * kBlock + threadIdx.x;
     uint32_t inv = 0;
-    if (u < N) {
+    if (a < N) {
+        const uint32_t u = perm[a];
         const uint32_t d = V[u + 1] - V[u];
-        deg[u] = d;
-        key[u] = kAlive;
-        alive0[u] = u;
+        degr[a] = d;
+        keyr[a] = kAlive;
+        alive0[a] = a;
         inv = UINT32_MAX - d;
     }
-    if (u == 0) {
+    if (a == 0) {
         ctl->sl_alive[0] = N;
     }
     // Min degree goes to slot 2 (the slot of "round -1").
@@ -69,11 +103,19 @@ __device__ __forceinline__ uint32_t round_delta(const gcxk_ctl *ctl, int log_var
     return prev + 1 > mn ? prev + 1 : mn;
 }
 
-__global__ __launch_bounds__(kBlock) void sl_scan_kernel(uint32_t *__restrict__ deg,
+// alive_in / alive_out hold ranks. A removed vertex's threshold goes to key
+// (by id); its row goes to gone when it has 1 .. kLaneRow entries, else to
+// chunks as ranges of at most kChunk entries, so that the decrement pass has
+// only equal work items and no workgroup-wide steps.
+__global__ __launch_bounds__(kBlock) void sl_scan_kernel(const uint32_t *__restrict__ V,
+                                                         const uint32_t *__restrict__ degr,
+                                                         uint32_t *__restrict__ keyr,
                                                          uint32_t *__restrict__ key,
+                                                         const uint32_t *__restrict__ perm,
                                                          const uint32_t *__restrict__ alive_in,
                                                          uint32_t *__restrict__ alive_out,
-                                                         uint32_t *__restrict__ gone, gcxk_ctl *ctl,
+                                                         uint32_t *__restrict__ gone,
+                                                         uint2 *__restrict__ chunks, gcxk_ctl *ctl,
                                                          int log_variant, uint32_t r)
 {
     const uint32_t s = r % 3, s1 = (r + 1) % 3, s2 = (r + 2) % 3;
@@ -81,8 +123,9 @@ __global__ __launch_bounds__(kBlock) void sl_scan_kernel(uint32_t *__restrict__ 
     const uint32_t delta = round_delta(ctl, log_variant, r);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         ctl->sl_delta[s] = delta;
-        // Slot s1 is next round's: its gone list and min start empty.
+        // Slot s1 is next round's: its work lists and min start empty.
         ctl->sl_gone[s1] = 0;
+        ctl->sl_chunks[s1] = 0;
         ctl->sl_mininv[s1] = 0;
         // Slot s2 held round r-1's input; it becomes round r+1's input.
         ctl->sl_alive[s2] = 0;
@@ -95,305 +138,264 @@ __global__ __launch_bounds__(kBlock) void sl_scan_kernel(uint32_t *__restrict__ 
         return;
     const uint64_t thr = log_variant ? (delta >= 32 ? (1ull << 32) : (1ull << delta)) : delta;
     // Tiles of kBlock * kScanPer entries: one reservation per list per tile
-    // (all blocks append to the same two counters, whose atomics serialise).
-    __shared__ uint32_t s_cnt[2][kWavesPerBlock];
-    __shared__ uint32_t s_base[2];
+    // (all blocks append to the same counters, whose atomics serialise).
+    constexpr int kW = kBlock / kWave;
+    __shared__ uint32_t s_cnt[3][kW];
+    __shared__ uint32_t s_base[3];
     const uint32_t tile = kBlock * kScanPer;
     const uint32_t ntiles = (n + tile - 1) / tile;
     const int lane = lane_id(), w = threadIdx.x / kWave;
     uint32_t inv = 0;
     for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) { // block-uniform trip count
-        uint32_t u[kScanPer];
+        uint32_t a[kScanPer], u[kScanPer], beg[kScanPer], len[kScanPer];
         bool rm[kScanPer], keep[kScanPer];
-        uint32_t cr = 0, ck = 0;
+#pragma unroll
+        for (int q = 0; q < kScanPer; q++) {
+            const uint32_t i = t * tile + (uint32_t)q * kBlock + threadIdx.x;
+            a[q] = i < n ? alive_in[i] : 0u;
+        }
 #pragma unroll
         for (int q = 0; q < kScanPer; q++) {
             const uint32_t i = t * tile + (uint32_t)q * kBlock + threadIdx.x;
             rm[q] = keep[q] = false;
-            u[q] = i < n ? alive_in[i] : 0u;
             if (i < n) {
-                const uint32_t d = deg[u[q]];
+                const uint32_t d = degr[a[q]];
                 if ((uint64_t)d <= thr) {
                     rm[q] = true;
-                    key[u[q]] = delta;
+                    keyr[a[q]] = delta;
                 } else {
                     keep[q] = true;
-                    const uint32_t m = UINT32_MAX - d;
-                    inv = m > inv ? m : inv;
+                    fold_min(d, inv);
                 }
             }
-            cr += rm[q] ? 1u : 0u;
-            ck += keep[q] ? 1u : 0u;
         }
-        uint32_t ir = cr, ik = ck; // inclusive scans over the wave
+#pragma unroll
+        for (int q = 0; q < kScanPer; q++)
+            u[q] = rm[q] ? perm[a[q]] : 0u;
+#pragma unroll
+        for (int q = 0; q < kScanPer; q++) {
+            beg[q] = rm[q] ? V[u[q]] : 0u;
+            len[q] = rm[q] ? V[u[q] + 1] - beg[q] : 0u;
+        }
+        uint32_t cr = 0, ck = 0, cc = 0;
+#pragma unroll
+        for (int q = 0; q < kScanPer; q++) {
+            cr += (len[q] >= 1u && len[q] <= kLaneRow) ? 1u : 0u;
+            ck += keep[q] ? 1u : 0u;
+            cc += len[q] > kLaneRow ? (len[q] + kChunk - 1u) / kChunk : 0u;
+        }
+        uint32_t ir = cr, ik = ck, ic = cc; // inclusive scans over the wave
 #pragma unroll
         for (int off = 1; off < kWave; off <<= 1) {
-            const uint32_t a = __shfl_up(ir, off), b = __shfl_up(ik, off);
+            const uint32_t x = __shfl_up(ir, off), y = __shfl_up(ik, off), z = __shfl_up(ic, off);
             if (lane >= off) {
-                ir += a;
-                ik += b;
+                ir += x;
+                ik += y;
+                ic += z;
             }
         }
         if (lane == kWave - 1) {
             s_cnt[0][w] = ir;
             s_cnt[1][w] = ik;
+            s_cnt[2][w] = ic;
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t tr = 0, tk = 0;
-            for (int q = 0; q < kWavesPerBlock; q++) {
-                const uint32_t a = s_cnt[0][q], b = s_cnt[1][q];
-                s_cnt[0][q] = tr;
-                s_cnt[1][q] = tk;
-                tr += a;
-                tk += b;
+        if (threadIdx.x < 3) {
+            uint32_t tot = 0;
+            for (int q = 0; q < kW; q++) {
+                const uint32_t x = s_cnt[threadIdx.x][q];
+                s_cnt[threadIdx.x][q] = tot;
+                tot += x;
             }
-            s_base[0] = tr ? atomicAdd(&ctl->sl_gone[s], tr) : 0u;
-            s_base[1] = tk ? atomicAdd(&ctl->sl_alive[s1], tk) : 0u;
+            uint32_t *ctr = threadIdx.x == 0   ? &ctl->sl_gone[s]
+                            : threadIdx.x == 1 ? &ctl->sl_alive[s1]
+                                               : &ctl->sl_chunks[s];
+            s_base[threadIdx.x] = tot ? atomicAdd(ctr, tot) : 0u;
         }
         __syncthreads();
         uint32_t pr = s_base[0] + s_cnt[0][w] + ir - cr;
         uint32_t pk = s_base[1] + s_cnt[1][w] + ik - ck;
+        uint32_t pc = s_base[2] + s_cnt[2][w] + ic - cc;
 #pragma unroll
         for (int q = 0; q < kScanPer; q++) {
-            if (rm[q])
-                gone[pr++] = u[q];
-            else if (keep[q])
-                alive_out[pk++] = u[q];
+            if (rm[q]) {
+                key[u[q]] = delta;
+                if (len[q] > kLaneRow) {
+                    const uint32_t end = beg[q] + len[q];
+                    for (uint32_t b = beg[q]; b < end; b += kChunk)
+                        chunks[pc++] = make_uint2(b, end - b > kChunk ? b + kChunk : end);
+                } else if (len[q]) {
+                    gone[pr++] = u[q];
+                }
+            } else if (keep[q]) {
+                alive_out[pk++] = a[q];
+            }
         }
         __syncthreads(); // s_cnt and s_base are rewritten by the next tile
     }
     wave_min_inv(inv, &ctl->sl_mininv[s]);
 }
 
-// In rounds that remove at least kAggMinGone vertices, decrements to the
-// kHubSlots highest-degree vertices (hub_bit set, slot = degree-order rank)
-// are summed in LDS and applied once per block: in R-MAT graphs most removed
-// vertices hang off the same few hubs, whose counters would otherwise take
-// thousands of serialised atomics per round.
-constexpr uint32_t kHubSlots = 1024;
-constexpr uint32_t kAggMinGone = 1u << 16;
-
-__device__ __forceinline__ bool is_hub(const uint32_t *__restrict__ hub_bit, uint32_t v)
-{
-    return (hub_bit[v >> 5] >> (v & 31u)) & 1u;
-}
-
 // Decrement the alive neighbours among entries beg + t0, beg + t0 + stride,
 // ... (< end) of a removed vertex's row, STEPS entries per thread in flight
-// (loads, then atomics, then uses); folds the new degrees into inv.
+// (column loads, then flag loads, then the decrements): ranks below nsum
+// into the workgroup's LDS sums, the others with a device atomic whose new
+// degree is folded into inv.
 template <int STEPS>
 __device__ __forceinline__ uint32_t dec_row(uint32_t beg, uint32_t end, uint32_t t0, uint32_t stride,
-                                            const uint32_t *__restrict__ E,
-                                            uint32_t *__restrict__ deg,
-                                            const uint32_t *__restrict__ key,
-                                            const uint32_t *__restrict__ hub_bit,
-                                            const uint32_t *__restrict__ rank, uint32_t *s_hub,
-                                            uint32_t inv)
+                                            const uint32_t *__restrict__ Er,
+                                            uint32_t *__restrict__ degr,
+                                            const uint32_t *__restrict__ keyr, uint32_t nsum,
+                                            uint32_t *s_sum, uint32_t inv)
 {
     for (uint32_t i0 = beg + t0; i0 < end; i0 += stride * STEPS) {
-        uint32_t v[STEPS], d[STEPS];
-        bool live[STEPS];
+        uint32_t t[STEPS], kv[STEPS];
 #pragma unroll
-        for (int t = 0; t < STEPS; t++) {
-            const uint32_t i = i0 + (uint32_t)t * stride;
-            live[t] = i < end;
-            v[t] = live[t] ? E[i] : 0u;
+        for (int q = 0; q < STEPS; q++) {
+            const uint32_t i = i0 + (uint32_t)q * stride;
+            t[q] = i < end ? Er[i] : 0u;
         }
 #pragma unroll
-        for (int t = 0; t < STEPS; t++) {
-            live[t] = live[t] && key[v[t]] == kAlive;
-            if (live[t] && hub_bit && is_hub(hub_bit, v[t])) {
-                atomicAdd(&s_hub[rank[v[t]]], 1u);
-                live[t] = false;
-            }
+        for (int q = 0; q < STEPS; q++) {
+            const uint32_t i = i0 + (uint32_t)q * stride;
+            kv[q] = i < end ? keyr[t[q]] : 0u;
         }
 #pragma unroll
-        for (int t = 0; t < STEPS; t++)
-            d[t] = live[t] ? atomicSub(deg + v[t], 1u) : 0u;
-#pragma unroll
-        for (int t = 0; t < STEPS; t++) {
-            const uint32_t m = live[t] ? UINT32_MAX - (d[t] - 1u) : 0u;
-            inv = m > inv ? m : inv;
+        for (int q = 0; q < STEPS; q++) {
+            if (kv[q] != kAlive)
+                continue;
+            if (t[q] < nsum)
+                atomicAdd(&s_sum[t[q] >> 1], 1u << ((t[q] & 1u) * 16u));
+            else
+                fold_min(atomicSub(degr + t[q], 1u) - 1u, inv);
         }
     }
     return inv;
 }
 
-// In rounds that remove many vertices, rows of at most kLaneRow entries are
-// decremented by one thread, rows of at most kBigRow by one wave, longer
-// rows by the whole block (one wave per removed vertex made R-MAT 22's first
-// rounds, 1.8 M isolated and 0.55 M degree-1 vertices, take 0.6-0.7 ms each);
-// in the others every removed vertex gets a wave (a thread per vertex left
-// most of the grid idle in rounds that remove a few thousand long rows).
-constexpr uint32_t kLaneRow = 16;
-constexpr uint32_t kBigRow = 2048;
-
-__global__ __launch_bounds__(kBlock) void sl_dec_kernel(const uint32_t *__restrict__ V,
-                                                        const uint32_t *__restrict__ E,
-                                                        uint32_t *__restrict__ deg,
-                                                        const uint32_t *__restrict__ key,
-                                                        const uint32_t *__restrict__ gone,
-                                                        const uint32_t *__restrict__ hub_bit,
-                                                        const uint32_t *__restrict__ rank,
-                                                        const uint32_t *__restrict__ perm,
-                                                        uint32_t nslot, gcxk_ctl *ctl, uint32_t r)
+// Short rows (at most kLaneRow entries): a thread each; long rows: a wave
+// per chunk of kChunk entries (a lane per entry), kChunkSteps chunks in
+// flight. No workgroup-wide steps until the LDS sums are flushed.
+__global__ __launch_bounds__(kDBlock) void sl_dec_kernel(const uint32_t *__restrict__ V,
+                                                         const uint32_t *__restrict__ Er,
+                                                         uint32_t *__restrict__ degr,
+                                                         const uint32_t *__restrict__ keyr,
+                                                         const uint32_t *__restrict__ gone,
+                                                         const uint2 *__restrict__ chunks,
+                                                         uint32_t nsum, gcxk_ctl *ctl, uint32_t r)
 {
-    __shared__ uint32_t s_beg[kBlock], s_end[kBlock];
-    __shared__ uint32_t s_nrow;
-    __shared__ uint32_t s_hub[kHubSlots];
+    __shared__ uint32_t s_sum[kSumWords];
     const uint32_t s = r % 3;
-    const uint32_t n = ctl->sl_gone[s];
-    if (n == 0)
+    const uint32_t n = ctl->sl_gone[s], nc = ctl->sl_chunks[s];
+    // nothing left alive (the last round): no degree to lower
+    if ((n == 0 && nc == 0) || ctl->sl_alive[(r + 1) % 3] == 0)
         return;
-    // only rounds that remove many vertices pile up on the hubs' counters;
-    // in the others the LDS clear and flush cost more than they save
-    if (n < kAggMinGone)
-        nslot = 0;
-    if (nslot == 0)
-        hub_bit = nullptr;
-    for (uint32_t i = threadIdx.x; i < nslot; i += kBlock)
-        s_hub[i] = 0u;
-    const int w = threadIdx.x / kWave;
+    const uint64_t rows_per = kDBlock, chunks_per = (uint64_t)kDWaves * kChunkSteps;
+    const uint64_t rows = (n + gridDim.x * rows_per - 1) / (gridDim.x * rows_per) * rows_per;
+    const uint64_t chs = (nc + gridDim.x * chunks_per - 1) / (gridDim.x * chunks_per) * chunks_per;
+    if (rows + chs > kMaxRowsPerGroup)
+        nsum = 0; // a 16-bit sum could overflow
+    const uint32_t words = (nsum + 1u) / 2u;
+    for (uint32_t i = threadIdx.x; i < words; i += kDBlock)
+        s_sum[i] = 0u;
+    __syncthreads();
     uint32_t inv = 0;
-    if (n >= gridDim.x * (kBlock / 4u)) {
-        // many removed vertices (early rounds: mostly degree 0 or 1): a
-        // thread per removed vertex; rows of at most kLaneRow entries are
-        // walked by their own thread, longer ones are listed in LDS and taken
-        // by a wave each, rows past kBigRow by the whole block
-        for (uint32_t g = blockIdx.x * kBlock; g < n; g += gridDim.x * kBlock) {
-            if (threadIdx.x == 0)
-                s_nrow = 0u;
-            __syncthreads();
-            const uint32_t k = g + threadIdx.x;
-            uint32_t beg = 0, end = 0;
-            if (k < n) {
-                const uint32_t u = gone[k];
-                beg = V[u];
-                end = V[u + 1];
-            }
-            if (end - beg <= kLaneRow) {
-                inv = dec_row<kDecSteps>(beg, end, 0u, 1u, E, deg, key, hub_bit, rank, s_hub, inv);
-            } else {
-                const uint32_t slot = atomicAdd(&s_nrow, 1u);
-                s_beg[slot] = beg;
-                s_end[slot] = end;
-            }
-            __syncthreads();
-            const uint32_t m = s_nrow;
-            for (uint32_t j = (uint32_t)w; j < m; j += kWavesPerBlock)
-                if (s_end[j] - s_beg[j] <= kBigRow)
-                    inv = dec_row<kDecSteps>(s_beg[j], s_end[j], (uint32_t)lane_id(), kWave, E, deg,
-                                             key, hub_bit, rank, s_hub, inv);
-            for (uint32_t j = 0; j < m; j++)
-                if (s_end[j] - s_beg[j] > kBigRow) // block-uniform
-                    inv = dec_row<kDecSteps>(s_beg[j], s_end[j], threadIdx.x, kBlock, E, deg, key,
-                                             hub_bit, rank, s_hub, inv);
-            __syncthreads(); // s_nrow, s_beg and s_end are rewritten by the next group
+    for (uint32_t k = blockIdx.x * kDBlock + threadIdx.x; k < n; k += gridDim.x * kDBlock) {
+        const uint32_t u = gone[k];
+        inv = dec_row<kDecSteps>(V[u], V[u + 1], 0u, 1u, Er, degr, keyr, nsum, s_sum, inv);
+    }
+    const uint32_t nw = gridDim.x * kDWaves;
+    const uint32_t lane = (uint32_t)lane_id();
+    for (uint32_t c0 = blockIdx.x * kDWaves + threadIdx.x / kWave; c0 < nc; c0 += nw * kChunkSteps) {
+        uint32_t t[kChunkSteps], kv[kChunkSteps];
+        bool in[kChunkSteps];
+#pragma unroll
+        for (int q = 0; q < kChunkSteps; q++) {
+            const uint32_t c = c0 + (uint32_t)q * nw;
+            const uint2 ch = c < nc ? chunks[c] : make_uint2(0u, 0u);
+            in[q] = ch.x + lane < ch.y;
+            t[q] = in[q] ? Er[ch.x + lane] : 0u;
         }
-    } else {
-        // few removed vertices (late rounds: long rows): a wave per removed
-        // vertex, so that every wave of the grid has a row; rows past kBigRow
-        // by the whole block after the group's short rows
-        __syncthreads(); // s_hub cleared
-        for (uint32_t g = blockIdx.x * kWavesPerBlock; g < n; g += gridDim.x * kWavesPerBlock) {
-            const uint32_t k = g + (uint32_t)w;
-            uint32_t beg = 0, end = 0;
-            if (k < n) {
-                const uint32_t u = gone[k];
-                beg = V[u];
-                end = V[u + 1];
-            }
-            const bool big = end - beg > kBigRow;
-            if (!big)
-                inv = dec_row<kDecSteps>(beg, end, (uint32_t)lane_id(), kWave, E, deg, key, hub_bit,
-                                         rank, s_hub, inv);
-            if (lane_id() == 0) {
-                s_beg[w] = big ? beg : 0u;
-                s_end[w] = big ? end : 0u;
-            }
-            __syncthreads();
-            for (int q = 0; q < kWavesPerBlock; q++)
-                if (s_end[q]) // block-uniform
-                    inv = dec_row<kDecSteps>(s_beg[q], s_end[q], threadIdx.x, kBlock, E, deg, key,
-                                             hub_bit, rank, s_hub, inv);
-            __syncthreads(); // s_beg / s_end are rewritten by the next group
+#pragma unroll
+        for (int q = 0; q < kChunkSteps; q++)
+            kv[q] = in[q] ? keyr[t[q]] : 0u;
+#pragma unroll
+        for (int q = 0; q < kChunkSteps; q++) {
+            if (kv[q] != kAlive)
+                continue;
+            if (t[q] < nsum)
+                atomicAdd(&s_sum[t[q] >> 1], 1u << ((t[q] & 1u) * 16u));
+            else
+                fold_min(atomicSub(degr + t[q], 1u) - 1u, inv);
         }
     }
-    for (uint32_t i = threadIdx.x; i < nslot; i += kBlock) {
-        const uint32_t c = s_hub[i];
-        if (c) {
-            const uint32_t d = atomicSub(deg + perm[i], c) - c;
-            const uint32_t m = UINT32_MAX - d;
-            inv = m > inv ? m : inv;
-        }
+    __syncthreads();
+    // one atomic per rank with a non-zero sum; a wave covers 128 consecutive
+    // ranks
+    for (uint32_t i = threadIdx.x; i < words; i += kDBlock) {
+        const uint32_t c = s_sum[i];
+        const uint32_t lo = c & 0xFFFFu, hi = c >> 16;
+        if (lo)
+            fold_min(atomicSub(degr + 2u * i, lo) - lo, inv);
+        if (hi)
+            fold_min(atomicSub(degr + 2u * i + 1u, hi) - hi, inv);
     }
     wave_min_inv(inv, &ctl->sl_mininv[s]);
 }
 
-// hub_bit: one bit per vertex, set for perm[0 .. nslot).
-__global__ __launch_bounds__(kBlock) void sl_hub_bits(uint32_t words, const uint32_t *__restrict__ perm,
-                                                      uint32_t nslot, uint32_t *__restrict__ hub_bit,
-                                                      int phase)
-{
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (phase == 0) {
-        if (i < words)
-            hub_bit[i] = 0u;
-    } else if (i < nslot) {
-        const uint32_t v = perm[i];
-        atomicOr(&hub_bit[v >> 5], 1u << (v & 31u));
-    }
-}
+uint32_t g_dec_grid = 0;
 
 } // namespace
 
-extern "C" int gcxk_sl_init(uint32_t N, const uint32_t *dV, uint32_t *deg, uint32_t *key,
-                            uint32_t *alive0, gcxk_ctl *ctl, void *stream)
+extern "C" int gcxk_sl_init(uint32_t N, const uint32_t *dV, const uint32_t *perm, uint32_t *degr,
+                            uint32_t *keyr, uint32_t *alive0, gcxk_ctl *ctl, void *stream)
 {
-    sl_init_kernel<<<div_up(N ? N : 1, kBlock), kBlock, 0, as_stream(stream)>>>(N, dV, deg, key,
-                                                                                 alive0, ctl);
+    sl_init_kernel<<<div_up(N ? N : 1, kBlock), kBlock, 0, as_stream(stream)>>>(N, dV, perm, degr,
+                                                                                 keyr, alive0, ctl);
     GCXK_LAUNCH_CHECK();
     return GCX_OK;
 }
 
-extern "C" int gcxk_sl_scan(uint32_t *deg, uint32_t *key, const uint32_t *alive_in,
-                            uint32_t *alive_out, uint32_t *gone, gcxk_ctl *ctl, int log_variant,
-                            uint32_t r, uint32_t grid, void *stream)
+extern "C" uint64_t gcxk_sl_chunk_capacity(uint64_t M)
 {
-    sl_scan_kernel<<<grid, kBlock, 0, as_stream(stream)>>>(deg, key, alive_in, alive_out, gone, ctl,
-                                                           log_variant, r);
+    // a row of len > kLaneRow entries takes ceil(len / kChunk) <= len / 16
+    // chunks (kChunk = 64 = 4 * kLaneRow): at most M / 16 over all rows
+    return M / kLaneRow + 1;
+}
+
+extern "C" int gcxk_sl_scan(const uint32_t *dV, const uint32_t *degr, uint32_t *keyr,
+                            uint32_t *key, const uint32_t *perm, const uint32_t *alive_in,
+                            uint32_t *alive_out, uint32_t *gone, void *chunks, gcxk_ctl *ctl,
+                            int log_variant, uint32_t r, uint32_t grid, void *stream)
+{
+    sl_scan_kernel<<<grid, kBlock, 0, as_stream(stream)>>>(
+        dV, degr, keyr, key, perm, alive_in, alive_out, gone, static_cast<uint2 *>(chunks), ctl,
+        log_variant, r);
     GCXK_LAUNCH_CHECK();
     return GCX_OK;
 }
 
-extern "C" int gcxk_sl_dec(const uint32_t *dV, const uint32_t *dE, uint32_t *deg,
-                           const uint32_t *key, const uint32_t *gone, const uint32_t *hub_bit,
-                           const uint32_t *rank, const uint32_t *perm, uint32_t nslot,
-                           gcxk_ctl *ctl, uint32_t r, uint32_t grid, void *stream)
+extern "C" int gcxk_sl_dec(uint32_t N, const uint32_t *dV, const uint32_t *Er, uint32_t *degr,
+                           const uint32_t *keyr, const uint32_t *gone, const void *chunks,
+                           gcxk_ctl *ctl, int log_variant, uint32_t r, void *stream)
 {
-    if (nslot > kHubSlots)
-        nslot = kHubSlots;
-    if (!hub_bit)
-        nslot = 0;
-    sl_dec_kernel<<<grid, kBlock, 0, as_stream(stream)>>>(dV, dE, deg, key, gone, hub_bit, rank,
-                                                          perm, nslot, ctl, r);
-    GCXK_LAUNCH_CHECK();
-    return GCX_OK;
-}
-
-extern "C" uint32_t gcxk_sl_hub_slots(uint32_t N) { return N < kHubSlots ? N : kHubSlots; }
-
-extern "C" int gcxk_sl_hub_bits(uint32_t N, const uint32_t *perm, uint32_t *hub_bit, void *stream)
-{
-    const uint32_t words = N / 32u + 1u, nslot = gcxk_sl_hub_slots(N);
-    sl_hub_bits<<<div_up(words, kBlock), kBlock, 0, as_stream(stream)>>>(words, perm, nslot,
-                                                                         hub_bit, 0);
-    GCXK_LAUNCH_CHECK();
-    if (nslot) {
-        sl_hub_bits<<<div_up(nslot, kBlock), kBlock, 0, as_stream(stream)>>>(words, perm, nslot,
-                                                                             hub_bit, 1);
-        GCXK_LAUNCH_CHECK();
+    if (!g_dec_grid) {
+        int dev = 0;
+        hipDeviceProp_t prop;
+        if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+            return GCX_ERR_HIP;
+        g_dec_grid = (uint32_t)prop.multiProcessorCount;
     }
+    // Fewer workgroups sum more decrements per LDS table (fewer flushed
+    // atomics) but keep fewer loads in flight. R-MAT 22 SL (thin degree
+    // layers, 0.5-1.2 M decrements per middle round): 2 per CU 17.9 ms, 1 per
+    // CU 15.4, 1 per 2 CUs 14.0, 1 per 4 CUs 14.4. SLL (layers of doubling
+    // width, up to several M decrements a round): 3.9, 3.5, 3.8, 5.1 ms.
+    // The 27-point stencil (160^3) is within 0.1 ms of its best at either.
+    const uint32_t grid = log_variant ? g_dec_grid : (g_dec_grid + 1) / 2;
+    const uint32_t nsum = N < kSumSlots ? N : kSumSlots;
+    sl_dec_kernel<<<grid, kDBlock, 0, as_stream(stream)>>>(
+        dV, Er, degr, keyr, gone, static_cast<const uint2 *>(chunks), nsum, ctl, r);
+    GCXK_LAUNCH_CHECK();
     return GCX_OK;
 }
