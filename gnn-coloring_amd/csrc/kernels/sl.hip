@@ -46,6 +46,7 @@ constexpr uint64_t kMaxRowsPerGroup = 65535;
 constexpr uint32_t kLaneRow = 16;   // rows up to this length: a thread each
 constexpr uint32_t kChunk = kWave;  // longer rows: chunks of a lane per entry
 constexpr int kChunkSteps = 4;
+constexpr uint32_t kOwnChunks = 4;  // chunks of a row its own thread lists
 
 __device__ __forceinline__ void wave_min_inv(uint32_t m, uint32_t *slot)
 {
@@ -218,17 +219,34 @@ __global__ __launch_bounds__(kBlock) void sl_scan_kernel(const uint32_t *__restr
         uint32_t pc = s_base[2] + s_cnt[2][w] + ic - cc;
 #pragma unroll
         for (int q = 0; q < kScanPer; q++) {
+            const uint32_t nch = len[q] > kLaneRow ? (len[q] + kChunk - 1u) / kChunk : 0u;
             if (rm[q]) {
                 key[u[q]] = delta;
-                if (len[q] > kLaneRow) {
+                if (nch && nch <= kOwnChunks) {
                     const uint32_t end = beg[q] + len[q];
                     for (uint32_t b = beg[q]; b < end; b += kChunk)
                         chunks[pc++] = make_uint2(b, end - b > kChunk ? b + kChunk : end);
-                } else if (len[q]) {
+                } else if (nch == 0 && len[q]) {
                     gone[pr++] = u[q];
                 }
             } else if (keep[q]) {
                 alive_out[pk++] = a[q];
+            }
+            // rows of more than kOwnChunks chunks (removed hubs: up to
+            // thousands) are written by the whole wave, one row at a time
+            uint64_t big = __ballot(rm[q] && nch > kOwnChunks);
+            const uint32_t pc_row = pc;
+            if (rm[q] && nch > kOwnChunks)
+                pc += nch;
+            while (big) {
+                const int src = __builtin_ctzll(big);
+                big &= big - 1;
+                const uint32_t b0 = __shfl(beg[q], src), e0 = b0 + __shfl(len[q], src);
+                const uint32_t p0 = __shfl(pc_row, src), n0 = __shfl(nch, src);
+                for (uint32_t j = (uint32_t)lane; j < n0; j += kWave) {
+                    const uint32_t b = b0 + j * kChunk;
+                    chunks[p0 + j] = make_uint2(b, e0 - b > kChunk ? b + kChunk : e0);
+                }
             }
         }
         __syncthreads(); // s_cnt and s_base are rewritten by the next tile
