@@ -1486,8 +1486,12 @@ static int launch_ordered(gcx_workspace *ws, const gcx_graph *g, int mb, int kin
                     : kind == LAUNCH_JP_PULL ? gcxk_pull_grid()
                                              : gcxk_jp_ordered_grid();
     if (k) {
-        const uint32_t cap = kind == LAUNCH_JP_PULL ? gcxk_pull_cap() : gcxk_jp_ordered_cap();
-        const uint32_t share = cap / (uint32_t)k;
+        /* k launches share the default grid, which leaves workgroup slots
+         * free on every CU (not the occupancy capacity): two plain pull
+         * launches of capacity / 2 each under 8 serving workers once stayed
+         * 328 and 505 of 512 workgroups resident for the whole 2 s wait bound */
+        const uint32_t dflt = kind == LAUNCH_JP_PULL ? gcxk_pull_grid() : gcxk_jp_ordered_grid();
+        const uint32_t share = dflt / (uint32_t)k;
         if (grid > share)
             grid = share;
         if (grid == 0)
