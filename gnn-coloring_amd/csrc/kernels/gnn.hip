@@ -24,6 +24,8 @@
 //
 // This file is compiled with -ffp-contract=off: the only fused operations
 // are the explicit fmaf calls of the dense step.
+#include <type_traits>
+
 #include "common.hip.h"
 #include "hub_sum.hip.h"
 #include "input_sum.hip.h"
@@ -286,11 +288,12 @@ __global__ __launch_bounds__(kHubBlock) void sage_hubs(GnnCsr gr, const float *_
                                                        float *__restrict__ out,
                                                        uint32_t *__restrict__ key)
 {
-    // the input layer stages 2 features per neighbour, the others 16
-    __shared__ __attribute__((aligned(16))) union {
-        hubsum::Stage<kHubBlock> rows;
-        insum::PairStage<kHubBlock, kHubInRows> feats;
-    } st;
+    // the input layer stages 2 features per neighbour, the others 16 (each
+    // kind reserves only its own stage: the 33 KB row stage held the input
+    // layer's hub workgroups to fewer per CU)
+    using Stage = std::conditional_t<kind == kInput, insum::PairStage<kHubBlock, kHubInRows>,
+                                     hubsum::Stage<kHubBlock>>;
+    __shared__ __attribute__((aligned(16))) Stage st;
     const uint32_t u = gr.perm[blockIdx.x]; // hubs by degree, largest first
     const uint32_t beg = gr.V[u], end = gr.V[u + 1];
     const uint32_t deg = end - beg;
@@ -300,11 +303,11 @@ __global__ __launch_bounds__(kHubBlock) void sage_hubs(GnnCsr gr, const float *_
     // feature j.
     float acc;
     uint2 sp = make_uint2(0u, gr.N);
-    if (kind == kInput) {
+    if constexpr (kind == kInput) {
         sp = insum::span_of(u, gr.N, gr.gspan);
-        acc = insum::pair_sums(st.feats, beg, end, gr.E, gr.Ed, fmd, sp);
+        acc = insum::pair_sums(st, beg, end, gr.E, gr.Ed, fmd, sp);
     } else {
-        acc = hubsum::ordered_sum<kHubBlock, false>(st.rows, beg, end, gr.Er,
+        acc = hubsum::ordered_sum<kHubBlock, false>(st, beg, end, gr.Er,
                                                     [&](uint32_t v, float4 *row) {
             const float4 *src = reinterpret_cast<const float4 *>(x + (size_t)v * F);
 #pragma unroll

@@ -36,6 +36,8 @@
 // stream: the same CSR-order sums and the same MFMA sequence, so results do
 // not depend on the hub degree; both kernels only read the previous layer and
 // write disjoint rows.
+#include <type_traits>
+
 #include "common.hip.h"
 #include "hub_sum.hip.h"
 #include "input_sum.hip.h"
@@ -50,7 +52,7 @@ constexpr int kBlock = 256;
 constexpr int kUnroll = 8;        // neighbour rows in flight per lane
 constexpr int kGrid = 2048;       // persistent workgroups of the quad kernel
 constexpr int kLaneSteps = 1;     // input layer: blocks of 8 neighbours per load step
-constexpr int kHubInRows = 4;     // input layer hubs: neighbours per loader thread per chunk
+constexpr int kHubInRows = 8;     // input layer hubs: neighbours per loader thread per chunk
 
 enum { kInput = 0, kHidden = 1, kOutput = 2 };
 enum { kBf16 = 0, kSplit = 1 };
@@ -326,13 +328,22 @@ __global__ __launch_bounds__(kBlock) void sage_mfma_lanes_input(GnnCsr gr,
 // sequence with the hub as row 0 of the A tile (the other rows zero): each
 // output row of an MFMA depends only on its own A row, so the priorities and
 // rows do not depend on the hub degree.
-// Threads (= rows per staged chunk) of the hub kernel. Measured on R-MAT
-// scale 22 (GNN3 ordering, BF16): 256 -> 4.91 ms, 512 -> 4.63 ms,
-// 1024 -> 5.80 ms.
+// Threads (= rows per staged chunk) of the hub kernel. Hidden and output
+// layers, measured on R-MAT scale 22 (GNN3 ordering, BF16): 256 -> 4.91 ms,
+// 512 -> 4.63 ms, 1024 -> 5.80 ms. The input layer stages two features per
+// neighbour, 8 neighbours per loader thread, in 256-thread workgroups, as
+// the FP32 input layer does (the hidden layers' 512 threads and 4
+// neighbours per loader made it 0.81 ms against FP32's 0.69).
 constexpr int kHubChunk = 512;
 
+template <int kind>
+constexpr int hub_threads()
+{
+    return kind == kInput ? 256 : kHubChunk;
+}
+
 template <int kind, int mode>
-__global__ __launch_bounds__(kHubChunk) void sage_mfma_hubs(GnnCsr gr,
+__global__ __launch_bounds__(hub_threads<kind>()) void sage_mfma_hubs(GnnCsr gr,
                                                             const uint16_t *__restrict__ Wpack,
                                                             const float *__restrict__ bias,
                                                             float fmd, const void *__restrict__ x,
@@ -340,10 +351,10 @@ __global__ __launch_bounds__(kHubChunk) void sage_mfma_hubs(GnnCsr gr,
                                                             uint32_t *__restrict__ key)
 {
     // the input layer stages 2 features per neighbour, the others 16
-    __shared__ __attribute__((aligned(16))) union {
-        hubsum::BlockedStage<kHubChunk> rows;
-        insum::PairStage<kHubChunk, kHubInRows> feats;
-    } st;
+    constexpr int T = hub_threads<kind>();
+    using Stage = std::conditional_t<kind == kInput, insum::PairStage<T, kHubInRows>,
+                                     hubsum::BlockedStage<T>>;
+    __shared__ __attribute__((aligned(16))) Stage st;
     const uint32_t u = gr.perm[blockIdx.x]; // hubs by degree, largest first
     const uint32_t beg = gr.V[u], end = gr.V[u + 1];
     const uint32_t deg = end - beg;
@@ -351,11 +362,11 @@ __global__ __launch_bounds__(kHubChunk) void sage_mfma_hubs(GnnCsr gr,
     // layer 0: lane 0 sums fdeg, lane 1 fid; else lane j feature j
     float acc;
     uint2 sp = make_uint2(0u, gr.N);
-    if (kind == kInput) {
+    if constexpr (kind == kInput) {
         sp = insum::span_of(u, gr.N, gr.gspan);
-        acc = insum::pair_sums(st.feats, beg, end, gr.E, gr.Ed, fmd, sp);
+        acc = insum::pair_sums(st, beg, end, gr.E, gr.Ed, fmd, sp);
     } else {
-        acc = hubsum::ordered_sum<kHubChunk, false, true>(st.rows, beg, end, gr.Er,
+        acc = hubsum::ordered_sum<T, false, true>(st, beg, end, gr.Er,
                                                     [&](uint32_t v, float4 *row) {
 #pragma unroll
             for (int q = 0; q < 4; q++)
@@ -442,7 +453,7 @@ int launch(const GnnCsr &gr, const uint16_t *Wpack, const float *b, float fmd, b
     if (gr.N == 0)
         return GCX_OK;
     if (gr.nhub)
-        sage_mfma_hubs<kind, mode><<<gr.nhub, kHubChunk, 0, as_stream(hub_stream)>>>(
+        sage_mfma_hubs<kind, mode><<<gr.nhub, hub_threads<kind>(), 0, as_stream(hub_stream)>>>(
             gr, Wpack, b, fmd, x, out, key);
     if constexpr (kind == kInput) {
         if (gr.N > gr.nhub) {
