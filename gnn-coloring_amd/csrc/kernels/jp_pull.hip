@@ -581,7 +581,11 @@ __global__ __launch_bounds__(kBlock, 4) void jp_pull_kernel(uint32_t N, const ui
     const int lane = lane_id(), w = threadIdx.x / kWave;
     uint32_t *bm = s_bm[w];
     const uint32_t n = ctl->err ? 0u : N;
-    const uint32_t w0 = blockIdx.x * kWavesPerBlock + (uint32_t)w;
+    // Wave ids run across workgroups first: consecutive positions go to
+    // consecutive workgroups, which the dispatcher places on the XCDs in turn
+    // (R-MAT 22: 4.41 -> 4.17 ms against four consecutive positions per
+    // workgroup; keeping runs of consecutive positions on one XCD: 5.62 ms).
+    const uint32_t w0 = (uint32_t)w * gridDim.x + blockIdx.x;
     const uint32_t nw = gridDim.x * kWavesPerBlock;
     const uint32_t T = (uint64_t)kSingleRounds * nw < n ? kSingleRounds * nw : n;
     uint2 h = w0 < T ? hdr[w0] : make_uint2(0u, 0u);
