@@ -543,7 +543,7 @@ def main():
                     help="worker streams of the resident R-MAT serving leg")
     ap.add_argument("--serving-mixed-workers", type=int, default=16,
                     help="worker streams of the mixed-trace serving leg")
-    ap.add_argument("--serving-mixed-slots", type=int, default=4,
+    ap.add_argument("--serving-mixed-slots", type=int, default=2,
                     help="ordered-launch slots of the mixed-trace BF16 passes")
     ap.add_argument("--serving-budget-gb", type=float, default=48.0)
     ap.add_argument("--serving-express-workers", type=int, default=4,
