@@ -1540,14 +1540,14 @@ static int color_pull(gcx_workspace *ws, const gcx_graph *g, int mb, hipStream_t
     void *scratch = ws->order_buf ? ws->order_buf : (void *)ws->x;
     uint32_t *ids = NULL;
     TRY(gcxk_jp_sort_ids(g->N, ws->key, ws->key_bits, scratch, ws->order_bytes, &ids, s));
-    TRY(gcxk_pull_rank(g->N, ids, ws->count, g->nhub, ws->hpos, ws->ctl, s));
+    TRY(gcxk_pull_rank(g->N, ids, ws->count, ws->wof, g->nhub, ws->hpos, ws->ctl, s));
     void *hs = gcx__fork_hubs(ws, g, 1, s);
     if (!hs)
         return GCX_ERR_HIP;
     TRY(gcx__join_hubs(ws, hs, s,
                        gcxk_pull_orient(g->N, g->V, g->E, ws->count, g->perm, g->hubs, g->nhub,
                                         g->hchunk, g->nhchunk, ws->hpos, ws->esplit,
-                                        ws->items[0], ws->wof, s, hs)));
+                                        ws->items[0], s, hs)));
     TRY(gcx__prof_end(tok, s));
     TRY(gcx__prof_begin(GCX_ST_JP_ROUNDS, s, &tok));
     TRY(launch_ordered(ws, g, mb, LAUNCH_JP_PULL, ids, s));

@@ -308,17 +308,17 @@ uint32_t gcxk_jp_ordered_cap(void);
 
 /* ---- ordered schedule with pulled colours (jp_pull.hip) ----
  * Step 1, after gcxk_jp_sort_ids: zero ctl and hcnt[0 .. nhub), then
- * rank[ids[j]] = j (greedy-order positions). */
-int gcxk_pull_rank(uint32_t N, const uint32_t *ids, uint32_t *rank, uint32_t nhub, uint32_t *hcnt,
-                   gcxk_ctl *ctl, void *stream);
+ * rank[ids[j]] = j (greedy-order positions) and pcol[j] = 0. */
+int gcxk_pull_rank(uint32_t N, const uint32_t *ids, uint32_t *rank, uint32_t *pcol, uint32_t nhub,
+                   uint32_t *hcnt, gcxk_ctl *ctl, void *stream);
 /* Step 2: every row's predecessor positions to its front in esplit, hdr[j] =
- * (V[ids[j]], predecessor count) (8 bytes per position), pcol[j] = 0. Hub
+ * (V[ids[j]], predecessor count) (8 bytes per position). Hub
  * rows (chunk table hchunk) run on hub_stream (may equal stream; the caller
  * forks it after step 1 and joins it before step 3). */
 int gcxk_pull_orient(uint32_t N, const uint32_t *dV, const uint32_t *dE, const uint32_t *rank,
                      const uint32_t *perm, const uint32_t *hubs, uint32_t nhub,
                      const uint32_t *hchunk, uint32_t nhchunk, uint32_t *hcnt, uint32_t *esplit,
-                     void *hdr, uint32_t *pcol, void *stream, void *hub_stream);
+                     void *hdr, void *stream, void *hub_stream);
 /* Default workgroups of the pull kernel (6 per CU, at most the resident
  * maximum; 0 on error) and the resident maximum. */
 uint32_t gcxk_pull_grid(void);

@@ -101,26 +101,29 @@ __global__ __launch_bounds__(kBlock) void pull_clear(uint32_t nhub, uint32_t *__
         reinterpret_cast<uint32_t *>(ctl)[t] = 0u;
 }
 
+// rank[ids[i]] = i, and position i's colour slot cleared (in order here,
+// rather than at a random position per vertex in the orientation pass).
 __global__ __launch_bounds__(kBlock) void pull_rank(uint32_t N, const uint32_t *__restrict__ ids,
-                                                    uint32_t *__restrict__ rank)
+                                                    uint32_t *__restrict__ rank,
+                                                    uint32_t *__restrict__ pcol)
 {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < N)
+    if (i < N) {
         rank[ids[i]] = i;
+        pcol[i] = 0u;
+    }
 }
 
 // Non-hub vertices in degree order (perm[nhub .. N)), 4 lanes per vertex so a
 // wave's 16 vertices have near-equal degrees: predecessor positions to the
-// front of the row, the header and a cleared colour slot at the vertex's
-// position.
+// front of the row, and the header at the vertex's position.
 __global__ __launch_bounds__(kBlock) void pull_orient_quads(uint32_t N, const uint32_t *__restrict__ V,
                                                             const uint32_t *__restrict__ E,
                                                             const uint32_t *__restrict__ rank,
                                                             const uint32_t *__restrict__ perm,
                                                             uint32_t nhub,
                                                             uint32_t *__restrict__ esplit,
-                                                            uint2 *__restrict__ hdr,
-                                                            uint32_t *__restrict__ pcol)
+                                                            uint2 *__restrict__ hdr)
 {
     const uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     const uint32_t pi = (uint32_t)(t / kQuad) + nhub;
@@ -154,10 +157,8 @@ __global__ __launch_bounds__(kBlock) void pull_orient_quads(uint32_t N, const ui
             np += __popc(gp);
         }
     }
-    if (gl == 0) {
+    if (gl == 0)
         hdr[ru] = make_uint2(beg, np);
-        pcol[ru] = 0u;
-    }
 }
 
 // Hubs: chunk b of hub h covers entries [V[u] + b*kHubEdges, ...); one
@@ -236,14 +237,12 @@ __global__ __launch_bounds__(kBlock) void pull_hub_headers(const uint32_t *__res
                                                            const uint32_t *__restrict__ hubs,
                                                            uint32_t nhub,
                                                            const uint32_t *__restrict__ hcnt,
-                                                           uint2 *__restrict__ hdr,
-                                                           uint32_t *__restrict__ pcol)
+                                                           uint2 *__restrict__ hdr)
 {
     const uint32_t h = blockIdx.x * kBlock + threadIdx.x;
     if (h < nhub) {
         const uint32_t u = hubs[h], ru = rank[u];
         hdr[ru] = make_uint2(V[u], hcnt[h]);
-        pcol[ru] = 0u;
     }
 }
 
@@ -658,15 +657,15 @@ uint32_t g_pull_grid = 0, g_pull_cap = 0;
 
 } // namespace
 
-extern "C" int gcxk_pull_rank(uint32_t N, const uint32_t *ids, uint32_t *rank, uint32_t nhub,
-                              uint32_t *hcnt, gcxk_ctl *ctl, void *stream)
+extern "C" int gcxk_pull_rank(uint32_t N, const uint32_t *ids, uint32_t *rank, uint32_t *pcol,
+                              uint32_t nhub, uint32_t *hcnt, gcxk_ctl *ctl, void *stream)
 {
     hipStream_t s = as_stream(stream);
     const uint32_t words = (uint32_t)(sizeof(gcxk_ctl) / 4);
     pull_clear<<<div_up(nhub > words ? nhub : words, kBlock), kBlock, 0, s>>>(nhub, hcnt, ctl);
     GCXK_LAUNCH_CHECK();
     if (N) {
-        pull_rank<<<div_up(N, kBlock), kBlock, 0, s>>>(N, ids, rank);
+        pull_rank<<<div_up(N, kBlock), kBlock, 0, s>>>(N, ids, rank, pcol);
         GCXK_LAUNCH_CHECK();
     }
     return GCX_OK;
@@ -675,8 +674,8 @@ extern "C" int gcxk_pull_rank(uint32_t N, const uint32_t *ids, uint32_t *rank, u
 extern "C" int gcxk_pull_orient(uint32_t N, const uint32_t *dV, const uint32_t *dE,
                                 const uint32_t *rank, const uint32_t *perm, const uint32_t *hubs,
                                 uint32_t nhub, const uint32_t *hchunk, uint32_t nhchunk,
-                                uint32_t *hcnt, uint32_t *esplit, void *hdr, uint32_t *pcol,
-                                void *stream, void *hub_stream)
+                                uint32_t *hcnt, uint32_t *esplit, void *hdr, void *stream,
+                                void *hub_stream)
 {
     if (N == 0)
         return GCX_OK;
@@ -684,7 +683,7 @@ extern "C" int gcxk_pull_orient(uint32_t N, const uint32_t *dV, const uint32_t *
     const uint64_t quads = (uint64_t)(N - nhub) * kQuad;
     if (quads) {
         pull_orient_quads<<<div_up(quads, kBlock), kBlock, 0, as_stream(stream)>>>(
-            N, dV, dE, rank, perm, nhub, esplit, h, pcol);
+            N, dV, dE, rank, perm, nhub, esplit, h);
         GCXK_LAUNCH_CHECK();
     }
     if (nhub) {
@@ -694,8 +693,7 @@ extern "C" int gcxk_pull_orient(uint32_t N, const uint32_t *dV, const uint32_t *
                                                                nhub, hcnt);
             GCXK_LAUNCH_CHECK();
         }
-        pull_hub_headers<<<div_up(nhub, kBlock), kBlock, 0, hs>>>(dV, rank, hubs, nhub, hcnt, h,
-                                                                  pcol);
+        pull_hub_headers<<<div_up(nhub, kBlock), kBlock, 0, hs>>>(dV, rank, hubs, nhub, hcnt, h);
         GCXK_LAUNCH_CHECK();
     }
     return GCX_OK;
