@@ -47,6 +47,7 @@ struct gcx_workspace {
     void *key;        /* 8 * capN bytes: key32 or key64 view */
     int key_bits;     /* 32 / 64 for the current priorities, 0 = none */
     int key_kind;     /* 32-bit keys: 0 unsigned integer, 1 float order */
+    int id_order;     /* the keys are first-fit's: the greedy order is the id order */
     uint32_t *count;  /* SL: remaining degrees */
     int *color;
     uint32_t *q[2];   /* SL alive lists */
