@@ -42,6 +42,7 @@ constexpr int kScanPer = 8;
 // low half never carries into the high one.
 constexpr uint32_t kSumWords = 16384;
 constexpr uint32_t kSumSlots = 2 * kSumWords;
+constexpr uint32_t kMinSumSlots = 1024;
 constexpr uint64_t kMaxRowsPerGroup = 65535;
 constexpr uint32_t kLaneRow = 16;   // rows up to this length: a thread each
 constexpr uint32_t kChunk = kWave;  // longer rows: chunks of a lane per entry
@@ -308,6 +309,21 @@ __global__ __launch_bounds__(kDBlock) void sl_dec_kernel(const uint32_t *__restr
     // nothing left alive (the last round): no degree to lower
     if ((n == 0 && nc == 0) || ctl->sl_alive[(r + 1) % 3] == 0)
         return;
+    // LDS sums over the first nsum ranks, nsum sized to the round: the
+    // power of two from kMinSumSlots that covers the entries a workgroup
+    // takes. Every workgroup clears and flushes its table once per round,
+    // which light rounds do not repay. R-MAT 22 with fixed tables of 8192 /
+    // 16384 / 32768 ranks: SL 12.6 / 13.0 / 13.8 ms, SLL (heavy rounds)
+    // 3.85 / 3.60 / 3.53; sized per round (4x / 1x the entries per
+    // workgroup, from 8192; 1x from 4096; 1x from 1024): SL 12.9 / 12.7 /
+    // 12.5 / 12.3, SLL 3.53 - 3.54
+    {
+        const uint64_t per_wg = ((uint64_t)n * (kLaneRow / 2u) + (uint64_t)nc * kChunk) / gridDim.x;
+        uint32_t want = kMinSumSlots;
+        while (want < nsum && (uint64_t)want < per_wg)
+            want <<= 1;
+        nsum = want < nsum ? want : nsum;
+    }
     const uint64_t rows_per = kDBlock, chunks_per = (uint64_t)kDWaves * kChunkSteps;
     const uint64_t rows = (n + gridDim.x * rows_per - 1) / (gridDim.x * rows_per) * rows_per;
     const uint64_t chs = (nc + gridDim.x * chunks_per - 1) / (gridDim.x * chunks_per) * chunks_per;
