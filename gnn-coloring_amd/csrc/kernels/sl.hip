@@ -49,12 +49,25 @@ constexpr uint32_t kChunk = kWave;  // longer rows: chunks of a lane per entry
 constexpr int kChunkSteps = 4;
 constexpr uint32_t kOwnChunks = 4;  // chunks of a row its own thread lists
 
-__device__ __forceinline__ void wave_min_inv(uint32_t m, uint32_t *slot)
+// Folds m = UINT32_MAX - value (0 = no contribution) into *slot with
+// atomicMax, one atomic per workgroup of T threads (one per wave put
+// thousands of atomics per round on one word). Every thread of the
+// workgroup calls it.
+template <int T>
+__device__ __forceinline__ void block_min_inv(uint32_t m, uint32_t *slot)
 {
-    // m = UINT32_MAX - value (0 = no contribution)
+    __shared__ uint32_t s_m[T / kWave];
     m = group_max32<kWave>(m);
-    if (lane_id() == 0 && m)
-        atomicMax(slot, m);
+    if (lane_id() == 0)
+        s_m[threadIdx.x / kWave] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t b = 0;
+        for (int w = 0; w < T / kWave; w++)
+            b = s_m[w] > b ? s_m[w] : b;
+        if (b)
+            atomicMax(slot, b);
+    }
 }
 
 __device__ __forceinline__ void fold_min(uint32_t d, uint32_t &inv)
@@ -71,20 +84,20 @@ __global__ __launch_bounds__(kBlock) void sl_init_kernel(uint32_t N, const uint3
                                                          gcxk_ctl *ctl)
 {
     const uint32_t a = blockIdx.x * kBlock + threadIdx.x;
-    uint32_t inv = 0;
     if (a < N) {
         const uint32_t u = perm[a];
         const uint32_t d = V[u + 1] - V[u];
         degr[a] = d;
         keyr[a] = kAlive;
         alive0[a] = a;
-        inv = UINT32_MAX - d;
+        // the last rank has the smallest degree: it goes to slot 2 (the
+        // slot of "round -1")
+        if (a == N - 1u)
+            ctl->sl_mininv[2] = UINT32_MAX - d;
     }
     if (a == 0) {
         ctl->sl_alive[0] = N;
     }
-    // Min degree goes to slot 2 (the slot of "round -1").
-    wave_min_inv(inv, &ctl->sl_mininv[2]);
 }
 
 __device__ __forceinline__ uint32_t round_delta(const gcxk_ctl *ctl, int log_variant, uint32_t r)
@@ -252,7 +265,7 @@ __global__ __launch_bounds__(kBlock) void sl_scan_kernel(const uint32_t *__restr
         }
         __syncthreads(); // s_cnt and s_base are rewritten by the next tile
     }
-    wave_min_inv(inv, &ctl->sl_mininv[s]);
+    block_min_inv<kBlock>(inv, &ctl->sl_mininv[s]);
 }
 
 // Decrement the alive neighbours among entries beg + t0, beg + t0 + stride,
@@ -374,7 +387,7 @@ __global__ __launch_bounds__(kDBlock) void sl_dec_kernel(const uint32_t *__restr
         if (hi)
             fold_min(atomicSub(degr + 2u * i + 1u, hi) - hi, inv);
     }
-    wave_min_inv(inv, &ctl->sl_mininv[s]);
+    block_min_inv<kDBlock>(inv, &ctl->sl_mininv[s]);
 }
 
 uint32_t g_dec_grid = 0;
