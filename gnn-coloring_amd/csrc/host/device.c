@@ -1230,10 +1230,11 @@ static int sl_round(void *p, uint32_t r, hipStream_t s)
     uint32_t *keyr = (uint32_t *)ws->key + (g->N + 1);
     /* long rows' chunks (gcxk_sl_chunk_capacity(M) pairs) in E_split, which
      * only the colouring uses */
-    TRY(gcxk_sl_scan(g->V, ws->count, keyr, (uint32_t *)ws->key, g->perm, ws->q[r & 1],
-                     ws->q[(r + 1) & 1], ws->h[0], ws->esplit, ws->ctl, c->log_variant, r, c->grid,
+    /* rows by rank in x, the short-row list in y (the GNN row buffers) */
+    TRY(gcxk_sl_scan(ws->x, ws->count, keyr, (uint32_t *)ws->key, g->perm, ws->q[r & 1],
+                     ws->q[(r + 1) & 1], ws->y, ws->esplit, ws->ctl, c->log_variant, r, c->grid,
                      s));
-    return gcxk_sl_dec(g->N, g->V, g->Er, ws->count, keyr, ws->h[0], ws->esplit, ws->ctl,
+    return gcxk_sl_dec(g->N, g->V, g->Er, ws->count, keyr, ws->y, ws->esplit, ws->ctl,
                        c->log_variant, r, s);
 }
 
@@ -1246,7 +1247,7 @@ static int order_sl(gcx_workspace *ws, const gcx_graph *g, int log_variant, hipS
     /* degrees and alive flags by degree-order rank: count and the upper half
      * of key (key holds 2 (N + 1) words) */
     TRY(gcxk_sl_init(g->N, g->V, g->perm, ws->count, (uint32_t *)ws->key + (g->N + 1), ws->q[0],
-                     ws->ctl, s));
+                     ws->x, ws->ctl, s));
     sl_ctx c = {ws, g, log_variant, g_round_grid};
     TRY(drive_rounds(ws, s, sl_round, &c, offsetof(gcxk_ctl, sl_done), g->N + 2 * ROUND_BATCH + 2));
     return GCX_OK;

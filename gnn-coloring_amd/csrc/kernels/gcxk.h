@@ -136,21 +136,22 @@ int gcxk_copy_bytes(uint64_t bytes, const void *src, void *dst, void *stream);
  * ranks; key (by vertex id) receives each vertex's threshold (delta) at its
  * removal. ctl must be zeroed. */
 int gcxk_sl_init(uint32_t N, const uint32_t *dV, const uint32_t *perm, uint32_t *degr,
-                 uint32_t *keyr, uint32_t *alive0, gcxk_ctl *ctl, void *stream);
-/* Round r, phase 1: remove alive vertices with degree <= threshold. gone
- * receives the removed vertices whose rows have 1 .. 16 entries, chunks
- * (capacity gcxk_sl_chunk_capacity(M) pairs) the CSR ranges [begin, end) of
- * at most 64 entries that cover the longer rows. */
-int gcxk_sl_scan(const uint32_t *dV, const uint32_t *degr, uint32_t *keyr, uint32_t *key,
-                 const uint32_t *perm, const uint32_t *alive_in, uint32_t *alive_out,
-                 uint32_t *gone, void *chunks, gcxk_ctl *ctl, int log_variant, uint32_t r,
-                 uint32_t grid, void *stream);
+                 uint32_t *keyr, uint32_t *alive0, void *rrow, gcxk_ctl *ctl, void *stream);
+/* Round r, phase 1: remove alive vertices with degree <= threshold. rrow (N
+ * pairs, from gcxk_sl_init): each rank's (first entry, degree). gone (N
+ * pairs) receives (first entry, length) of the removed rows with 1 .. 16
+ * entries, chunks (capacity gcxk_sl_chunk_capacity(M) pairs) the CSR ranges
+ * [begin, end) of at most 64 entries that cover the longer rows. */
+int gcxk_sl_scan(const void *rrow, const uint32_t *degr, uint32_t *keyr, uint32_t *key,
+                 const uint32_t *perm, const uint32_t *alive_in, uint32_t *alive_out, void *gone,
+                 void *chunks, gcxk_ctl *ctl, int log_variant, uint32_t r, uint32_t grid,
+                 void *stream);
 uint64_t gcxk_sl_chunk_capacity(uint64_t M);
 /* Round r, phase 2: decrement unremoved neighbours of removed vertices,
  * reading neighbour ranks from Er = rank[E]. Decrements to the first 32768
  * ranks are summed per workgroup in LDS and applied once per rank. */
 int gcxk_sl_dec(uint32_t N, const uint32_t *dV, const uint32_t *Er, uint32_t *degr,
-                const uint32_t *keyr, const uint32_t *gone, const void *chunks, gcxk_ctl *ctl,
+                const uint32_t *keyr, const void *gone, const void *chunks, gcxk_ctl *ctl,
                 int log_variant, uint32_t r, void *stream);
 
 /* ---- GraphSAGE-16 (gnn.hip) ---- */

@@ -76,18 +76,21 @@ __device__ __forceinline__ void fold_min(uint32_t d, uint32_t &inv)
     inv = m > inv ? m : inv;
 }
 
+// rrow[a] = (first entry, degree) of rank a's row, read by the scan in one
+// load instead of perm -> V.
 __global__ __launch_bounds__(kBlock) void sl_init_kernel(uint32_t N, const uint32_t *__restrict__ V,
                                                          const uint32_t *__restrict__ perm,
                                                          uint32_t *__restrict__ degr,
                                                          uint32_t *__restrict__ keyr,
                                                          uint32_t *__restrict__ alive0,
-                                                         gcxk_ctl *ctl)
+                                                         uint2 *__restrict__ rrow, gcxk_ctl *ctl)
 {
     const uint32_t a = blockIdx.x * kBlock + threadIdx.x;
     if (a < N) {
         const uint32_t u = perm[a];
-        const uint32_t d = V[u + 1] - V[u];
+        const uint32_t b = V[u], d = V[u + 1] - b;
         degr[a] = d;
+        rrow[a] = make_uint2(b, d);
         keyr[a] = kAlive;
         alive0[a] = a;
         // the last rank has the smallest degree: it goes to slot 2 (the
@@ -119,17 +122,18 @@ __device__ __forceinline__ uint32_t round_delta(const gcxk_ctl *ctl, int log_var
 }
 
 // alive_in / alive_out hold ranks. A removed vertex's threshold goes to key
-// (by id); its row goes to gone when it has 1 .. kLaneRow entries, else to
-// chunks as ranges of at most kChunk entries, so that the decrement pass has
-// only equal work items and no workgroup-wide steps.
-__global__ __launch_bounds__(kBlock) void sl_scan_kernel(const uint32_t *__restrict__ V,
+// (by id); its row (first entry, length) goes to gone when it has 1 ..
+// kLaneRow entries, else to chunks as ranges of at most kChunk entries, so
+// that the decrement pass has only equal work items and no workgroup-wide
+// steps.
+__global__ __launch_bounds__(kBlock) void sl_scan_kernel(const uint2 *__restrict__ rrow,
                                                          const uint32_t *__restrict__ degr,
                                                          uint32_t *__restrict__ keyr,
                                                          uint32_t *__restrict__ key,
                                                          const uint32_t *__restrict__ perm,
                                                          const uint32_t *__restrict__ alive_in,
                                                          uint32_t *__restrict__ alive_out,
-                                                         uint32_t *__restrict__ gone,
+                                                         uint2 *__restrict__ gone,
                                                          uint2 *__restrict__ chunks, gcxk_ctl *ctl,
                                                          int log_variant, uint32_t r)
 {
@@ -185,12 +189,11 @@ __global__ __launch_bounds__(kBlock) void sl_scan_kernel(const uint32_t *__restr
             }
         }
 #pragma unroll
-        for (int q = 0; q < kScanPer; q++)
-            u[q] = rm[q] ? perm[a[q]] : 0u;
-#pragma unroll
         for (int q = 0; q < kScanPer; q++) {
-            beg[q] = rm[q] ? V[u[q]] : 0u;
-            len[q] = rm[q] ? V[u[q] + 1] - beg[q] : 0u;
+            const uint2 rw = rm[q] ? rrow[a[q]] : make_uint2(0u, 0u);
+            u[q] = rm[q] ? perm[a[q]] : 0u; // for key only
+            beg[q] = rw.x;
+            len[q] = rw.y;
         }
         uint32_t cr = 0, ck = 0, cc = 0;
 #pragma unroll
@@ -241,7 +244,7 @@ __global__ __launch_bounds__(kBlock) void sl_scan_kernel(const uint32_t *__restr
                     for (uint32_t b = beg[q]; b < end; b += kChunk)
                         chunks[pc++] = make_uint2(b, end - b > kChunk ? b + kChunk : end);
                 } else if (nch == 0 && len[q]) {
-                    gone[pr++] = u[q];
+                    gone[pr++] = make_uint2(beg[q], len[q]);
                 }
             } else if (keep[q]) {
                 alive_out[pk++] = a[q];
@@ -312,7 +315,7 @@ __global__ __launch_bounds__(kDBlock) void sl_dec_kernel(const uint32_t *__restr
                                                          const uint32_t *__restrict__ Er,
                                                          uint32_t *__restrict__ degr,
                                                          const uint32_t *__restrict__ keyr,
-                                                         const uint32_t *__restrict__ gone,
+                                                         const uint2 *__restrict__ gone,
                                                          const uint2 *__restrict__ chunks,
                                                          uint32_t nsum, gcxk_ctl *ctl, uint32_t r)
 {
@@ -348,8 +351,8 @@ __global__ __launch_bounds__(kDBlock) void sl_dec_kernel(const uint32_t *__restr
     __syncthreads();
     uint32_t inv = 0;
     for (uint32_t k = blockIdx.x * kDBlock + threadIdx.x; k < n; k += gridDim.x * kDBlock) {
-        const uint32_t u = gone[k];
-        inv = dec_row<kDecSteps>(V[u], V[u + 1], 0u, 1u, Er, degr, keyr, nsum, s_sum, inv);
+        const uint2 row = gone[k];
+        inv = dec_row<kDecSteps>(row.x, row.x + row.y, 0u, 1u, Er, degr, keyr, nsum, s_sum, inv);
     }
     const uint32_t nw = gridDim.x * kDWaves;
     const uint32_t lane = (uint32_t)lane_id();
@@ -395,10 +398,11 @@ uint32_t g_dec_grid = 0;
 } // namespace
 
 extern "C" int gcxk_sl_init(uint32_t N, const uint32_t *dV, const uint32_t *perm, uint32_t *degr,
-                            uint32_t *keyr, uint32_t *alive0, gcxk_ctl *ctl, void *stream)
+                            uint32_t *keyr, uint32_t *alive0, void *rrow, gcxk_ctl *ctl,
+                            void *stream)
 {
-    sl_init_kernel<<<div_up(N ? N : 1, kBlock), kBlock, 0, as_stream(stream)>>>(N, dV, perm, degr,
-                                                                                 keyr, alive0, ctl);
+    sl_init_kernel<<<div_up(N ? N : 1, kBlock), kBlock, 0, as_stream(stream)>>>(
+        N, dV, perm, degr, keyr, alive0, static_cast<uint2 *>(rrow), ctl);
     GCXK_LAUNCH_CHECK();
     return GCX_OK;
 }
@@ -410,20 +414,20 @@ extern "C" uint64_t gcxk_sl_chunk_capacity(uint64_t M)
     return M / kLaneRow + 1;
 }
 
-extern "C" int gcxk_sl_scan(const uint32_t *dV, const uint32_t *degr, uint32_t *keyr,
+extern "C" int gcxk_sl_scan(const void *rrow, const uint32_t *degr, uint32_t *keyr,
                             uint32_t *key, const uint32_t *perm, const uint32_t *alive_in,
-                            uint32_t *alive_out, uint32_t *gone, void *chunks, gcxk_ctl *ctl,
+                            uint32_t *alive_out, void *gone, void *chunks, gcxk_ctl *ctl,
                             int log_variant, uint32_t r, uint32_t grid, void *stream)
 {
     sl_scan_kernel<<<grid, kBlock, 0, as_stream(stream)>>>(
-        dV, degr, keyr, key, perm, alive_in, alive_out, gone, static_cast<uint2 *>(chunks), ctl,
-        log_variant, r);
+        static_cast<const uint2 *>(rrow), degr, keyr, key, perm, alive_in, alive_out,
+        static_cast<uint2 *>(gone), static_cast<uint2 *>(chunks), ctl, log_variant, r);
     GCXK_LAUNCH_CHECK();
     return GCX_OK;
 }
 
 extern "C" int gcxk_sl_dec(uint32_t N, const uint32_t *dV, const uint32_t *Er, uint32_t *degr,
-                           const uint32_t *keyr, const uint32_t *gone, const void *chunks,
+                           const uint32_t *keyr, const void *gone, const void *chunks,
                            gcxk_ctl *ctl, int log_variant, uint32_t r, void *stream)
 {
     if (!g_dec_grid) {
@@ -442,7 +446,8 @@ extern "C" int gcxk_sl_dec(uint32_t N, const uint32_t *dV, const uint32_t *Er, u
     const uint32_t grid = log_variant ? g_dec_grid : (g_dec_grid + 1) / 2;
     const uint32_t nsum = N < kSumSlots ? N : kSumSlots;
     sl_dec_kernel<<<grid, kDBlock, 0, as_stream(stream)>>>(
-        dV, Er, degr, keyr, gone, static_cast<const uint2 *>(chunks), nsum, ctl, r);
+        dV, Er, degr, keyr, static_cast<const uint2 *>(gone), static_cast<const uint2 *>(chunks),
+        nsum, ctl, r);
     GCXK_LAUNCH_CHECK();
     return GCX_OK;
 }
