@@ -1228,13 +1228,14 @@ static int sl_round(void *p, uint32_t r, hipStream_t s)
     gcx_workspace *ws = c->ws;
     const gcx_graph *g = c->g;
     uint32_t *keyr = (uint32_t *)ws->key + (g->N + 1);
-    /* long rows' chunks (gcxk_sl_chunk_capacity(M) pairs) in E_split, which
-     * only the colouring uses */
-    /* rows by rank in x, the short-row list in y (the GNN row buffers) */
+    /* long rows' chunks (gcxk_sl_chunk_capacity(M) pairs, at most M / 16 +
+     * M / 2048 + 2 <= M / 2 once a row is long) in E_split, which only the
+     * colouring uses; rows by rank in x, the short-row list in y (the GNN row
+     * buffers) */
     TRY(gcxk_sl_scan(ws->x, ws->count, keyr, (uint32_t *)ws->key, g->perm, ws->q[r & 1],
-                     ws->q[(r + 1) & 1], ws->y, ws->esplit, ws->ctl, c->log_variant, r, c->grid,
-                     s));
-    return gcxk_sl_dec(g->N, g->V, g->Er, ws->count, keyr, ws->y, ws->esplit, ws->ctl,
+                     ws->q[(r + 1) & 1], ws->y, ws->esplit, g->M, ws->ctl, c->log_variant, r,
+                     c->grid, s));
+    return gcxk_sl_dec(g->N, g->V, g->Er, ws->count, keyr, ws->y, ws->esplit, g->M, ws->ctl,
                        c->log_variant, r, s);
 }
 

@@ -37,6 +37,7 @@ typedef struct {
     uint32_t err;           /* JP (ordered): a wave hit its wait limit */
     uint32_t head_n;        /* JP (ordered): greedy-order positions the head kernel colours */
     uint32_t sl_chunks[3];  /* SL: removed-this-round long-row chunk list length */
+    uint32_t sl_supers[3];  /* SL: removed-this-round super-chunk list length */
 } gcxk_ctl;
 
 /* ---- graph helpers (csr.hip) ---- */
@@ -140,19 +141,21 @@ int gcxk_sl_init(uint32_t N, const uint32_t *dV, const uint32_t *perm, uint32_t 
 /* Round r, phase 1: remove alive vertices with degree <= threshold. rrow (N
  * pairs, from gcxk_sl_init): each rank's (first entry, degree). gone (N
  * pairs) receives (first entry, length) of the removed rows with 1 .. 16
- * entries, chunks (capacity gcxk_sl_chunk_capacity(M) pairs) the CSR ranges
- * [begin, end) of at most 64 entries that cover the longer rows. */
+ * entries; chunks (capacity gcxk_sl_chunk_capacity(M) pairs, M = CSR
+ * entries) the CSR ranges [begin, end) of at most 64 entries that cover the
+ * rows of 17 .. 4096 entries, followed by the ranges of at most 4096 entries
+ * that cover the longer rows. */
 int gcxk_sl_scan(const void *rrow, const uint32_t *degr, uint32_t *keyr, uint32_t *key,
                  const uint32_t *perm, const uint32_t *alive_in, uint32_t *alive_out, void *gone,
-                 void *chunks, gcxk_ctl *ctl, int log_variant, uint32_t r, uint32_t grid,
-                 void *stream);
+                 void *chunks, uint64_t M, gcxk_ctl *ctl, int log_variant, uint32_t r,
+                 uint32_t grid, void *stream);
 uint64_t gcxk_sl_chunk_capacity(uint64_t M);
 /* Round r, phase 2: decrement unremoved neighbours of removed vertices,
  * reading neighbour ranks from Er = rank[E]. Decrements to the first 32768
  * ranks are summed per workgroup in LDS and applied once per rank. */
 int gcxk_sl_dec(uint32_t N, const uint32_t *dV, const uint32_t *Er, uint32_t *degr,
-                const uint32_t *keyr, const void *gone, const void *chunks, gcxk_ctl *ctl,
-                int log_variant, uint32_t r, void *stream);
+                const uint32_t *keyr, const void *gone, const void *chunks, uint64_t M,
+                gcxk_ctl *ctl, int log_variant, uint32_t r, void *stream);
 
 /* ---- GraphSAGE-16 (gnn.hip) ---- */
 /* FP32, bit-exact with the reference op order. kind 0 = input layer (reads

@@ -48,6 +48,13 @@ constexpr uint32_t kLaneRow = 16;   // rows up to this length: a thread each
 constexpr uint32_t kChunk = kWave;  // longer rows: chunks of a lane per entry
 constexpr int kChunkSteps = 4;
 constexpr uint32_t kOwnChunks = 4;  // chunks of a row its own thread lists
+// Rows of more than kSuperChunks chunks (removed hubs, up to hundreds of
+// thousands of entries) are listed as super chunks of kSuperChunks * kChunk
+// entries, which the decrement pass cuts into chunks: writing a hub's
+// thousands of chunk descriptors fell to one wave of the scan and made the
+// late SL rounds' scans 5-10x longer than their median.
+constexpr uint32_t kSuperChunks = 64;
+constexpr uint32_t kSuperLen = kSuperChunks * kChunk;
 
 // Folds m = UINT32_MAX - value (0 = no contribution) into *slot with
 // atomicMax, one atomic per workgroup of T threads (one per wave put
@@ -123,7 +130,8 @@ __device__ __forceinline__ uint32_t round_delta(const gcxk_ctl *ctl, int log_var
 
 // alive_in / alive_out hold ranks. A removed vertex's threshold goes to key
 // (by id); its row (first entry, length) goes to gone when it has 1 ..
-// kLaneRow entries, else to chunks as ranges of at most kChunk entries, so
+// kLaneRow entries, else to chunks as ranges of at most kChunk entries, or,
+// past kSuperChunks chunks, to supers as ranges of at most kSuperLen, so
 // that the decrement pass has only equal work items and no workgroup-wide
 // steps.
 __global__ __launch_bounds__(kBlock) void sl_scan_kernel(const uint2 *__restrict__ rrow,
@@ -134,7 +142,8 @@ __global__ __launch_bounds__(kBlock) void sl_scan_kernel(const uint2 *__restrict
                                                          const uint32_t *__restrict__ alive_in,
                                                          uint32_t *__restrict__ alive_out,
                                                          uint2 *__restrict__ gone,
-                                                         uint2 *__restrict__ chunks, gcxk_ctl *ctl,
+                                                         uint2 *__restrict__ chunks,
+                                                         uint2 *__restrict__ supers, gcxk_ctl *ctl,
                                                          int log_variant, uint32_t r)
 {
     const uint32_t s = r % 3, s1 = (r + 1) % 3, s2 = (r + 2) % 3;
@@ -145,6 +154,7 @@ __global__ __launch_bounds__(kBlock) void sl_scan_kernel(const uint2 *__restrict
         // Slot s1 is next round's: its work lists and min start empty.
         ctl->sl_gone[s1] = 0;
         ctl->sl_chunks[s1] = 0;
+        ctl->sl_supers[s1] = 0;
         ctl->sl_mininv[s1] = 0;
         // Slot s2 held round r-1's input; it becomes round r+1's input.
         ctl->sl_alive[s2] = 0;
@@ -159,8 +169,8 @@ __global__ __launch_bounds__(kBlock) void sl_scan_kernel(const uint2 *__restrict
     // Tiles of kBlock * kScanPer entries: one reservation per list per tile
     // (all blocks append to the same counters, whose atomics serialise).
     constexpr int kW = kBlock / kWave;
-    __shared__ uint32_t s_cnt[3][kW];
-    __shared__ uint32_t s_base[3];
+    __shared__ uint32_t s_cnt[4][kW];
+    __shared__ uint32_t s_base[4];
     const uint32_t tile = kBlock * kScanPer;
     const uint32_t ntiles = (n + tile - 1) / tile;
     const int lane = lane_id(), w = threadIdx.x / kWave;
@@ -195,30 +205,35 @@ __global__ __launch_bounds__(kBlock) void sl_scan_kernel(const uint2 *__restrict
             beg[q] = rw.x;
             len[q] = rw.y;
         }
-        uint32_t cr = 0, ck = 0, cc = 0;
+        uint32_t cr = 0, ck = 0, cc = 0, cs = 0;
 #pragma unroll
         for (int q = 0; q < kScanPer; q++) {
+            const uint32_t nch = len[q] > kLaneRow ? (len[q] + kChunk - 1u) / kChunk : 0u;
             cr += (len[q] >= 1u && len[q] <= kLaneRow) ? 1u : 0u;
             ck += keep[q] ? 1u : 0u;
-            cc += len[q] > kLaneRow ? (len[q] + kChunk - 1u) / kChunk : 0u;
+            cc += nch <= kSuperChunks ? nch : 0u;
+            cs += nch > kSuperChunks ? (len[q] + kSuperLen - 1u) / kSuperLen : 0u;
         }
-        uint32_t ir = cr, ik = ck, ic = cc; // inclusive scans over the wave
+        uint32_t ir = cr, ik = ck, ic = cc, is = cs; // inclusive scans over the wave
 #pragma unroll
         for (int off = 1; off < kWave; off <<= 1) {
-            const uint32_t x = __shfl_up(ir, off), y = __shfl_up(ik, off), z = __shfl_up(ic, off);
+            const uint32_t x = __shfl_up(ir, off), y = __shfl_up(ik, off), z = __shfl_up(ic, off),
+                           v = __shfl_up(is, off);
             if (lane >= off) {
                 ir += x;
                 ik += y;
                 ic += z;
+                is += v;
             }
         }
         if (lane == kWave - 1) {
             s_cnt[0][w] = ir;
             s_cnt[1][w] = ik;
             s_cnt[2][w] = ic;
+            s_cnt[3][w] = is;
         }
         __syncthreads();
-        if (threadIdx.x < 3) {
+        if (threadIdx.x < 4) {
             uint32_t tot = 0;
             for (int q = 0; q < kW; q++) {
                 const uint32_t x = s_cnt[threadIdx.x][q];
@@ -227,13 +242,15 @@ __global__ __launch_bounds__(kBlock) void sl_scan_kernel(const uint2 *__restrict
             }
             uint32_t *ctr = threadIdx.x == 0   ? &ctl->sl_gone[s]
                             : threadIdx.x == 1 ? &ctl->sl_alive[s1]
-                                               : &ctl->sl_chunks[s];
+                            : threadIdx.x == 2 ? &ctl->sl_chunks[s]
+                                               : &ctl->sl_supers[s];
             s_base[threadIdx.x] = tot ? atomicAdd(ctr, tot) : 0u;
         }
         __syncthreads();
         uint32_t pr = s_base[0] + s_cnt[0][w] + ir - cr;
         uint32_t pk = s_base[1] + s_cnt[1][w] + ik - ck;
         uint32_t pc = s_base[2] + s_cnt[2][w] + ic - cc;
+        uint32_t ps = s_base[3] + s_cnt[3][w] + is - cs;
 #pragma unroll
         for (int q = 0; q < kScanPer; q++) {
             const uint32_t nch = len[q] > kLaneRow ? (len[q] + kChunk - 1u) / kChunk : 0u;
@@ -249,20 +266,30 @@ __global__ __launch_bounds__(kBlock) void sl_scan_kernel(const uint2 *__restrict
             } else if (keep[q]) {
                 alive_out[pk++] = a[q];
             }
-            // rows of more than kOwnChunks chunks (removed hubs: up to
-            // thousands) are written by the whole wave, one row at a time
+            // rows of more than kOwnChunks chunks are written by the whole
+            // wave, one row at a time: as chunks up to kSuperChunks, longer
+            // rows as super chunks
             uint64_t big = __ballot(rm[q] && nch > kOwnChunks);
-            const uint32_t pc_row = pc;
-            if (rm[q] && nch > kOwnChunks)
-                pc += nch;
+            const bool sup = nch > kSuperChunks;
+            const uint32_t nunit = sup ? (len[q] + kSuperLen - 1u) / kSuperLen : nch;
+            const uint32_t p_row = sup ? ps : pc;
+            if (rm[q] && nch > kOwnChunks) {
+                if (sup)
+                    ps += nunit;
+                else
+                    pc += nunit;
+            }
             while (big) {
                 const int src = __builtin_ctzll(big);
                 big &= big - 1;
                 const uint32_t b0 = __shfl(beg[q], src), e0 = b0 + __shfl(len[q], src);
-                const uint32_t p0 = __shfl(pc_row, src), n0 = __shfl(nch, src);
+                const uint32_t p0 = __shfl(p_row, src), n0 = __shfl(nunit, src);
+                const bool s0 = __shfl((int)sup, src) != 0;
+                uint2 *dst = s0 ? supers : chunks;
+                const uint32_t unit = s0 ? kSuperLen : kChunk;
                 for (uint32_t j = (uint32_t)lane; j < n0; j += kWave) {
-                    const uint32_t b = b0 + j * kChunk;
-                    chunks[p0 + j] = make_uint2(b, e0 - b > kChunk ? b + kChunk : e0);
+                    const uint32_t b = b0 + j * unit;
+                    dst[p0 + j] = make_uint2(b, e0 - b > unit ? b + unit : e0);
                 }
             }
         }
@@ -310,20 +337,24 @@ __device__ __forceinline__ uint32_t dec_row(uint32_t beg, uint32_t end, uint32_t
 
 // Short rows (at most kLaneRow entries): a thread each; long rows: a wave
 // per chunk of kChunk entries (a lane per entry), kChunkSteps chunks in
-// flight. No workgroup-wide steps until the LDS sums are flushed.
+// flight. Super chunk x stands for chunks nc + x * kSuperChunks .. + 63 of
+// the chunk sequence (empty past its end). No workgroup-wide steps until the
+// LDS sums are flushed.
 __global__ __launch_bounds__(kDBlock) void sl_dec_kernel(const uint32_t *__restrict__ V,
                                                          const uint32_t *__restrict__ Er,
                                                          uint32_t *__restrict__ degr,
                                                          const uint32_t *__restrict__ keyr,
                                                          const uint2 *__restrict__ gone,
                                                          const uint2 *__restrict__ chunks,
+                                                         const uint2 *__restrict__ supers,
                                                          uint32_t nsum, gcxk_ctl *ctl, uint32_t r)
 {
     __shared__ uint32_t s_sum[kSumWords];
     const uint32_t s = r % 3;
     const uint32_t n = ctl->sl_gone[s], nc = ctl->sl_chunks[s];
+    const uint32_t nu = nc + ctl->sl_supers[s] * kSuperChunks; // chunks in all
     // nothing left alive (the last round): no degree to lower
-    if ((n == 0 && nc == 0) || ctl->sl_alive[(r + 1) % 3] == 0)
+    if ((n == 0 && nu == 0) || ctl->sl_alive[(r + 1) % 3] == 0)
         return;
     // LDS sums over the first nsum ranks, nsum sized to the round: the
     // power of two from kMinSumSlots that covers the entries a workgroup
@@ -334,7 +365,7 @@ __global__ __launch_bounds__(kDBlock) void sl_dec_kernel(const uint32_t *__restr
     // workgroup, from 8192; 1x from 4096; 1x from 1024): SL 12.9 / 12.7 /
     // 12.5 / 12.3, SLL 3.53 - 3.54
     {
-        const uint64_t per_wg = ((uint64_t)n * (kLaneRow / 2u) + (uint64_t)nc * kChunk) / gridDim.x;
+        const uint64_t per_wg = ((uint64_t)n * (kLaneRow / 2u) + (uint64_t)nu * kChunk) / gridDim.x;
         uint32_t want = kMinSumSlots;
         while (want < nsum && (uint64_t)want < per_wg)
             want <<= 1;
@@ -342,7 +373,7 @@ __global__ __launch_bounds__(kDBlock) void sl_dec_kernel(const uint32_t *__restr
     }
     const uint64_t rows_per = kDBlock, chunks_per = (uint64_t)kDWaves * kChunkSteps;
     const uint64_t rows = (n + gridDim.x * rows_per - 1) / (gridDim.x * rows_per) * rows_per;
-    const uint64_t chs = (nc + gridDim.x * chunks_per - 1) / (gridDim.x * chunks_per) * chunks_per;
+    const uint64_t chs = (nu + gridDim.x * chunks_per - 1) / (gridDim.x * chunks_per) * chunks_per;
     if (rows + chs > kMaxRowsPerGroup)
         nsum = 0; // a 16-bit sum could overflow
     const uint32_t words = (nsum + 1u) / 2u;
@@ -356,13 +387,22 @@ __global__ __launch_bounds__(kDBlock) void sl_dec_kernel(const uint32_t *__restr
     }
     const uint32_t nw = gridDim.x * kDWaves;
     const uint32_t lane = (uint32_t)lane_id();
-    for (uint32_t c0 = blockIdx.x * kDWaves + threadIdx.x / kWave; c0 < nc; c0 += nw * kChunkSteps) {
+    for (uint32_t c0 = blockIdx.x * kDWaves + threadIdx.x / kWave; c0 < nu; c0 += nw * kChunkSteps) {
         uint32_t t[kChunkSteps], kv[kChunkSteps];
         bool in[kChunkSteps];
 #pragma unroll
         for (int q = 0; q < kChunkSteps; q++) {
-            const uint32_t c = c0 + (uint32_t)q * nw;
-            const uint2 ch = c < nc ? chunks[c] : make_uint2(0u, 0u);
+            const uint32_t c = c0 + (uint32_t)q * nw; // wave-uniform
+            uint2 ch = make_uint2(0u, 0u);
+            if (c < nc) {
+                ch = chunks[c];
+            } else if (c < nu) {
+                const uint32_t x = c - nc;
+                const uint2 sp = supers[x / kSuperChunks];
+                const uint32_t b = sp.x + (x % kSuperChunks) * kChunk;
+                if (b < sp.y) // the last super chunk of a row can end early
+                    ch = make_uint2(b, sp.y - b > kChunk ? b + kChunk : sp.y);
+            }
             in[q] = ch.x + lane < ch.y;
             t[q] = in[q] ? Er[ch.x + lane] : 0u;
         }
@@ -407,27 +447,39 @@ extern "C" int gcxk_sl_init(uint32_t N, const uint32_t *dV, const uint32_t *perm
     return GCX_OK;
 }
 
+namespace {
+
+// A row of kLaneRow < len <= kSuperLen entries takes ceil(len / kChunk) <=
+// len / 16 chunks (kChunk = 64 = 4 * kLaneRow): at most M / 16 over all
+// rows. A longer row takes ceil(len / kSuperLen) <= len / 2048 super chunks
+// (len > kSuperLen = 4096): at most M / 2048. The super list follows the
+// chunk list.
+uint64_t chunk_list_capacity(uint64_t M) { return M / kLaneRow + 1; }
+uint64_t super_list_capacity(uint64_t M) { return M / (kSuperLen / 2u) + 1; }
+
+} // namespace
+
 extern "C" uint64_t gcxk_sl_chunk_capacity(uint64_t M)
 {
-    // a row of len > kLaneRow entries takes ceil(len / kChunk) <= len / 16
-    // chunks (kChunk = 64 = 4 * kLaneRow): at most M / 16 over all rows
-    return M / kLaneRow + 1;
+    return chunk_list_capacity(M) + super_list_capacity(M);
 }
 
 extern "C" int gcxk_sl_scan(const void *rrow, const uint32_t *degr, uint32_t *keyr,
                             uint32_t *key, const uint32_t *perm, const uint32_t *alive_in,
-                            uint32_t *alive_out, void *gone, void *chunks, gcxk_ctl *ctl,
-                            int log_variant, uint32_t r, uint32_t grid, void *stream)
+                            uint32_t *alive_out, void *gone, void *chunks, uint64_t M,
+                            gcxk_ctl *ctl, int log_variant, uint32_t r, uint32_t grid,
+                            void *stream)
 {
+    uint2 *ch = static_cast<uint2 *>(chunks);
     sl_scan_kernel<<<grid, kBlock, 0, as_stream(stream)>>>(
         static_cast<const uint2 *>(rrow), degr, keyr, key, perm, alive_in, alive_out,
-        static_cast<uint2 *>(gone), static_cast<uint2 *>(chunks), ctl, log_variant, r);
+        static_cast<uint2 *>(gone), ch, ch + chunk_list_capacity(M), ctl, log_variant, r);
     GCXK_LAUNCH_CHECK();
     return GCX_OK;
 }
 
 extern "C" int gcxk_sl_dec(uint32_t N, const uint32_t *dV, const uint32_t *Er, uint32_t *degr,
-                           const uint32_t *keyr, const void *gone, const void *chunks,
+                           const uint32_t *keyr, const void *gone, const void *chunks, uint64_t M,
                            gcxk_ctl *ctl, int log_variant, uint32_t r, void *stream)
 {
     if (!g_dec_grid) {
@@ -445,8 +497,9 @@ extern "C" int gcxk_sl_dec(uint32_t N, const uint32_t *dV, const uint32_t *Er, u
     // The 27-point stencil (160^3) is within 0.1 ms of its best at either.
     const uint32_t grid = log_variant ? g_dec_grid : (g_dec_grid + 1) / 2;
     const uint32_t nsum = N < kSumSlots ? N : kSumSlots;
+    const uint2 *ch = static_cast<const uint2 *>(chunks);
     sl_dec_kernel<<<grid, kDBlock, 0, as_stream(stream)>>>(
-        dV, Er, degr, keyr, static_cast<const uint2 *>(gone), static_cast<const uint2 *>(chunks),
+        dV, Er, degr, keyr, static_cast<const uint2 *>(gone), ch, ch + chunk_list_capacity(M),
         nsum, ctl, r);
     GCXK_LAUNCH_CHECK();
     return GCX_OK;

@@ -187,6 +187,42 @@ def test_hub_heavy_random_graph(gmodels):
         np.testing.assert_array_equal(ws.colors(n), want, err_msg=h)
 
 
+@pytest.mark.parametrize("seed", [0, 1])
+def test_sl_rows_past_4096_entries(seed, gmodels):
+    # SL / SLL list rows longer than 4096 entries as 4096-entry ranges that
+    # the decrement pass cuts into 64-entry chunks: rows of exactly 4096 (the
+    # longest plain chunk list), 4097 and 8193 (a one-entry last chunk), 8192
+    # and 12000 entries, joined to each other; a 60-clique outlives them, so
+    # their removal round still lowers degrees
+    rng = np.random.default_rng(40 + seed)
+    n = 30000
+    lens = [4096, 4097, 8192, 8193, 12000, 5000]
+    pick = rng.permutation(n)
+    hubs, core, rest = pick[:len(lens)], pick[len(lens):len(lens) + 60], pick[len(lens):]
+    I = [np.repeat(hubs, len(hubs)), np.repeat(core, len(core))]
+    J = [np.tile(hubs, len(hubs)), np.tile(core, len(core))]
+    for h, l in zip(hubs, lens):
+        I.append(np.full(l - (len(hubs) - 1), h))
+        J.append(rng.choice(rest, l - (len(hubs) - 1), replace=False))
+    I.append(rng.choice(rest, 90000))
+    J.append(rng.choice(rest, 90000))
+    I = np.concatenate(I).astype(np.uint32)
+    J = np.concatenate(J).astype(np.uint32)
+    V, E = oracle.csr_from_pairs(n, I, J)
+    assert np.diff(V)[hubs].tolist() == lens
+    g = gcx.Graph.from_csr(V, E)
+    ws = gcx.Workspace(n, len(E))
+    for h in ["SL", "SLL"]:
+        order(ws, g, h, gmodels)
+        p = ws.priorities(n)
+        want_p = oracle.key(h, V, E)[0]
+        assert want_p[hubs].max() < want_p[core].min(), h
+        assert p.view(np.uint64).tolist() == want_p.view(np.uint64).tolist(), h
+        ws.color(g)
+        want, _ = oracle.greedy(V, E, p)
+        np.testing.assert_array_equal(ws.colors(n), want, err_msg=h)
+
+
 def test_single_vertex_and_isolated(gmodels):
     for n in [1, 2, 17]:
         V = np.zeros(n + 1, np.uint32)
