@@ -103,7 +103,7 @@ __device__ __forceinline__ void load_weights(const float *__restrict__ W, int j,
 // The weights sit in LDS as float4 W[i][4q .. 4q+3]; the dense step takes
 // A[i] from the owning lane of the quad.
 constexpr int kQuads = kBlock / 4; // vertices per workgroup step
-constexpr int kUnroll = 8;         // neighbour rows in flight per lane
+constexpr int kUnroll = 8;         // neighbour rows in flight per lane (16: slower)
 
 template <int kind>
 __global__ __launch_bounds__(kBlock) void sage_quads(const uint32_t *__restrict__ V,
@@ -153,12 +153,24 @@ __global__ __launch_bounds__(kBlock) void sage_quads(const uint32_t *__restrict_
                 s3 = s3 + r[t].w;
             }
         }
-        for (; i < end; i++) {
-            const float4 r = X[(size_t)Er[i] * 4 + q];
-            s0 = s0 + r.x;
-            s1 = s1 + r.y;
-            s2 = s2 + r.z;
-            s3 = s3 + r.w;
+        if (i < end) { // the last < kUnroll rows, loads in flight together
+            uint32_t v[kUnroll];
+#pragma unroll
+            for (int t = 0; t < kUnroll; t++)
+                v[t] = i + t < end ? Er[i + t] : 0u;
+            float4 r[kUnroll];
+#pragma unroll
+            for (int t = 0; t < kUnroll; t++)
+                r[t] = i + t < end ? X[(size_t)v[t] * 4 + q] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+            for (int t = 0; t < kUnroll; t++) {
+                if (i + t < end) {
+                    s0 = s0 + r[t].x;
+                    s1 = s1 + r[t].y;
+                    s2 = s2 + r[t].z;
+                    s3 = s3 + r[t].w;
+                }
+            }
         }
         if (fdeg_f > 0.0f) {
             s0 = s0 / fdeg_f;
