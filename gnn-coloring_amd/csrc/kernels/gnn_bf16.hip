@@ -153,12 +153,25 @@ __global__ __launch_bounds__(kBlock) void sage_mfma_quads(const uint32_t *__rest
                     t3 = t3 + r[t].w;
                 }
             }
-            for (; i < be; i++) {
-                const float4 r = load_slice<mode>(x, Er[i], q);
-                t0 = t0 + r.x;
-                t1 = t1 + r.y;
-                t2 = t2 + r.z;
-                t3 = t3 + r.w;
+            if (i < be) { // the last < kUnroll rows, loads in flight together
+                uint32_t v[kUnroll];
+#pragma unroll
+                for (int t = 0; t < kUnroll; t++)
+                    v[t] = i + t < be ? Er[i + t] : 0u;
+                float4 r[kUnroll];
+#pragma unroll
+                for (int t = 0; t < kUnroll; t++)
+                    r[t] = i + t < be ? load_slice<mode>(x, v[t], q)
+                                      : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+                for (int t = 0; t < kUnroll; t++) {
+                    if (i + t < be) {
+                        t0 = t0 + r[t].x;
+                        t1 = t1 + r[t].y;
+                        t2 = t2 + r[t].z;
+                        t3 = t3 + r[t].w;
+                    }
+                }
             }
             s0 = s0 + t0;
             s1 = s1 + t1;
